@@ -1,0 +1,84 @@
+"""ctypes binding of libawk_hip.so (the C-ABI declared in include/awk_hip.h).
+
+The product path is the HIP library only: if ``libawk_hip.so`` is missing or
+fails to load this module raises; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libawk_hip.so")
+
+AWK_OK, AWK_ERR_INVALID, AWK_ERR_HIP, AWK_ERR_ALLOC, AWK_ERR_UNSUPPORTED = range(5)
+BG_ID, BG_ZERO, BG_PARTIAL_ID, BG_PARTIAL_ZERO, BG_CLAMP, BG_WRAP, BG_VAL = range(7)
+FK_EIGEN, FK_DIVFREE = 1, 2
+SPLAT_FIXED64, SPLAT_FIXED20, SPLAT_FLOAT = range(3)
+
+# (name, argtypes); every function returns awk_status (int) unless listed in _RESTYPE.
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+_P3 = ctypes.c_void_p * 3
+SIGNATURES = {
+    "awk_last_error": [],
+    "awk_version": [],
+    "awk_fluid_kernel_create": [ctypes.POINTER(_P), _I, _I, _I, _F, _F, _F, _F, _F, _F, _F, _I],
+    "awk_fluid_kernel_set_params": [_P, _F, _F, _F, _F, _I],
+    "awk_fluid_kernel_destroy": [_P],
+    "awk_fluid_kernel_scratch_bytes": [_P, ctypes.POINTER(ctypes.c_ulonglong)],
+    "awk_fluid_kernel_apply": [_P, _P, _P, _P, _I, _P],
+    "awk_hfield_set_to_identity": [_P, _P, _P, _I, _I, _I, _P],
+    "awk_hfield_velocity_to_h": [_P, _P, _P, _I, _I, _I, _F, _F, _F, _P],
+    "awk_hfield_h_to_velocity": [_P, _P, _P, _I, _I, _I, _F, _F, _F, _P],
+    "awk_hfield_inverse_zeroth_order": [_P, _P, _P, _P, _P, _P, _I, _I, _I, _P],
+    "awk_hfield_apply": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _F, _P],
+    "awk_hfield_compose": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _P],
+    "awk_hfield_compose_hv": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _F, _F, _F, _F, _I,
+                              _I, _P],
+    "awk_hfield_compose_vh": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _F, _F, _F, _I, _I,
+                              _P],
+    "awk_gradient": [_P, _P, _P, _P, _I, _I, _I, _F, _F, _F, _I, _P],
+    "awk_splat": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _P],
+    "awk_greedy_create": [ctypes.POINTER(_P), _I, _I, _I, _F, _F, _F, _F, _F, _F, _F],
+    "awk_greedy_destroy": [_P],
+    "awk_greedy_update_step_next": [_P],
+    "awk_greedy_set_step": [_P, _F],
+    "awk_greedy_get_step": [_P, ctypes.POINTER(_F)],
+    "awk_greedy_set_auto_step_reduce": [_P, _I],
+    "awk_greedy_iterate": [_P, _P, _P, ctypes.POINTER(_P3), ctypes.POINTER(_P3),
+                           ctypes.POINTER(_P3), ctypes.POINTER(_P3), _P, _P],
+}
+_RESTYPE = {"awk_last_error": ctypes.c_char_p, "awk_version": ctypes.c_int}
+
+_lib = None
+
+
+class AwkError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("awk status %d: %s" % (code, msg))
+        self.code = code
+
+
+def load():
+    """Load libawk_hip.so, raising if it is absent (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libawk_hip.so not built (run __graft_entry__.build()): %s" % LIB_PATH)
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPE.get(name, ctypes.c_int)
+        _lib = lib
+    return _lib
+
+
+def check(code):
+    if code != AWK_OK:
+        raise AwkError(code, load().awk_last_error().decode())
+    return code
+
+
+def call(name, *args):
+    return check(getattr(load(), name)(*args))

@@ -1,0 +1,210 @@
+// awk_common.h -- shared device/host helpers for the MI355X (gfx950) hot path.
+//
+// Layout conventions (device):
+//   * scalar image: float[nz][ny][nx], x fastest (the reference Array3D order).
+//   * vector field: three separate float planes x, y, z of the same shape (SoA,
+//     as the reference cplVector3DArray, CUDACommon/src/cudaVector3DArray.cu:46-57).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/awk_hip.h"
+
+namespace awk {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+awk_status fail(awk_status code, const std::string &msg);
+awk_status hip_check(hipError_t e, const char *what);
+
+#define AWK_HIP(call)                                          \
+  do {                                                         \
+    hipError_t _e = (call);                                    \
+    if (_e != hipSuccess) return ::awk::hip_check(_e, #call);  \
+  } while (0)
+
+#define AWK_LAUNCH_CHECK(name)                                 \
+  do {                                                         \
+    hipError_t _e = hipGetLastError();                         \
+    if (_e != hipSuccess) return ::awk::hip_check(_e, name);   \
+  } while (0)
+
+inline hipStream_t as_stream(awk_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Grid size rounded up to a multiple of the XCD count so that xcd_block() is a
+// bijection; blocks past the end exit early.
+constexpr int kNumXcd = 8;
+inline unsigned grid_for(long long items, int per_block) {
+  long long b = (items + per_block - 1) / per_block;
+  b = (b + kNumXcd - 1) / kNumXcd * kNumXcd;
+  return (unsigned)b;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).
+// Remap so that XCD k works through one contiguous 1/8 of the logical blocks in
+// order: spatial neighbours then share an L2.  Requires gridDim.x % 8 == 0.
+__device__ __forceinline__ unsigned xcd_block() {
+  const unsigned b = blockIdx.x, nb = gridDim.x;
+  return (b % kNumXcd) * (nb / kNumXcd) + b / kNumXcd;
+}
+
+// ---------------------------------------------------------------- complex
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+// ---------------------------------------------------------------- trilerp
+// Background strategies (numbering of include/awk_hip.h):
+//   ID, ZERO, PARTIAL_ID, PARTIAL_ZERO, CLAMP, WRAP, VAL(scalar)
+struct Dims {
+  int nx, ny, nz;
+};
+
+__device__ __forceinline__ int wrap_mod(int r, int b) {
+  int m = r % b;
+  return m < 0 ? m + b : m;
+}
+
+// Vector trilerp of a SoA field at (x, y, z) in voxel coordinates, following
+// HField3DUtils::trilerp (UtilitiesDataTypes/HField3DUtils.txx:1969-2300):
+// corner order v0..v7, nested lerp, CPU semantics for PARTIAL_ID (out-of-volume
+// corners take their own integer coordinates).  FP32 weights.
+template <int BG>
+__device__ __forceinline__ float3 trilerp3(const float *__restrict__ fx,
+                                           const float *__restrict__ fy,
+                                           const float *__restrict__ fz, Dims d, float x,
+                                           float y, float z) {
+  const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
+  int x0 = (int)flx, y0 = (int)fly, z0 = (int)flz;
+  int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
+  const float t = x - flx, u = y - fly, v = z - flz;
+  const float omt = 1.f - t, omu = 1.f - u, omv = 1.f - v;
+  bool inside = true;
+  if (BG == AWK_BG_WRAP) {
+    x0 = wrap_mod(x0, d.nx); x1 = wrap_mod(x1, d.nx);
+    y0 = wrap_mod(y0, d.ny); y1 = wrap_mod(y1, d.ny);
+    z0 = wrap_mod(z0, d.nz); z1 = wrap_mod(z1, d.nz);
+  } else if (BG == AWK_BG_CLAMP) {
+    x0 = min(max(x0, 0), d.nx - 1); x1 = min(max(x1, 0), d.nx - 1);
+    y0 = min(max(y0, 0), d.ny - 1); y1 = min(max(y1, 0), d.ny - 1);
+    z0 = min(max(z0, 0), d.nz - 1); z1 = min(max(z1, 0), d.nz - 1);
+  } else {
+    // x0 < n-1 rather than x1 < n: stays correct when a huge coordinate saturates
+    // the float->int conversion (x1 = x0 + 1 would overflow).
+    inside = (x0 >= 0) & (x0 < d.nx - 1) & (y0 >= 0) & (y0 < d.ny - 1) & (z0 >= 0) & (z0 < d.nz - 1);
+  }
+  float c[8][3];
+  if (inside) {
+    const int sxy = d.nx * d.ny;
+    const int b00 = z0 * sxy + y0 * d.nx, b10 = z0 * sxy + y1 * d.nx;
+    const int b01 = z1 * sxy + y0 * d.nx, b11 = z1 * sxy + y1 * d.nx;
+    const int idx[8] = {b00 + x0, b00 + x1, b10 + x1, b10 + x0, b11 + x0, b11 + x1, b01 + x1, b01 + x0};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      c[i][0] = fx[idx[i]];
+      c[i][1] = fy[idx[i]];
+      c[i][2] = fz[idx[i]];
+    }
+  } else {
+    if (BG == AWK_BG_ID) return make_float3(x, y, z);
+    if (BG == AWK_BG_ZERO) return make_float3(0.f, 0.f, 0.f);
+    // PARTIAL_ID / PARTIAL_ZERO
+    // corner in-range flags per axis: floor corner in [0, n), ceil corner in [0, n)
+    const bool xin[2] = {(x0 >= 0) && (x0 < d.nx), (x0 >= -1) && (x0 < d.nx - 1)};
+    const bool yin[2] = {(y0 >= 0) && (y0 < d.ny), (y0 >= -1) && (y0 < d.ny - 1)};
+    const bool zin[2] = {(z0 >= 0) && (z0 < d.nz), (z0 >= -1) && (z0 < d.nz - 1)};
+    const int ox[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+    const int oy[8] = {0, 0, 1, 1, 1, 1, 0, 0};
+    const int oz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const bool in = xin[ox[i]] & yin[oy[i]] & zin[oz[i]];
+      if (in) {
+        const int id = ((z0 + oz[i]) * d.ny + (y0 + oy[i])) * d.nx + (x0 + ox[i]);
+        c[i][0] = fx[id];
+        c[i][1] = fy[id];
+        c[i][2] = fz[id];
+      } else if (BG == AWK_BG_PARTIAL_ID) {
+        // the corner's own integer coordinates (taken from the float floor so that
+        // they stay exact for any finite coordinate)
+        c[i][0] = flx + (float)ox[i];
+        c[i][1] = fly + (float)oy[i];
+        c[i][2] = flz + (float)oz[i];
+      } else {
+        c[i][0] = c[i][1] = c[i][2] = 0.f;
+      }
+    }
+  }
+  float r[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    r[q] = omt * (omu * (c[0][q] * omv + c[7][q] * v) + u * (c[3][q] * omv + c[4][q] * v)) +
+           t * (omu * (c[1][q] * omv + c[6][q] * v) + u * (c[2][q] * omv + c[5][q] * v));
+  return make_float3(r[0], r[1], r[2]);
+}
+
+// Scalar trilerp (Array3DUtils::trilerp, UtilitiesDataTypes/Array3DUtils.txx:59-232).
+// BG: AWK_BG_VAL / AWK_BG_ZERO (background value), AWK_BG_CLAMP, AWK_BG_WRAP.
+template <int BG>
+__device__ __forceinline__ float trilerp1(const float *__restrict__ a, Dims d, float x, float y,
+                                          float z, float background) {
+  const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
+  int x0 = (int)flx, y0 = (int)fly, z0 = (int)flz;
+  int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
+  const float t = x - flx, u = y - fly, v = z - flz;
+  const float omt = 1.f - t, omu = 1.f - u, omv = 1.f - v;
+  float c[8];
+  if (BG == AWK_BG_WRAP || BG == AWK_BG_CLAMP) {
+    if (BG == AWK_BG_WRAP) {
+      x0 = wrap_mod(x0, d.nx); x1 = wrap_mod(x1, d.nx);
+      y0 = wrap_mod(y0, d.ny); y1 = wrap_mod(y1, d.ny);
+      z0 = wrap_mod(z0, d.nz); z1 = wrap_mod(z1, d.nz);
+    } else {
+      x0 = min(max(x0, 0), d.nx - 1); x1 = min(max(x1, 0), d.nx - 1);
+      y0 = min(max(y0, 0), d.ny - 1); y1 = min(max(y1, 0), d.ny - 1);
+      z0 = min(max(z0, 0), d.nz - 1); z1 = min(max(z1, 0), d.nz - 1);
+    }
+    const int sxy = d.nx * d.ny;
+    const int b00 = z0 * sxy + y0 * d.nx, b10 = z0 * sxy + y1 * d.nx;
+    const int b01 = z1 * sxy + y0 * d.nx, b11 = z1 * sxy + y1 * d.nx;
+    c[0] = a[b00 + x0]; c[1] = a[b00 + x1]; c[2] = a[b10 + x1]; c[3] = a[b10 + x0];
+    c[4] = a[b11 + x0]; c[5] = a[b11 + x1]; c[6] = a[b01 + x1]; c[7] = a[b01 + x0];
+  } else {
+    const float bgv = (BG == AWK_BG_ZERO) ? 0.f : background;
+    const bool inside = (x0 >= 0) & (x0 < d.nx - 1) & (y0 >= 0) & (y0 < d.ny - 1) & (z0 >= 0) &
+                        (z0 < d.nz - 1);
+    if (inside) {
+      const int sxy = d.nx * d.ny;
+      const int b00 = z0 * sxy + y0 * d.nx, b10 = z0 * sxy + y1 * d.nx;
+      const int b01 = z1 * sxy + y0 * d.nx, b11 = z1 * sxy + y1 * d.nx;
+      c[0] = a[b00 + x0]; c[1] = a[b00 + x1]; c[2] = a[b10 + x1]; c[3] = a[b10 + x0];
+      c[4] = a[b11 + x0]; c[5] = a[b11 + x1]; c[6] = a[b01 + x1]; c[7] = a[b01 + x0];
+    } else {
+      const bool xin[2] = {(x0 >= 0) && (x0 < d.nx), (x0 >= -1) && (x0 < d.nx - 1)};
+      const bool yin[2] = {(y0 >= 0) && (y0 < d.ny), (y0 >= -1) && (y0 < d.ny - 1)};
+      const bool zin[2] = {(z0 >= 0) && (z0 < d.nz), (z0 >= -1) && (z0 < d.nz - 1)};
+      const int ox[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+      const int oy[8] = {0, 0, 1, 1, 1, 1, 0, 0};
+      const int oz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const bool in = xin[ox[i]] & yin[oy[i]] & zin[oz[i]];
+        c[i] = in ? a[((z0 + oz[i]) * d.ny + (y0 + oy[i])) * d.nx + (x0 + ox[i])] : bgv;
+      }
+    }
+  }
+  return omt * (omu * (c[0] * omv + c[7] * v) + u * (c[3] * omv + c[4] * v)) +
+         t * (omu * (c[1] * omv + c[6] * v) + u * (c[2] * omv + c[5] * v));
+}
+
+}  // namespace awk

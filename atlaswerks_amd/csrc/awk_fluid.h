@@ -1,0 +1,86 @@
+// awk_fluid.h -- host-side structures of the FluidKernelFFT operator and the
+// pass launchers shared by the standalone operator and the fused greedy step.
+#pragma once
+
+#include <vector>
+
+#include "awk_fft.h"
+
+namespace awk {
+
+// Per-axis host plan: radix list and the device twiddle table.
+struct AxisPlan {
+  int n = 0;
+  std::vector<int> rad;
+  Axis dev() const;
+  float2 *tw = nullptr;  // device
+};
+
+// Epilogue of the inverse x pass.
+enum XStore {
+  XSTORE_PLAIN = 0,    // write v to the output planes
+  XSTORE_MAXNORM = 1,  // write v and a per-block max |v|
+  XSTORE_COMPOSE = 2   // h_new(x) = h(x + step * v(x) / sp) (PARTIAL_ID); v not written
+};
+
+// Prologue of the forward x pass.
+enum XLoad {
+  XLOAD_PLAIN = 0,  // read f from the input planes
+  XLOAD_FORCE = 1   // f = (I1 - Idef) * grad(Idef); per-block FP64 partial sums of (I1-Idef)^2
+};
+
+struct XFwdArgs {
+  int mode;
+  const float *in[3];   // PLAIN: input planes
+  const float *idef;    // FORCE: deformed image
+  const float *i1;      // FORCE: target image
+  float sp[3];          // FORCE: spacing for the gradient
+  double *partials;     // FORCE: one double per block (may be null)
+};
+
+struct XInvArgs {
+  int mode;
+  float *out[3];            // PLAIN / MAXNORM: output planes
+  float *maxpart;           // MAXNORM: one float per block
+  const float *h[3];        // COMPOSE: current hfield
+  float *hnew[3];           // COMPOSE: updated hfield
+  const float *step;        // COMPOSE: step size (device scalar)
+  float sp[3];              // COMPOSE: spacing
+};
+
+struct FluidKernel {
+  int nx = 0, ny = 0, nz = 0;
+  int nxh = 0, nxh_pad = 0;
+  float sp[3] = {1, 1, 1};
+  float alpha = 0, beta = 0, gamma = 0, lpow = 1;
+  int flags = 0;
+  AxisPlan ax[3];
+  // launch shapes
+  int rt = 8;          // spatial rows per x-pass block (even)
+  int tx_y = 16;       // kx columns per y-pass block
+  int tx_z = 8;        // kx columns per z-pass block
+  int bd_x = 256, bd_y = 256, bd_z = 256;
+  // device buffers
+  float2 *spec = nullptr;  // 3 planes of nz*ny*nxh_pad complex
+  float *lut = nullptr;    // cwx[nxh] swx[nxh] cwy[ny] swy[ny] cwz[nz] swz[nz]
+  float *eig = nullptr;    // 6 floats per frequency: symmetric L^p (eigen path)
+  int eig_dir = 0;         // which power eig holds: +1 (L^lpow) or -1 (L^-lpow), 0 none
+  size_t spec_plane() const { return (size_t)nz * ny * nxh_pad; }
+  size_t scratch_bytes = 0;
+};
+
+awk_status fk_init(FluidKernel &k, int nx, int ny, int nz, const float sp[3]);
+void fk_release(FluidKernel &k);
+awk_status fk_prepare_eigen(FluidKernel &k, int dir, hipStream_t s);
+// Pass launchers.
+awk_status fk_xfwd(FluidKernel &k, const XFwdArgs &a, hipStream_t s, int *nblocks);
+awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s);
+awk_status fk_zsolve(FluidKernel &k, bool inverse_op, hipStream_t s);
+awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblocks);
+int fk_xblocks(const FluidKernel &k);
+
+}  // namespace awk
+
+struct awk_fluid_kernel {
+  awk::FluidKernel k;
+};

@@ -1,0 +1,668 @@
+// awk_fluid.hip -- FluidKernelFFT: L and L^-1 of the Navier operator
+//   L = -alpha*Lap - beta*grad(div) + gamma
+// applied through a 3-D real FFT, on gfx950.
+//
+// Semantics follow DiffOper::ApplyInverseOperatorOnTheFly / ApplyOperatorOnTheFly
+// (Algorithms/Common/DiffOper.cxx:747-905): scale by 1/N, forward r2c of each
+// component, per-frequency 3x3 L built from the LUT of DiffOper.cxx:587-639,
+// Cholesky solve (DiffOper.cxx:1002-1083) or multiply (1085-1116), c2r.
+//
+// Pass structure (each pass is one kernel, one HBM round trip of the spectrum):
+//   X  r2c along x of row triples (x, y, z components of the same rows), two real
+//      lines packed per complex line; writes the half spectrum [c][z][y][kx].
+//   Y  complex FFT along y for a tile of kx columns.
+//   Z  forward FFT along z for all 3 components of a kx tile, the per-frequency
+//      solve (with the 1/N scale folded in), inverse FFT along z.
+//   Y' inverse along y.   X' c2r along x with an epilogue (plain store, max-norm,
+//      or the fused greedy hfield update).
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "awk_fluid.h"
+
+namespace awk {
+
+Axis AxisPlan::dev() const {
+  Axis a;
+  a.tw = tw;
+  a.n = n;
+  a.nst = (int)rad.size();
+  for (int i = 0; i < a.nst; ++i) a.rad[i] = rad[i];
+  return a;
+}
+
+static std::vector<int> factor(int n) {
+  std::vector<int> r;
+  int m = n;
+  while (m % 16 == 0) { r.push_back(16); m /= 16; }
+  if (m % 8 == 0) { r.push_back(8); m /= 8; }
+  if (m % 4 == 0) { r.push_back(4); m /= 4; }
+  if (m % 2 == 0) { r.push_back(2); m /= 2; }
+  for (int p = 3; m > 1; p += 2)
+    while (m % p == 0) { r.push_back(p); m /= p; }
+  return r;
+}
+
+// ------------------------------------------------------------------ kernels
+
+// X forward: r2c of RT spatial rows (3 components each) per block.
+// Complex line layout per row pair (q = 2p, 2p+1):
+//   line 3p   = (c0[2p],   c1[2p])
+//   line 3p+1 = (c2[2p],   c2[2p+1])
+//   line 3p+2 = (c0[2p+1], c1[2p+1])
+__device__ __forceinline__ void line_targets(int line, int r0, int &rowA, int &cA, int &rowB,
+                                             int &cB) {
+  const int p = line / 3, e = line - 3 * p;
+  const int ra = r0 + 2 * p;
+  if (e == 0) { rowA = ra; cA = 0; rowB = ra; cB = 1; }
+  else if (e == 1) { rowA = ra; cA = 2; rowB = ra + 1; cB = 2; }
+  else { rowA = ra + 1; cA = 0; rowB = ra + 1; cB = 1; }
+}
+
+__global__ void __launch_bounds__(1024)
+k_xfwd(XFwdArgs a, Axis ax, int nx, int ny, int nz, int nxh, int nxh_pad, int RT,
+       float2 *__restrict__ spec, size_t plane) {
+  extern __shared__ float2 sm[];
+  __shared__ double red[16];
+  const int NL = 3 * RT / 2, LS = NL + 1;
+  const int NR = ny * nz;
+  const int lb = (int)xcd_block();
+  const int r0 = lb * RT;
+  if (r0 >= NR) return;
+  double ssd = 0.0;
+  float *smf = reinterpret_cast<float *>(sm);
+  for (int w = threadIdx.x; w < RT * nx; w += blockDim.x) {
+    const int q = w / nx, x = w - q * nx;
+    const int row = r0 + q;
+    float f0 = 0.f, f1 = 0.f, f2 = 0.f;
+    if (row < NR) {
+      const size_t base = (size_t)row * nx + x;
+      if (a.mode == XLOAD_PLAIN) {
+        f0 = a.in[0][base];
+        f1 = a.in[1][base];
+        f2 = a.in[2][base];
+      } else {
+        // gradient of the deformed image (Array3DUtils::computeGradient, wrap=false,
+        // Array3DUtils.txx:783-901), residual r = I1 - Idef and force r*grad
+        // (GreedyIteratorCPU.cxx:108-117).
+        const int y = row % ny, z = row / ny;
+        const float *I = a.idef;
+        const float c = I[base];
+        float gx = 0.f, gy = 0.f, gz = 0.f;
+        if (nx > 1) {
+          if (x > 0 && x < nx - 1) gx = (I[base + 1] - I[base - 1]) / (2.f * a.sp[0]);
+          else if (x == 0) gx = (I[base + 1] - c) / a.sp[0];
+          else gx = (c - I[base - 1]) / a.sp[0];
+        }
+        if (ny > 1) {
+          if (y > 0 && y < ny - 1) gy = (I[base + nx] - I[base - nx]) / (2.f * a.sp[1]);
+          else if (y == 0) gy = (I[base + nx] - c) / a.sp[1];
+          else gy = (c - I[base - nx]) / a.sp[1];
+        }
+        if (nz > 1) {
+          const size_t s = (size_t)nx * ny;
+          if (z > 0 && z < nz - 1) gz = (I[base + s] - I[base - s]) / (2.f * a.sp[2]);
+          else if (z == 0) gz = (I[base + s] - c) / a.sp[2];
+          else gz = (c - I[base - s]) / a.sp[2];
+        }
+        const float r = a.i1[base] - c;
+        ssd += (double)r * (double)r;
+        f0 = gx * r;
+        f1 = gy * r;
+        f2 = gz * r;
+      }
+    }
+    const int p = q >> 1;
+    if ((q & 1) == 0) {
+      sm[x * LS + 3 * p] = make_float2(f0, f1);
+      smf[2 * (x * LS + 3 * p + 1)] = f2;
+    } else {
+      sm[x * LS + 3 * p + 2] = make_float2(f0, f1);
+      smf[2 * (x * LS + 3 * p + 1) + 1] = f2;
+    }
+  }
+  __syncthreads();
+  lds_fft(sm, ax, NL, LS, false);
+  for (int w = threadIdx.x; w < NL * nxh; w += blockDim.x) {
+    const int line = w / nxh, k = w - line * nxh;
+    const float2 Z = sm[k * LS + line];
+    const float2 Zc = cconj(sm[((nx - k) % nx) * LS + line]);
+    const float2 A = cscale(cadd(Z, Zc), 0.5f);
+    const float2 d = cscale(csub(Z, Zc), 0.5f);
+    const float2 B = make_float2(d.y, -d.x);
+    int rowA, cA, rowB, cB;
+    line_targets(line, r0, rowA, cA, rowB, cB);
+    if (rowA < NR) spec[cA * plane + (size_t)rowA * nxh_pad + k] = A;
+    if (rowB < NR) spec[cB * plane + (size_t)rowB * nxh_pad + k] = B;
+  }
+  if (a.mode == XLOAD_FORCE && a.partials) {
+    // deterministic block reduction of the FP64 SSD partial
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) ssd += __shfl_down(ssd, off, 64);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) red[wave] = ssd;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      const int nw = (blockDim.x + 63) >> 6;
+      for (int i = 0; i < nw; ++i) s += red[i];
+      a.partials[lb] = s;
+    }
+  }
+}
+
+// X inverse: c2r of RT rows (3 components) with an epilogue.
+__global__ void __launch_bounds__(1024)
+k_xinv(XInvArgs a, Axis ax, int nx, int ny, int nz, int nxh, int nxh_pad, int RT,
+       const float2 *__restrict__ spec, size_t plane) {
+  extern __shared__ float2 sm[];
+  __shared__ float redf[16];
+  const int NL = 3 * RT / 2, LS = NL + 1;
+  const int NR = ny * nz;
+  const int lb = (int)xcd_block();
+  const int r0 = lb * RT;
+  if (r0 >= NR) return;
+  for (int w = threadIdx.x; w < NL * nxh; w += blockDim.x) {
+    const int line = w / nxh, k = w - line * nxh;
+    int rowA, cA, rowB, cB;
+    line_targets(line, r0, rowA, cA, rowB, cB);
+    float2 A = rowA < NR ? spec[cA * plane + (size_t)rowA * nxh_pad + k] : make_float2(0.f, 0.f);
+    float2 B = rowB < NR ? spec[cB * plane + (size_t)rowB * nxh_pad + k] : make_float2(0.f, 0.f);
+    if (k == 0 || 2 * k == nx) {  // the imaginary part c2r ignores
+      A.y = 0.f;
+      B.y = 0.f;
+    }
+    sm[k * LS + line] = make_float2(A.x - B.y, A.y + B.x);               // A + iB
+    if (k > 0 && nx - k >= nxh) sm[(nx - k) * LS + line] = make_float2(A.x + B.y, B.x - A.y);  // conj(A) + i conj(B)
+  }
+  __syncthreads();
+  lds_fft(sm, ax, NL, LS, true);
+  float mx = 0.f;
+  for (int w = threadIdx.x; w < RT * nx; w += blockDim.x) {
+    const int q = w / nx, x = w - q * nx;
+    const int row = r0 + q;
+    if (row >= NR) continue;
+    const int p = q >> 1;
+    float v0, v1, v2;
+    if ((q & 1) == 0) {
+      const float2 t = sm[x * LS + 3 * p];
+      v0 = t.x; v1 = t.y; v2 = sm[x * LS + 3 * p + 1].x;
+    } else {
+      const float2 t = sm[x * LS + 3 * p + 2];
+      v0 = t.x; v1 = t.y; v2 = sm[x * LS + 3 * p + 1].y;
+    }
+    const size_t base = (size_t)row * nx + x;
+    if (a.mode == XSTORE_COMPOSE) {
+      // updateDeformation (GreedyIteratorCPU.cxx:81-95): v *= step; h <- composeHV(h, v, sp, PARTIAL_ID)
+      const float st = *a.step;
+      const int y = row % ny, z = row / ny;
+      const Dims d{nx, ny, nz};
+      const float px = (float)x + (v0 * st) / a.sp[0];
+      const float py = (float)y + (v1 * st) / a.sp[1];
+      const float pz = (float)z + (v2 * st) / a.sp[2];
+      const float3 hn = trilerp3<AWK_BG_PARTIAL_ID>(a.h[0], a.h[1], a.h[2], d, px, py, pz);
+      a.hnew[0][base] = hn.x;
+      a.hnew[1][base] = hn.y;
+      a.hnew[2][base] = hn.z;
+    } else {
+      a.out[0][base] = v0;
+      a.out[1][base] = v1;
+      a.out[2][base] = v2;
+      if (a.mode == XSTORE_MAXNORM) mx = fmaxf(mx, sqrtf(v0 * v0 + v1 * v1 + v2 * v2));
+    }
+  }
+  if (a.mode == XSTORE_MAXNORM) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_down(mx, off, 64));
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) redf[wave] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float m = 0.f;
+      const int nw = (blockDim.x + 63) >> 6;
+      for (int i = 0; i < nw; ++i) m = fmaxf(m, redf[i]);
+      a.maxpart[lb] = m;
+    }
+  }
+}
+
+// Y pass: complex FFT along y for TX kx columns of one (component, z) slab.
+__global__ void __launch_bounds__(1024)
+k_ypass(Axis ax, int ny, int nz, int nxh, int nxh_pad, int TX, float2 *__restrict__ spec,
+        size_t plane, int inv) {
+  extern __shared__ float2 sm[];
+  const int ntile = (nxh + TX - 1) / TX;
+  const int lb = (int)xcd_block();
+  if (lb >= 3 * nz * ntile) return;
+  const int tile = lb % ntile, rest = lb / ntile;
+  const int z = rest % nz, c = rest / nz;
+  const int kx0 = tile * TX;
+  float2 *base = spec + c * plane + (size_t)z * ny * nxh_pad + kx0;
+  for (int w = threadIdx.x; w < TX * ny; w += blockDim.x) {
+    const int t = w % TX, y = w / TX;
+    sm[y * TX + t] = (kx0 + t < nxh) ? base[(size_t)y * nxh_pad + t] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  lds_fft(sm, ax, TX, TX, inv != 0);
+  for (int w = threadIdx.x; w < TX * ny; w += blockDim.x) {
+    const int t = w % TX, y = w / TX;
+    if (kx0 + t < nxh) base[(size_t)y * nxh_pad + t] = sm[y * TX + t];
+  }
+}
+
+struct SolveArgs {
+  const float *cwx, *swx, *cwy, *swy, *cwz, *swz;
+  const float *eig;  // 6 per frequency, or null
+  float alpha, beta, gamma, scale;
+  int inverse_op;  // 1: L^-1 (Cholesky), 0: L
+  int divfree;
+};
+
+// Z pass: forward FFT along z of 3 components x TX kx columns at one ky, the
+// per-frequency solve, inverse FFT along z.
+__global__ void __launch_bounds__(1024)
+k_zsolve(Axis az, SolveArgs s, int ny, int nz, int nxh, int nxh_pad, int TX,
+         float2 *__restrict__ spec, size_t plane) {
+  extern __shared__ float2 sm[];
+  const int NL = 3 * TX;
+  const int ntile = (nxh + TX - 1) / TX;
+  const int lb = (int)xcd_block();
+  if (lb >= ny * ntile) return;
+  const int tile = lb % ntile, ky = lb / ntile;
+  const int kx0 = tile * TX;
+  for (int w = threadIdx.x; w < NL * nz; w += blockDim.x) {
+    const int t = w % TX, rest = w / TX;
+    const int c = rest % 3, z = rest / 3;
+    sm[z * NL + c * TX + t] = (kx0 + t < nxh)
+                                  ? spec[c * plane + ((size_t)z * ny + ky) * nxh_pad + kx0 + t]
+                                  : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  lds_fft(sm, az, NL, NL, false);
+  for (int w = threadIdx.x; w < TX * nz; w += blockDim.x) {
+    const int t = w % TX, kz = w / TX;
+    const int kx = kx0 + t;
+    if (kx >= nxh) continue;
+    float2 b0 = sm[kz * NL + t], b1 = sm[kz * NL + TX + t], b2 = sm[kz * NL + 2 * TX + t];
+    float2 v0, v1, v2;
+    const float sx = s.swx[kx], sy = s.swy[ky], sz = s.swz[kz];
+    if (s.eig) {
+      const float *M = s.eig + ((size_t)(kz * ny + ky) * nxh + kx) * 6;
+      const float M00 = M[0], M10 = M[1], M11 = M[2], M20 = M[3], M21 = M[4], M22 = M[5];
+      v0 = make_float2(M00 * b0.x + M10 * b1.x + M20 * b2.x, M00 * b0.y + M10 * b1.y + M20 * b2.y);
+      v1 = make_float2(M10 * b0.x + M11 * b1.x + M21 * b2.x, M10 * b0.y + M11 * b1.y + M21 * b2.y);
+      v2 = make_float2(M20 * b0.x + M21 * b1.x + M22 * b2.x, M20 * b0.y + M21 * b1.y + M22 * b2.y);
+    } else {
+      const float cx = s.cwx[kx], cy = s.cwy[ky], cz = s.cwz[kz];
+      // DiffOper.cxx:861-879
+      const float lam = -s.alpha * (cx + cy + cz) + s.gamma;
+      const float L00 = lam - s.beta * cx, L11 = lam - s.beta * cy, L22 = lam - s.beta * cz;
+      const float L10 = s.beta * sx * sy, L20 = s.beta * sx * sz, L21 = s.beta * sy * sz;
+      if (s.inverse_op) {
+        // Cholesky G G^T = L and two triangular solves (DiffOper.cxx:1002-1083)
+        const float G00 = sqrtf(L00);
+        const float G10 = L10 / G00, G20 = L20 / G00;
+        const float G11 = sqrtf(L11 - G10 * G10);
+        const float G21 = (L21 - G20 * G10) / G11;
+        const float G22 = sqrtf(L22 - (G20 * G20 + G21 * G21));
+        float y0 = b0.x / G00, y1 = (b1.x - G10 * y0) / G11, y2 = (b2.x - G20 * y0 - G21 * y1) / G22;
+        v2.x = y2 / G22; v1.x = (y1 - G21 * v2.x) / G11; v0.x = (y0 - G10 * v1.x - G20 * v2.x) / G00;
+        y0 = b0.y / G00; y1 = (b1.y - G10 * y0) / G11; y2 = (b2.y - G20 * y0 - G21 * y1) / G22;
+        v2.y = y2 / G22; v1.y = (y1 - G21 * v2.y) / G11; v0.y = (y0 - G10 * v1.y - G20 * v2.y) / G00;
+      } else {
+        v0 = make_float2(L00 * b0.x + L10 * b1.x + L20 * b2.x, L00 * b0.y + L10 * b1.y + L20 * b2.y);
+        v1 = make_float2(L10 * b0.x + L11 * b1.x + L21 * b2.x, L10 * b0.y + L11 * b1.y + L21 * b2.y);
+        v2 = make_float2(L20 * b0.x + L21 * b1.x + L22 * b2.x, L20 * b0.y + L21 * b1.y + L22 * b2.y);
+      }
+    }
+    if (s.divfree && (kx | ky | kz)) {
+      // ProjectIncomp (DiffOper.cxx:948-1000)
+      const float nsq = sx * sx + sy * sy + sz * sz;
+      if (nsq > 0.f) {
+        const float re = (v0.x * sx + v1.x * sy + v2.x * sz) / nsq;
+        const float im = (v0.y * sx + v1.y * sy + v2.y * sz) / nsq;
+        v0.x -= re * sx; v1.x -= re * sy; v2.x -= re * sz;
+        v0.y -= im * sx; v1.y -= im * sy; v2.y -= im * sz;
+      }
+    }
+    sm[kz * NL + t] = cscale(v0, s.scale);
+    sm[kz * NL + TX + t] = cscale(v1, s.scale);
+    sm[kz * NL + 2 * TX + t] = cscale(v2, s.scale);
+  }
+  __syncthreads();
+  lds_fft(sm, az, NL, NL, true);
+  for (int w = threadIdx.x; w < NL * nz; w += blockDim.x) {
+    const int t = w % TX, rest = w / TX;
+    const int c = rest % 3, z = rest / 3;
+    if (kx0 + t < nxh) spec[c * plane + ((size_t)z * ny + ky) * nxh_pad + kx0 + t] = sm[z * NL + c * TX + t];
+  }
+}
+
+// FP64 table of L^p (symmetric, 6 entries) per frequency for the eigen path
+// (DiffOper::InitializeEigenLookup + PowL, DiffOper.cxx:641-745, 1176-1218).
+// Jacobi rotations on the 3x3 symmetric matrix in double.
+__global__ void k_eigen_table(float *__restrict__ out, int nx, int ny, int nz, int nxh, double spx,
+                              double spy, double spz, double alpha, double beta, double gamma,
+                              double p) {
+  const size_t total = (size_t)nxh * ny * nz;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(i % nxh);
+    const int ky = (int)((i / nxh) % ny);
+    const int kz = (int)(i / ((size_t)nxh * ny));
+    const double PI2 = 6.283185307179586476925286766559;
+    const double cx = (2.0 * cos(PI2 * kx / nx) - 2.0) / (spx * spx);
+    const double cy = (2.0 * cos(PI2 * ky / ny) - 2.0) / (spy * spy);
+    const double cz = (2.0 * cos(PI2 * kz / nz) - 2.0) / (spz * spz);
+    const double sx = sin(PI2 * kx / nx) / spx, sy = sin(PI2 * ky / ny) / spy,
+                 sz = sin(PI2 * kz / nz) / spz;
+    const double lam = -alpha * (cx + cy + cz) + gamma;
+    double A[3][3] = {{lam - beta * cx, beta * sx * sy, beta * sx * sz},
+                      {beta * sx * sy, lam - beta * cy, beta * sy * sz},
+                      {beta * sx * sz, beta * sy * sz, lam - beta * cz}};
+    double Q[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int sweep = 0; sweep < 32; ++sweep) {
+      const double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+      const double dg = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+      if (off <= 1e-34 * dg) break;
+      for (int pp = 0; pp < 2; ++pp)
+        for (int q = pp + 1; q < 3; ++q) {
+          if (A[pp][q] == 0.0) continue;
+          const double th = (A[q][q] - A[pp][pp]) / (2.0 * A[pp][q]);
+          const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+          const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+          for (int k = 0; k < 3; ++k) {
+            const double akp = A[k][pp], akq = A[k][q];
+            A[k][pp] = c * akp - sn * akq;
+            A[k][q] = sn * akp + c * akq;
+          }
+          for (int k = 0; k < 3; ++k) {
+            const double apk = A[pp][k], aqk = A[q][k];
+            A[pp][k] = c * apk - sn * aqk;
+            A[q][k] = sn * apk + c * aqk;
+          }
+          for (int k = 0; k < 3; ++k) {
+            const double qkp = Q[k][pp], qkq = Q[k][q];
+            Q[k][pp] = c * qkp - sn * qkq;
+            Q[k][q] = sn * qkp + c * qkq;
+          }
+        }
+    }
+    double e[3];
+    for (int j = 0; j < 3; ++j) e[j] = pow(A[j][j], p);
+    double M[3][3];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c)
+        M[r][c] = Q[r][0] * Q[c][0] * e[0] + Q[r][1] * Q[c][1] * e[1] + Q[r][2] * Q[c][2] * e[2];
+    float *o = out + i * 6;
+    o[0] = (float)M[0][0];
+    o[1] = (float)M[1][0];
+    o[2] = (float)M[1][1];
+    o[3] = (float)M[2][0];
+    o[4] = (float)M[2][1];
+    o[5] = (float)M[2][2];
+  }
+}
+
+// ------------------------------------------------------------------ host
+
+static int pick_rt(int nx) {
+  // largest even RT <= 8 with LDS nx * (3RT/2 + 1) * 8 B <= 64 KiB
+  for (int rt = 8; rt >= 2; rt -= 2)
+    if ((size_t)nx * (3 * rt / 2 + 1) * sizeof(float2) <= 65536) return rt;
+  return 0;
+}
+
+static int pick_tx(int n, int lines_per_col, int max_tx) {
+  for (int tx = max_tx; tx >= 1; tx /= 2)
+    if ((size_t)n * lines_per_col * tx * sizeof(float2) <= 65536) return tx;
+  return 0;
+}
+
+static int pick_bd(int n, int nl) {
+  // n * NL <= 16 * bd, bd multiple of 64, 64..1024
+  int bd = (int)(((long long)n * nl + kMaxElemsPerThread - 1) / kMaxElemsPerThread);
+  bd = (bd + 63) / 64 * 64;
+  if (bd < 256) bd = 256;
+  return bd;
+}
+
+awk_status fk_init(FluidKernel &k, int nx, int ny, int nz, const float sp[3]) {
+  if (nx < 1 || ny < 1 || nz < 1) return fail(AWK_ERR_INVALID, "fluid kernel: sizes must be >= 1");
+  if ((long long)nx * ny * nz >= (1LL << 31)) return fail(AWK_ERR_INVALID, "fluid kernel: volume too large");
+  k.nx = nx; k.ny = ny; k.nz = nz;
+  k.nxh = nx / 2 + 1;
+  k.nxh_pad = (k.nxh + 7) / 8 * 8;
+  for (int i = 0; i < 3; ++i) k.sp[i] = sp[i];
+  const int ns[3] = {nx, ny, nz};
+  for (int i = 0; i < 3; ++i) {
+    k.ax[i].n = ns[i];
+    k.ax[i].rad = factor(ns[i]);
+    if ((int)k.ax[i].rad.size() > kMaxStages) return fail(AWK_ERR_INVALID, "fluid kernel: too many FFT stages");
+    for (int r : k.ax[i].rad)
+      if (r > 64) return fail(AWK_ERR_UNSUPPORTED, "fluid kernel: axis length has a prime factor > 64");
+  }
+  k.rt = pick_rt(nx);
+  k.tx_y = pick_tx(ny, 1, 16);
+  k.tx_z = pick_tx(nz, 3, 8);
+  if (!k.rt || !k.tx_y || !k.tx_z) return fail(AWK_ERR_UNSUPPORTED, "fluid kernel: axis too long for LDS staging");
+  k.bd_x = pick_bd(nx, 3 * k.rt / 2);
+  k.bd_y = pick_bd(ny, k.tx_y);
+  k.bd_z = pick_bd(nz, 3 * k.tx_z);
+  if (k.bd_x > 1024 || k.bd_y > 1024 || k.bd_z > 1024)
+    return fail(AWK_ERR_UNSUPPORTED, "fluid kernel: block too large");
+  // twiddles and LUT (double -> float)
+  size_t ntw = (size_t)nx + ny + nz;
+  std::vector<float2> tw(ntw);
+  size_t off = 0;
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < ns[i]; ++j) {
+      const double a = -2.0 * M_PI * (double)j / (double)ns[i];
+      tw[off + j] = make_float2((float)cos(a), (float)sin(a));
+    }
+    off += ns[i];
+  }
+  const size_t nlut = 2 * (size_t)k.nxh + 2 * (size_t)ny + 2 * (size_t)nz;
+  std::vector<float> lut(nlut);
+  {
+    // DiffOper.cxx:602-625: cosW = (2cos(2 pi k/n) - 2)/sp^2, sinW = sin(2 pi k/n)/sp
+    float *p = lut.data();
+    const int lens[3] = {k.nxh, ny, nz};
+    for (int i = 0; i < 3; ++i) {
+      const double s = 2.0 * M_PI / ns[i];
+      for (int j = 0; j < lens[i]; ++j) {
+        p[j] = (float)((2.0 * cos(s * j) - 2.0) / ((double)sp[i] * sp[i]));
+        p[lens[i] + j] = (float)(sin(s * j) / (double)sp[i]);
+      }
+      p += 2 * lens[i];
+    }
+  }
+  const size_t spec_bytes = 3 * k.spec_plane() * sizeof(float2);
+  AWK_HIP(hipMalloc(&k.spec, spec_bytes));
+  float2 *twd = nullptr;
+  AWK_HIP(hipMalloc(&twd, ntw * sizeof(float2)));
+  AWK_HIP(hipMemcpy(twd, tw.data(), ntw * sizeof(float2), hipMemcpyHostToDevice));
+  k.ax[0].tw = twd;
+  k.ax[1].tw = twd + nx;
+  k.ax[2].tw = twd + nx + ny;
+  AWK_HIP(hipMalloc(&k.lut, nlut * sizeof(float)));
+  AWK_HIP(hipMemcpy(k.lut, lut.data(), nlut * sizeof(float), hipMemcpyHostToDevice));
+  k.scratch_bytes = spec_bytes + ntw * sizeof(float2) + nlut * sizeof(float);
+  return AWK_OK;
+}
+
+void fk_release(FluidKernel &k) {
+  if (k.spec) (void)hipFree(k.spec);
+  if (k.ax[0].tw) (void)hipFree(k.ax[0].tw);
+  if (k.lut) (void)hipFree(k.lut);
+  if (k.eig) (void)hipFree(k.eig);
+  k.spec = nullptr;
+  k.ax[0].tw = k.ax[1].tw = k.ax[2].tw = nullptr;
+  k.lut = nullptr;
+  k.eig = nullptr;
+}
+
+awk_status fk_prepare_eigen(FluidKernel &k, int dir, hipStream_t s) {
+  if (k.eig && k.eig_dir == dir) return AWK_OK;
+  const size_t nf = (size_t)k.nxh * k.ny * k.nz;
+  if (!k.eig) {
+    AWK_HIP(hipMalloc(&k.eig, nf * 6 * sizeof(float)));
+    k.scratch_bytes += nf * 6 * sizeof(float);
+  }
+  const double p = dir > 0 ? (double)k.lpow : -(double)k.lpow;
+  unsigned grid = (unsigned)std::min<size_t>((nf + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_eigen_table, dim3(grid), dim3(256), 0, s, k.eig, k.nx, k.ny, k.nz, k.nxh,
+                     (double)k.sp[0], (double)k.sp[1], (double)k.sp[2], (double)k.alpha,
+                     (double)k.beta, (double)k.gamma, p);
+  AWK_LAUNCH_CHECK("k_eigen_table");
+  k.eig_dir = dir;
+  return AWK_OK;
+}
+
+int fk_xblocks(const FluidKernel &k) {
+  return (int)grid_for((long long)k.ny * k.nz, k.rt);
+}
+
+awk_status fk_xfwd(FluidKernel &k, const XFwdArgs &a, hipStream_t s, int *nblocks) {
+  const unsigned grid = grid_for((long long)k.ny * k.nz, k.rt);
+  const size_t lds = (size_t)k.nx * (3 * k.rt / 2 + 1) * sizeof(float2);
+  hipLaunchKernelGGL(k_xfwd, dim3(grid), dim3(k.bd_x), lds, s, a, k.ax[0].dev(), k.nx, k.ny, k.nz,
+                     k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane());
+  AWK_LAUNCH_CHECK("k_xfwd");
+  if (nblocks) *nblocks = (int)grid;
+  return AWK_OK;
+}
+
+awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s) {
+  const int ntile = (k.nxh + k.tx_y - 1) / k.tx_y;
+  const unsigned grid = grid_for((long long)3 * k.nz * ntile, 1);
+  const size_t lds = (size_t)k.ny * k.tx_y * sizeof(float2);
+  hipLaunchKernelGGL(k_ypass, dim3(grid), dim3(k.bd_y), lds, s, k.ax[1].dev(), k.ny, k.nz, k.nxh,
+                     k.nxh_pad, k.tx_y, k.spec, k.spec_plane(), inv ? 1 : 0);
+  AWK_LAUNCH_CHECK("k_ypass");
+  return AWK_OK;
+}
+
+awk_status fk_zsolve(FluidKernel &k, bool inverse_op, hipStream_t s) {
+  const bool eigen = (k.flags & AWK_FK_EIGEN) || k.lpow != 1.0f;
+  if (eigen) {
+    awk_status st = fk_prepare_eigen(k, inverse_op ? -1 : 1, s);
+    if (st) return st;
+  }
+  SolveArgs sa;
+  sa.cwx = k.lut;
+  sa.swx = k.lut + k.nxh;
+  sa.cwy = k.lut + 2 * k.nxh;
+  sa.swy = sa.cwy + k.ny;
+  sa.cwz = sa.swy + k.ny;
+  sa.swz = sa.cwz + k.nz;
+  sa.eig = eigen ? k.eig : nullptr;
+  sa.alpha = k.alpha;
+  sa.beta = k.beta;
+  sa.gamma = k.gamma;
+  sa.scale = (float)(1.0 / ((double)k.nx * k.ny * k.nz));
+  sa.inverse_op = inverse_op ? 1 : 0;
+  sa.divfree = (k.flags & AWK_FK_DIVFREE) ? 1 : 0;
+  const int ntile = (k.nxh + k.tx_z - 1) / k.tx_z;
+  const unsigned grid = grid_for((long long)k.ny * ntile, 1);
+  const size_t lds = (size_t)k.nz * 3 * k.tx_z * sizeof(float2);
+  hipLaunchKernelGGL(k_zsolve, dim3(grid), dim3(k.bd_z), lds, s, k.ax[2].dev(), sa, k.ny, k.nz,
+                     k.nxh, k.nxh_pad, k.tx_z, k.spec, k.spec_plane());
+  AWK_LAUNCH_CHECK("k_zsolve");
+  return AWK_OK;
+}
+
+awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblocks) {
+  const unsigned grid = grid_for((long long)k.ny * k.nz, k.rt);
+  const size_t lds = (size_t)k.nx * (3 * k.rt / 2 + 1) * sizeof(float2);
+  hipLaunchKernelGGL(k_xinv, dim3(grid), dim3(k.bd_x), lds, s, a, k.ax[0].dev(), k.nx, k.ny, k.nz,
+                     k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane());
+  AWK_LAUNCH_CHECK("k_xinv");
+  if (nblocks) *nblocks = (int)grid;
+  return AWK_OK;
+}
+
+}  // namespace awk
+
+using namespace awk;
+
+extern "C" {
+
+awk_status awk_fluid_kernel_create(awk_fk_t *out, int nx, int ny, int nz, float spx, float spy,
+                                   float spz, float alpha, float beta, float gamma, float lpow,
+                                   int flags) {
+  if (!out) return fail(AWK_ERR_INVALID, "awk_fluid_kernel_create: null handle pointer");
+  *out = nullptr;
+  if (!(spx > 0 && spy > 0 && spz > 0)) return fail(AWK_ERR_INVALID, "awk_fluid_kernel_create: spacing must be > 0");
+  awk_fluid_kernel *h = new (std::nothrow) awk_fluid_kernel();
+  if (!h) return fail(AWK_ERR_ALLOC, "awk_fluid_kernel_create: host allocation failed");
+  const float sp[3] = {spx, spy, spz};
+  awk_status st = fk_init(h->k, nx, ny, nz, sp);
+  if (st) {
+    fk_release(h->k);
+    delete h;
+    return st;
+  }
+  h->k.alpha = alpha;
+  h->k.beta = beta;
+  h->k.gamma = gamma;
+  h->k.lpow = lpow;
+  h->k.flags = flags;
+  *out = h;
+  return AWK_OK;
+}
+
+awk_status awk_fluid_kernel_set_params(awk_fk_t h, float alpha, float beta, float gamma, float lpow,
+                                       int flags) {
+  if (!h) return fail(AWK_ERR_INVALID, "awk_fluid_kernel_set_params: null handle");
+  FluidKernel &k = h->k;
+  if (alpha != k.alpha || beta != k.beta || gamma != k.gamma || lpow != k.lpow) k.eig_dir = 0;
+  k.alpha = alpha;
+  k.beta = beta;
+  k.gamma = gamma;
+  k.lpow = lpow;
+  k.flags = flags;
+  return AWK_OK;
+}
+
+awk_status awk_fluid_kernel_destroy(awk_fk_t h) {
+  if (!h) return AWK_OK;
+  fk_release(h->k);
+  delete h;
+  return AWK_OK;
+}
+
+awk_status awk_fluid_kernel_scratch_bytes(awk_fk_t h, unsigned long long *bytes) {
+  if (!h || !bytes) return fail(AWK_ERR_INVALID, "awk_fluid_kernel_scratch_bytes: null argument");
+  *bytes = h->k.scratch_bytes;
+  return AWK_OK;
+}
+
+awk_status awk_fluid_kernel_apply(awk_fk_t h, float *x, float *y, float *z, int inverse,
+                                  awk_stream_t stream) {
+  if (!h || !x || !y || !z) return fail(AWK_ERR_INVALID, "awk_fluid_kernel_apply: null argument");
+  FluidKernel &k = h->k;
+  hipStream_t s = as_stream(stream);
+  XFwdArgs fa;
+  std::memset(&fa, 0, sizeof(fa));
+  fa.mode = XLOAD_PLAIN;
+  fa.in[0] = x;
+  fa.in[1] = y;
+  fa.in[2] = z;
+  awk_status st;
+  if ((st = fk_xfwd(k, fa, s, nullptr))) return st;
+  if ((st = fk_ypass(k, false, s))) return st;
+  if ((st = fk_zsolve(k, inverse != 0, s))) return st;
+  if ((st = fk_ypass(k, true, s))) return st;
+  XInvArgs ia;
+  std::memset(&ia, 0, sizeof(ia));
+  ia.mode = XSTORE_PLAIN;
+  ia.out[0] = x;
+  ia.out[1] = y;
+  ia.out[2] = z;
+  return fk_xinv(k, ia, s, nullptr);
+}
+
+}  // extern "C"

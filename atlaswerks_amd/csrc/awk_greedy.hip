@@ -1,0 +1,268 @@
+// awk_greedy.hip -- one greedy-fluid iteration on gfx950.
+//
+// GreedyIteratorGPU::Iterate (Algorithms/Greedy/GreedyIteratorGPU.cxx:92-135) runs
+// about fifteen kernels and two host syncs per iteration.  Here one iteration is
+// six kernels and no host sync:
+//   1. apply          Idef = I0(h)                                  (gather)
+//   2. X  (force)     f = (I1 - Idef) grad(Idef), FP64 SSD partials, r2c along x
+//   3. energy         fixed-order FP64 sum of the partials (+ AutoStepReduce)
+//   4. Y, Z+solve, Y' the Navier solve in the spectrum
+//   5. X' (compose)   c2r along x and h' = h(x + step v / sp) in the epilogue
+// On the first iteration of a scale level (step not yet known) step 5 writes v
+// and a per-block max |v|; the step MaxPert / max|v| is then formed on the device
+// and a separate composeHV applies it.  The same unfused tail serves the inverse
+// hfield update.
+#include <cstring>
+#include <new>
+
+#include "awk_fluid.h"
+#include "awk_ops.h"
+
+namespace awk {
+
+// Fixed-order FP64 reduction of per-block partial SSD sums (one block).
+__global__ void __launch_bounds__(256)
+k_energy(const double *__restrict__ partials, int n, double scale, double *__restrict__ out,
+         double *__restrict__ prev, int *__restrict__ have_prev, float *__restrict__ step,
+         int auto_reduce) {
+  __shared__ double s[256];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) a += partials[i];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double e = scale * s[0];
+    if (out) *out = e;
+    // GreedyDeformationData::AddEnergy (GreedyDeformationData.cxx:162-184)
+    if (auto_reduce && *have_prev && e - *prev > 0.0) *step = *step * 0.5f;
+    *prev = e;
+    *have_prev = 1;
+  }
+}
+
+// step = MaxPert / max |v| (GreedyIteratorCPU.cxx:65-79, 129-134)
+__global__ void __launch_bounds__(256)
+k_step_from_max(const float *__restrict__ maxpart, int n, float max_pert, float *__restrict__ step) {
+  __shared__ float s[256];
+  float m = 0.f;
+  for (int i = threadIdx.x; i < n; i += 256) m = fmaxf(m, maxpart[i]);
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *step = max_pert / s[0];
+}
+
+}  // namespace awk
+
+struct awk_greedy {
+  awk::FluidKernel k;
+  int nx = 0, ny = 0, nz = 0;
+  float sp[3] = {1, 1, 1};
+  float max_pert = 0.1f;
+  bool update_step = true;
+  int auto_reduce = 0;
+  float *idef = nullptr;
+  float *v = nullptr;     // 3 planes (first iteration / inverse update)
+  float *hbuf = nullptr;  // 3 planes (in-place update staging)
+  double *partials = nullptr;
+  float *maxpart = nullptr;
+  float *step = nullptr;  // device scalar
+  double *prev = nullptr;
+  int *have_prev = nullptr;
+  int nblk = 0;
+};
+
+using namespace awk;
+
+static void greedy_free(awk_greedy *g) {
+  fk_release(g->k);
+  if (g->idef) (void)hipFree(g->idef);
+  if (g->v) (void)hipFree(g->v);
+  if (g->hbuf) (void)hipFree(g->hbuf);
+  if (g->partials) (void)hipFree(g->partials);
+  if (g->maxpart) (void)hipFree(g->maxpart);
+  if (g->step) (void)hipFree(g->step);
+  if (g->prev) (void)hipFree(g->prev);
+  if (g->have_prev) (void)hipFree(g->have_prev);
+}
+
+extern "C" {
+
+awk_status awk_greedy_create(awk_greedy_t *out, int nx, int ny, int nz, float spx, float spy,
+                             float spz, float alpha, float beta, float gamma, float max_pert) {
+  if (!out) return fail(AWK_ERR_INVALID, "awk_greedy_create: null handle pointer");
+  *out = nullptr;
+  if (!(spx > 0 && spy > 0 && spz > 0)) return fail(AWK_ERR_INVALID, "awk_greedy_create: spacing must be > 0");
+  if (!(max_pert > 0)) return fail(AWK_ERR_INVALID, "awk_greedy_create: MaxPert must be > 0");
+  awk_greedy *g = new (std::nothrow) awk_greedy();
+  if (!g) return fail(AWK_ERR_ALLOC, "awk_greedy_create: host allocation failed");
+  const float sp[3] = {spx, spy, spz};
+  awk_status st = fk_init(g->k, nx, ny, nz, sp);
+  if (st) {
+    greedy_free(g);
+    delete g;
+    return st;
+  }
+  g->k.alpha = alpha;
+  g->k.beta = beta;
+  g->k.gamma = gamma;
+  g->k.lpow = 1.f;
+  g->k.flags = 0;
+  g->nx = nx;
+  g->ny = ny;
+  g->nz = nz;
+  for (int i = 0; i < 3; ++i) g->sp[i] = sp[i];
+  g->max_pert = max_pert;
+  const size_t N = (size_t)nx * ny * nz;
+  g->nblk = fk_xblocks(g->k);
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = hipMalloc(&g->idef, N * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&g->v, 3 * N * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&g->hbuf, 3 * N * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&g->partials, (size_t)g->nblk * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&g->maxpart, (size_t)g->nblk * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&g->step, sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&g->prev, sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&g->have_prev, sizeof(int));
+  if (e == hipSuccess) e = hipMemset(g->step, 0, sizeof(float));
+  if (e == hipSuccess) e = hipMemset(g->prev, 0, sizeof(double));
+  if (e == hipSuccess) e = hipMemset(g->have_prev, 0, sizeof(int));
+  if (e != hipSuccess) {
+    st = hip_check(e, "awk_greedy_create: allocation");
+    greedy_free(g);
+    delete g;
+    return st;
+  }
+  *out = g;
+  return AWK_OK;
+}
+
+awk_status awk_greedy_destroy(awk_greedy_t g) {
+  if (!g) return AWK_OK;
+  greedy_free(g);
+  delete g;
+  return AWK_OK;
+}
+
+awk_status awk_greedy_update_step_next(awk_greedy_t g) {
+  if (!g) return fail(AWK_ERR_INVALID, "awk_greedy_update_step_next: null handle");
+  g->update_step = true;
+  return AWK_OK;
+}
+
+awk_status awk_greedy_set_step(awk_greedy_t g, float step) {
+  if (!g) return fail(AWK_ERR_INVALID, "awk_greedy_set_step: null handle");
+  AWK_HIP(hipMemcpy(g->step, &step, sizeof(float), hipMemcpyHostToDevice));
+  g->update_step = false;
+  return AWK_OK;
+}
+
+awk_status awk_greedy_get_step(awk_greedy_t g, float *step) {
+  if (!g || !step) return fail(AWK_ERR_INVALID, "awk_greedy_get_step: null argument");
+  AWK_HIP(hipDeviceSynchronize());
+  AWK_HIP(hipMemcpy(step, g->step, sizeof(float), hipMemcpyDeviceToHost));
+  return AWK_OK;
+}
+
+awk_status awk_greedy_set_auto_step_reduce(awk_greedy_t g, int on) {
+  if (!g) return fail(AWK_ERR_INVALID, "awk_greedy_set_auto_step_reduce: null handle");
+  g->auto_reduce = on ? 1 : 0;
+  return AWK_OK;
+}
+
+awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
+                              const float *const h_in[3], float *const h_out[3],
+                              const float *const hinv_in[3], float *const hinv_out[3],
+                              double *energy_dev, awk_stream_t stream) {
+  if (!g || !I0 || !I1 || !h_in || !h_out) return fail(AWK_ERR_INVALID, "awk_greedy_iterate: null argument");
+  for (int c = 0; c < 3; ++c)
+    if (!h_in[c] || !h_out[c]) return fail(AWK_ERR_INVALID, "awk_greedy_iterate: null hfield plane");
+  const bool inv = hinv_in && hinv_out;
+  if (inv)
+    for (int c = 0; c < 3; ++c)
+      if (!hinv_in[c] || !hinv_out[c]) return fail(AWK_ERR_INVALID, "awk_greedy_iterate: null hinv plane");
+  hipStream_t s = as_stream(stream);
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const size_t N = (size_t)nx * ny * nz;
+  awk_status st;
+  // 1. deformed image (VAL background 0, HField3DUtils::apply defaults)
+  if ((st = launch_apply(g->idef, I0, h_in[0], h_in[1], h_in[2], nx, ny, nz, AWK_BG_VAL, 0.f, s))) return st;
+  // 2. force + forward x pass
+  XFwdArgs fa;
+  std::memset(&fa, 0, sizeof(fa));
+  fa.mode = XLOAD_FORCE;
+  fa.idef = g->idef;
+  fa.i1 = I1;
+  for (int c = 0; c < 3; ++c) fa.sp[c] = g->sp[c];
+  fa.partials = g->partials;
+  int nb = 0;
+  if ((st = fk_xfwd(g->k, fa, s, &nb))) return st;
+  // 3. energy (ImageUtils::l2NormSqr = prod(spacing) * sum r^2, ImageUtils.txx:55-58)
+  const double scale = (double)g->sp[0] * g->sp[1] * g->sp[2];
+  hipLaunchKernelGGL(k_energy, dim3(1), dim3(256), 0, s, g->partials, nb, scale, energy_dev, g->prev,
+                     g->have_prev, g->step, g->auto_reduce);
+  AWK_LAUNCH_CHECK("k_energy");
+  // 4. solve
+  if ((st = fk_ypass(g->k, false, s))) return st;
+  if ((st = fk_zsolve(g->k, true, s))) return st;
+  if ((st = fk_ypass(g->k, true, s))) return st;
+  // 5. velocity -> hfield update
+  const bool in_place = (h_out[0] == h_in[0]);
+  float *dst[3];
+  for (int c = 0; c < 3; ++c) dst[c] = in_place ? g->hbuf + c * N : h_out[c];
+  XInvArgs ia;
+  std::memset(&ia, 0, sizeof(ia));
+  for (int c = 0; c < 3; ++c) ia.sp[c] = g->sp[c];
+  if (g->update_step || inv) {
+    ia.mode = g->update_step ? XSTORE_MAXNORM : XSTORE_PLAIN;
+    for (int c = 0; c < 3; ++c) ia.out[c] = g->v + c * N;
+    ia.maxpart = g->maxpart;
+    if ((st = fk_xinv(g->k, ia, s, &nb))) return st;
+    if (g->update_step) {
+      hipLaunchKernelGGL(k_step_from_max, dim3(1), dim3(256), 0, s, g->maxpart, nb, g->max_pert, g->step);
+      AWK_LAUNCH_CHECK("k_step_from_max");
+      g->update_step = false;
+    }
+    float *vx = g->v, *vy = g->v + N, *vz = g->v + 2 * N;
+    if ((st = launch_compose_hv(dst[0], dst[1], dst[2], h_in[0], h_in[1], h_in[2], vx, vy, vz, nx,
+                                ny, nz, g->sp, 1.f, g->step, 0, AWK_BG_PARTIAL_ID, s)))
+      return st;
+    if (in_place)
+      for (int c = 0; c < 3; ++c)
+        AWK_HIP(hipMemcpyAsync(h_out[c], g->hbuf + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, s));
+    if (inv) {
+      // composeVHInv(v * step, hinv, PARTIAL_ZERO) (GreedyIteratorCPU.cxx:89-93)
+      const bool inv_in_place = (hinv_out[0] == hinv_in[0]);
+      float *idst[3];
+      for (int c = 0; c < 3; ++c) idst[c] = inv_in_place ? g->hbuf + c * N : hinv_out[c];
+      if ((st = launch_compose_vh(idst[0], idst[1], idst[2], vx, vy, vz, hinv_in[0], hinv_in[1],
+                                  hinv_in[2], nx, ny, nz, g->sp, g->step, 1, AWK_BG_PARTIAL_ZERO, s)))
+        return st;
+      if (inv_in_place)
+        for (int c = 0; c < 3; ++c)
+          AWK_HIP(hipMemcpyAsync(hinv_out[c], g->hbuf + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, s));
+    }
+    return AWK_OK;
+  } else {
+    ia.mode = XSTORE_COMPOSE;
+    for (int c = 0; c < 3; ++c) {
+      ia.h[c] = h_in[c];
+      ia.hnew[c] = dst[c];
+    }
+    ia.step = g->step;
+    if ((st = fk_xinv(g->k, ia, s, &nb))) return st;
+  }
+  if (in_place)
+    for (int c = 0; c < 3; ++c)
+      AWK_HIP(hipMemcpyAsync(h_out[c], g->hbuf + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, s));
+  return AWK_OK;
+}
+
+}  // extern "C"

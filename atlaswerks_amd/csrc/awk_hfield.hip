@@ -1,0 +1,332 @@
+// awk_hfield.hip -- hfield utilities on gfx950: identity / conversions, apply
+// (trilinear gather of an image), the compose family, gradient, and the error
+// plumbing of the C-ABI.
+//
+// Every per-voxel kernel uses one thread per voxel, 256 threads per block, and
+// the XCD-aware block order of awk_common.h so that the 8-corner gathers of
+// neighbouring blocks share one XCD's L2.
+#include <cstring>
+#include <string>
+
+#include "awk_ops.h"
+
+namespace awk {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+awk_status fail(awk_status code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+awk_status hip_check(hipError_t e, const char *what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return AWK_ERR_HIP;
+}
+
+constexpr int kBlock = 256;
+
+static awk_status check_dims(int nx, int ny, int nz, const char *who) {
+  if (nx < 1 || ny < 1 || nz < 1) return fail(AWK_ERR_INVALID, std::string(who) + ": sizes must be >= 1");
+  if ((long long)nx * ny * nz >= (1LL << 31)) return fail(AWK_ERR_INVALID, std::string(who) + ": volume too large");
+  return AWK_OK;
+}
+
+__device__ __forceinline__ bool voxel(int N, int nx, int ny, int &i, int &x, int &y, int &z) {
+  i = (int)(xcd_block() * blockDim.x + threadIdx.x);
+  if (i >= N) return false;
+  x = i % nx;
+  const int r = i / nx;
+  y = r % ny;
+  z = r / ny;
+  return true;
+}
+
+// ------------------------------------------------------------ identity etc.
+__global__ void k_identity(float *__restrict__ hx, float *__restrict__ hy, float *__restrict__ hz,
+                           int nx, int ny, int nz) {
+  int i, x, y, z;
+  if (!voxel(nx * ny * nz, nx, ny, i, x, y, z)) return;
+  hx[i] = (float)x;
+  hy[i] = (float)y;
+  hz[i] = (float)z;
+}
+
+// mode 0: v -> h (x + v/sp), 1: h -> v ((h - x)*sp)
+__global__ void k_hv_convert(float *__restrict__ a, float *__restrict__ b, float *__restrict__ c,
+                             int nx, int ny, int nz, float spx, float spy, float spz, int mode) {
+  int i, x, y, z;
+  if (!voxel(nx * ny * nz, nx, ny, i, x, y, z)) return;
+  if (mode == 0) {
+    a[i] = (float)x + a[i] / spx;
+    b[i] = (float)y + b[i] / spy;
+    c[i] = (float)z + c[i] / spz;
+  } else {
+    a[i] = (a[i] - (float)x) * spx;
+    b[i] = (b[i] - (float)y) * spy;
+    c[i] = (c[i] - (float)z) * spz;
+  }
+}
+
+__global__ void k_inv0(float *__restrict__ ox, float *__restrict__ oy, float *__restrict__ oz,
+                       const float *__restrict__ hx, const float *__restrict__ hy,
+                       const float *__restrict__ hz, int nx, int ny, int nz) {
+  int i, x, y, z;
+  if (!voxel(nx * ny * nz, nx, ny, i, x, y, z)) return;
+  ox[i] = (float)(x + x) - hx[i];
+  oy[i] = (float)(y + y) - hy[i];
+  oz[i] = (float)(z + z) - hz[i];
+}
+
+// ------------------------------------------------------------ apply
+template <int BG>
+__global__ void __launch_bounds__(kBlock)
+k_apply(float *__restrict__ out, const float *__restrict__ img, const float *__restrict__ hx,
+        const float *__restrict__ hy, const float *__restrict__ hz, int nx, int ny, int nz,
+        float background) {
+  const int N = nx * ny * nz;
+  const int i = (int)(xcd_block() * blockDim.x + threadIdx.x);
+  if (i >= N) return;
+  const Dims d{nx, ny, nz};
+  out[i] = trilerp1<BG>(img, d, hx[i], hy[i], hz[i], background);
+}
+
+// ------------------------------------------------------------ compose family
+// op 0: h = f(g(x))
+// op 1: h = g(x + s*delta*v(x)/sp)                 (composeHV / composeHVInv)
+// op 2: h = g(x) + s*v(g(x))/sp                    (composeVH / composeVHInv)
+template <int BG, int OP>
+__global__ void __launch_bounds__(kBlock)
+k_compose(float *__restrict__ ox, float *__restrict__ oy, float *__restrict__ oz,
+          const float *__restrict__ fx, const float *__restrict__ fy, const float *__restrict__ fz,
+          const float *__restrict__ gx, const float *__restrict__ gy, const float *__restrict__ gz,
+          int nx, int ny, int nz, float spx, float spy, float spz, float delta_h,
+          const float *__restrict__ delta_dev, float sgn) {
+  int i, x, y, z;
+  if (!voxel(nx * ny * nz, nx, ny, i, x, y, z)) return;
+  const Dims d{nx, ny, nz};
+  const float delta = delta_dev ? *delta_dev : delta_h;
+  if (OP == 0) {
+    const float3 r = trilerp3<BG>(fx, fy, fz, d, gx[i], gy[i], gz[i]);
+    ox[i] = r.x; oy[i] = r.y; oz[i] = r.z;
+  } else if (OP == 1) {
+    // f = g (field gathered), (gx,gy,gz) = v.  The reference scales v by the step
+    // first (GreedyIteratorCPU.cxx:84), then divides by the spacing.
+    const float vx = gx[i] * delta, vy = gy[i] * delta, vz = gz[i] * delta;
+    const float px = (float)x + sgn * (vx / spx);
+    const float py = (float)y + sgn * (vy / spy);
+    const float pz = (float)z + sgn * (vz / spz);
+    const float3 r = trilerp3<BG>(fx, fy, fz, d, px, py, pz);
+    ox[i] = r.x; oy[i] = r.y; oz[i] = r.z;
+  } else {
+    // f = v (field gathered), (gx,gy,gz) = g
+    const float ax = gx[i], ay = gy[i], az = gz[i];
+    float3 r = trilerp3<BG>(fx, fy, fz, d, ax, ay, az);
+    if (delta_dev || delta_h != 1.f) {  // v scaled by the step (greedy inverse update)
+      r.x *= delta; r.y *= delta; r.z *= delta;
+    }
+    ox[i] = r.x / (sgn * spx) + ax;
+    oy[i] = r.y / (sgn * spy) + ay;
+    oz[i] = r.z / (sgn * spz) + az;
+  }
+}
+
+// ------------------------------------------------------------ gradient
+__global__ void __launch_bounds__(kBlock)
+k_gradient(float *__restrict__ gx, float *__restrict__ gy, float *__restrict__ gz,
+           const float *__restrict__ a, int nx, int ny, int nz, float spx, float spy, float spz,
+           int wrap) {
+  int i, x, y, z;
+  if (!voxel(nx * ny * nz, nx, ny, i, x, y, z)) return;
+  const float c = a[i];
+  const int s1 = nx, s2 = nx * ny;
+  auto comp = [&](int p, int n, int stride, float sp) -> float {
+    if (n == 1) return 0.f;
+    if (p > 0 && p < n - 1) return (a[i + stride] - a[i - stride]) / (2.f * sp);
+    if (wrap) {
+      const int ip = (p + 1 < n) ? i + stride : i - (n - 1) * stride;
+      const int im = (p > 0) ? i - stride : i + (n - 1) * stride;
+      return (a[ip] - a[im]) / (2.f * sp);
+    }
+    if (p == 0) return (a[i + stride] - c) / sp;
+    return (c - a[i - stride]) / sp;
+  };
+  gx[i] = comp(x, nx, 1, spx);
+  gy[i] = comp(y, ny, s1, spy);
+  gz[i] = comp(z, nz, s2, spz);
+}
+
+template <int OP>
+static awk_status launch_compose(float *ox, float *oy, float *oz, const float *fx, const float *fy,
+                                 const float *fz, const float *gx, const float *gy,
+                                 const float *gz, int nx, int ny, int nz, float spx, float spy,
+                                 float spz, float delta, const float *delta_dev, float sgn, int bg,
+                                 hipStream_t s) {
+  const unsigned grid = grid_for((long long)nx * ny * nz, kBlock);
+#define AWK_CMP(B)                                                                            \
+  hipLaunchKernelGGL((k_compose<B, OP>), dim3(grid), dim3(kBlock), 0, s, ox, oy, oz, fx, fy, fz, \
+                     gx, gy, gz, nx, ny, nz, spx, spy, spz, delta, delta_dev, sgn)
+  switch (bg) {
+    case AWK_BG_ID: AWK_CMP(AWK_BG_ID); break;
+    case AWK_BG_ZERO: AWK_CMP(AWK_BG_ZERO); break;
+    case AWK_BG_PARTIAL_ID: AWK_CMP(AWK_BG_PARTIAL_ID); break;
+    case AWK_BG_PARTIAL_ZERO: AWK_CMP(AWK_BG_PARTIAL_ZERO); break;
+    case AWK_BG_CLAMP: AWK_CMP(AWK_BG_CLAMP); break;
+    case AWK_BG_WRAP: AWK_CMP(AWK_BG_WRAP); break;
+    default: return fail(AWK_ERR_INVALID, "compose: invalid background strategy");
+  }
+#undef AWK_CMP
+  AWK_LAUNCH_CHECK("k_compose");
+  return AWK_OK;
+}
+
+awk_status launch_apply(float *out, const float *img, const float *hx, const float *hy,
+                        const float *hz, int nx, int ny, int nz, int bg, float background,
+                        hipStream_t s) {
+  const unsigned grid = grid_for((long long)nx * ny * nz, kBlock);
+  switch (bg) {
+    case AWK_BG_VAL:
+      hipLaunchKernelGGL(k_apply<AWK_BG_VAL>, dim3(grid), dim3(kBlock), 0, s, out, img, hx, hy, hz, nx, ny, nz, background);
+      break;
+    case AWK_BG_ZERO:
+      hipLaunchKernelGGL(k_apply<AWK_BG_ZERO>, dim3(grid), dim3(kBlock), 0, s, out, img, hx, hy, hz, nx, ny, nz, background);
+      break;
+    case AWK_BG_CLAMP:
+      hipLaunchKernelGGL(k_apply<AWK_BG_CLAMP>, dim3(grid), dim3(kBlock), 0, s, out, img, hx, hy, hz, nx, ny, nz, background);
+      break;
+    case AWK_BG_WRAP:
+      hipLaunchKernelGGL(k_apply<AWK_BG_WRAP>, dim3(grid), dim3(kBlock), 0, s, out, img, hx, hy, hz, nx, ny, nz, background);
+      break;
+    default:
+      return fail(AWK_ERR_INVALID, "apply: background must be VAL, ZERO, CLAMP or WRAP");
+  }
+  AWK_LAUNCH_CHECK("k_apply");
+  return AWK_OK;
+}
+
+awk_status launch_compose_hv(float *ox, float *oy, float *oz, const float *gx, const float *gy,
+                             const float *gz, const float *vx, const float *vy, const float *vz,
+                             int nx, int ny, int nz, const float sp[3], float delta,
+                             const float *delta_dev, int inverse, int bg, hipStream_t s) {
+  return launch_compose<1>(ox, oy, oz, gx, gy, gz, vx, vy, vz, nx, ny, nz, sp[0], sp[1], sp[2],
+                           delta, delta_dev, inverse ? -1.f : 1.f, bg, s);
+}
+
+awk_status launch_compose_vh(float *ox, float *oy, float *oz, const float *vx, const float *vy,
+                             const float *vz, const float *gx, const float *gy, const float *gz,
+                             int nx, int ny, int nz, const float sp[3], const float *delta_dev,
+                             int inverse, int bg, hipStream_t s) {
+  return launch_compose<2>(ox, oy, oz, vx, vy, vz, gx, gy, gz, nx, ny, nz, sp[0], sp[1], sp[2],
+                           1.f, delta_dev, inverse ? -1.f : 1.f, bg, s);
+}
+
+}  // namespace awk
+
+using namespace awk;
+
+extern "C" {
+
+const char *awk_last_error(void) { return g_err.c_str(); }
+
+int awk_version(void) { return 100; }
+
+awk_status awk_hfield_set_to_identity(float *hx, float *hy, float *hz, int nx, int ny, int nz,
+                                      awk_stream_t stream) {
+  if (awk_status st = check_dims(nx, ny, nz, "awk_hfield_set_to_identity")) return st;
+  if (!hx || !hy || !hz) return fail(AWK_ERR_INVALID, "awk_hfield_set_to_identity: null pointer");
+  hipLaunchKernelGGL(k_identity, dim3(grid_for((long long)nx * ny * nz, kBlock)), dim3(kBlock), 0,
+                     as_stream(stream), hx, hy, hz, nx, ny, nz);
+  AWK_LAUNCH_CHECK("k_identity");
+  return AWK_OK;
+}
+
+awk_status awk_hfield_velocity_to_h(float *x, float *y, float *z, int nx, int ny, int nz, float spx,
+                                    float spy, float spz, awk_stream_t stream) {
+  if (awk_status st = check_dims(nx, ny, nz, "awk_hfield_velocity_to_h")) return st;
+  if (!x || !y || !z) return fail(AWK_ERR_INVALID, "awk_hfield_velocity_to_h: null pointer");
+  hipLaunchKernelGGL(k_hv_convert, dim3(grid_for((long long)nx * ny * nz, kBlock)), dim3(kBlock), 0,
+                     as_stream(stream), x, y, z, nx, ny, nz, spx, spy, spz, 0);
+  AWK_LAUNCH_CHECK("k_hv_convert");
+  return AWK_OK;
+}
+
+awk_status awk_hfield_h_to_velocity(float *x, float *y, float *z, int nx, int ny, int nz, float spx,
+                                    float spy, float spz, awk_stream_t stream) {
+  if (awk_status st = check_dims(nx, ny, nz, "awk_hfield_h_to_velocity")) return st;
+  if (!x || !y || !z) return fail(AWK_ERR_INVALID, "awk_hfield_h_to_velocity: null pointer");
+  hipLaunchKernelGGL(k_hv_convert, dim3(grid_for((long long)nx * ny * nz, kBlock)), dim3(kBlock), 0,
+                     as_stream(stream), x, y, z, nx, ny, nz, spx, spy, spz, 1);
+  AWK_LAUNCH_CHECK("k_hv_convert");
+  return AWK_OK;
+}
+
+awk_status awk_hfield_inverse_zeroth_order(float *ox, float *oy, float *oz, const float *hx,
+                                           const float *hy, const float *hz, int nx, int ny, int nz,
+                                           awk_stream_t stream) {
+  if (awk_status st = check_dims(nx, ny, nz, "awk_hfield_inverse_zeroth_order")) return st;
+  if (!ox || !oy || !oz || !hx || !hy || !hz)
+    return fail(AWK_ERR_INVALID, "awk_hfield_inverse_zeroth_order: null pointer");
+  hipLaunchKernelGGL(k_inv0, dim3(grid_for((long long)nx * ny * nz, kBlock)), dim3(kBlock), 0,
+                     as_stream(stream), ox, oy, oz, hx, hy, hz, nx, ny, nz);
+  AWK_LAUNCH_CHECK("k_inv0");
+  return AWK_OK;
+}
+
+awk_status awk_hfield_apply(float *out, const float *img, const float *hx, const float *hy,
+                            const float *hz, int nx, int ny, int nz, int bg, float background,
+                            awk_stream_t stream) {
+  if (awk_status st = check_dims(nx, ny, nz, "awk_hfield_apply")) return st;
+  if (!out || !img || !hx || !hy || !hz) return fail(AWK_ERR_INVALID, "awk_hfield_apply: null pointer");
+  return launch_apply(out, img, hx, hy, hz, nx, ny, nz, bg, background, as_stream(stream));
+}
+
+awk_status awk_hfield_compose(float *ox, float *oy, float *oz, const float *fx, const float *fy,
+                              const float *fz, const float *gx, const float *gy, const float *gz,
+                              int nx, int ny, int nz, int bg, awk_stream_t stream) {
+  if (awk_status st = check_dims(nx, ny, nz, "awk_hfield_compose")) return st;
+  if (!ox || !oy || !oz || !fx || !fy || !fz || !gx || !gy || !gz)
+    return fail(AWK_ERR_INVALID, "awk_hfield_compose: null pointer");
+  if (ox == fx || oy == fy || oz == fz) return fail(AWK_ERR_INVALID, "awk_hfield_compose: output aliases f");
+  return launch_compose<0>(ox, oy, oz, fx, fy, fz, gx, gy, gz, nx, ny, nz, 1.f, 1.f, 1.f, 1.f,
+                           nullptr, 1.f, bg, as_stream(stream));
+}
+
+awk_status awk_hfield_compose_hv(float *ox, float *oy, float *oz, const float *gx, const float *gy,
+                                 const float *gz, const float *vx, const float *vy, const float *vz,
+                                 int nx, int ny, int nz, float spx, float spy, float spz,
+                                 float delta, int inverse, int bg, awk_stream_t stream) {
+  if (awk_status st = check_dims(nx, ny, nz, "awk_hfield_compose_hv")) return st;
+  if (!ox || !oy || !oz || !gx || !gy || !gz || !vx || !vy || !vz)
+    return fail(AWK_ERR_INVALID, "awk_hfield_compose_hv: null pointer");
+  if (ox == gx || oy == gy || oz == gz) return fail(AWK_ERR_INVALID, "awk_hfield_compose_hv: output aliases g");
+  const float sp[3] = {spx, spy, spz};
+  return launch_compose_hv(ox, oy, oz, gx, gy, gz, vx, vy, vz, nx, ny, nz, sp, delta, nullptr,
+                           inverse, bg, as_stream(stream));
+}
+
+awk_status awk_hfield_compose_vh(float *ox, float *oy, float *oz, const float *vx, const float *vy,
+                                 const float *vz, const float *gx, const float *gy, const float *gz,
+                                 int nx, int ny, int nz, float spx, float spy, float spz, int inverse,
+                                 int bg, awk_stream_t stream) {
+  if (awk_status st = check_dims(nx, ny, nz, "awk_hfield_compose_vh")) return st;
+  if (!ox || !oy || !oz || !gx || !gy || !gz || !vx || !vy || !vz)
+    return fail(AWK_ERR_INVALID, "awk_hfield_compose_vh: null pointer");
+  if (ox == vx || oy == vy || oz == vz) return fail(AWK_ERR_INVALID, "awk_hfield_compose_vh: output aliases v");
+  const float sp[3] = {spx, spy, spz};
+  return launch_compose_vh(ox, oy, oz, vx, vy, vz, gx, gy, gz, nx, ny, nz, sp, nullptr, inverse,
+                           bg, as_stream(stream));
+}
+
+awk_status awk_gradient(float *gx, float *gy, float *gz, const float *img, int nx, int ny, int nz,
+                        float spx, float spy, float spz, int wrap, awk_stream_t stream) {
+  if (awk_status st = check_dims(nx, ny, nz, "awk_gradient")) return st;
+  if (!gx || !gy || !gz || !img) return fail(AWK_ERR_INVALID, "awk_gradient: null pointer");
+  hipLaunchKernelGGL(k_gradient, dim3(grid_for((long long)nx * ny * nz, kBlock)), dim3(kBlock), 0,
+                     as_stream(stream), gx, gy, gz, img, nx, ny, nz, spx, spy, spz, wrap);
+  AWK_LAUNCH_CHECK("k_gradient");
+  return AWK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+// awk_splat.hip -- splatting (forward apply) on gfx950.
+//
+// dst(y) = sum_x w(h(x), y) * src(x) over the 8 trilinear corners of h(x)
+// (cplSplat3DH, CUDACommon/src/cudaSplatAtomic.cu:185-247 and 569-599, with the
+// floor fix ADJUST_INT of cudaSplatAtomic.cu:27).  Three accumulation modes:
+//   FIXED64  int64 fixed point with 32 fractional bits: integer adds commute, so
+//            the result is bit-identical for any scheduling (deterministic);
+//            per-contribution rounding error <= 2^-33, |sum| < 2^31.
+//   FIXED20  the reference's int32 fixed point S2p20 = int(a * 2^20 + 0.5f)
+//            (CUDACommon/inc/cplMacro.h:17, 67-91), accumulated in the dst buffer
+//            itself and converted in place (FixedToFloating); |sum| < 2048.
+//   FLOAT    float atomics (fast; summation order, hence rounding, varies).
+#include "awk_ops.h"
+
+namespace awk {
+
+constexpr int kSplatBlock = 256;
+
+template <int MODE>
+__global__ void __launch_bounds__(kSplatBlock)
+k_splat(void *__restrict__ acc, const float *__restrict__ src, const float *__restrict__ hx,
+        const float *__restrict__ hy, const float *__restrict__ hz, int nx, int ny, int nz) {
+  const int N = nx * ny * nz;
+  const int i = (int)(xcd_block() * blockDim.x + threadIdx.x);
+  if (i >= N) return;
+  const float x = hx[i], y = hy[i], z = hz[i];
+  const float mass = src[i];
+  const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+  const int xi = (int)fx, yi = (int)fy, zi = (int)fz;
+  const float dx = 1.f - (x - fx), dy = 1.f - (y - fy), dz = 1.f - (z - fz);
+  const float wx[2] = {dx, 1.f - dx}, wy[2] = {dy, 1.f - dy}, wz[2] = {dz, 1.f - dz};
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int ox = c & 1, oy = (c >> 1) & 1, oz = c >> 2;
+    const int X = xi + ox, Y = yi + oy, Z = zi + oz;
+    if (X < 0 || X >= nx || Y < 0 || Y >= ny || Z < 0 || Z >= nz) continue;
+    const int id = (Z * ny + Y) * nx + X;
+    // same factor order as the reference: mass * wx * wy * wz
+    const float w = mass * wx[ox] * wy[oy] * wz[oz];
+    if (MODE == AWK_SPLAT_FIXED64) {
+      const long long q = llrint((double)w * 4294967296.0);
+      atomicAdd(reinterpret_cast<unsigned long long *>(acc) + id, (unsigned long long)q);
+    } else if (MODE == AWK_SPLAT_FIXED20) {
+      atomicAdd(reinterpret_cast<int *>(acc) + id, (int)(w * 1048576.f + 0.5f));
+    } else {
+      atomicAdd(reinterpret_cast<float *>(acc) + id, w);
+    }
+  }
+}
+
+__global__ void k_fixed64_to_float(float *__restrict__ dst, const long long *__restrict__ acc, int N) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < N) dst[i] = (float)((double)acc[i] * (1.0 / 4294967296.0));
+}
+
+__global__ void k_fixed20_to_float(float *__restrict__ dst, int N) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < N) dst[i] = (float)reinterpret_cast<const int *>(dst)[i] / 1048576.f;
+}
+
+awk_status launch_splat(float *dst, const float *src, const float *hx, const float *hy,
+                        const float *hz, int nx, int ny, int nz, int mode, hipStream_t s) {
+  const long long N = (long long)nx * ny * nz;
+  const unsigned grid = grid_for(N, kSplatBlock);
+  const unsigned cgrid = (unsigned)((N + 255) / 256);
+  if (mode == AWK_SPLAT_FIXED64) {
+    void *acc = nullptr;
+    AWK_HIP(hipMallocAsync(&acc, (size_t)N * 8, s));
+    AWK_HIP(hipMemsetAsync(acc, 0, (size_t)N * 8, s));
+    hipLaunchKernelGGL(k_splat<AWK_SPLAT_FIXED64>, dim3(grid), dim3(kSplatBlock), 0, s, acc, src, hx,
+                       hy, hz, nx, ny, nz);
+    AWK_LAUNCH_CHECK("k_splat<FIXED64>");
+    hipLaunchKernelGGL(k_fixed64_to_float, dim3(cgrid), dim3(256), 0, s, dst,
+                       reinterpret_cast<const long long *>(acc), (int)N);
+    AWK_LAUNCH_CHECK("k_fixed64_to_float");
+    AWK_HIP(hipFreeAsync(acc, s));
+  } else if (mode == AWK_SPLAT_FIXED20) {
+    if ((const void *)dst == (const void *)src) return fail(AWK_ERR_INVALID, "awk_splat: dst aliases src");
+    AWK_HIP(hipMemsetAsync(dst, 0, (size_t)N * 4, s));
+    hipLaunchKernelGGL(k_splat<AWK_SPLAT_FIXED20>, dim3(grid), dim3(kSplatBlock), 0, s, (void *)dst,
+                       src, hx, hy, hz, nx, ny, nz);
+    AWK_LAUNCH_CHECK("k_splat<FIXED20>");
+    hipLaunchKernelGGL(k_fixed20_to_float, dim3(cgrid), dim3(256), 0, s, dst, (int)N);
+    AWK_LAUNCH_CHECK("k_fixed20_to_float");
+  } else if (mode == AWK_SPLAT_FLOAT) {
+    if ((const void *)dst == (const void *)src) return fail(AWK_ERR_INVALID, "awk_splat: dst aliases src");
+    AWK_HIP(hipMemsetAsync(dst, 0, (size_t)N * 4, s));
+    hipLaunchKernelGGL(k_splat<AWK_SPLAT_FLOAT>, dim3(grid), dim3(kSplatBlock), 0, s, (void *)dst,
+                       src, hx, hy, hz, nx, ny, nz);
+    AWK_LAUNCH_CHECK("k_splat<FLOAT>");
+  } else {
+    return fail(AWK_ERR_INVALID, "awk_splat: unknown mode");
+  }
+  return AWK_OK;
+}
+
+}  // namespace awk
+
+using namespace awk;
+
+extern "C" awk_status awk_splat(float *dst, const float *src, const float *hx, const float *hy,
+                                const float *hz, int nx, int ny, int nz, int mode,
+                                awk_stream_t stream) {
+  if (nx < 1 || ny < 1 || nz < 1 || (long long)nx * ny * nz >= (1LL << 31))
+    return fail(AWK_ERR_INVALID, "awk_splat: bad sizes");
+  if (!dst || !src || !hx || !hy || !hz) return fail(AWK_ERR_INVALID, "awk_splat: null pointer");
+  return launch_splat(dst, src, hx, hy, hz, nx, ny, nz, mode, as_stream(stream));
+}

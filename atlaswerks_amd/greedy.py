@@ -1,0 +1,141 @@
+"""Greedy-fluid registration on the GPU.
+
+Mirrors the reference driver layer for the greedy path:
+``GreedyIteratorGPU`` (Algorithms/Greedy/GreedyIteratorGPU.cxx:24-135),
+``GreedyDeformationDataGPU`` (per-registration device state) and
+``GreedyWarp<IteratorT>::RunWarp`` (Algorithms/Greedy/GreedyWarp.txx:69-125).
+Each ``Iterate`` is one call of ``awk_greedy_iterate`` (six HIP kernels, no host
+sync); energies stay on the device until ``energies()`` is read.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from .device import dims, planes_arr, ptr, stream_handle
+from . import hfield
+
+
+class GreedyIteratorParam:
+    """GreedyIteratorParam (GreedyIteratorParam.h:20-40) + DiffOperParam."""
+
+    def __init__(self, max_pert=0.1, alpha=0.01, beta=0.01, gamma=0.001):
+        self.MaxPert = max_pert
+        self.alpha, self.beta, self.gamma = alpha, beta, gamma
+
+
+class GreedyDeformationDataGPU:
+    """Device state of one registration: I0 (moving), I1 (fixed), Def1To0, Def0To1."""
+
+    def __init__(self, I0, I1, spacing=(1.0, 1.0, 1.0), compute_inverse=False, max_iters=4096,
+                 h0=None):
+        if I0.shape != I1.shape:
+            raise ValueError("I0 and I1 must have the same size")
+        self.I0, self.I1 = I0, I1
+        self.spacing = tuple(float(s) for s in spacing)
+        nz, ny, nx = I0.shape
+        dev = I0.device
+        self.Def1To0 = torch.empty((3, nz, ny, nx), dtype=torch.float32, device=dev)
+        self._hbuf = torch.empty_like(self.Def1To0)
+        if h0 is None:
+            hfield.setToIdentity(self.Def1To0)
+        else:
+            self.Def1To0.copy_(h0)
+        self.compute_inverse = compute_inverse
+        self.Def0To1 = None
+        self._hinvbuf = None
+        if compute_inverse:
+            self.Def0To1 = torch.empty_like(self.Def1To0)
+            self._hinvbuf = torch.empty_like(self.Def1To0)
+            hfield.setToIdentity(self.Def0To1)
+        self._energy = torch.zeros(max_iters, dtype=torch.float64, device=dev)
+        self.cur_iter = 0
+
+    def energy_slot(self):
+        if self.cur_iter >= self._energy.numel():
+            grown = torch.zeros(2 * self._energy.numel(), dtype=torch.float64, device=self._energy.device)
+            grown[: self._energy.numel()] = self._energy
+            self._energy = grown
+        return self._energy[self.cur_iter:self.cur_iter + 1]
+
+    def energies(self):
+        """Per-iteration energies (host numpy, synchronises)."""
+        return self._energy[: self.cur_iter].cpu().numpy()
+
+
+class GreedyIteratorGPU:
+    def __init__(self, size, spacing=(1.0, 1.0, 1.0), param=None):
+        self.param = param or GreedyIteratorParam()
+        self._h = None
+        self.SetScaleLevel(size, spacing, self.param)
+
+    def SetScaleLevel(self, size, spacing, param=None):
+        if param is not None:
+            self.param = param
+        self._destroy()
+        self.size = tuple(int(s) for s in size)
+        self.spacing = tuple(float(s) for s in spacing)
+        h = ctypes.c_void_p()
+        p = self.param
+        _lib.call("awk_greedy_create", ctypes.byref(h), *self.size, *self.spacing, p.alpha, p.beta,
+                  p.gamma, p.MaxPert)
+        self._h = h
+
+    def UpdateStepSizeNextIteration(self):
+        _lib.call("awk_greedy_update_step_next", self._h)
+
+    def SetAutoStepReduce(self, on):
+        _lib.call("awk_greedy_set_auto_step_reduce", self._h, int(bool(on)))
+
+    def StepSize(self):
+        s = ctypes.c_float()
+        _lib.call("awk_greedy_get_step", self._h, ctypes.byref(s))
+        return s.value
+
+    def SetStepSize(self, step):
+        _lib.call("awk_greedy_set_step", self._h, float(step))
+
+    def Iterate(self, data, stream=None):
+        if dims(data.I0) != self.size:
+            raise ValueError("deformation data size does not match the iterator")
+        hinv_in = planes_arr(data.Def0To1) if data.compute_inverse else None
+        hinv_out = planes_arr(data._hinvbuf) if data.compute_inverse else None
+        e = data.energy_slot()
+        _lib.call("awk_greedy_iterate", self._h, ptr(data.I0), ptr(data.I1),
+                  planes_arr(data.Def1To0), planes_arr(data._hbuf), hinv_in, hinv_out,
+                  ctypes.c_void_p(e.data_ptr()), stream_handle(stream))
+        # ping-pong instead of copying the update back
+        data.Def1To0, data._hbuf = data._hbuf, data.Def1To0
+        if data.compute_inverse:
+            data.Def0To1, data._hinvbuf = data._hinvbuf, data.Def0To1
+        data.cur_iter += 1
+
+    def _destroy(self):
+        if self._h is not None:
+            _lib.load().awk_greedy_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self._destroy()
+        except Exception:
+            pass
+
+
+class GreedyWarp:
+    """Single-scale GreedyWarp::RunWarp (GreedyWarp.txx:69-125)."""
+
+    def __init__(self, I0, I1, n_iterations=50, spacing=(1.0, 1.0, 1.0), param=None,
+                 compute_inverse=False, auto_step_reduce=False, h0=None):
+        self.data = GreedyDeformationDataGPU(I0, I1, spacing, compute_inverse,
+                                             max_iters=max(n_iterations, 1), h0=h0)
+        self.iterator = GreedyIteratorGPU(dims(I0), spacing, param)
+        self.iterator.SetAutoStepReduce(auto_step_reduce)
+        self.n_iterations = n_iterations
+
+    def RunWarp(self, stream=None):
+        for it in range(self.n_iterations):
+            if it == 0:
+                self.iterator.UpdateStepSizeNextIteration()
+            self.iterator.Iterate(self.data, stream)
+        return self.data

@@ -1,0 +1,110 @@
+"""HField3DUtils on the GPU (the reference's ``cudaHField3DUtils`` namespace,
+CUDACommon/inc/cudaHField3DUtils.h).  All functions are asynchronous on the
+current torch stream and write into caller-provided tensors.
+
+Background strategies: BACKGROUND_STRATEGY_{ID, ZERO, PARTIAL_ID, PARTIAL_ZERO,
+CLAMP, WRAP} for vector interpolation, VAL/ZERO/CLAMP/WRAP for images.
+"""
+from . import _lib
+from .device import dims, planes, ptr, stream_handle
+
+BACKGROUND_STRATEGY_ID = _lib.BG_ID
+BACKGROUND_STRATEGY_ZERO = _lib.BG_ZERO
+BACKGROUND_STRATEGY_PARTIAL_ID = _lib.BG_PARTIAL_ID
+BACKGROUND_STRATEGY_PARTIAL_ZERO = _lib.BG_PARTIAL_ZERO
+BACKGROUND_STRATEGY_CLAMP = _lib.BG_CLAMP
+BACKGROUND_STRATEGY_WRAP = _lib.BG_WRAP
+BACKGROUND_STRATEGY_VAL = _lib.BG_VAL
+
+
+def _same(*ts):
+    d = dims(ts[0])
+    for t in ts[1:]:
+        if dims(t) != d:
+            raise ValueError("size mismatch: %s vs %s" % (d, dims(t)))
+    return d
+
+
+def setToIdentity(h, stream=None):
+    nx, ny, nz = dims(h)
+    _lib.call("awk_hfield_set_to_identity", *planes(h), nx, ny, nz, stream_handle(stream))
+
+
+def velocityToH(v, spacing=(1.0, 1.0, 1.0), stream=None):
+    nx, ny, nz = dims(v)
+    _lib.call("awk_hfield_velocity_to_h", *planes(v), nx, ny, nz, *[float(s) for s in spacing],
+              stream_handle(stream))
+
+
+def hToVelocity(h, spacing=(1.0, 1.0, 1.0), stream=None):
+    nx, ny, nz = dims(h)
+    _lib.call("awk_hfield_h_to_velocity", *planes(h), nx, ny, nz, *[float(s) for s in spacing],
+              stream_handle(stream))
+
+
+def computeInverseZerothOrder(h, hinv, stream=None):
+    nx, ny, nz = _same(h, hinv)
+    _lib.call("awk_hfield_inverse_zeroth_order", *planes(hinv), *planes(h), nx, ny, nz,
+              stream_handle(stream))
+
+
+def apply(out, image, h, bg=BACKGROUND_STRATEGY_VAL, background=0.0, stream=None):
+    """out(x) = image(h(x))."""
+    nx, ny, nz = _same(out, image, h)
+    _lib.call("awk_hfield_apply", ptr(out), ptr(image), *planes(h), nx, ny, nz, int(bg),
+              float(background), stream_handle(stream))
+
+
+def compose(out, f, g, bg=BACKGROUND_STRATEGY_ID, stream=None):
+    """out(x) = f(g(x))."""
+    nx, ny, nz = _same(out, f, g)
+    _lib.call("awk_hfield_compose", *planes(out), *planes(f), *planes(g), nx, ny, nz, int(bg),
+              stream_handle(stream))
+
+
+def composeHV(out, g, v, spacing=(1.0, 1.0, 1.0), bg=BACKGROUND_STRATEGY_PARTIAL_ID, delta=1.0,
+              stream=None):
+    """out(x) = g(x + delta*v(x)/spacing)."""
+    nx, ny, nz = _same(out, g, v)
+    _lib.call("awk_hfield_compose_hv", *planes(out), *planes(g), *planes(v), nx, ny, nz,
+              *[float(s) for s in spacing], float(delta), 0, int(bg), stream_handle(stream))
+
+
+def composeHVInv(out, g, v, spacing=(1.0, 1.0, 1.0), bg=BACKGROUND_STRATEGY_PARTIAL_ID, delta=1.0,
+                 stream=None):
+    """out(x) = g(x - delta*v(x)/spacing)."""
+    nx, ny, nz = _same(out, g, v)
+    _lib.call("awk_hfield_compose_hv", *planes(out), *planes(g), *planes(v), nx, ny, nz,
+              *[float(s) for s in spacing], float(delta), 1, int(bg), stream_handle(stream))
+
+
+def composeVH(out, v, g, spacing=(1.0, 1.0, 1.0), bg=BACKGROUND_STRATEGY_PARTIAL_ZERO, stream=None):
+    """out(x) = g(x) + v(g(x))/spacing."""
+    nx, ny, nz = _same(out, v, g)
+    _lib.call("awk_hfield_compose_vh", *planes(out), *planes(v), *planes(g), nx, ny, nz,
+              *[float(s) for s in spacing], 0, int(bg), stream_handle(stream))
+
+
+def composeVHInv(out, v, g, spacing=(1.0, 1.0, 1.0), bg=BACKGROUND_STRATEGY_PARTIAL_ZERO,
+                 stream=None):
+    """out(x) = g(x) - v(g(x))/spacing."""
+    nx, ny, nz = _same(out, v, g)
+    _lib.call("awk_hfield_compose_vh", *planes(out), *planes(v), *planes(g), nx, ny, nz,
+              *[float(s) for s in spacing], 1, int(bg), stream_handle(stream))
+
+
+def computeGradient(grad, image, spacing=(1.0, 1.0, 1.0), wrap=False, stream=None):
+    """Array3DUtils::computeGradient / cplComputeGradient."""
+    nx, ny, nz = _same(grad, image)
+    _lib.call("awk_gradient", *planes(grad), ptr(image), nx, ny, nz, *[float(s) for s in spacing],
+              int(bool(wrap)), stream_handle(stream))
+
+
+def splat(dst, src, h, mode=_lib.SPLAT_FIXED64, stream=None):
+    """cplSplat3DH / HField3DUtils::forwardApply (normalize=false): dst(h(x)) += src(x)."""
+    nx, ny, nz = _same(dst, src, h)
+    _lib.call("awk_splat", ptr(dst), ptr(src), *planes(h), nx, ny, nz, int(mode),
+              stream_handle(stream))
+
+
+cplSplat3DH = splat

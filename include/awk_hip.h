@@ -1,0 +1,209 @@
+/*
+ * awk_hip.h -- C-ABI of the MI355X greedy-fluid / LDMM hot path (libawk_hip.so).
+ *
+ * Plain C: pointers, sizes, status codes.  No HIP or torch types cross this
+ * boundary; streams are passed as opaque `awk_stream_t` (a hipStream_t, NULL =
+ * the null stream).  All device pointers are float32 device memory owned by the
+ * caller.  Every call is asynchronous on its stream unless it returns a host
+ * scalar (documented per call).  Errors never throw: a non-zero awk_status is
+ * returned and awk_last_error() gives the message (thread-local).
+ *
+ * Layouts (device):
+ *   image         float[nz][ny][nx], x fastest (reference Array3D order,
+ *                 DataTypes/Array3D.txx:426-447).
+ *   vector field  three planes (x, y, z), each float[nz][ny][nx] (SoA, the
+ *                 reference cplVector3DArray, CUDACommon/src/cudaVector3DArray.cu:46-57).
+ *   hfield        vector field of voxel-index coordinates (h(x) = x + u(x)).
+ *
+ * Reference citations are relative to the AtlasWerks source root.
+ */
+#ifndef AWK_HIP_H
+#define AWK_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int awk_status;
+enum {
+  AWK_OK = 0,
+  AWK_ERR_INVALID = 1,      /* bad argument / size / null pointer */
+  AWK_ERR_HIP = 2,          /* HIP runtime error */
+  AWK_ERR_ALLOC = 3,        /* device allocation failed */
+  AWK_ERR_UNSUPPORTED = 4   /* parameter combination not implemented */
+};
+
+typedef void *awk_stream_t;
+
+/* Background strategies: HField3DUtils::VectorBackgroundStrategy
+ * (UtilitiesDataTypes/HField3DUtils.h) plus the scalar VAL of
+ * Array3DUtils::ScalarBackgroundStrategyT (Array3DUtils.h:59-63). */
+enum {
+  AWK_BG_ID = 0,
+  AWK_BG_ZERO = 1,
+  AWK_BG_PARTIAL_ID = 2,
+  AWK_BG_PARTIAL_ZERO = 3,
+  AWK_BG_CLAMP = 4,
+  AWK_BG_WRAP = 5,
+  AWK_BG_VAL = 6
+};
+
+/* Last error message of the calling host thread ("" if none). */
+const char *awk_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int awk_version(void);
+
+/* ====================================================================== */
+/* FluidKernelFFT: the Navier operator L = -alpha*Lap - beta*grad(div) + gamma */
+/* ====================================================================== */
+
+typedef struct awk_fluid_kernel *awk_fk_t;
+
+enum {
+  AWK_FK_EIGEN = 1,    /* use precomputed L^(+-LPow) tables (DiffOperParam UseEigenLUT) */
+  AWK_FK_DIVFREE = 2   /* project onto divergence-free fields (DiffOperParam DivergenceFree) */
+};
+
+/* Replaces DiffOperGPU::SetSize + SetParams (Algorithms/Common/DiffOperGPU.cxx:32-125)
+ * and FFTSolverPlan3D::setSize/setParams (CUDACommon/inc/cudaFFTSolver.h:99-129).
+ * The operator owns its spectrum scratch and tables; any nx, ny, nz >= 1. */
+awk_status awk_fluid_kernel_create(awk_fk_t *out, int nx, int ny, int nz, float spx, float spy,
+                                   float spz, float alpha, float beta, float gamma, float lpow,
+                                   int flags);
+/* DiffOperGPU::SetAlpha/SetBeta/SetGamma/SetLPow/SetUseEigenLUT/SetDivergenceFree
+ * (Algorithms/Common/DiffOperGPU.h:40-115). */
+awk_status awk_fluid_kernel_set_params(awk_fk_t k, float alpha, float beta, float gamma,
+                                       float lpow, int flags);
+awk_status awk_fluid_kernel_destroy(awk_fk_t k);
+/* Bytes of device scratch the operator holds. */
+awk_status awk_fluid_kernel_scratch_bytes(awk_fk_t k, unsigned long long *bytes);
+
+/* In place on the SoA planes x, y, z.  inverse != 0: v = L^-1 f
+ * (DiffOperGPU::ApplyInverseOperator(float*,float*,float*), DiffOperGPU.cxx:175-183;
+ * KernelInterfaceGPU::ApplyInverseOperator, KernelInterfaceGPU.h:36-41);
+ * inverse == 0: f = L v (ApplyOperator, DiffOperGPU.cxx:165-173).
+ * Matches DiffOper::ApplyInverseOperatorOnTheFly / ApplyOperatorOnTheFly
+ * (DiffOper.cxx:747-905): the 1/N scaling is part of the operator. */
+awk_status awk_fluid_kernel_apply(awk_fk_t k, float *x, float *y, float *z, int inverse,
+                                  awk_stream_t stream);
+
+/* ====================================================================== */
+/* HField3DUtils (cudaHField3DUtils namespace, CUDACommon/inc/cudaHField3DUtils.h) */
+/* ====================================================================== */
+
+/* h(x) = x (HField3DUtils::setToIdentity, HField3DUtils.txx:19-35). */
+awk_status awk_hfield_set_to_identity(float *hx, float *hy, float *hz, int nx, int ny, int nz,
+                                      awk_stream_t stream);
+/* v -> h: h = x + v/sp (velocityToH, HField3DUtils.txx:74-90); in place. */
+awk_status awk_hfield_velocity_to_h(float *x, float *y, float *z, int nx, int ny, int nz,
+                                    float spx, float spy, float spz, awk_stream_t stream);
+/* h -> v: v = (h - x)*sp (hToVelocity, HField3DUtils.txx:92-108); in place. */
+awk_status awk_hfield_h_to_velocity(float *x, float *y, float *z, int nx, int ny, int nz,
+                                    float spx, float spy, float spz, awk_stream_t stream);
+/* hinv(x) = 2x - h(x) (computeInverseZerothOrder, HField3DUtils.txx:585-604). */
+awk_status awk_hfield_inverse_zeroth_order(float *ox, float *oy, float *oz, const float *hx,
+                                           const float *hy, const float *hz, int nx, int ny,
+                                           int nz, awk_stream_t stream);
+
+/* def(x) = image(h(x)): HField3DUtils::apply (HField3DUtils.txx:728-752),
+ * cudaHField3DUtils::apply (CUDACommon/src/cudaHField3DUtils.cu:580-640).
+ * bg: AWK_BG_VAL (with `background`), AWK_BG_ZERO, AWK_BG_CLAMP, AWK_BG_WRAP. */
+awk_status awk_hfield_apply(float *out, const float *img, const float *hx, const float *hy,
+                            const float *hz, int nx, int ny, int nz, int bg, float background,
+                            awk_stream_t stream);
+
+/* h(x) = f(g(x)): HField3DUtils::compose (HField3DUtils.txx:290-310),
+ * cudaHField3DUtils::compose (cudaHField3DUtils.cu:1163-1226).  out may not
+ * alias f. */
+awk_status awk_hfield_compose(float *ox, float *oy, float *oz, const float *fx, const float *fy,
+                              const float *fz, const float *gx, const float *gy, const float *gz,
+                              int nx, int ny, int nz, int bg, awk_stream_t stream);
+
+/* h(x) = g(x + s*delta*v(x)/sp), s = +1 (composeHV, HField3DUtils.txx:432-455) or
+ * -1 if inverse (composeHVInv, HField3DUtils.txx:457-480); the spacing variant of
+ * cudaHField3DUtilsSp.cu:454-470.  `delta` scales v (1 for the plain call; the
+ * greedy update passes its step size).  out may not alias g. */
+awk_status awk_hfield_compose_hv(float *ox, float *oy, float *oz, const float *gx,
+                                 const float *gy, const float *gz, const float *vx,
+                                 const float *vy, const float *vz, int nx, int ny, int nz,
+                                 float spx, float spy, float spz, float delta, int inverse,
+                                 int bg, awk_stream_t stream);
+
+/* h(x) = g(x) + s*v(g(x))/sp, s = +1 (composeVH) or -1 (composeVHInv)
+ * (HField3DUtils.txx:370-430; cudaHField3DUtilsSp.cu:360-390).  out may not alias v. */
+awk_status awk_hfield_compose_vh(float *ox, float *oy, float *oz, const float *vx,
+                                 const float *vy, const float *vz, const float *gx,
+                                 const float *gy, const float *gz, int nx, int ny, int nz,
+                                 float spx, float spy, float spz, int inverse, int bg,
+                                 awk_stream_t stream);
+
+/* Gradient (central differences inside, one-sided at the edges, or periodic if
+ * wrap): Array3DUtils::computeGradient (Array3DUtils.txx:783-901),
+ * cplComputeGradient (CUDACommon/src/cudaImage3D.cu:620-656). */
+awk_status awk_gradient(float *gx, float *gy, float *gz, const float *img, int nx, int ny,
+                        int nz, float spx, float spy, float spz, int wrap,
+                        awk_stream_t stream);
+
+/* ====================================================================== */
+/* Splatting (forward apply)                                                */
+/* ====================================================================== */
+
+enum {
+  AWK_SPLAT_FIXED64 = 0,   /* deterministic: int64 fixed point (2^-32), any order */
+  AWK_SPLAT_FIXED20 = 1,   /* the reference's int32 S2p20 fixed point (cplMacro.h:17,67-91) */
+  AWK_SPLAT_FLOAT = 2      /* float atomics (fast, order-dependent rounding) */
+};
+
+/* dst(y) = sum over x with h(x) near y of w(h(x), y) * src(x): positions are an
+ * hfield (voxel coordinates).  cplSplat3DH (CUDACommon/inc/cudaSplat.h:127-128,
+ * cudaSplatAtomic.cu:569-599) / HField3DUtils::forwardApply with normalize=false
+ * (HField3DUtils.txx:1281-1413).  Corners outside the volume are dropped. */
+awk_status awk_splat(float *dst, const float *src, const float *hx, const float *hy,
+                     const float *hz, int nx, int ny, int nz, int mode, awk_stream_t stream);
+
+/* ====================================================================== */
+/* Greedy-fluid iteration                                                   */
+/* ====================================================================== */
+
+typedef struct awk_greedy *awk_greedy_t;
+
+/* One registration's device state: the caller's I0, I1 (float images) and
+ * Def1To0 hfield (SoA, updated in place), optional Def0To1; the iterator owns
+ * its operator and scratch.  GreedyIteratorGPU (Algorithms/Greedy/GreedyIteratorGPU.cxx:24-70)
+ * + GreedyDeformationDataGPU.  Parameters: GreedyIteratorParam::MaxPert
+ * (GreedyIteratorParam.h:29-34), DiffOperParam alpha/beta/gamma. */
+awk_status awk_greedy_create(awk_greedy_t *out, int nx, int ny, int nz, float spx, float spy,
+                             float spz, float alpha, float beta, float gamma, float max_pert);
+awk_status awk_greedy_destroy(awk_greedy_t g);
+
+/* Request a new step size (MaxPert / max|v|) at the next iteration
+ * (DeformationIteratorInterface::UpdateStepSizeNextIteration; GreedyWarp.txx:99). */
+awk_status awk_greedy_update_step_next(awk_greedy_t g);
+awk_status awk_greedy_set_step(awk_greedy_t g, float step);
+awk_status awk_greedy_get_step(awk_greedy_t g, float *step);
+
+/* One GreedyIteratorCPU::Iterate / GreedyIteratorGPU::Iterate
+ * (GreedyIteratorCPU.cxx:97-138; GreedyIteratorGPU.cxx:92-135):
+ *   Idef = I0(h); f = (I1 - Idef) * grad(Idef); E = prod(sp) * sum (I1-Idef)^2;
+ *   v = L^-1 f; [step = MaxPert/max|v| if requested]; h' = h(x + step*v/sp)
+ *   (PARTIAL_ID); if hinv given: hinv' = hinv - step*v(hinv)/sp (PARTIAL_ZERO).
+ * h_in -> h_out (SoA planes): passing distinct buffers and swapping them between
+ * iterations avoids the copy the reference makes (GreedyIteratorGPU.cxx:78-80);
+ * h_out == h_in is allowed (the update then goes through iterator scratch).
+ * hinv_in/hinv_out may be NULL (no Def0To1).  The energy is summed in FP64 on
+ * the device in a fixed order (deterministic) and, if energy_dev is non-NULL,
+ * stored there (device double) without a host sync. */
+awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
+                              const float *const h_in[3], float *const h_out[3],
+                              const float *const hinv_in[3], float *const hinv_out[3],
+                              double *energy_dev, awk_stream_t stream);
+
+/* AutoStepReduce (GreedyParam.h:61-65, GreedyDeformationData.cxx:176-183): when
+ * on, an energy increase over the previous iteration halves the step before
+ * this iteration's update (done on the device). */
+awk_status awk_greedy_set_auto_step_reduce(awk_greedy_t g, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AWK_HIP_H */

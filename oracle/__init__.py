@@ -1,0 +1,249 @@
+"""CPU oracle for the greedy-fluid hot path -- TEST INFRASTRUCTURE ONLY.
+
+This package is the checker the HIP path is compared against.  Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import it,
+and only as the checker; the product library never calls into it.
+
+It wraps ``liboracle.so`` (built from ``awk_oracle.cpp`` by ``oracle/Makefile``),
+a C++ restatement of the reference CPU path; every function there cites the
+reference file:line it follows.  Arrays here are numpy, indexed ``[z, y, x]``
+(x fastest, as ``Array3D`` stores them); vector fields are AoS ``[z, y, x, 3]``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+BG_ID, BG_ZERO, BG_PARTIAL_ID, BG_PARTIAL_ZERO, BG_CLAMP, BG_WRAP, BG_VAL = range(7)
+DO_EIGEN, DO_DIVFREE = 1, 2
+
+
+class GreedyParams(ctypes.Structure):
+    _fields_ = [
+        ("size", ctypes.c_int * 3),
+        ("sp", ctypes.c_double * 3),
+        ("alpha", ctypes.c_double),
+        ("beta", ctypes.c_double),
+        ("gamma", ctypes.c_double),
+        ("max_pert", ctypes.c_double),
+        ("n_iter", ctypes.c_int),
+        ("compute_inverse", ctypes.c_int),
+        ("auto_step_reduce", ctypes.c_int),
+    ]
+
+
+def build():
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = ctypes.CDLL(path)
+        _LIB.orc_trilerp_scalar_d.restype = ctypes.c_double
+        _LIB.orc_greedy_time_f.restype = ctypes.c_double
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _size(shape3):
+    nz, ny, nx = shape3
+    return (ctypes.c_int * 3)(nx, ny, nz)
+
+
+def _sp(sp):
+    return (ctypes.c_double * 3)(*[float(s) for s in sp])
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def set_mutation(m):
+    lib().orc_set_mutation(int(m))
+
+
+def fft(x, inverse=False, dtype=np.float64):
+    """Unnormalised 1-D complex FFT of the oracle (forward = exp(-2 pi i jk/n))."""
+    x = np.asarray(x)
+    if dtype == np.float64:
+        buf = np.ascontiguousarray(x, dtype=np.complex128).copy()
+        lib().orc_fft_d(len(buf), _p(buf), int(inverse))
+    else:
+        buf = np.ascontiguousarray(x, dtype=np.complex64).copy()
+        lib().orc_fft_f(len(buf), _p(buf), int(inverse))
+    return buf
+
+
+def lut(n, sp):
+    c = np.zeros(n)
+    s = np.zeros(n)
+    lib().orc_lut(int(n), ctypes.c_double(sp), _p(c), _p(s))
+    return c, s
+
+
+def diffoper(vf, sp=(1, 1, 1), alpha=0.01, beta=0.01, gamma=0.001, inverse=True,
+             lpow=1.0, eigen=False, divfree=False, dtype=np.float64):
+    """DiffOper L or L^-1 of an AoS field vf[z,y,x,3]; returns a new array."""
+    assert vf.ndim == 4 and vf.shape[3] == 3
+    flags = (DO_EIGEN if eigen else 0) | (DO_DIVFREE if divfree else 0)
+    out = np.ascontiguousarray(vf, dtype=dtype).copy()
+    fn = lib().orc_diffoper_d if dtype == np.float64 else lib().orc_diffoper_f
+    rc = fn(_size(vf.shape[:3]), _sp(sp), ctypes.c_double(alpha), ctypes.c_double(beta),
+            ctypes.c_double(gamma), ctypes.c_double(lpow), int(flags), _p(out), int(inverse))
+    if rc != 0:
+        raise ValueError("oracle diffoper failed")
+    return out
+
+
+def apply(img, h, bg=BG_VAL, background=0.0, dtype=np.float64):
+    img = _f32(img)
+    h = _f32(h)
+    out = np.zeros(img.shape, dtype=dtype)
+    fn = lib().orc_apply_d if dtype == np.float64 else lib().orc_apply_f
+    fn(_p(img), _p(h), _size(h.shape[:3]), _p(out), int(bg), ctypes.c_double(background))
+    return out
+
+
+def compose(f, g, bg=BG_PARTIAL_ID):
+    f, g = _f32(f), _f32(g)
+    out = np.zeros(g.shape, dtype=np.float64)
+    lib().orc_compose_d(_p(f), _p(g), _size(g.shape[:3]), _p(out), int(bg))
+    return out
+
+
+def compose_hv(g, v, sp=(1, 1, 1), bg=BG_PARTIAL_ID, inv=False):
+    g, v = _f32(g), _f32(v)
+    out = np.zeros(g.shape, dtype=np.float64)
+    lib().orc_compose_hv_d(_p(g), _p(v), _size(g.shape[:3]), _sp(sp), _p(out), int(bg), int(inv))
+    return out
+
+
+def compose_vh(v, g, sp=(1, 1, 1), bg=BG_PARTIAL_ZERO, inv=False):
+    v, g = _f32(v), _f32(g)
+    out = np.zeros(g.shape, dtype=np.float64)
+    lib().orc_compose_vh_d(_p(v), _p(g), _size(g.shape[:3]), _sp(sp), _p(out), int(bg), int(inv))
+    return out
+
+
+def trilerp_vec(h, x, y, z, bg):
+    h = _f32(h)
+    out = np.zeros(3)
+    lib().orc_trilerp_vec_d(_p(h), _size(h.shape[:3]), ctypes.c_double(x), ctypes.c_double(y),
+                            ctypes.c_double(z), _p(out), int(bg))
+    return out
+
+
+def trilerp_scalar(img, x, y, z, bg=BG_VAL, background=0.0):
+    img = _f32(img)
+    return lib().orc_trilerp_scalar_d(_p(img), _size(img.shape), ctypes.c_double(x),
+                                      ctypes.c_double(y), ctypes.c_double(z), int(bg),
+                                      ctypes.c_double(background))
+
+
+def forward_apply(img, h, background=0.0, normalize=False):
+    img, h = _f32(img), _f32(h)
+    out = np.zeros(img.shape, dtype=np.float32)
+    lib().orc_forward_apply_f(_p(img), _p(h), _size(img.shape), _p(out),
+                              ctypes.c_double(background), int(normalize))
+    return out
+
+
+def splat(src, h):
+    """cpuSplat3D: double accumulation; h AoS positions."""
+    src = _f32(src)
+    h = _f32(h)
+    px = np.ascontiguousarray(h[..., 0])
+    py = np.ascontiguousarray(h[..., 1])
+    pz = np.ascontiguousarray(h[..., 2])
+    out = np.zeros(src.shape, dtype=np.float64)
+    lib().orc_splat_d(_p(src), _p(px), _p(py), _p(pz), _size(src.shape), _p(out))
+    return out
+
+
+def splat_index(src, v, floor_fix=True):
+    """createIndex keys/values (8N each) for positions voxel + v (AoS v)."""
+    src = _f32(src)
+    vx = np.ascontiguousarray(v[..., 0], dtype=np.float32)
+    vy = np.ascontiguousarray(v[..., 1], dtype=np.float32)
+    vz = np.ascontiguousarray(v[..., 2], dtype=np.float32)
+    n = src.size
+    keys = np.zeros(8 * n, dtype=np.uint32)
+    vals = np.zeros(8 * n, dtype=np.float32)
+    lib().orc_splat_index(_p(src), _p(vx), _p(vy), _p(vz), _size(src.shape), int(floor_fix),
+                          _p(keys), _p(vals))
+    return keys, vals
+
+
+def gradient(img, sp=(1, 1, 1), wrap=False, dtype=np.float64):
+    img = _f32(img)
+    out = np.zeros(img.shape + (3,), dtype=dtype)
+    fn = lib().orc_gradient_d if dtype == np.float64 else lib().orc_gradient_f
+    fn(_p(img), _size(img.shape), _sp(sp), int(wrap), _p(out))
+    return out
+
+
+def identity(shape3, dtype=np.float32):
+    nz, ny, nx = shape3
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    return np.stack([x, y, z], axis=-1).astype(dtype)
+
+
+def greedy_run(I0, I1, n_iter, sp=(1, 1, 1), alpha=0.01, beta=0.01, gamma=0.001,
+               max_pert=0.1, h0=None, compute_inverse=False, auto_step_reduce=False,
+               dtype=np.float32):
+    """GreedyWarp single scale level (GreedyWarp.txx:69-125) in float or double.
+
+    Returns (h, hinv or None, energies (reference definition), energies summed in
+    double, step)."""
+    I0, I1 = _f32(I0), _f32(I1)
+    p = GreedyParams()
+    nz, ny, nx = I0.shape
+    p.size[:] = [nx, ny, nz]
+    p.sp[:] = [float(s) for s in sp]
+    p.alpha, p.beta, p.gamma, p.max_pert = alpha, beta, gamma, max_pert
+    p.n_iter = int(n_iter)
+    p.compute_inverse = int(compute_inverse)
+    p.auto_step_reduce = int(auto_step_reduce)
+    h = identity(I0.shape, dtype) if h0 is None else np.ascontiguousarray(h0, dtype=dtype).copy()
+    hinv = identity(I0.shape, dtype) if compute_inverse else None
+    ef = np.zeros(n_iter)
+    ed = np.zeros(n_iter)
+    step = ctypes.c_double(0)
+    if dtype == np.float32:
+        lib().orc_greedy_run_f(ctypes.byref(p), _p(I0), _p(I1), _p(h),
+                               _p(hinv) if hinv is not None else None, _p(ef), _p(ed),
+                               ctypes.byref(step))
+    else:
+        lib().orc_greedy_run_d(ctypes.byref(p), _p(I0), _p(I1), _p(h),
+                               _p(hinv) if hinv is not None else None, _p(ed),
+                               ctypes.byref(step))
+        ef = ed.copy()
+    return h, hinv, ef, ed, step.value
+
+
+def greedy_time(shape3, n_iter=1, seed=0):
+    """Seconds per FP32 greedy iteration of the oracle on one host thread."""
+    nz, ny, nx = shape3
+    size = (ctypes.c_int * 3)(nx, ny, nz)
+    return lib().orc_greedy_time_f(size, int(n_iter), int(seed))
+
+
+def fluidwarp_asym(fixed, moving, n_iter=20, alpha=0.01, beta=0.01, gamma=1e-4, max_pert=0.5):
+    fixed, moving = _f32(fixed), _f32(moving)
+    h = identity(fixed.shape)
+    hinv = identity(fixed.shape)
+    lib().orc_fluidwarp_asym_f(_p(fixed), _p(moving), _size(fixed.shape), ctypes.c_double(alpha),
+                               ctypes.c_double(beta), ctypes.c_double(gamma),
+                               ctypes.c_double(max_pert), int(n_iter), _p(h), _p(hinv))
+    return h, hinv

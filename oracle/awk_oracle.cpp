@@ -1,0 +1,997 @@
+/*
+ * awk_oracle.cpp -- CPU restatement of the AtlasWerks greedy-fluid hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product).  See awk_oracle.h.
+ * Citations are relative to the AtlasWerks source root.
+ *
+ * The reference CPU path links FFTW3 (Code/Libraries/Algorithms/Common/DiffOper.cxx:30-99),
+ * which is not available here, so the FFT below is this file's own mixed-radix
+ * Stockham transform (any length; radix 4/2 butterflies plus a table-driven DFT
+ * for every other prime factor).  Its twiddles are computed in double.
+ */
+#include "awk_oracle.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <vector>
+
+static int g_mut = 0;
+extern "C" void orc_set_mutation(int m) { g_mut = m; }
+
+namespace {
+
+/* ====================================================================== */
+/* FFT                                                                    */
+/* ====================================================================== */
+
+template <class T>
+struct Fft {
+  typedef std::complex<T> C;
+  int n;
+  std::vector<int> rad;
+  std::vector<C> tw;   // tw[k] = exp(-2 pi i k / n), computed in double
+  std::vector<C> buf;
+
+  explicit Fft(int n_) : n(n_), tw(n_), buf(n_) {
+    int m = n;
+    while (m % 4 == 0) { rad.push_back(4); m /= 4; }
+    while (m % 2 == 0) { rad.push_back(2); m /= 2; }
+    for (int p = 3; m > 1; p += 2) {
+      while (m % p == 0) { rad.push_back(p); m /= p; }
+    }
+    for (int k = 0; k < n; ++k) {
+      double a = -2.0 * M_PI * (double)k / (double)n;
+      tw[k] = C((T)std::cos(a), (T)std::sin(a));
+    }
+  }
+
+  inline C w(long idx, bool inv) const {
+    C t = tw[idx % n];
+    return inv ? std::conj(t) : t;
+  }
+
+  /* Stockham autosort, out-of-place ping-pong; result left in x. */
+  void run(C *x, bool inv) {
+    if (n == 1) return;
+    C *a = x, *b = buf.data();
+    int Ns = 1;
+    C v[128];
+    C o[128];
+    for (int R : rad) {
+      const int m = n / R;
+      const int twstep = n / (Ns * R);
+      for (int j = 0; j < m; ++j) {
+        const int k = j % Ns;
+        for (int r = 0; r < R; ++r) {
+          C t = a[j + r * m];
+          if (k) t *= w((long)r * k * twstep, inv);
+          v[r] = t;
+        }
+        if (R == 2) {
+          o[0] = v[0] + v[1];
+          o[1] = v[0] - v[1];
+        } else if (R == 4) {
+          C s02 = v[0] + v[2], d02 = v[0] - v[2];
+          C s13 = v[1] + v[3], d13 = v[1] - v[3];
+          // forward: multiply d13 by -i ; inverse by +i
+          C jd13 = inv ? C(-d13.imag(), d13.real()) : C(d13.imag(), -d13.real());
+          o[0] = s02 + s13;
+          o[1] = d02 + jd13;
+          o[2] = s02 - s13;
+          o[3] = d02 - jd13;
+        } else {
+          const int step = n / R;
+          for (int q = 0; q < R; ++q) {
+            C acc(0, 0);
+            for (int r = 0; r < R; ++r) acc += v[r] * w((long)((r * q) % R) * step, inv);
+            o[q] = acc;
+          }
+        }
+        const int idxD = (j / Ns) * Ns * R + k;
+        for (int r = 0; r < R; ++r) b[idxD + r * Ns] = o[r];
+      }
+      std::swap(a, b);
+      Ns *= R;
+    }
+    if (a != x) std::memcpy(x, a, sizeof(C) * n);
+  }
+};
+
+/* 3-D real <-> half-complex transforms of one scalar plane.  Spectrum layout is
+ * [z][y][kx] with kx < nx/2+1, the layout FFTW's r2c produces for rank 3,
+ * n = {z, y, x} (DiffOper.cxx:38-66). */
+template <class T>
+struct Fft3 {
+  typedef std::complex<T> C;
+  int nx, ny, nz, nxh;
+  Fft<T> fx, fy, fz;
+  std::vector<C> line, lb;
+  Fft3(int x, int y, int z)
+      : nx(x), ny(y), nz(z), nxh(x / 2 + 1), fx(x), fy(y), fz(z),
+        line(std::max(std::max(x, y), z)), lb(std::max(std::max(x, y), z)) {}
+
+  size_t spec_size() const { return (size_t)nxh * ny * nz; }
+
+  void r2c(const T *in, C *S) {
+    const int nl = ny * nz;
+    for (int l = 0; l < nl; l += 2) {
+      const T *a = in + (size_t)l * nx;
+      const T *b = (l + 1 < nl) ? in + (size_t)(l + 1) * nx : nullptr;
+      for (int i = 0; i < nx; ++i) line[i] = C(a[i], b ? b[i] : (T)0);
+      fx.run(line.data(), false);
+      C *A = S + (size_t)l * nxh;
+      C *B = (l + 1 < nl) ? S + (size_t)(l + 1) * nxh : nullptr;
+      for (int k = 0; k < nxh; ++k) {
+        C zk = line[k];
+        C zc = std::conj(line[(nx - k) % nx]);
+        A[k] = (zk + zc) * (T)0.5;
+        if (B) {
+          C d = (zk - zc) * (T)0.5;        // = i * B_k
+          B[k] = C(d.imag(), -d.real());   // B_k = d / i
+        }
+      }
+    }
+    axis_pass(S, false);
+  }
+
+  void axis_pass(C *S, bool inv) {
+    // y axis
+    for (int z = 0; z < nz; ++z)
+      for (int k = 0; k < nxh; ++k) {
+        C *base = S + (size_t)z * ny * nxh + k;
+        for (int y = 0; y < ny; ++y) line[y] = base[(size_t)y * nxh];
+        fy.run(line.data(), inv);
+        for (int y = 0; y < ny; ++y) base[(size_t)y * nxh] = line[y];
+      }
+    // z axis
+    const size_t pz = (size_t)ny * nxh;
+    for (int y = 0; y < ny; ++y)
+      for (int k = 0; k < nxh; ++k) {
+        C *base = S + (size_t)y * nxh + k;
+        for (int z = 0; z < nz; ++z) line[z] = base[z * pz];
+        fz.run(line.data(), inv);
+        for (int z = 0; z < nz; ++z) base[z * pz] = line[z];
+      }
+  }
+
+  /* Unnormalised inverse; the DC and Nyquist bins of every x-line are taken as
+   * real (the imaginary part FFTW's c2r ignores). Destroys S. */
+  void c2r(C *S, T *out) {
+    axis_pass(S, true);
+    const int nl = ny * nz;
+    for (int l = 0; l < nl; l += 2) {
+      C *A = S + (size_t)l * nxh;
+      C *B = (l + 1 < nl) ? S + (size_t)(l + 1) * nxh : nullptr;
+      for (int k = 0; k < nx; ++k) {
+        C ak, bk;
+        if (k < nxh) {
+          ak = A[k];
+          bk = B ? B[k] : C(0, 0);
+        } else {
+          ak = std::conj(A[nx - k]);
+          bk = B ? std::conj(B[nx - k]) : C(0, 0);
+        }
+        if (k == 0 || 2 * k == nx) {
+          ak = C(ak.real(), 0);
+          bk = C(bk.real(), 0);
+        }
+        line[k] = ak + C(-bk.imag(), bk.real());  // ak + i*bk
+      }
+      fx.run(line.data(), true);
+      T *a = out + (size_t)l * nx;
+      for (int i = 0; i < nx; ++i) a[i] = line[i].real();
+      if (B) {
+        T *b = out + (size_t)(l + 1) * nx;
+        for (int i = 0; i < nx; ++i) b[i] = line[i].imag();
+      }
+    }
+  }
+};
+
+/* ====================================================================== */
+/* Navier operator                                                         */
+/* ====================================================================== */
+
+/* DiffOperT::InitializeOperatorLookupTable (DiffOper.cxx:587-639): computed in
+ * double, stored as T. */
+template <class T>
+void make_lut(int n, double sp, std::vector<T> &cw, std::vector<T> &sw) {
+  cw.resize(n);
+  sw.resize(n);
+  const double s = 2.0 * M_PI / n;
+  for (int k = 0; k < n; ++k) {
+    cw[k] = (T)((2.0 * std::cos(s * (double)k) - 2.0) / (sp * sp));
+    sw[k] = (T)(std::sin(s * (double)k) / sp);
+  }
+}
+
+/* Jacobi eigen-decomposition of a symmetric 3x3 (double).  Stands in for
+ * Matrix3D::factorEVsym / LAPACK dsyev (DataTypes/Matrix3D.txx:273-303): any
+ * orthonormal eigenbasis gives the same L^p = Q diag(e^p) Q^T. */
+void jacobi3(double A[3][3], double e[3], double Q[3][3]) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) Q[i][j] = (i == j);
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+    double diag = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+    if (off <= 1e-34 * diag || off == 0.0) break;
+    for (int p = 0; p < 2; ++p)
+      for (int q = p + 1; q < 3; ++q) {
+        if (A[p][q] == 0.0) continue;
+        double theta = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < 3; ++k) {  // A = J^T A J
+          double akp = A[k][p], akq = A[k][q];
+          A[k][p] = c * akp - s * akq;
+          A[k][q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < 3; ++k) {
+          double apk = A[p][k], aqk = A[q][k];
+          A[p][k] = c * apk - s * aqk;
+          A[q][k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < 3; ++k) {
+          double qkp = Q[k][p], qkq = Q[k][q];
+          Q[k][p] = c * qkp - s * qkq;
+          Q[k][q] = s * qkp + c * qkq;
+        }
+      }
+  }
+  for (int i = 0; i < 3; ++i) e[i] = A[i][i];
+}
+
+template <class T>
+int diffoper(const int *size, const double *sp, double alpha, double beta, double gamma,
+             double lpow, int flags, T *vf, bool inverse) {
+  typedef std::complex<T> C;
+  const int nx = size[0], ny = size[1], nz = size[2];
+  if (nx <= 0 || ny <= 0 || nz <= 0) return -1;
+  const bool eigen = (flags & ORC_DO_EIGEN) || lpow != 1.0;
+  const bool divfree = flags & ORC_DO_DIVFREE;
+  const size_t N = (size_t)nx * ny * nz;
+  Fft3<T> F(nx, ny, nz);
+  const int nxh = F.nxh;
+  std::vector<T> cwx, swx, cwy, swy, cwz, swz;
+  make_lut<T>(nx, sp[0], cwx, swx);
+  make_lut<T>(ny, sp[1], cwy, swy);
+  make_lut<T>(nz, sp[2], cwz, swz);
+  if (g_mut == 6) std::swap(cwy, swy);  // self-test: swapped LUT tables
+
+  // scale(1/N) then forward transform of each component
+  // (DiffOper.cxx:834-836: mFFTWVectorField->scale(1.0/N); ExecuteForward)
+  std::vector<T> plane(N);
+  std::vector<C> S[3];
+  const T scale = (T)(1.0 / (double)N);
+  for (int c = 0; c < 3; ++c) {
+    for (size_t i = 0; i < N; ++i) plane[i] = (T)(vf[3 * i + c] * scale);
+    S[c].resize(F.spec_size());
+    F.r2c(plane.data(), S[c].data());
+  }
+
+  const T a = (T)alpha, b = (T)beta, g = (T)gamma;
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nxh; ++x) {
+        const size_t idx = ((size_t)z * ny + y) * nxh + x;
+        C bx = S[0][idx], by = S[1][idx], bz = S[2][idx];
+        // DiffOper.cxx:861-879 (lambda held in double as the reference does)
+        double lambda = -(double)a * ((double)cwx[x] + (double)cwy[y] + (double)cwz[z]) + (double)g;
+        T L00 = (T)(lambda - (double)b * cwx[x]);
+        T L11 = (T)(lambda - (double)b * cwy[y]);
+        T L22 = (T)(lambda - (double)b * cwz[z]);
+        T L10 = b * swx[x] * swy[y];
+        T L20 = b * swx[x] * swz[z];
+        T L21 = b * swy[y] * swz[z];
+        if (g_mut == 1) { L10 = L20 = L21 = 0; }           // dropped grad-div coupling
+        if (g_mut == 2) { L10 = -L10; }                      // wrong sign
+        if (g_mut == 3) { std::swap(L20, L21); }             // transposed operand
+        C vx, vy, vz;
+        if (eigen) {
+          // InitializeEigenLookup + PowL (DiffOper.cxx:641-745, 1176-1218)
+          double A[3][3] = {{(double)L00, (double)L10, (double)L20},
+                            {(double)L10, (double)L11, (double)L21},
+                            {(double)L20, (double)L21, (double)L22}};
+          double e[3], Q[3][3];
+          jacobi3(A, e, Q);
+          double p = inverse ? -lpow : lpow;
+          for (int i = 0; i < 3; ++i) e[i] = std::pow(e[i], p);
+          double M[3][3];
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+              M[i][j] = Q[i][0] * Q[j][0] * e[0] + Q[i][1] * Q[j][1] * e[1] + Q[i][2] * Q[j][2] * e[2];
+          T M00 = (T)M[0][0], M10 = (T)M[1][0], M20 = (T)M[2][0];
+          T M11 = (T)M[1][1], M21 = (T)M[2][1], M22 = (T)M[2][2];
+          vx = M00 * bx + M10 * by + M20 * bz;
+          vy = M10 * bx + M11 * by + M21 * bz;
+          vz = M20 * bx + M21 * by + M22 * bz;
+        } else if (inverse) {
+          // InverseOperatorMultiply: Cholesky GG^T = L, two triangular solves
+          // on real and imaginary parts (DiffOper.cxx:1002-1083)
+          T G00 = std::sqrt(L00);
+          T G10 = L10 / G00;
+          T G20 = L20 / G00;
+          T G11 = L11 - G10 * G10;
+          T G21 = L21 - G20 * G10;
+          G11 = std::sqrt(G11);
+          G21 = G21 / G11;
+          T G22 = L22 - (G20 * G20 + G21 * G21);
+          G22 = std::sqrt(G22);
+          T br[3] = {bx.real(), by.real(), bz.real()};
+          T bi[3] = {bx.imag(), by.imag(), bz.imag()};
+          T xr[3], xi[3];
+          for (int pass = 0; pass < 2; ++pass) {
+            T *bb = pass ? bi : br;
+            T *xx = pass ? xi : xr;
+            T y0 = bb[0] / G00;
+            T y1 = (bb[1] - G10 * y0) / G11;
+            T y2 = (bb[2] - G20 * y0 - G21 * y1) / G22;
+            xx[2] = y2 / G22;
+            xx[1] = (y1 - G21 * xx[2]) / G11;
+            xx[0] = (y0 - G10 * xx[1] - G20 * xx[2]) / G00;
+          }
+          vx = C(xr[0], xi[0]);
+          vy = C(xr[1], xi[1]);
+          vz = C(xr[2], xi[2]);
+        } else {
+          // OperatorMultiply (DiffOper.cxx:1085-1116)
+          vx = L00 * bx + L10 * by + L20 * bz;
+          vy = L10 * bx + L11 * by + L21 * bz;
+          vz = L20 * bx + L21 * by + L22 * bz;
+        }
+        if (divfree && (x | y | z)) {
+          // ProjectIncomp (DiffOper.cxx:948-1000)
+          double nsq = (double)swx[x] * swx[x] + (double)swy[y] * swy[y] + (double)swz[z] * swz[z];
+          double re = vx.real() * (double)swx[x] + vy.real() * (double)swy[y] + vz.real() * (double)swz[z];
+          double im = vx.imag() * (double)swx[x] + vy.imag() * (double)swy[y] + vz.imag() * (double)swz[z];
+          if (nsq > 0) {
+            vx = C((T)(vx.real() - re * swx[x] / nsq), (T)(vx.imag() - im * swx[x] / nsq));
+            vy = C((T)(vy.real() - re * swy[y] / nsq), (T)(vy.imag() - im * swy[y] / nsq));
+            vz = C((T)(vz.real() - re * swz[z] / nsq), (T)(vz.imag() - im * swz[z] / nsq));
+          }
+        }
+        S[0][idx] = vx;
+        S[1][idx] = vy;
+        S[2][idx] = vz;
+      }
+
+  for (int c = 0; c < 3; ++c) {
+    F.c2r(S[c].data(), plane.data());
+    if (g_mut == 7) for (size_t i = 0; i < N; ++i) plane[i] *= (T)N;  // self-test: double scaling
+    for (size_t i = 0; i < N; ++i) vf[3 * i + c] = plane[i];
+  }
+  return 0;
+}
+
+/* ====================================================================== */
+/* Interpolation                                                           */
+/* ====================================================================== */
+
+inline int safe_mod(int r, int b) {  // Numerics/Arithmetic.h:22-33
+  int m = r % b;
+  return m < 0 ? m + b : m;
+}
+
+/* floor with the reference's negative fix (HField3DUtils.txx:1977-1983) */
+inline int ref_floor(double x) {
+  int f = (int)x;
+  if (x < 0 && x != (double)(int)x) --f;
+  return f;
+}
+
+/* HField3DUtils::trilerp (HField3DUtils.txx:1969-2300).  h is AoS float. */
+template <class TF>
+void trilerp_vec(const TF *h, const int *size, double x, double y, double z, double out[3],
+                 int bg) {
+  int fX = ref_floor(x), fY = ref_floor(y), fZ = ref_floor(z);
+  int cX = fX + 1, cY = fY + 1, cZ = fZ + 1;
+  const double t = x - fX, u = y - fY, v = z - fZ;
+  const double omt = 1.0 - t, omu = 1.0 - u, omv = 1.0 - v;
+  const int sX = size[0], sY = size[1], sZ = size[2];
+  double c[8][3];
+  for (int i = 0; i < 8; ++i) c[i][0] = c[i][1] = c[i][2] = 0;
+  bool inside = true;
+  if (bg == ORC_BG_WRAP) {
+    fX = safe_mod(fX, sX); fY = safe_mod(fY, sY); fZ = safe_mod(fZ, sZ);
+    cX = safe_mod(cX, sX); cY = safe_mod(cY, sY); cZ = safe_mod(cZ, sZ);
+  } else if (bg == ORC_BG_CLAMP) {
+    if (fX < 0) { fX = 0; if (cX < 0) cX = 0; }
+    if (cX >= sX) { cX = sX - 1; if (fX >= sX) fX = sX - 1; }
+    if (fY < 0) { fY = 0; if (cY < 0) cY = 0; }
+    if (cY >= sY) { cY = sY - 1; if (fY >= sY) fY = sY - 1; }
+    if (fZ < 0) { fZ = 0; if (cZ < 0) cZ = 0; }
+    if (cZ >= sZ) { cZ = sZ - 1; if (fZ >= sZ) fZ = sZ - 1; }
+  } else {
+    inside = (fX >= 0 && cX < sX && fY >= 0 && cY < sY && fZ >= 0 && cZ < sZ);
+  }
+  // corner order v0..v7 (HField3DUtils.txx:2014-2020)
+  const int cx[8] = {fX, cX, cX, fX, fX, cX, cX, fX};
+  const int cy[8] = {fY, fY, cY, cY, cY, cY, fY, fY};
+  const int cz[8] = {fZ, fZ, fZ, fZ, cZ, cZ, cZ, cZ};
+  auto at = [&](int i, int j, int k, int comp) -> double {
+    return (double)h[3 * (((size_t)k * sY + j) * sX + i) + comp];
+  };
+  if (inside) {
+    for (int i = 0; i < 8; ++i)
+      for (int q = 0; q < 3; ++q) c[i][q] = at(cx[i], cy[i], cz[i], q);
+  } else if (bg == ORC_BG_ID) {
+    out[0] = x; out[1] = y; out[2] = z;
+    return;
+  } else if (bg == ORC_BG_ZERO) {
+    out[0] = out[1] = out[2] = 0;
+    return;
+  } else {  // PARTIAL_ID / PARTIAL_ZERO
+    for (int i = 0; i < 8; ++i) {
+      bool in = cx[i] >= 0 && cx[i] < sX && cy[i] >= 0 && cy[i] < sY && cz[i] >= 0 && cz[i] < sZ;
+      if (in) {
+        for (int q = 0; q < 3; ++q) c[i][q] = at(cx[i], cy[i], cz[i], q);
+      } else if (bg == ORC_BG_PARTIAL_ID) {
+        c[i][0] = cx[i]; c[i][1] = cy[i]; c[i][2] = cz[i];
+      }
+    }
+  }
+  if (g_mut == 4) {  // self-test: swapped corners v3/v4
+    for (int q = 0; q < 3; ++q) std::swap(c[3][q], c[4][q]);
+  }
+  for (int q = 0; q < 3; ++q)
+    out[q] = omt * (omu * (c[0][q] * omv + c[7][q] * v) + u * (c[3][q] * omv + c[4][q] * v)) +
+             t * (omu * (c[1][q] * omv + c[6][q] * v) + u * (c[2][q] * omv + c[5][q] * v));
+}
+
+/* Array3DUtils::trilerp<T,bg> (Array3DUtils.txx:59-232).  Scalar strategies:
+ * VAL (background value), CLAMP, WRAP; ZERO maps to VAL with 0.  The scalar
+ * CLAMP in the reference tests floor > size (it can read one past the end);
+ * this restatement clamps with >= as the vector version does. */
+template <class TF>
+double trilerp_scalar(const TF *a, const int *size, double x, double y, double z, int bg,
+                      double background) {
+  int fX = ref_floor(x), fY = ref_floor(y), fZ = ref_floor(z);
+  int cX = fX + 1, cY = fY + 1, cZ = fZ + 1;
+  const double t = x - fX, u = y - fY, v = z - fZ;
+  const double omt = 1.0 - t, omu = 1.0 - u, omv = 1.0 - v;
+  const int sX = size[0], sY = size[1], sZ = size[2];
+  bool inside = true;
+  if (bg == ORC_BG_WRAP) {
+    fX = safe_mod(fX, sX); fY = safe_mod(fY, sY); fZ = safe_mod(fZ, sZ);
+    cX = safe_mod(cX, sX); cY = safe_mod(cY, sY); cZ = safe_mod(cZ, sZ);
+  } else if (bg == ORC_BG_CLAMP) {
+    if (fX < 0) { fX = 0; if (cX < 0) cX = 0; }
+    if (cX >= sX) { cX = sX - 1; if (fX >= sX) fX = sX - 1; }
+    if (fY < 0) { fY = 0; if (cY < 0) cY = 0; }
+    if (cY >= sY) { cY = sY - 1; if (fY >= sY) fY = sY - 1; }
+    if (fZ < 0) { fZ = 0; if (cZ < 0) cZ = 0; }
+    if (cZ >= sZ) { cZ = sZ - 1; if (fZ >= sZ) fZ = sZ - 1; }
+  } else {
+    if (bg == ORC_BG_ZERO) background = 0;
+    inside = (fX >= 0 && cX < sX && fY >= 0 && cY < sY && fZ >= 0 && cZ < sZ);
+  }
+  const int cx[8] = {fX, cX, cX, fX, fX, cX, cX, fX};
+  const int cy[8] = {fY, fY, cY, cY, cY, cY, fY, fY};
+  const int cz[8] = {fZ, fZ, fZ, fZ, cZ, cZ, cZ, cZ};
+  double c[8];
+  for (int i = 0; i < 8; ++i) {
+    bool in = inside || (cx[i] >= 0 && cx[i] < sX && cy[i] >= 0 && cy[i] < sY && cz[i] >= 0 &&
+                         cz[i] < sZ);
+    c[i] = in ? (double)a[((size_t)cz[i] * sY + cy[i]) * sX + cx[i]] : background;
+  }
+  if (g_mut == 4) std::swap(c[3], c[4]);
+  return omt * (omu * (c[0] * omv + c[7] * v) + u * (c[3] * omv + c[4] * v)) +
+         t * (omu * (c[1] * omv + c[6] * v) + u * (c[2] * omv + c[5] * v));
+}
+
+template <class TO>
+void apply_t(const float *img, const float *h, const int *size, TO *out, int bg,
+             double background) {
+  const size_t N = (size_t)size[0] * size[1] * size[2];
+  for (size_t i = 0; i < N; ++i)
+    out[i] = (TO)trilerp_scalar(img, size, (double)h[3 * i], (double)h[3 * i + 1],
+                                (double)h[3 * i + 2], bg, background);
+}
+
+/* Array3DUtils::computeGradient (Array3DUtils.txx:783-901) -> AoS */
+template <class TO>
+void gradient_t(const float *a, const int *size, const double *sp, int wrap, TO *g) {
+  const int sx = size[0], sy = size[1], sz = size[2];
+  auto A = [&](int x, int y, int z) -> double { return (double)a[((size_t)z * sy + y) * sx + x]; };
+  auto G = [&](int x, int y, int z, int c) -> TO & { return g[3 * (((size_t)z * sy + y) * sx + x) + c]; };
+  const int nd[3] = {sx, sy, sz};
+  for (int z = 0; z < sz; ++z)
+    for (int y = 0; y < sy; ++y)
+      for (int x = 0; x < sx; ++x) {
+        const int p[3] = {x, y, z};
+        for (int c = 0; c < 3; ++c) {
+          const int n = nd[c];
+          int pm[3] = {x, y, z}, pp[3] = {x, y, z};
+          double val;
+          if (n == 1) {
+            val = 0;  // degenerate axis: the reference reads out of range here
+          } else if (p[c] > 0 && p[c] < n - 1) {
+            pp[c] += 1; pm[c] -= 1;
+            val = (A(pp[0], pp[1], pp[2]) - A(pm[0], pm[1], pm[2])) / (2.0 * sp[c]);
+          } else if (wrap) {
+            pp[c] = safe_mod(p[c] + 1, n); pm[c] = safe_mod(p[c] - 1, n);
+            val = (A(pp[0], pp[1], pp[2]) - A(pm[0], pm[1], pm[2])) / (2.0 * sp[c]);
+          } else if (p[c] == 0) {
+            pp[c] = 1;
+            val = (A(pp[0], pp[1], pp[2]) - A(x, y, z)) / (g_mut == 5 ? 2.0 * sp[c] : sp[c]);
+          } else {
+            pm[c] = n - 2;
+            val = (A(x, y, z) - A(pm[0], pm[1], pm[2])) / (g_mut == 5 ? 2.0 * sp[c] : sp[c]);
+          }
+          G(x, y, z, c) = (TO)val;
+        }
+      }
+}
+
+/* ====================================================================== */
+/* Greedy loop                                                             */
+/* ====================================================================== */
+
+/* One GreedyIteratorCPU::Iterate (GreedyIteratorCPU.cxx:97-138) in precision T.
+ * h / hinv: AoS T, I0/I1 float.  vf, scratchI, scratchV: caller scratch. */
+template <class T>
+struct GreedyState {
+  int size[3];
+  double sp[3];
+  size_t N;
+  std::vector<T> Idef, vf, hnew;
+  std::vector<float> hf;  // float view of h for trilerp input when T=double
+  GreedyState(const int *s, const double *p) {
+    for (int i = 0; i < 3; ++i) { size[i] = s[i]; sp[i] = p[i]; }
+    N = (size_t)s[0] * s[1] * s[2];
+    Idef.resize(N);
+    vf.resize(3 * N);
+    hnew.resize(3 * N);
+  }
+};
+
+template <class T>
+void trilerp_vec_T(const T *h, const int *size, double x, double y, double z, double out[3], int bg) {
+  trilerp_vec<T>(h, size, x, y, z, out, bg);
+}
+
+template <class T>
+double greedy_iterate(GreedyState<T> &st, const orc_greedy_params *p, const float *I0,
+                      const float *I1, T *h, T *hinv, bool update_step, double &step,
+                      double *energy_float_sum) {
+  const int *size = st.size;
+  const size_t N = st.N;
+  const int sx = size[0], sy = size[1];
+  // 1. deformed image (HField3DUtils::apply, VAL background 0)
+  for (size_t i = 0; i < N; ++i)
+    st.Idef[i] = (T)trilerp_scalar(I0, size, (double)h[3 * i], (double)h[3 * i + 1],
+                                   (double)h[3 * i + 2], ORC_BG_VAL, 0.0);
+  // 2. gradient into the operator's field
+  {
+    std::vector<float> idf;
+    const float *src;
+    if (sizeof(T) == sizeof(float)) {
+      src = reinterpret_cast<const float *>(st.Idef.data());
+    } else {
+      // FP64 oracle: gradient of the double image
+      std::vector<T> &I = st.Idef;
+      const int nd[3] = {size[0], size[1], size[2]};
+      for (int z = 0; z < size[2]; ++z)
+        for (int y = 0; y < sy; ++y)
+          for (int x = 0; x < sx; ++x) {
+            const int pp[3] = {x, y, z};
+            for (int c = 0; c < 3; ++c) {
+              const int n = nd[c];
+              int a[3] = {x, y, z}, b[3] = {x, y, z};
+              double den;
+              if (n == 1) { st.vf[3 * (((size_t)z * sy + y) * sx + x) + c] = 0; continue; }
+              if (pp[c] > 0 && pp[c] < n - 1) { a[c] += 1; b[c] -= 1; den = 2.0 * p->sp[c]; }
+              else if (pp[c] == 0) { a[c] = 1; den = p->sp[c]; }
+              else { b[c] = n - 2; den = p->sp[c]; }
+              double va = I[((size_t)a[2] * sy + a[1]) * sx + a[0]];
+              double vb = I[((size_t)b[2] * sy + b[1]) * sx + b[0]];
+              st.vf[3 * (((size_t)z * sy + y) * sx + x) + c] = (T)((va - vb) / den);
+            }
+          }
+      src = nullptr;
+    }
+    if (src) gradient_t<T>(src, size, p->sp, 0, st.vf.data());
+  }
+  // 3-4. residual r = -(Idef - I1); force = grad * r
+  //      (GreedyIteratorCPU.cxx:112-117, DiffOper.cxx:512-525)
+  T fsum = 0;
+  double dsum = 0;
+  for (size_t i = 0; i < N; ++i) {
+    T r = (T)((T)(st.Idef[i] - (T)I1[i]) * (T)-1.0);
+    st.Idef[i] = r;
+    st.vf[3 * i] *= r;
+    st.vf[3 * i + 1] *= r;
+    st.vf[3 * i + 2] *= r;
+    fsum += r * r;  // Array3DUtils.txx:1624-1638 (accumulated in T)
+    dsum += (double)r * (double)r;
+  }
+  const double spprod = p->sp[0] * p->sp[1] * p->sp[2];
+  if (energy_float_sum) *energy_float_sum = spprod * (double)fsum;  // ImageUtils.txx:55-58
+  // 5. v = L^-1 f
+  diffoper<T>(size, p->sp, p->alpha, p->beta, p->gamma, 1.0, 0, st.vf.data(), true);
+  // 6. step size at the first iteration of a scale level (GreedyIteratorCPU.cxx:65-79,129-134)
+  if (update_step) {
+    T mx = 0;
+    for (size_t i = 0; i < N; ++i) {
+      T l = std::sqrt(st.vf[3 * i] * st.vf[3 * i] + st.vf[3 * i + 1] * st.vf[3 * i + 1] +
+                      st.vf[3 * i + 2] * st.vf[3 * i + 2]);
+      if (l > mx) mx = l;
+    }
+    step = (double)((T)p->max_pert / mx);
+  }
+  // 7. updateDeformation (GreedyIteratorCPU.cxx:81-95)
+  const T stepT = (T)step;
+  for (size_t i = 0; i < 3 * N; ++i) st.vf[i] = (T)(st.vf[i] * stepT);
+  const T spx = (T)p->sp[0], spy = (T)p->sp[1], spz = (T)p->sp[2];
+  size_t i = 0;
+  for (int z = 0; z < size[2]; ++z)
+    for (int y = 0; y < sy; ++y)
+      for (int x = 0; x < sx; ++x, ++i) {
+        double o[3];
+        T qx = (T)x + st.vf[3 * i] / spx;
+        T qy = (T)y + st.vf[3 * i + 1] / spy;
+        T qz = (T)z + st.vf[3 * i + 2] / spz;
+        if (g_mut == 8) { qx = (T)x - st.vf[3 * i] / spx; }  // self-test: wrong direction
+        trilerp_vec<T>(h, size, (double)qx, (double)qy, (double)qz, o, ORC_BG_PARTIAL_ID);
+        st.hnew[3 * i] = (T)o[0];
+        st.hnew[3 * i + 1] = (T)o[1];
+        st.hnew[3 * i + 2] = (T)o[2];
+      }
+  std::memcpy(h, st.hnew.data(), sizeof(T) * 3 * N);
+  if (p->compute_inverse && hinv) {
+    // composeVHInv(v, hinv, PARTIAL_ZERO): hinv(x) - v(hinv(x))/sp
+    for (size_t j = 0; j < N; ++j) {
+      double o[3];
+      trilerp_vec<T>(st.vf.data(), size, (double)hinv[3 * j], (double)hinv[3 * j + 1],
+                     (double)hinv[3 * j + 2], o, ORC_BG_PARTIAL_ZERO);
+      st.hnew[3 * j] = (T)((T)o[0] / -spx + hinv[3 * j]);
+      st.hnew[3 * j + 1] = (T)((T)o[1] / -spy + hinv[3 * j + 1]);
+      st.hnew[3 * j + 2] = (T)((T)o[2] / -spz + hinv[3 * j + 2]);
+    }
+    std::memcpy(hinv, st.hnew.data(), sizeof(T) * 3 * N);
+  }
+  return spprod * dsum;
+}
+
+template <class T>
+int greedy_run(const orc_greedy_params *p, const float *I0, const float *I1, T *h, T *hinv,
+               double *energy_f, double *energy_d, double *step_out) {
+  GreedyState<T> st(p->size, p->sp);
+  double step = 0;
+  double prev = 0;
+  for (int it = 0; it < p->n_iter; ++it) {
+    double ef = 0;
+    double ed = greedy_iterate<T>(st, p, I0, I1, h, hinv, it == 0, step, &ef);
+    if (energy_f) energy_f[it] = ef;
+    if (energy_d) energy_d[it] = ed;
+    // GreedyDeformationData::AddEnergy (GreedyDeformationData.cxx:162-184): the
+    // halved step takes effect from the next iteration.
+    double e = (sizeof(T) == sizeof(float)) ? ef : ed;
+    if (it > 0 && p->auto_step_reduce && e - prev > 0) step = step / 2.0;
+    prev = e;
+  }
+  if (step_out) *step_out = step;
+  return 0;
+}
+
+}  // namespace
+
+/* ====================================================================== */
+/* C entry points                                                          */
+/* ====================================================================== */
+
+extern "C" {
+
+int orc_fft_d(int n, double *data, int inverse) {
+  if (n <= 0) return -1;
+  Fft<double> f(n);
+  f.run(reinterpret_cast<std::complex<double> *>(data), inverse != 0);
+  return 0;
+}
+
+int orc_fft_f(int n, float *data, int inverse) {
+  if (n <= 0) return -1;
+  Fft<float> f(n);
+  f.run(reinterpret_cast<std::complex<float> *>(data), inverse != 0);
+  return 0;
+}
+
+void orc_lut(int n, double sp, double *cosW, double *sinW) {
+  std::vector<double> c, s;
+  make_lut<double>(n, sp, c, s);
+  std::copy(c.begin(), c.end(), cosW);
+  std::copy(s.begin(), s.end(), sinW);
+}
+
+int orc_diffoper_d(const int *size, const double *sp, double alpha, double beta, double gamma,
+                   double lpow, int flags, double *vf, int inverse) {
+  return diffoper<double>(size, sp, alpha, beta, gamma, lpow, flags, vf, inverse != 0);
+}
+
+int orc_diffoper_f(const int *size, const double *sp, double alpha, double beta, double gamma,
+                   double lpow, int flags, float *vf, int inverse) {
+  return diffoper<float>(size, sp, alpha, beta, gamma, lpow, flags, vf, inverse != 0);
+}
+
+void orc_trilerp_vec_d(const float *h, const int *size, double x, double y, double z,
+                       double *out3, int bg) {
+  trilerp_vec<float>(h, size, x, y, z, out3, bg);
+}
+
+double orc_trilerp_scalar_d(const float *img, const int *size, double x, double y, double z,
+                            int bg, double background) {
+  return trilerp_scalar<float>(img, size, x, y, z, bg, background);
+}
+
+void orc_apply_d(const float *img, const float *h, const int *size, double *out, int bg,
+                 double background) {
+  apply_t<double>(img, h, size, out, bg, background);
+}
+
+void orc_apply_f(const float *img, const float *h, const int *size, float *out, int bg,
+                 double background) {
+  apply_t<float>(img, h, size, out, bg, background);
+}
+
+void orc_compose_d(const float *f, const float *g, const int *size, double *h, int bg) {
+  const size_t N = (size_t)size[0] * size[1] * size[2];
+  for (size_t i = 0; i < N; ++i)
+    trilerp_vec<float>(f, size, g[3 * i], g[3 * i + 1], g[3 * i + 2], h + 3 * i, bg);
+}
+
+void orc_compose_hv_d(const float *g, const float *v, const int *size, const double *sp,
+                      double *h, int bg, int inv) {
+  const int sx = size[0], sy = size[1], sz = size[2];
+  const double s = inv ? -1.0 : 1.0;
+  size_t i = 0;
+  for (int z = 0; z < sz; ++z)
+    for (int y = 0; y < sy; ++y)
+      for (int x = 0; x < sx; ++x, ++i) {
+        double qx = x + s * (double)v[3 * i] / sp[0];
+        double qy = y + s * (double)v[3 * i + 1] / sp[1];
+        double qz = z + s * (double)v[3 * i + 2] / sp[2];
+        if (g_mut == 8) qx = x - s * (double)v[3 * i] / sp[0];
+        trilerp_vec<float>(g, size, qx, qy, qz, h + 3 * i, bg);
+      }
+}
+
+void orc_compose_vh_d(const float *v, const float *g, const int *size, const double *sp,
+                      double *h, int bg, int inv) {
+  const size_t N = (size_t)size[0] * size[1] * size[2];
+  const double s = inv ? -1.0 : 1.0;
+  for (size_t i = 0; i < N; ++i) {
+    double o[3];
+    trilerp_vec<float>(v, size, g[3 * i], g[3 * i + 1], g[3 * i + 2], o, bg);
+    for (int c = 0; c < 3; ++c) h[3 * i + c] = (double)g[3 * i + c] + s * o[c] / sp[c];
+  }
+}
+
+void orc_forward_apply_f(const float *img, const float *h, const int *size, float *out,
+                         double background, int normalize) {
+  // HField3DUtils::forwardApply (HField3DUtils.txx:1281-1413), float accumulation
+  const int sx = size[0], sy = size[1], sz = size[2];
+  const size_t N = (size_t)sx * sy * sz;
+  std::vector<float> count(N, 0.0f);
+  for (size_t i = 0; i < N; ++i) out[i] = 0.0f;
+  auto O = [&](int x, int y, int z) -> size_t { return ((size_t)z * sy + y) * sx + x; };
+  for (size_t i = 0; i < N; ++i) {
+    float ix = img[i];
+    float hx = h[3 * i], hy = h[3 * i + 1], hz = h[3 * i + 2];
+    int fx = (int)hx, fy = (int)hy, fz = (int)hz;
+    if (hx < 0 && hx != (float)(int)hx) --fx;
+    if (hy < 0 && hy != (float)(int)hy) --fy;
+    if (hz < 0 && hz != (float)(int)hz) --fz;
+    if (hx > -1 && hx < sx && hy > -1 && hy < sy && hz > -1 && hz < sz) {
+      float rx = hx - fx, ry = hy - fy, rz = hz - fz;
+      if (g_mut == 9) rx = 1.0f - rx;  // self-test: swapped x weights
+      float w000 = (float)((1.0 - rx) * (1.0 - ry) * (1.0 - rz));
+      float w001 = (float)((1.0 - rx) * (1.0 - ry) * (rz));
+      float w010 = (float)((1.0 - rx) * (ry) * (1.0 - rz));
+      float w011 = (float)((1.0 - rx) * (ry) * (rz));
+      float w100 = (float)((rx) * (1.0 - ry) * (1.0 - rz));
+      float w101 = (float)((rx) * (1.0 - ry) * (rz));
+      float w110 = (float)((rx) * (ry) * (1.0 - rz));
+      float w111 = (float)((rx) * (ry) * (rz));
+      bool fxi = fx >= 0, cxi = fx < sx - 1, fyi = fy >= 0, cyi = fy < sy - 1, fzi = fz >= 0,
+           czi = fz < sz - 1;
+      if (fxi && fyi && fzi) { out[O(fx, fy, fz)] += w000 * ix; count[O(fx, fy, fz)] += w000; }
+      if (fxi && fyi && czi) { out[O(fx, fy, fz + 1)] += w001 * ix; count[O(fx, fy, fz + 1)] += w001; }
+      if (fxi && cyi && fzi) { out[O(fx, fy + 1, fz)] += w010 * ix; count[O(fx, fy + 1, fz)] += w010; }
+      if (fxi && cyi && czi) { out[O(fx, fy + 1, fz + 1)] += w011 * ix; count[O(fx, fy + 1, fz + 1)] += w011; }
+      if (cxi && fyi && fzi) { out[O(fx + 1, fy, fz)] += w100 * ix; count[O(fx + 1, fy, fz)] += w100; }
+      if (cxi && fyi && czi) { out[O(fx + 1, fy, fz + 1)] += w101 * ix; count[O(fx + 1, fy, fz + 1)] += w101; }
+      if (cxi && cyi && fzi) { out[O(fx + 1, fy + 1, fz)] += w110 * ix; count[O(fx + 1, fy + 1, fz)] += w110; }
+      if (cxi && cyi && czi) { out[O(fx + 1, fy + 1, fz + 1)] += w111 * ix; count[O(fx + 1, fy + 1, fz + 1)] += w111; }
+    }
+  }
+  if (normalize)
+    for (size_t e = 0; e < N; ++e) out[e] = count[e] > 0 ? out[e] / count[e] : (float)background;
+}
+
+void orc_splat_d(const float *src, const float *px, const float *py, const float *pz,
+                 const int *size, double *out) {
+  // cpuSplat3D (cudaSplatAtomic.cu:602-663) with ADJUST_INT (cudaSplatAtomic.cu:27)
+  const int w = size[0], hh = size[1], l = size[2];
+  const size_t N = (size_t)w * hh * l;
+  for (size_t i = 0; i < N; ++i) out[i] = 0.0;
+  auto inside = [&](int x, int y, int z) { return x >= 0 && x < w && y >= 0 && y < hh && z >= 0 && z < l; };
+  for (size_t i = 0; i < N; ++i) {
+    float x = px[i], y = py[i], z = pz[i];
+    int xi = (int)x, yi = (int)y, zi = (int)z;
+    if (x < 0 && x != (float)xi) --xi;
+    if (y < 0 && y != (float)yi) --yi;
+    if (z < 0 && z != (float)zi) --zi;
+    float dx = 1.f - (x - xi), dy = 1.f - (y - yi), dz = 1.f - (z - zi);
+    if (g_mut == 9) dx = 1.f - dx;
+    double m = src[i];
+    const double wx[2] = {dx, 1.0 - (double)dx}, wy[2] = {dy, 1.0 - (double)dy},
+                 wz[2] = {dz, 1.0 - (double)dz};
+    for (int c = 0; c < 8; ++c) {
+      int ox = c & 1, oy = (c >> 1) & 1, oz = c >> 2;
+      if (inside(xi + ox, yi + oy, zi + oz))
+        out[((size_t)(zi + oz) * hh + (yi + oy)) * w + (xi + ox)] += m * wx[ox] * wy[oy] * wz[oz];
+    }
+  }
+}
+
+void orc_splat_index(const float *src, const float *vx, const float *vy, const float *vz,
+                     const int *size, int floor_fix, unsigned int *keys, float *vals) {
+  // createIndex_kernel (cudaSplat.cu:31-117).  Flattened order: key[4*id + c]
+  // for the z-floor corners c = 0..3, key[4*(N + id) + c] for the z-ceil corners;
+  // corner c: (x + (c&1), y + (c>>1)).  Sentinel = N.
+  const int w = size[0], h = size[1], l = size[2];
+  const unsigned int N = (unsigned int)((size_t)w * h * l);
+  auto inside = [&](int x, int y, int z) { return x >= 0 && x < w && y >= 0 && y < h && z >= 0 && z < l; };
+  unsigned int id = 0;
+  for (int k = 0; k < l; ++k)
+    for (int j = 0; j < h; ++j)
+      for (int i = 0; i < w; ++i, ++id) {
+        float mass = src[id];
+        float x = i + vx[id], y = j + vy[id], z = k + vz[id];
+        int xi = (int)x, yi = (int)y, zi = (int)z;
+        if (floor_fix) {
+          if (x < 0 && x != (float)xi) --xi;
+          if (y < 0 && y != (float)yi) --yi;
+          if (z < 0 && z != (float)zi) --zi;
+        }
+        float dx = 1.f - (x - xi), dy = 1.f - (y - yi), dz = 1.f - (z - zi);
+        for (int half = 0; half < 2; ++half) {
+          size_t base = 4 * ((size_t)half * N + id);
+          float wz = half ? (1.f - dz) : dz;
+          for (int c = 0; c < 4; ++c) {
+            int ox = c & 1, oy = c >> 1;
+            float wx = ox ? (1.f - dx) : dx, wy = oy ? (1.f - dy) : dy;
+            if (inside(xi + ox, yi + oy, zi + half)) {
+              keys[base + c] = (unsigned int)(((size_t)(zi + half) * h + (yi + oy)) * w + (xi + ox));
+              vals[base + c] = mass * wx * wy * wz;
+            } else {
+              keys[base + c] = N;
+              vals[base + c] = 0.f;
+            }
+          }
+        }
+      }
+}
+
+void orc_gradient_d(const float *img, const int *size, const double *sp, int wrap, double *grad) {
+  gradient_t<double>(img, size, sp, wrap, grad);
+}
+
+void orc_gradient_f(const float *img, const int *size, const double *sp, int wrap, float *grad) {
+  gradient_t<float>(img, size, sp, wrap, grad);
+}
+
+int orc_greedy_run_f(const orc_greedy_params *p, const float *I0, const float *I1, float *h,
+                     float *hinv, double *energy_f, double *energy_d, double *step_out) {
+  return greedy_run<float>(p, I0, I1, h, hinv, energy_f, energy_d, step_out);
+}
+
+int orc_greedy_run_d(const orc_greedy_params *p, const float *I0, const float *I1, double *h,
+                     double *hinv, double *energy_d, double *step_out) {
+  return greedy_run<double>(p, I0, I1, h, hinv, nullptr, energy_d, step_out);
+}
+
+double orc_greedy_time_f(const int *size, int n_iter, int seed) {
+  // Synthetic blob pair, identity start; times n_iter FP32 iterations (1 thread).
+  orc_greedy_params p;
+  for (int i = 0; i < 3; ++i) { p.size[i] = size[i]; p.sp[i] = 1.0; }
+  p.alpha = p.beta = 0.01; p.gamma = 0.001; p.max_pert = 0.1;
+  p.n_iter = n_iter; p.compute_inverse = 0; p.auto_step_reduce = 0;
+  const size_t N = (size_t)size[0] * size[1] * size[2];
+  std::vector<float> I0(N), I1(N), h(3 * N);
+  std::mt19937 rng(seed);
+  std::uniform_real_distribution<float> U(0.f, 1.f);
+  float cx = size[0] * 0.5f, cy = size[1] * 0.5f, cz = size[2] * 0.5f;
+  float s0 = size[0] * 0.15f;
+  size_t i = 0;
+  for (int z = 0; z < size[2]; ++z)
+    for (int y = 0; y < size[1]; ++y)
+      for (int x = 0; x < size[0]; ++x, ++i) {
+        float d0 = (x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz);
+        float d1 = (x - cx - 2) * (x - cx - 2) + (y - cy) * (y - cy) + (z - cz + 1) * (z - cz + 1);
+        I0[i] = std::exp(-d0 / (2 * s0 * s0));
+        I1[i] = std::exp(-d1 / (2 * s0 * s0)) + 0.001f * U(rng);
+        h[3 * i] = (float)x; h[3 * i + 1] = (float)y; h[3 * i + 2] = (float)z;
+      }
+  auto t0 = std::chrono::steady_clock::now();
+  greedy_run<float>(&p, I0.data(), I1.data(), h.data(), nullptr, nullptr, nullptr, nullptr);
+  auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double>(t1 - t0).count() / n_iter;
+}
+
+int orc_fluidwarp_asym_f(const float *fixed, const float *moving, const int *size, double alpha,
+                         double beta, double gamma, double max_pert, int n_iter, float *h,
+                         float *hinv) {
+  // FluidWarp::_computeHFieldAsymmetric (FluidWarp.cxx:925-1175), no Jacobian
+  // scaling; L^-1 via the eigen table path (FluidWarp.cxx:973).
+  const int sx = size[0], sy = size[1], sz = size[2];
+  const size_t N = (size_t)sx * sy * sz;
+  const double sp[3] = {1, 1, 1};
+  std::vector<float> def(N), vf(3 * N), hinc(3 * N), old(3 * N), tmp(3 * N);
+  float delta = 0;
+  bool have_delta = false;
+  for (int it = 0; it < n_iter; ++it) {
+    apply_t<float>(moving, h, size, def.data(), ORC_BG_VAL, 0.0);
+    gradient_t<float>(def.data(), size, sp, 0, vf.data());
+    // _generateBodyForce (FluidWarp.cxx:2076-2098): di in double
+    for (size_t i = 0; i < N; ++i) {
+      double di = (double)fixed[i] - (double)def[i];
+      for (int c = 0; c < 3; ++c) vf[3 * i + c] = (float)(vf[3 * i + c] * di);
+    }
+    diffoper<float>(size, sp, alpha, beta, gamma, 1.0, ORC_DO_EIGEN, vf.data(), true);
+    if (!have_delta) {
+      // _computeDelta (FluidWarp.cxx:2126-2155)
+      double mx = 0;
+      for (size_t i = 0; i < N; ++i) {
+        double l = (double)vf[3 * i] * vf[3 * i] + (double)vf[3 * i + 1] * vf[3 * i + 1] +
+                   (double)vf[3 * i + 2] * vf[3 * i + 2];
+        if (mx < l) mx = l;
+      }
+      delta = (float)(max_pert / std::sqrt(mx));
+      have_delta = true;
+      if (delta > 100) break;
+    }
+    // _updateHField(h, hinv, v, delta) (FluidWarp.cxx:2236-2275)
+    size_t i = 0;
+    for (int z = 0; z < sz; ++z)
+      for (int y = 0; y < sy; ++y)
+        for (int x = 0; x < sx; ++x, ++i) {
+          hinc[3 * i] = x + vf[3 * i] * delta;
+          hinc[3 * i + 1] = y + vf[3 * i + 1] * delta;
+          hinc[3 * i + 2] = z + vf[3 * i + 2] * delta;
+        }
+    std::memcpy(old.data(), h, sizeof(float) * 3 * N);
+    for (size_t j = 0; j < N; ++j) {
+      double o[3];
+      trilerp_vec<float>(old.data(), size, hinc[3 * j], hinc[3 * j + 1], hinc[3 * j + 2], o,
+                         ORC_BG_PARTIAL_ID);
+      for (int c = 0; c < 3; ++c) h[3 * j + c] = (float)o[c];
+    }
+    if (hinv) {
+      i = 0;
+      for (int z = 0; z < sz; ++z)
+        for (int y = 0; y < sy; ++y)
+          for (int x = 0; x < sx; ++x, ++i) {
+            // computeInverseZerothOrder (HField3DUtils.txx:585-604)
+            old[3 * i] = x + x - hinc[3 * i];
+            old[3 * i + 1] = y + y - hinc[3 * i + 1];
+            old[3 * i + 2] = z + z - hinc[3 * i + 2];
+          }
+      for (size_t j = 0; j < N; ++j) {
+        double o[3];
+        trilerp_vec<float>(old.data(), size, hinv[3 * j], hinv[3 * j + 1], hinv[3 * j + 2], o,
+                           ORC_BG_PARTIAL_ID);
+        for (int c = 0; c < 3; ++c) tmp[3 * j + c] = (float)o[c];
+      }
+      std::memcpy(hinv, tmp.data(), sizeof(float) * 3 * N);
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"
