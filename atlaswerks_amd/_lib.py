@@ -45,6 +45,8 @@ SIGNATURES = {
     "awk_greedy_set_step": [_P, _F],
     "awk_greedy_get_step": [_P, ctypes.POINTER(_F)],
     "awk_greedy_set_auto_step_reduce": [_P, _I],
+    "awk_greedy_set_profiling": [_P, _I],
+    "awk_greedy_get_profile": [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)],
     "awk_greedy_iterate": [_P, _P, _P, ctypes.POINTER(_P3), ctypes.POINTER(_P3),
                            ctypes.POINTER(_P3), ctypes.POINTER(_P3), _P, _P],
 }

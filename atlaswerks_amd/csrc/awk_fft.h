@@ -226,3 +226,62 @@ __device__ __forceinline__ void lds_fft(float2 *buf, const Axis &ax, int NL, int
 }
 
 }  // namespace awk
+
+namespace awk {
+
+// ---------------------------------------------------------------------------
+// Compile-time variants for power-of-two lengths: the same Stockham stages with
+// N, the line count NL, the line stride LS and the block size BD known to the
+// compiler (index division becomes shifts / multiply-shifts, loops unroll).
+constexpr int first_radix(int rem) {
+  return rem % 16 == 0 ? 16 : rem % 8 == 0 ? 8 : rem % 4 == 0 ? 4 : 2;
+}
+
+template <int N, int NL, int LS, int BD, int R, int NS>
+__device__ __forceinline__ void ct_stage(float2 *buf, const float2 *__restrict__ tw, bool inv) {
+  constexpr int M = N / R;
+  constexpr int TOTAL = M * NL;
+  constexpr int K = (TOTAL + BD - 1) / BD;
+  constexpr int TWSTEP = N / (NS * R);
+  float2 v[K][R];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const int w = threadIdx.x + i * BD;
+    if (TOTAL % BD == 0 || w < TOTAL) {
+      const int line = w % NL, j = w / NL, k = j % NS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i][r] = buf[(j + r * M) * LS + line];
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const float2 t = tw[r * k * TWSTEP];
+          v[i][r] = inv ? cmulc(v[i][r], t) : cmul(v[i][r], t);
+        }
+      }
+      dft<R>(v[i], inv);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const int w = threadIdx.x + i * BD;
+    if (TOTAL % BD == 0 || w < TOTAL) {
+      const int line = w % NL, j = w / NL, k = j % NS;
+      const int idxD = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(idxD + r * NS) * LS + line] = v[i][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int NL, int LS, int BD, int NS = 1>
+__device__ __forceinline__ void ct_fft(float2 *buf, const float2 *__restrict__ tw, bool inv) {
+  if constexpr (NS < N) {
+    constexpr int R = first_radix(N / NS);
+    ct_stage<N, NL, LS, BD, R, NS>(buf, tw, inv);
+    ct_fft<N, NL, LS, BD, NS * R>(buf, tw, inv);
+  }
+}
+
+}  // namespace awk

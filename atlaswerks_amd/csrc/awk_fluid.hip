@@ -61,11 +61,15 @@ __device__ __forceinline__ void line_targets(int line, int r0, int &rowA, int &c
   else { rowA = ra + 1; cA = 0; rowB = ra + 1; cB = 1; }
 }
 
-__global__ void __launch_bounds__(1024)
-k_xfwd(XFwdArgs a, Axis ax, int nx, int ny, int nz, int nxh, int nxh_pad, int RT,
+// NC, RTC, BDC: compile-time length, rows per block and block size (0 = runtime).
+template <int NC, int RTC, int BDC>
+__global__ void __launch_bounds__(BDC ? BDC : 1024)
+k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, int RT_rt,
        float2 *__restrict__ spec, size_t plane) {
   extern __shared__ float2 sm[];
   __shared__ double red[16];
+  const int nx = NC ? NC : nx_rt, nxh = NC ? NC / 2 + 1 : nxh_rt, RT = RTC ? RTC : RT_rt;
+  const int bdim = BDC ? BDC : (int)blockDim.x;
   const int NL = 3 * RT / 2, LS = NL + 1;
   const int NR = ny * nz;
   const int lb = (int)xcd_block();
@@ -73,7 +77,7 @@ k_xfwd(XFwdArgs a, Axis ax, int nx, int ny, int nz, int nxh, int nxh_pad, int RT
   if (r0 >= NR) return;
   double ssd = 0.0;
   float *smf = reinterpret_cast<float *>(sm);
-  for (int w = threadIdx.x; w < RT * nx; w += blockDim.x) {
+  for (int w = threadIdx.x; w < RT * nx; w += bdim) {
     const int q = w / nx, x = w - q * nx;
     const int row = r0 + q;
     float f0 = 0.f, f1 = 0.f, f2 = 0.f;
@@ -124,8 +128,9 @@ k_xfwd(XFwdArgs a, Axis ax, int nx, int ny, int nz, int nxh, int nxh_pad, int RT
     }
   }
   __syncthreads();
-  lds_fft(sm, ax, NL, LS, false);
-  for (int w = threadIdx.x; w < NL * nxh; w += blockDim.x) {
+  if constexpr (NC != 0) ct_fft<NC, 3 * RTC / 2, 3 * RTC / 2 + 1, BDC>(sm, ax.tw, false);
+  else lds_fft(sm, ax, NL, LS, false);
+  for (int w = threadIdx.x; w < NL * nxh; w += bdim) {
     const int line = w / nxh, k = w - line * nxh;
     const float2 Z = sm[k * LS + line];
     const float2 Zc = cconj(sm[((nx - k) % nx) * LS + line]);
@@ -146,7 +151,7 @@ k_xfwd(XFwdArgs a, Axis ax, int nx, int ny, int nz, int nxh, int nxh_pad, int RT
     __syncthreads();
     if (threadIdx.x == 0) {
       double s = 0.0;
-      const int nw = (blockDim.x + 63) >> 6;
+      const int nw = (bdim + 63) >> 6;
       for (int i = 0; i < nw; ++i) s += red[i];
       a.partials[lb] = s;
     }
@@ -154,17 +159,20 @@ k_xfwd(XFwdArgs a, Axis ax, int nx, int ny, int nz, int nxh, int nxh_pad, int RT
 }
 
 // X inverse: c2r of RT rows (3 components) with an epilogue.
-__global__ void __launch_bounds__(1024)
-k_xinv(XInvArgs a, Axis ax, int nx, int ny, int nz, int nxh, int nxh_pad, int RT,
+template <int NC, int RTC, int BDC>
+__global__ void __launch_bounds__(BDC ? BDC : 1024)
+k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, int RT_rt,
        const float2 *__restrict__ spec, size_t plane) {
   extern __shared__ float2 sm[];
   __shared__ float redf[16];
+  const int nx = NC ? NC : nx_rt, nxh = NC ? NC / 2 + 1 : nxh_rt, RT = RTC ? RTC : RT_rt;
+  const int bdim = BDC ? BDC : (int)blockDim.x;
   const int NL = 3 * RT / 2, LS = NL + 1;
   const int NR = ny * nz;
   const int lb = (int)xcd_block();
   const int r0 = lb * RT;
   if (r0 >= NR) return;
-  for (int w = threadIdx.x; w < NL * nxh; w += blockDim.x) {
+  for (int w = threadIdx.x; w < NL * nxh; w += bdim) {
     const int line = w / nxh, k = w - line * nxh;
     int rowA, cA, rowB, cB;
     line_targets(line, r0, rowA, cA, rowB, cB);
@@ -178,9 +186,10 @@ k_xinv(XInvArgs a, Axis ax, int nx, int ny, int nz, int nxh, int nxh_pad, int RT
     if (k > 0 && nx - k >= nxh) sm[(nx - k) * LS + line] = make_float2(A.x + B.y, B.x - A.y);  // conj(A) + i conj(B)
   }
   __syncthreads();
-  lds_fft(sm, ax, NL, LS, true);
+  if constexpr (NC != 0) ct_fft<NC, 3 * RTC / 2, 3 * RTC / 2 + 1, BDC>(sm, ax.tw, true);
+  else lds_fft(sm, ax, NL, LS, true);
   float mx = 0.f;
-  for (int w = threadIdx.x; w < RT * nx; w += blockDim.x) {
+  for (int w = threadIdx.x; w < RT * nx; w += bdim) {
     const int q = w / nx, x = w - q * nx;
     const int row = r0 + q;
     if (row >= NR) continue;
@@ -221,7 +230,7 @@ k_xinv(XInvArgs a, Axis ax, int nx, int ny, int nz, int nxh, int nxh_pad, int RT
     __syncthreads();
     if (threadIdx.x == 0) {
       float m = 0.f;
-      const int nw = (blockDim.x + 63) >> 6;
+      const int nw = (bdim + 63) >> 6;
       for (int i = 0; i < nw; ++i) m = fmaxf(m, redf[i]);
       a.maxpart[lb] = m;
     }
@@ -252,6 +261,52 @@ k_ypass(Axis ax, int ny, int nz, int nxh, int nxh_pad, int TX, float2 *__restric
   }
 }
 
+
+// ------------------------------------------------------------------
+// Register-resident two-stage passes for N = RA * RB (RA, RB in {8, 16}).
+// Thread (t, j): t = kx column within the tile (lanes, 128-B rows for TX = 16),
+// j = butterfly index.  Stage 1 reads its RA inputs straight from HBM, stage 2
+// gets its inputs through one LDS transpose; the outputs of the forward stage 2
+// stay in registers (Z pass: for the solve and the inverse stage 1).
+
+template <int N, int RA, int RB, int TX>
+__global__ void __launch_bounds__(TX *(N / RA > N / RB ? N / RA : N / RB))
+k_ypass_reg(const float2 *__restrict__ tw, int nz, int nxh, int nxh_pad, float2 *__restrict__ spec,
+            size_t plane, int inv) {
+  constexpr int PA = N / RA, PB = N / RB;
+  __shared__ float2 sm[N * TX];
+  const int t = threadIdx.x % TX, j = threadIdx.x / TX;
+  const int ntile = (nxh + TX - 1) / TX;
+  const int lb = (int)xcd_block();
+  if (lb >= 3 * nz * ntile) return;
+  const int tile = lb % ntile, rest = lb / ntile;
+  const int z = rest % nz, c = rest / nz;
+  const int kx = tile * TX + t;
+  const bool valid = kx < nxh;
+  float2 *base = spec + c * plane + (size_t)z * N * nxh_pad + kx;
+  const bool iv = inv != 0;
+  if (j < PA) {  // stage 1: radix RA, Ns = 1, inputs y = j + r * PA
+    float2 v[RA];
+#pragma unroll
+    for (int r = 0; r < RA; ++r) v[r] = valid ? base[(size_t)(j + r * PA) * nxh_pad] : make_float2(0.f, 0.f);
+    dft<RA>(v, iv);
+#pragma unroll
+    for (int r = 0; r < RA; ++r) sm[(j * RA + r) * TX + t] = v[r];
+  }
+  __syncthreads();
+  if (j < PB) {  // stage 2: radix RB, Ns = RA, inputs j + r * PB, twiddle W^(r j), outputs j + r * RA
+    float2 v[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) v[r] = sm[(j + r * PB) * TX + t];
+#pragma unroll
+    for (int r = 1; r < RB; ++r) v[r] = iv ? cmulc(v[r], tw[r * j]) : cmul(v[r], tw[r * j]);
+    dft<RB>(v, iv);
+    if (valid) {
+#pragma unroll
+      for (int r = 0; r < RB; ++r) base[(size_t)(j + r * RA) * nxh_pad] = v[r];
+    }
+  }
+}
 struct SolveArgs {
   const float *cwx, *swx, *cwy, *swy, *cwz, *swz;
   const float *eig;  // 6 per frequency, or null
@@ -259,6 +314,122 @@ struct SolveArgs {
   int inverse_op;  // 1: L^-1 (Cholesky), 0: L
   int divfree;
 };
+
+
+// Per-frequency operator: Cholesky solve of L v = b (DiffOper.cxx:1002-1083), or
+// v = L b (1085-1116), or v = M b with the eigen table M = L^(+-p); then the
+// divergence-free projection (948-1000) and the 1/N scale.
+__device__ __forceinline__ void solve_one(const SolveArgs &s, int kx, int ky, int kz, int ny,
+                                          int nxh, float2 &b0, float2 &b1, float2 &b2) {
+  float2 v0, v1, v2;
+  const float sx = s.swx[kx], sy = s.swy[ky], sz = s.swz[kz];
+  if (s.eig) {
+    const float *M = s.eig + ((size_t)(kz * ny + ky) * nxh + kx) * 6;
+    const float M00 = M[0], M10 = M[1], M11 = M[2], M20 = M[3], M21 = M[4], M22 = M[5];
+    v0 = make_float2(M00 * b0.x + M10 * b1.x + M20 * b2.x, M00 * b0.y + M10 * b1.y + M20 * b2.y);
+    v1 = make_float2(M10 * b0.x + M11 * b1.x + M21 * b2.x, M10 * b0.y + M11 * b1.y + M21 * b2.y);
+    v2 = make_float2(M20 * b0.x + M21 * b1.x + M22 * b2.x, M20 * b0.y + M21 * b1.y + M22 * b2.y);
+  } else {
+    const float cx = s.cwx[kx], cy = s.cwy[ky], cz = s.cwz[kz];
+    const float lam = -s.alpha * (cx + cy + cz) + s.gamma;
+    const float L00 = lam - s.beta * cx, L11 = lam - s.beta * cy, L22 = lam - s.beta * cz;
+    const float L10 = s.beta * sx * sy, L20 = s.beta * sx * sz, L21 = s.beta * sy * sz;
+    if (s.inverse_op) {
+      const float G00 = sqrtf(L00);
+      const float G10 = L10 / G00, G20 = L20 / G00;
+      const float G11 = sqrtf(L11 - G10 * G10);
+      const float G21 = (L21 - G20 * G10) / G11;
+      const float G22 = sqrtf(L22 - (G20 * G20 + G21 * G21));
+      float y0 = b0.x / G00, y1 = (b1.x - G10 * y0) / G11, y2 = (b2.x - G20 * y0 - G21 * y1) / G22;
+      v2.x = y2 / G22; v1.x = (y1 - G21 * v2.x) / G11; v0.x = (y0 - G10 * v1.x - G20 * v2.x) / G00;
+      y0 = b0.y / G00; y1 = (b1.y - G10 * y0) / G11; y2 = (b2.y - G20 * y0 - G21 * y1) / G22;
+      v2.y = y2 / G22; v1.y = (y1 - G21 * v2.y) / G11; v0.y = (y0 - G10 * v1.y - G20 * v2.y) / G00;
+    } else {
+      v0 = make_float2(L00 * b0.x + L10 * b1.x + L20 * b2.x, L00 * b0.y + L10 * b1.y + L20 * b2.y);
+      v1 = make_float2(L10 * b0.x + L11 * b1.x + L21 * b2.x, L10 * b0.y + L11 * b1.y + L21 * b2.y);
+      v2 = make_float2(L20 * b0.x + L21 * b1.x + L22 * b2.x, L20 * b0.y + L21 * b1.y + L22 * b2.y);
+    }
+  }
+  if (s.divfree && (kx | ky | kz)) {
+    const float nsq = sx * sx + sy * sy + sz * sz;
+    if (nsq > 0.f) {
+      const float re = (v0.x * sx + v1.x * sy + v2.x * sz) / nsq;
+      const float im = (v0.y * sx + v1.y * sy + v2.y * sz) / nsq;
+      v0.x -= re * sx; v1.x -= re * sy; v2.x -= re * sz;
+      v0.y -= im * sx; v1.y -= im * sy; v2.y -= im * sz;
+    }
+  }
+  b0 = cscale(v0, s.scale);
+  b1 = cscale(v1, s.scale);
+  b2 = cscale(v2, s.scale);
+}
+
+template <int N, int RA, int RB, int TX>
+__global__ void __launch_bounds__(TX *(N / RA > N / RB ? N / RA : N / RB))
+k_zsolve_reg(const float2 *__restrict__ tw, SolveArgs sa, int ny, int nxh, int nxh_pad,
+             float2 *__restrict__ spec, size_t plane) {
+  constexpr int PA = N / RA, PB = N / RB;  // PB = RA: threads holding the spectrum
+  __shared__ float2 sm[N * TX];
+  const int t = threadIdx.x % TX, j = threadIdx.x / TX;
+  const int ntile = (nxh + TX - 1) / TX;
+  const int lb = (int)xcd_block();
+  if (lb >= ny * ntile) return;
+  const int tile = lb % ntile, ky = lb / ntile;
+  const int kx = tile * TX + t;
+  const bool valid = kx < nxh;
+  const size_t zst = (size_t)ny * nxh_pad;
+  float2 *base = spec + (size_t)ky * nxh_pad + kx;
+  float2 X[3][RB];
+  // forward: radix RA then RB; the spectrum at kz = j + r * RA lands in X[c][r]
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (j < PA) {
+      float2 v[RA];
+#pragma unroll
+      for (int r = 0; r < RA; ++r)
+        v[r] = valid ? base[c * plane + (size_t)(j + r * PA) * zst] : make_float2(0.f, 0.f);
+      dft<RA>(v, false);
+#pragma unroll
+      for (int r = 0; r < RA; ++r) sm[(j * RA + r) * TX + t] = v[r];
+    }
+    __syncthreads();
+    if (j < PB) {
+#pragma unroll
+      for (int r = 0; r < RB; ++r) X[c][r] = sm[(j + r * PB) * TX + t];
+#pragma unroll
+      for (int r = 1; r < RB; ++r) X[c][r] = cmul(X[c][r], tw[r * j]);
+      dft<RB>(X[c], false);
+    }
+    __syncthreads();
+  }
+  if (j < PB && valid) {
+#pragma unroll
+    for (int r = 0; r < RB; ++r) solve_one(sa, kx, ky, j + r * RA, ny, nxh, X[0][r], X[1][r], X[2][r]);
+  }
+  // inverse: radix RB (inputs j + r * RA = the registers) then RA; outputs z = j + r * RB
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (j < PB) {
+      dft<RB>(X[c], true);
+#pragma unroll
+      for (int r = 0; r < RB; ++r) sm[(j * RB + r) * TX + t] = X[c][r];
+    }
+    __syncthreads();
+    if (j < PA) {
+      float2 v[RA];
+#pragma unroll
+      for (int r = 0; r < RA; ++r) v[r] = sm[(j + r * PA) * TX + t];
+#pragma unroll
+      for (int r = 1; r < RA; ++r) v[r] = cmulc(v[r], tw[r * j]);
+      dft<RA>(v, true);
+      if (valid) {
+#pragma unroll
+        for (int r = 0; r < RA; ++r) base[c * plane + (size_t)(j + r * PA) * zst] = v[r];
+      }
+    }
+    __syncthreads();
+  }
+}
 
 // Z pass: forward FFT along z of 3 components x TX kx columns at one ky, the
 // per-frequency solve, inverse FFT along z.
@@ -528,19 +699,39 @@ int fk_xblocks(const FluidKernel &k) {
 awk_status fk_xfwd(FluidKernel &k, const XFwdArgs &a, hipStream_t s, int *nblocks) {
   const unsigned grid = grid_for((long long)k.ny * k.nz, k.rt);
   const size_t lds = (size_t)k.nx * (3 * k.rt / 2 + 1) * sizeof(float2);
-  hipLaunchKernelGGL(k_xfwd, dim3(grid), dim3(k.bd_x), lds, s, a, k.ax[0].dev(), k.nx, k.ny, k.nz,
-                     k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane());
+  const Axis ax = k.ax[0].dev();
+#define AWK_XF(NC, RTC, BDC)                                                                     \
+  hipLaunchKernelGGL((k_xfwd<NC, RTC, BDC>), dim3(grid), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \
+                     k.nx, k.ny, k.nz, k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane())
+  if (k.nx == 256 && k.rt == 8) AWK_XF(256, 8, 256);
+  else if (k.nx == 128 && k.rt == 8) AWK_XF(128, 8, 256);
+  else if (k.nx == 64 && k.rt == 8) AWK_XF(64, 8, 256);
+  else AWK_XF(0, 0, 0);
+#undef AWK_XF
   AWK_LAUNCH_CHECK("k_xfwd");
   if (nblocks) *nblocks = (int)grid;
   return AWK_OK;
 }
 
 awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s) {
-  const int ntile = (k.nxh + k.tx_y - 1) / k.tx_y;
-  const unsigned grid = grid_for((long long)3 * k.nz * ntile, 1);
-  const size_t lds = (size_t)k.ny * k.tx_y * sizeof(float2);
-  hipLaunchKernelGGL(k_ypass, dim3(grid), dim3(k.bd_y), lds, s, k.ax[1].dev(), k.ny, k.nz, k.nxh,
-                     k.nxh_pad, k.tx_y, k.spec, k.spec_plane(), inv ? 1 : 0);
+  const float2 *tw = k.ax[1].tw;
+  const int ntile16 = (k.nxh + 15) / 16;
+  const unsigned g16 = grid_for((long long)3 * k.nz * ntile16, 1);
+#define AWK_YR(N, RA, RB)                                                                     \
+  hipLaunchKernelGGL((k_ypass_reg<N, RA, RB, 16>), dim3(g16), dim3(16 * (N / RA > N / RB ? N / RA : N / RB)), \
+                     0, s, tw, k.nz, k.nxh, k.nxh_pad, k.spec, k.spec_plane(), inv ? 1 : 0)
+  if (k.ny == 256) AWK_YR(256, 16, 16);
+  else if (k.ny == 128) AWK_YR(128, 16, 8);
+  else if (k.ny == 64) AWK_YR(64, 8, 8);
+  else if (k.ny == 32) AWK_YR(32, 8, 4);
+  else {
+    const int ntile = (k.nxh + k.tx_y - 1) / k.tx_y;
+    const unsigned grid = grid_for((long long)3 * k.nz * ntile, 1);
+    const size_t lds = (size_t)k.ny * k.tx_y * sizeof(float2);
+    hipLaunchKernelGGL(k_ypass, dim3(grid), dim3(k.bd_y), lds, s, k.ax[1].dev(), k.ny, k.nz, k.nxh,
+                       k.nxh_pad, k.tx_y, k.spec, k.spec_plane(), inv ? 1 : 0);
+  }
+#undef AWK_YR
   AWK_LAUNCH_CHECK("k_ypass");
   return AWK_OK;
 }
@@ -565,11 +756,24 @@ awk_status fk_zsolve(FluidKernel &k, bool inverse_op, hipStream_t s) {
   sa.scale = (float)(1.0 / ((double)k.nx * k.ny * k.nz));
   sa.inverse_op = inverse_op ? 1 : 0;
   sa.divfree = (k.flags & AWK_FK_DIVFREE) ? 1 : 0;
-  const int ntile = (k.nxh + k.tx_z - 1) / k.tx_z;
-  const unsigned grid = grid_for((long long)k.ny * ntile, 1);
-  const size_t lds = (size_t)k.nz * 3 * k.tx_z * sizeof(float2);
-  hipLaunchKernelGGL(k_zsolve, dim3(grid), dim3(k.bd_z), lds, s, k.ax[2].dev(), sa, k.ny, k.nz,
-                     k.nxh, k.nxh_pad, k.tx_z, k.spec, k.spec_plane());
+  const float2 *tw = k.ax[2].tw;
+  const int ntile16 = (k.nxh + 15) / 16;
+  const unsigned g16 = grid_for((long long)k.ny * ntile16, 1);
+#define AWK_ZR(N, RA, RB)                                                                     \
+  hipLaunchKernelGGL((k_zsolve_reg<N, RA, RB, 16>), dim3(g16), dim3(16 * (N / RA > N / RB ? N / RA : N / RB)), \
+                     0, s, tw, sa, k.ny, k.nxh, k.nxh_pad, k.spec, k.spec_plane())
+  if (k.nz == 256) AWK_ZR(256, 16, 16);
+  else if (k.nz == 128) AWK_ZR(128, 16, 8);
+  else if (k.nz == 64) AWK_ZR(64, 8, 8);
+  else if (k.nz == 32) AWK_ZR(32, 8, 4);
+  else {
+    const int ntile = (k.nxh + k.tx_z - 1) / k.tx_z;
+    const unsigned grid = grid_for((long long)k.ny * ntile, 1);
+    const size_t lds = (size_t)k.nz * 3 * k.tx_z * sizeof(float2);
+    hipLaunchKernelGGL(k_zsolve, dim3(grid), dim3(k.bd_z), lds, s, k.ax[2].dev(), sa, k.ny, k.nz,
+                       k.nxh, k.nxh_pad, k.tx_z, k.spec, k.spec_plane());
+  }
+#undef AWK_ZR
   AWK_LAUNCH_CHECK("k_zsolve");
   return AWK_OK;
 }
@@ -577,8 +781,15 @@ awk_status fk_zsolve(FluidKernel &k, bool inverse_op, hipStream_t s) {
 awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblocks) {
   const unsigned grid = grid_for((long long)k.ny * k.nz, k.rt);
   const size_t lds = (size_t)k.nx * (3 * k.rt / 2 + 1) * sizeof(float2);
-  hipLaunchKernelGGL(k_xinv, dim3(grid), dim3(k.bd_x), lds, s, a, k.ax[0].dev(), k.nx, k.ny, k.nz,
-                     k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane());
+  const Axis ax = k.ax[0].dev();
+#define AWK_XI(NC, RTC, BDC)                                                                     \
+  hipLaunchKernelGGL((k_xinv<NC, RTC, BDC>), dim3(grid), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \
+                     k.nx, k.ny, k.nz, k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane())
+  if (k.nx == 256 && k.rt == 8) AWK_XI(256, 8, 256);
+  else if (k.nx == 128 && k.rt == 8) AWK_XI(128, 8, 256);
+  else if (k.nx == 64 && k.rt == 8) AWK_XI(64, 8, 256);
+  else AWK_XI(0, 0, 0);
+#undef AWK_XI
   AWK_LAUNCH_CHECK("k_xinv");
   if (nblocks) *nblocks = (int)grid;
   return AWK_OK;

@@ -12,8 +12,10 @@
 // and a per-block max |v|; the step MaxPert / max|v| is then formed on the device
 // and a separate composeHV applies it.  The same unfused tail serves the inverse
 // hfield update.
+#include <array>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "awk_fluid.h"
 #include "awk_ops.h"
@@ -77,11 +79,21 @@ struct awk_greedy {
   double *prev = nullptr;
   int *have_prev = nullptr;
   int nblk = 0;
+  // optional per-stage HIP event timing (awk_greedy_set_profiling)
+  bool prof = false;
+  std::vector<std::array<hipEvent_t, AWK_GREEDY_NSTAGES + 1>> ev;
+  int ev_used = 0;
+  void mark(int i, hipStream_t s) {
+    if (prof) (void)hipEventRecord(ev[ev_used][i], s);
+  }
 };
 
 using namespace awk;
 
 static void greedy_free(awk_greedy *g) {
+  for (auto &set : g->ev)
+    for (hipEvent_t e : set) (void)hipEventDestroy(e);
+  g->ev.clear();
   fk_release(g->k);
   if (g->idef) (void)hipFree(g->idef);
   if (g->v) (void)hipFree(g->v);
@@ -192,6 +204,14 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
   const int nx = g->nx, ny = g->ny, nz = g->nz;
   const size_t N = (size_t)nx * ny * nz;
   awk_status st;
+  if (g->prof) {
+    if (g->ev_used == (int)g->ev.size()) {
+      std::array<hipEvent_t, AWK_GREEDY_NSTAGES + 1> set;
+      for (auto &e : set) AWK_HIP(hipEventCreate(&e));
+      g->ev.push_back(set);
+    }
+  }
+  g->mark(0, s);
   // 1. deformed image (VAL background 0, HField3DUtils::apply defaults)
   if ((st = launch_apply(g->idef, I0, h_in[0], h_in[1], h_in[2], nx, ny, nz, AWK_BG_VAL, 0.f, s))) return st;
   // 2. force + forward x pass
@@ -203,16 +223,22 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
   for (int c = 0; c < 3; ++c) fa.sp[c] = g->sp[c];
   fa.partials = g->partials;
   int nb = 0;
+  g->mark(1, s);
   if ((st = fk_xfwd(g->k, fa, s, &nb))) return st;
+  g->mark(2, s);
   // 3. energy (ImageUtils::l2NormSqr = prod(spacing) * sum r^2, ImageUtils.txx:55-58)
   const double scale = (double)g->sp[0] * g->sp[1] * g->sp[2];
   hipLaunchKernelGGL(k_energy, dim3(1), dim3(256), 0, s, g->partials, nb, scale, energy_dev, g->prev,
                      g->have_prev, g->step, g->auto_reduce);
   AWK_LAUNCH_CHECK("k_energy");
+  g->mark(3, s);
   // 4. solve
   if ((st = fk_ypass(g->k, false, s))) return st;
+  g->mark(4, s);
   if ((st = fk_zsolve(g->k, true, s))) return st;
+  g->mark(5, s);
   if ((st = fk_ypass(g->k, true, s))) return st;
+  g->mark(6, s);
   // 5. velocity -> hfield update
   const bool in_place = (h_out[0] == h_in[0]);
   float *dst[3];
@@ -249,6 +275,8 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
         for (int c = 0; c < 3; ++c)
           AWK_HIP(hipMemcpyAsync(hinv_out[c], g->hbuf + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, s));
     }
+    g->mark(7, s);
+    if (g->prof) ++g->ev_used;
     return AWK_OK;
   } else {
     ia.mode = XSTORE_COMPOSE;
@@ -262,6 +290,31 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
   if (in_place)
     for (int c = 0; c < 3; ++c)
       AWK_HIP(hipMemcpyAsync(h_out[c], g->hbuf + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, s));
+  g->mark(7, s);
+  if (g->prof) ++g->ev_used;
+  return AWK_OK;
+}
+
+awk_status awk_greedy_set_profiling(awk_greedy_t g, int on) {
+  if (!g) return fail(AWK_ERR_INVALID, "awk_greedy_set_profiling: null handle");
+  g->prof = on != 0;
+  g->ev_used = 0;
+  return AWK_OK;
+}
+
+awk_status awk_greedy_get_profile(awk_greedy_t g, double *stage_ms, int *iterations) {
+  if (!g || !stage_ms || !iterations) return fail(AWK_ERR_INVALID, "awk_greedy_get_profile: null argument");
+  for (int i = 0; i < AWK_GREEDY_NSTAGES; ++i) stage_ms[i] = 0.0;
+  for (int it = 0; it < g->ev_used; ++it) {
+    AWK_HIP(hipEventSynchronize(g->ev[it][AWK_GREEDY_NSTAGES]));
+    for (int i = 0; i < AWK_GREEDY_NSTAGES; ++i) {
+      float ms = 0.f;
+      AWK_HIP(hipEventElapsedTime(&ms, g->ev[it][i], g->ev[it][i + 1]));
+      stage_ms[i] += ms;
+    }
+  }
+  *iterations = g->ev_used;
+  g->ev_used = 0;
   return AWK_OK;
 }
 

@@ -95,6 +95,18 @@ class GreedyIteratorGPU:
     def SetStepSize(self, step):
         _lib.call("awk_greedy_set_step", self._h, float(step))
 
+    STAGES = ("apply", "xfwd_force", "energy", "y_fwd", "z_solve", "y_inv", "xinv_compose")
+
+    def SetProfiling(self, on):
+        _lib.call("awk_greedy_set_profiling", self._h, int(bool(on)))
+
+    def GetProfile(self):
+        """{stage: total ms} over the iterations recorded since the last read, and the count."""
+        ms = (ctypes.c_double * len(self.STAGES))()
+        n = ctypes.c_int()
+        _lib.call("awk_greedy_get_profile", self._h, ms, ctypes.byref(n))
+        return dict(zip(self.STAGES, list(ms))), n.value
+
     def Iterate(self, data, stream=None):
         if dims(data.I0) != self.size:
             raise ValueError("deformation data size does not match the iterator")

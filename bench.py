@@ -1,0 +1,220 @@
+"""Greedy-fluid iteration throughput at 256^3 on one MI355X.
+
+One step = one greedy-fluid iteration (GreedyIteratorGPU::Iterate semantics:
+apply, gradient/residual/force/SSD, FluidKernelFFT L^-1, step-scaled composeHV)
+of a 256^3 float32 pairwise registration, inputs resident in HBM.  Prints one
+JSON line with the iteration rate, the roofline of the dominant kernel (measured
+live with HIP events on the iteration's stream), per-stage times, the kernel
+microbenchmarks of the 256^3 workload (L^-1, apply, compose, splat) and the CPU
+baseline (the oracle's FP32 restatement of the reference CPU iteration, one
+host thread, timed on a bounded sample).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def stage_bytes(nx, ny, nz):
+    """Algorithmic HBM bytes of one launch of each stage of the fused iteration.
+
+    N voxels, spectrum = 3 components x (nx/2+1) * ny * nz complex64."""
+    N = nx * ny * nz
+    spec = 3 * (nx // 2 + 1) * ny * nz * 8
+    return {
+        "apply": 20 * N,                         # read h 12N + I0 4N, write Idef 4N
+        "xfwd_force": 8 * N + spec,              # read Idef + I1, write spectrum
+        "energy": 0,
+        "y_fwd": 2 * spec,                       # read + write spectrum
+        "z_solve": 2 * spec,
+        "y_inv": 2 * spec,
+        "xinv_compose": spec + 24 * N,           # read spectrum + h, write h
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-microbench", action="store_true")
+    ap.add_argument("--copy-ref", action="store_true",
+                    help="also run a plain 1 GiB device copy (counter scaling reference)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (--gpus %d requested); the "
+              "multi-GPU design is subject sharding, see DESIGN.md" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from atlaswerks_amd import _lib, hfield, synth
+    from atlaswerks_amd.fluid import DiffOperGPU
+    from atlaswerks_amd.greedy import GreedyDeformationDataGPU, GreedyIteratorGPU, GreedyIteratorParam
+
+    _lib.load()
+    dev = torch.device("cuda:0")
+    n = args.size
+    shape = (n, n, n)
+    nx = ny = nz = n
+    N = n ** 3
+    # ---- synthetic workload: 8-blob template, target = template(x + u), |u| <= 8
+    t0 = time.time()
+    tmpl = synth.blobs(shape, seed=1)
+    u = synth.sinusoidal_displacement(shape, max_disp=8.0, seed=1)
+    I0 = torch.from_numpy(tmpl).to(dev)
+    h = torch.from_numpy(synth.identity(shape) + u).to(dev)
+    I1 = torch.empty_like(I0)
+    hfield.apply(I1, I0, h)
+    del h
+    torch.cuda.synchronize()
+    gen_s = time.time() - t0
+
+    param = GreedyIteratorParam(max_pert=0.1, alpha=0.01, beta=0.01, gamma=0.001)
+    it = GreedyIteratorGPU((nx, ny, nz), (1.0, 1.0, 1.0), param)
+    data = GreedyDeformationDataGPU(I0, I1, max_iters=args.warmup + args.steps + 1)
+    it.UpdateStepSizeNextIteration()
+    for _ in range(args.warmup):
+        it.Iterate(data)
+    torch.cuda.synchronize()
+    it.SetProfiling(True)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        it.Iterate(data)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - ts
+    prof, n_prof = it.GetProfile()
+    it.SetProfiling(False)
+    energies = data.energies()
+    ms_per_step = 1e3 * elapsed / args.steps
+    iters_per_s = args.steps / elapsed
+
+    stage_avg = {k: v / max(n_prof, 1) for k, v in prof.items()}
+    sb = stage_bytes(nx, ny, nz)
+    dom = max(stage_avg, key=lambda k: stage_avg[k])
+    achieved = sb[dom] / (stage_avg[dom] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pmc_traffic_%d.json" % n)
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(dom)
+    yard = 196 * N  # fixed yardstick bytes of one greedy iteration
+    out = {
+        "metric": "greedy-fluid iter/s @256^3 (% HBM peak); time-to-SSD plain vs warm-start",
+        "value": round(iters_per_s, 3),
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "greedy-fluid pairwise iteration %d^3 float32: 8-blob template, target "
+                        "warped by sinusoidal displacement (max 8 voxels), alpha=beta=0.01, "
+                        "gamma=0.001, MaxPert 0.1" % n,
+            "size": [nx, ny, nz],
+        },
+        "hbm_frac_yardstick": round(yard * iters_per_s / (HBM_PEAK_GBS * 1e9), 4),
+        "yardstick_bytes_per_iter": yard,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "alg_bytes_per_launch": sb[dom],
+            "avg_launch_ms": round(stage_avg[dom], 5),
+        },
+        "stages_ms": {k: round(v, 5) for k, v in stage_avg.items()},
+        "stages_gbs": {k: round(sb[k] / (v * 1e-3) / 1e9, 1) for k, v in stage_avg.items() if sb[k]},
+        "energy_first_last": [float(energies[0]), float(energies[-1])],
+        "input_gen_s": round(gen_s, 2),
+    }
+
+    # ---- kernel microbenchmarks of the 256^3 workload (config-1 operations)
+    if not args.no_microbench:
+        def timeit(fn, reps=20):
+            fn()
+            torch.cuda.synchronize()
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(reps):
+                fn()
+            b.record()
+            torch.cuda.synchronize()
+            return a.elapsed_time(b) / reps
+
+        mb = {}
+        f = torch.randn((3, nz, ny, nx), device=dev)
+        op = DiffOperGPU((nx, ny, nz))
+        ms = timeit(lambda: op.ApplyInverseOperator(f))
+        mb["L_inverse"] = {"ms": round(ms, 4), "gbs_120N": round(120 * N / (ms * 1e-3) / 1e9, 1)}
+        g1 = torch.from_numpy(synth.identity(shape) + u).to(dev)
+        g2 = torch.from_numpy(synth.identity(shape) + synth.sinusoidal_displacement(shape, 8.0, seed=2)).to(dev)
+        outi = torch.empty_like(I0)
+        ms = timeit(lambda: hfield.apply(outi, I0, g1))
+        mb["apply"] = {"ms": round(ms, 4), "gbs_20N": round(20 * N / (ms * 1e-3) / 1e9, 1)}
+        outv = torch.empty_like(g1)
+        ms = timeit(lambda: hfield.compose(outv, g1, g2, hfield.BACKGROUND_STRATEGY_PARTIAL_ID))
+        mb["compose"] = {"ms": round(ms, 4), "gbs_36N": round(36 * N / (ms * 1e-3) / 1e9, 1)}
+        for name, mode in (("splat_fixed64", _lib.SPLAT_FIXED64), ("splat_float", _lib.SPLAT_FLOAT)):
+            ms = timeit(lambda: hfield.splat(outi, I0, g1, mode=mode))
+            mb[name] = {"ms": round(ms, 4), "gbs_20N": round(20 * N / (ms * 1e-3) / 1e9, 1)}
+        if args.copy_ref:
+            a = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev)
+            b = torch.empty_like(a)
+            ms = timeit(lambda: b.copy_(a))
+            mb["copy_1GiB"] = {"ms": round(ms, 4), "gbs_rw": round(2 * a.numel() * 4 / (ms * 1e-3) / 1e9, 1)}
+        out["microbench"] = mb
+        del f, g1, g2, outi, outv
+
+    # ---- CPU baseline: the oracle's FP32 restatement, one host thread, bounded sample
+    if not args.no_cpu_baseline:
+        import oracle
+        sec = oracle.greedy_time(shape, n_iter=1, seed=0)
+        out["cpu_baseline"] = {
+            "value": round(1.0 / sec, 5),
+            "unit": "iter/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": "1 FP32 greedy iteration at %d^3 (C++ restatement of GreedyIteratorCPU::"
+                      "Iterate, own mixed-radix FFT in place of FFTW, 1 thread) = %.2f s; host %s, "
+                      "nproc %d" % (n, sec, _cpu_model(), os.cpu_count() or 0),
+        }
+        out["gpu_vs_cpu_1core"] = round(iters_per_s * sec, 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -13,7 +13,8 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 
 SIZES = [(8, 8, 8), (6, 10, 12), (24, 40, 36), (32, 31, 36), (64, 64, 64), (17, 9, 5),
-         (96, 48, 40), (1, 16, 16), (128, 128, 128)]
+         (96, 48, 40), (1, 16, 16), (128, 128, 128), (32, 64, 256), (256, 256, 16),
+         (128, 32, 64)]
 
 
 def _field(shape, seed):
