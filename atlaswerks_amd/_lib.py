@@ -22,6 +22,8 @@ _P3 = ctypes.c_void_p * 3
 SIGNATURES = {
     "awk_last_error": [],
     "awk_version": [],
+    "awk_tune_set": [ctypes.c_char_p, _I],
+    "awk_tune_get": [ctypes.c_char_p, ctypes.POINTER(_I)],
     "awk_fluid_kernel_create": [ctypes.POINTER(_P), _I, _I, _I, _F, _F, _F, _F, _F, _F, _F, _I],
     "awk_fluid_kernel_set_params": [_P, _F, _F, _F, _F, _I],
     "awk_fluid_kernel_destroy": [_P],

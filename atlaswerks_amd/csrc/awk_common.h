@@ -34,6 +34,14 @@ awk_status hip_check(hipError_t e, const char *what);
 
 inline hipStream_t as_stream(awk_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Launch-shape knobs (set by the autotuner through awk_tune_set).
+struct Tuning {
+  int apply_vpt = 1;    // voxels per thread, apply
+  int compose_vpt = 1;  // voxels per thread, compose family
+  int xinv_unroll = 1;  // voxels per epilogue step of the inverse x pass
+};
+Tuning &tuning();
+
 // Grid size rounded up to a multiple of the XCD count so that xcd_block() is a
 // bijection; blocks past the end exit early.
 constexpr int kNumXcd = 8;

@@ -38,6 +38,18 @@ struct XFwdArgs {
   double *partials;     // FORCE: one double per block (may be null)
 };
 
+// Energy reduction of the greedy step (done by block 0 of the forward y pass).
+struct ReduceArgs {
+  const double *partials;  // one FP64 partial per x-pass block (null: no reduction)
+  int n;
+  double scale;            // product of the spacings (ImageUtils::l2NormSqr)
+  double *energy_out;      // may be null
+  double *prev;            // previous energy
+  int *have_prev;
+  float *step;             // halved on an energy increase when auto_reduce
+  int auto_reduce;
+};
+
 struct XInvArgs {
   int mode;
   float *out[3];            // PLAIN / MAXNORM: output planes
@@ -74,7 +86,7 @@ void fk_release(FluidKernel &k);
 awk_status fk_prepare_eigen(FluidKernel &k, int dir, hipStream_t s);
 // Pass launchers.
 awk_status fk_xfwd(FluidKernel &k, const XFwdArgs &a, hipStream_t s, int *nblocks);
-awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s);
+awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s, const ReduceArgs *red);
 awk_status fk_zsolve(FluidKernel &k, bool inverse_op, hipStream_t s);
 awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblocks);
 int fk_xblocks(const FluidKernel &k);

@@ -15,6 +15,7 @@
 //      solve (with the 1/N scale folded in), inverse FFT along z.
 //   Y' inverse along y.   X' c2r along x with an epilogue (plain store, max-norm,
 //      or the fused greedy hfield update).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -61,6 +62,29 @@ __device__ __forceinline__ void line_targets(int line, int r0, int &rowA, int &c
   else { rowA = ra + 1; cA = 0; rowB = ra + 1; cB = 1; }
 }
 
+// Energy bookkeeping of one greedy iteration, run by logical block 0 of the
+// forward y pass (the kernel boundary after the x pass makes every partial
+// visible): fixed-order FP64 sum of the x-pass block partials, then
+// GreedyDeformationData::AddEnergy (GreedyDeformationData.cxx:162-184).
+__device__ __forceinline__ void energy_block(const ReduceArgs &ra, double *tree, int bdim) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < ra.n; i += bdim) acc += ra.partials[i];
+  tree[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w && (int)threadIdx.x + w < bdim) tree[threadIdx.x] += tree[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double e = ra.scale * tree[0];
+    if (ra.energy_out) *ra.energy_out = e;
+    if (ra.auto_reduce && *ra.have_prev && e - *ra.prev > 0.0) *ra.step = *ra.step * 0.5f;
+    *ra.prev = e;
+    *ra.have_prev = 1;
+  }
+}
+
+
 // NC, RTC, BDC: compile-time length, rows per block and block size (0 = runtime).
 template <int NC, int RTC, int BDC>
 __global__ void __launch_bounds__(BDC ? BDC : 1024)
@@ -77,6 +101,7 @@ k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
   if (r0 >= NR) return;
   double ssd = 0.0;
   float *smf = reinterpret_cast<float *>(sm);
+#pragma unroll
   for (int w = threadIdx.x; w < RT * nx; w += bdim) {
     const int q = w / nx, x = w - q * nx;
     const int row = r0 + q;
@@ -130,6 +155,7 @@ k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
   __syncthreads();
   if constexpr (NC != 0) ct_fft<NC, 3 * RTC / 2, 3 * RTC / 2 + 1, BDC>(sm, ax.tw, false);
   else lds_fft(sm, ax, NL, LS, false);
+#pragma unroll
   for (int w = threadIdx.x; w < NL * nxh; w += bdim) {
     const int line = w / nxh, k = w - line * nxh;
     const float2 Z = sm[k * LS + line];
@@ -159,7 +185,7 @@ k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
 }
 
 // X inverse: c2r of RT rows (3 components) with an epilogue.
-template <int NC, int RTC, int BDC>
+template <int NC, int RTC, int BDC, int XU>
 __global__ void __launch_bounds__(BDC ? BDC : 1024)
 k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, int RT_rt,
        const float2 *__restrict__ spec, size_t plane) {
@@ -172,6 +198,7 @@ k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
   const int lb = (int)xcd_block();
   const int r0 = lb * RT;
   if (r0 >= NR) return;
+#pragma unroll
   for (int w = threadIdx.x; w < NL * nxh; w += bdim) {
     const int line = w / nxh, k = w - line * nxh;
     int rowA, cA, rowB, cB;
@@ -189,33 +216,66 @@ k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
   if constexpr (NC != 0) ct_fft<NC, 3 * RTC / 2, 3 * RTC / 2 + 1, BDC>(sm, ax.tw, true);
   else lds_fft(sm, ax, NL, LS, true);
   float mx = 0.f;
-  for (int w = threadIdx.x; w < RT * nx; w += bdim) {
-    const int q = w / nx, x = w - q * nx;
-    const int row = r0 + q;
-    if (row >= NR) continue;
-    const int p = q >> 1;
-    float v0, v1, v2;
-    if ((q & 1) == 0) {
-      const float2 t = sm[x * LS + 3 * p];
-      v0 = t.x; v1 = t.y; v2 = sm[x * LS + 3 * p + 1].x;
-    } else {
-      const float2 t = sm[x * LS + 3 * p + 2];
-      v0 = t.x; v1 = t.y; v2 = sm[x * LS + 3 * p + 1].y;
+  if (a.mode == XSTORE_COMPOSE) {
+    // updateDeformation (GreedyIteratorCPU.cxx:81-95): v *= step; h <- composeHV(h, v, sp, PARTIAL_ID).
+    // XU voxels per step: their 24*XU gathers are in flight together.
+    const float st = *a.step;
+    const Dims d{nx, ny, nz};
+#pragma unroll
+    for (int w0 = threadIdx.x; w0 < RT * nx; w0 += bdim * XU) {
+      float px[XU], py[XU], pz[XU];
+      int row_[XU], x_[XU];
+#pragma unroll
+      for (int u = 0; u < XU; ++u) {
+        const int w = w0 + u * bdim;
+        const int q = w / nx, x = w - q * nx;
+        row_[u] = (w < RT * nx) ? r0 + q : NR;
+        x_[u] = x;
+        if (row_[u] < NR) {
+          const int p = q >> 1;
+          float v0, v1, v2;
+          if ((q & 1) == 0) {
+            const float2 t = sm[x * LS + 3 * p];
+            v0 = t.x; v1 = t.y; v2 = sm[x * LS + 3 * p + 1].x;
+          } else {
+            const float2 t = sm[x * LS + 3 * p + 2];
+            v0 = t.x; v1 = t.y; v2 = sm[x * LS + 3 * p + 1].y;
+          }
+          const int y = row_[u] % ny, z = row_[u] / ny;
+          px[u] = (float)x + (v0 * st) / a.sp[0];
+          py[u] = (float)y + (v1 * st) / a.sp[1];
+          pz[u] = (float)z + (v2 * st) / a.sp[2];
+        }
+      }
+      float3 hn[XU];
+#pragma unroll
+      for (int u = 0; u < XU; ++u)
+        if (row_[u] < NR) hn[u] = trilerp3<AWK_BG_PARTIAL_ID>(a.h[0], a.h[1], a.h[2], d, px[u], py[u], pz[u]);
+#pragma unroll
+      for (int u = 0; u < XU; ++u)
+        if (row_[u] < NR) {
+          const size_t base = (size_t)row_[u] * nx + x_[u];
+          a.hnew[0][base] = hn[u].x;
+          a.hnew[1][base] = hn[u].y;
+          a.hnew[2][base] = hn[u].z;
+        }
     }
-    const size_t base = (size_t)row * nx + x;
-    if (a.mode == XSTORE_COMPOSE) {
-      // updateDeformation (GreedyIteratorCPU.cxx:81-95): v *= step; h <- composeHV(h, v, sp, PARTIAL_ID)
-      const float st = *a.step;
-      const int y = row % ny, z = row / ny;
-      const Dims d{nx, ny, nz};
-      const float px = (float)x + (v0 * st) / a.sp[0];
-      const float py = (float)y + (v1 * st) / a.sp[1];
-      const float pz = (float)z + (v2 * st) / a.sp[2];
-      const float3 hn = trilerp3<AWK_BG_PARTIAL_ID>(a.h[0], a.h[1], a.h[2], d, px, py, pz);
-      a.hnew[0][base] = hn.x;
-      a.hnew[1][base] = hn.y;
-      a.hnew[2][base] = hn.z;
-    } else {
+  } else {
+#pragma unroll
+    for (int w = threadIdx.x; w < RT * nx; w += bdim) {
+      const int q = w / nx, x = w - q * nx;
+      const int row = r0 + q;
+      if (row >= NR) continue;
+      const int p = q >> 1;
+      float v0, v1, v2;
+      if ((q & 1) == 0) {
+        const float2 t = sm[x * LS + 3 * p];
+        v0 = t.x; v1 = t.y; v2 = sm[x * LS + 3 * p + 1].x;
+      } else {
+        const float2 t = sm[x * LS + 3 * p + 2];
+        v0 = t.x; v1 = t.y; v2 = sm[x * LS + 3 * p + 1].y;
+      }
+      const size_t base = (size_t)row * nx + x;
       a.out[0][base] = v0;
       a.out[1][base] = v1;
       a.out[2][base] = v2;
@@ -240,7 +300,7 @@ k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
 // Y pass: complex FFT along y for TX kx columns of one (component, z) slab.
 __global__ void __launch_bounds__(1024)
 k_ypass(Axis ax, int ny, int nz, int nxh, int nxh_pad, int TX, float2 *__restrict__ spec,
-        size_t plane, int inv) {
+        size_t plane, int inv, ReduceArgs ra) {
   extern __shared__ float2 sm[];
   const int ntile = (nxh + TX - 1) / TX;
   const int lb = (int)xcd_block();
@@ -259,7 +319,12 @@ k_ypass(Axis ax, int ny, int nz, int nxh, int nxh_pad, int TX, float2 *__restric
     const int t = w % TX, y = w / TX;
     if (kx0 + t < nxh) base[(size_t)y * nxh_pad + t] = sm[y * TX + t];
   }
+  if (ra.partials && lb == 0) {
+    __syncthreads();
+    energy_block(ra, reinterpret_cast<double *>(sm), blockDim.x);
+  }
 }
+
 
 
 // ------------------------------------------------------------------
@@ -272,7 +337,7 @@ k_ypass(Axis ax, int ny, int nz, int nxh, int nxh_pad, int TX, float2 *__restric
 template <int N, int RA, int RB, int TX>
 __global__ void __launch_bounds__(TX *(N / RA > N / RB ? N / RA : N / RB))
 k_ypass_reg(const float2 *__restrict__ tw, int nz, int nxh, int nxh_pad, float2 *__restrict__ spec,
-            size_t plane, int inv) {
+            size_t plane, int inv, ReduceArgs ra) {
   constexpr int PA = N / RA, PB = N / RB;
   __shared__ float2 sm[N * TX];
   const int t = threadIdx.x % TX, j = threadIdx.x / TX;
@@ -306,6 +371,10 @@ k_ypass_reg(const float2 *__restrict__ tw, int nz, int nxh, int nxh_pad, float2 
       for (int r = 0; r < RB; ++r) base[(size_t)(j + r * RA) * nxh_pad] = v[r];
     }
   }
+  if (ra.partials && lb == 0) {
+    __syncthreads();
+    energy_block(ra, reinterpret_cast<double *>(sm), TX * (PA > PB ? PA : PB));
+  }
 }
 struct SolveArgs {
   const float *cwx, *swx, *cwy, *swy, *cwz, *swz;
@@ -335,15 +404,17 @@ __device__ __forceinline__ void solve_one(const SolveArgs &s, int kx, int ky, in
     const float L00 = lam - s.beta * cx, L11 = lam - s.beta * cy, L22 = lam - s.beta * cz;
     const float L10 = s.beta * sx * sy, L20 = s.beta * sx * sz, L21 = s.beta * sy * sz;
     if (s.inverse_op) {
-      const float G00 = sqrtf(L00);
-      const float G10 = L10 / G00, G20 = L20 / G00;
-      const float G11 = sqrtf(L11 - G10 * G10);
-      const float G21 = (L21 - G20 * G10) / G11;
-      const float G22 = sqrtf(L22 - (G20 * G20 + G21 * G21));
-      float y0 = b0.x / G00, y1 = (b1.x - G10 * y0) / G11, y2 = (b2.x - G20 * y0 - G21 * y1) / G22;
-      v2.x = y2 / G22; v1.x = (y1 - G21 * v2.x) / G11; v0.x = (y0 - G10 * v1.x - G20 * v2.x) / G00;
-      y0 = b0.y / G00; y1 = (b1.y - G10 * y0) / G11; y2 = (b2.y - G20 * y0 - G21 * y1) / G22;
-      v2.y = y2 / G22; v1.y = (y1 - G21 * v2.y) / G11; v0.y = (y0 - G10 * v1.y - G20 * v2.y) / G00;
+      // Same Cholesky factorisation, carried with the reciprocals g_ii = 1/G_ii
+      // (rsqrt): every division of the reference becomes a multiplication.
+      const float g00 = rsqrtf(L00);
+      const float G10 = L10 * g00, G20 = L20 * g00;
+      const float g11 = rsqrtf(L11 - G10 * G10);
+      const float G21 = (L21 - G20 * G10) * g11;
+      const float g22 = rsqrtf(L22 - (G20 * G20 + G21 * G21));
+      float y0 = b0.x * g00, y1 = (b1.x - G10 * y0) * g11, y2 = (b2.x - G20 * y0 - G21 * y1) * g22;
+      v2.x = y2 * g22; v1.x = (y1 - G21 * v2.x) * g11; v0.x = (y0 - G10 * v1.x - G20 * v2.x) * g00;
+      y0 = b0.y * g00; y1 = (b1.y - G10 * y0) * g11; y2 = (b2.y - G20 * y0 - G21 * y1) * g22;
+      v2.y = y2 * g22; v1.y = (y1 - G21 * v2.y) * g11; v0.y = (y0 - G10 * v1.y - G20 * v2.y) * g00;
     } else {
       v0 = make_float2(L00 * b0.x + L10 * b1.x + L20 * b2.x, L00 * b0.y + L10 * b1.y + L20 * b2.y);
       v1 = make_float2(L10 * b0.x + L11 * b1.x + L21 * b2.x, L10 * b0.y + L11 * b1.y + L21 * b2.y);
@@ -457,50 +528,10 @@ k_zsolve(Axis az, SolveArgs s, int ny, int nz, int nxh, int nxh_pad, int TX,
     const int kx = kx0 + t;
     if (kx >= nxh) continue;
     float2 b0 = sm[kz * NL + t], b1 = sm[kz * NL + TX + t], b2 = sm[kz * NL + 2 * TX + t];
-    float2 v0, v1, v2;
-    const float sx = s.swx[kx], sy = s.swy[ky], sz = s.swz[kz];
-    if (s.eig) {
-      const float *M = s.eig + ((size_t)(kz * ny + ky) * nxh + kx) * 6;
-      const float M00 = M[0], M10 = M[1], M11 = M[2], M20 = M[3], M21 = M[4], M22 = M[5];
-      v0 = make_float2(M00 * b0.x + M10 * b1.x + M20 * b2.x, M00 * b0.y + M10 * b1.y + M20 * b2.y);
-      v1 = make_float2(M10 * b0.x + M11 * b1.x + M21 * b2.x, M10 * b0.y + M11 * b1.y + M21 * b2.y);
-      v2 = make_float2(M20 * b0.x + M21 * b1.x + M22 * b2.x, M20 * b0.y + M21 * b1.y + M22 * b2.y);
-    } else {
-      const float cx = s.cwx[kx], cy = s.cwy[ky], cz = s.cwz[kz];
-      // DiffOper.cxx:861-879
-      const float lam = -s.alpha * (cx + cy + cz) + s.gamma;
-      const float L00 = lam - s.beta * cx, L11 = lam - s.beta * cy, L22 = lam - s.beta * cz;
-      const float L10 = s.beta * sx * sy, L20 = s.beta * sx * sz, L21 = s.beta * sy * sz;
-      if (s.inverse_op) {
-        // Cholesky G G^T = L and two triangular solves (DiffOper.cxx:1002-1083)
-        const float G00 = sqrtf(L00);
-        const float G10 = L10 / G00, G20 = L20 / G00;
-        const float G11 = sqrtf(L11 - G10 * G10);
-        const float G21 = (L21 - G20 * G10) / G11;
-        const float G22 = sqrtf(L22 - (G20 * G20 + G21 * G21));
-        float y0 = b0.x / G00, y1 = (b1.x - G10 * y0) / G11, y2 = (b2.x - G20 * y0 - G21 * y1) / G22;
-        v2.x = y2 / G22; v1.x = (y1 - G21 * v2.x) / G11; v0.x = (y0 - G10 * v1.x - G20 * v2.x) / G00;
-        y0 = b0.y / G00; y1 = (b1.y - G10 * y0) / G11; y2 = (b2.y - G20 * y0 - G21 * y1) / G22;
-        v2.y = y2 / G22; v1.y = (y1 - G21 * v2.y) / G11; v0.y = (y0 - G10 * v1.y - G20 * v2.y) / G00;
-      } else {
-        v0 = make_float2(L00 * b0.x + L10 * b1.x + L20 * b2.x, L00 * b0.y + L10 * b1.y + L20 * b2.y);
-        v1 = make_float2(L10 * b0.x + L11 * b1.x + L21 * b2.x, L10 * b0.y + L11 * b1.y + L21 * b2.y);
-        v2 = make_float2(L20 * b0.x + L21 * b1.x + L22 * b2.x, L20 * b0.y + L21 * b1.y + L22 * b2.y);
-      }
-    }
-    if (s.divfree && (kx | ky | kz)) {
-      // ProjectIncomp (DiffOper.cxx:948-1000)
-      const float nsq = sx * sx + sy * sy + sz * sz;
-      if (nsq > 0.f) {
-        const float re = (v0.x * sx + v1.x * sy + v2.x * sz) / nsq;
-        const float im = (v0.y * sx + v1.y * sy + v2.y * sz) / nsq;
-        v0.x -= re * sx; v1.x -= re * sy; v2.x -= re * sz;
-        v0.y -= im * sx; v1.y -= im * sy; v2.y -= im * sz;
-      }
-    }
-    sm[kz * NL + t] = cscale(v0, s.scale);
-    sm[kz * NL + TX + t] = cscale(v1, s.scale);
-    sm[kz * NL + 2 * TX + t] = cscale(v2, s.scale);
+    solve_one(s, kx, ky, kz, ny, nxh, b0, b1, b2);
+    sm[kz * NL + t] = b0;
+    sm[kz * NL + TX + t] = b1;
+    sm[kz * NL + 2 * TX + t] = b2;
   }
   __syncthreads();
   lds_fft(sm, az, NL, NL, true);
@@ -713,13 +744,16 @@ awk_status fk_xfwd(FluidKernel &k, const XFwdArgs &a, hipStream_t s, int *nblock
   return AWK_OK;
 }
 
-awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s) {
+awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s, const ReduceArgs *red) {
+  ReduceArgs ra;
+  std::memset(&ra, 0, sizeof(ra));
+  if (red) ra = *red;
   const float2 *tw = k.ax[1].tw;
   const int ntile16 = (k.nxh + 15) / 16;
   const unsigned g16 = grid_for((long long)3 * k.nz * ntile16, 1);
 #define AWK_YR(N, RA, RB)                                                                     \
   hipLaunchKernelGGL((k_ypass_reg<N, RA, RB, 16>), dim3(g16), dim3(16 * (N / RA > N / RB ? N / RA : N / RB)), \
-                     0, s, tw, k.nz, k.nxh, k.nxh_pad, k.spec, k.spec_plane(), inv ? 1 : 0)
+                     0, s, tw, k.nz, k.nxh, k.nxh_pad, k.spec, k.spec_plane(), inv ? 1 : 0, ra)
   if (k.ny == 256) AWK_YR(256, 16, 16);
   else if (k.ny == 128) AWK_YR(128, 16, 8);
   else if (k.ny == 64) AWK_YR(64, 8, 8);
@@ -727,9 +761,9 @@ awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s) {
   else {
     const int ntile = (k.nxh + k.tx_y - 1) / k.tx_y;
     const unsigned grid = grid_for((long long)3 * k.nz * ntile, 1);
-    const size_t lds = (size_t)k.ny * k.tx_y * sizeof(float2);
+    const size_t lds = std::max((size_t)k.ny * k.tx_y * sizeof(float2), (size_t)k.bd_y * sizeof(double));
     hipLaunchKernelGGL(k_ypass, dim3(grid), dim3(k.bd_y), lds, s, k.ax[1].dev(), k.ny, k.nz, k.nxh,
-                       k.nxh_pad, k.tx_y, k.spec, k.spec_plane(), inv ? 1 : 0);
+                       k.nxh_pad, k.tx_y, k.spec, k.spec_plane(), inv ? 1 : 0, ra);
   }
 #undef AWK_YR
   AWK_LAUNCH_CHECK("k_ypass");
@@ -782,13 +816,17 @@ awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblock
   const unsigned grid = grid_for((long long)k.ny * k.nz, k.rt);
   const size_t lds = (size_t)k.nx * (3 * k.rt / 2 + 1) * sizeof(float2);
   const Axis ax = k.ax[0].dev();
-#define AWK_XI(NC, RTC, BDC)                                                                     \
-  hipLaunchKernelGGL((k_xinv<NC, RTC, BDC>), dim3(grid), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \
+#define AWK_XI(NC, RTC, BDC, XU)                                                                 \
+  hipLaunchKernelGGL((k_xinv<NC, RTC, BDC, XU>), dim3(grid), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \
                      k.nx, k.ny, k.nz, k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane())
-  if (k.nx == 256 && k.rt == 8) AWK_XI(256, 8, 256);
-  else if (k.nx == 128 && k.rt == 8) AWK_XI(128, 8, 256);
-  else if (k.nx == 64 && k.rt == 8) AWK_XI(64, 8, 256);
-  else AWK_XI(0, 0, 0);
+  const int xu = tuning().xinv_unroll;
+  if (k.nx == 256 && k.rt == 8) {
+    if (xu == 4) AWK_XI(256, 8, 256, 4);
+    else if (xu == 2) AWK_XI(256, 8, 256, 2);
+    else AWK_XI(256, 8, 256, 1);
+  } else if (k.nx == 128 && k.rt == 8) AWK_XI(128, 8, 256, 2);
+  else if (k.nx == 64 && k.rt == 8) AWK_XI(64, 8, 256, 2);
+  else AWK_XI(0, 0, 0, 1);
 #undef AWK_XI
   AWK_LAUNCH_CHECK("k_xinv");
   if (nblocks) *nblocks = (int)grid;
@@ -864,9 +902,9 @@ awk_status awk_fluid_kernel_apply(awk_fk_t h, float *x, float *y, float *z, int 
   fa.in[2] = z;
   awk_status st;
   if ((st = fk_xfwd(k, fa, s, nullptr))) return st;
-  if ((st = fk_ypass(k, false, s))) return st;
+  if ((st = fk_ypass(k, false, s, nullptr))) return st;
   if ((st = fk_zsolve(k, inverse != 0, s))) return st;
-  if ((st = fk_ypass(k, true, s))) return st;
+  if ((st = fk_ypass(k, true, s, nullptr))) return st;
   XInvArgs ia;
   std::memset(&ia, 0, sizeof(ia));
   ia.mode = XSTORE_PLAIN;

@@ -2,12 +2,13 @@
 //
 // GreedyIteratorGPU::Iterate (Algorithms/Greedy/GreedyIteratorGPU.cxx:92-135) runs
 // about fifteen kernels and two host syncs per iteration.  Here one iteration is
-// six kernels and no host sync:
+// five kernels and no host sync:
 //   1. apply          Idef = I0(h)                                  (gather)
-//   2. X  (force)     f = (I1 - Idef) grad(Idef), FP64 SSD partials, r2c along x
-//   3. energy         fixed-order FP64 sum of the partials (+ AutoStepReduce)
-//   4. Y, Z+solve, Y' the Navier solve in the spectrum
-//   5. X' (compose)   c2r along x and h' = h(x + step v / sp) in the epilogue
+//   2. X  (force)     f = (I1 - Idef) grad(Idef), r2c along x, FP64 SSD partials
+//   3. Y, Z+solve, Y' the Navier solve in the spectrum; block 0 of Y also sums
+//                     the SSD partials in index order (deterministic) and applies
+//                     AutoStepReduce
+//   4. X' (compose)   c2r along x and h' = h(x + step v / sp) in the epilogue
 // On the first iteration of a scale level (step not yet known) step 5 writes v
 // and a per-block max |v|; the step MaxPert / max|v| is then formed on the device
 // and a separate composeHV applies it.  The same unfused tail serves the inverse
@@ -21,30 +22,6 @@
 #include "awk_ops.h"
 
 namespace awk {
-
-// Fixed-order FP64 reduction of per-block partial SSD sums (one block).
-__global__ void __launch_bounds__(256)
-k_energy(const double *__restrict__ partials, int n, double scale, double *__restrict__ out,
-         double *__restrict__ prev, int *__restrict__ have_prev, float *__restrict__ step,
-         int auto_reduce) {
-  __shared__ double s[256];
-  double a = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) a += partials[i];
-  s[threadIdx.x] = a;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const double e = scale * s[0];
-    if (out) *out = e;
-    // GreedyDeformationData::AddEnergy (GreedyDeformationData.cxx:162-184)
-    if (auto_reduce && *have_prev && e - *prev > 0.0) *step = *step * 0.5f;
-    *prev = e;
-    *have_prev = 1;
-  }
-}
 
 // step = MaxPert / max |v| (GreedyIteratorCPU.cxx:65-79, 129-134)
 __global__ void __launch_bounds__(256)
@@ -214,7 +191,7 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
   g->mark(0, s);
   // 1. deformed image (VAL background 0, HField3DUtils::apply defaults)
   if ((st = launch_apply(g->idef, I0, h_in[0], h_in[1], h_in[2], nx, ny, nz, AWK_BG_VAL, 0.f, s))) return st;
-  // 2. force + forward x pass
+  g->mark(1, s);
   XFwdArgs fa;
   std::memset(&fa, 0, sizeof(fa));
   fa.mode = XLOAD_FORCE;
@@ -223,21 +200,24 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
   for (int c = 0; c < 3; ++c) fa.sp[c] = g->sp[c];
   fa.partials = g->partials;
   int nb = 0;
-  g->mark(1, s);
   if ((st = fk_xfwd(g->k, fa, s, &nb))) return st;
   g->mark(2, s);
-  // 3. energy (ImageUtils::l2NormSqr = prod(spacing) * sum r^2, ImageUtils.txx:55-58)
-  const double scale = (double)g->sp[0] * g->sp[1] * g->sp[2];
-  hipLaunchKernelGGL(k_energy, dim3(1), dim3(256), 0, s, g->partials, nb, scale, energy_dev, g->prev,
-                     g->have_prev, g->step, g->auto_reduce);
-  AWK_LAUNCH_CHECK("k_energy");
   g->mark(3, s);
-  // 4. solve
-  if ((st = fk_ypass(g->k, false, s))) return st;
+  // 3. solve; block 0 of the forward y pass also reduces the energy
+  ReduceArgs ra;
+  ra.partials = g->partials;
+  ra.n = (ny * nz + g->k.rt - 1) / g->k.rt;
+  ra.scale = (double)g->sp[0] * g->sp[1] * g->sp[2];  // ImageUtils::l2NormSqr (ImageUtils.txx:55-58)
+  ra.energy_out = energy_dev;
+  ra.prev = g->prev;
+  ra.have_prev = g->have_prev;
+  ra.step = g->step;
+  ra.auto_reduce = g->auto_reduce;
+  if ((st = fk_ypass(g->k, false, s, &ra))) return st;
   g->mark(4, s);
   if ((st = fk_zsolve(g->k, true, s))) return st;
   g->mark(5, s);
-  if ((st = fk_ypass(g->k, true, s))) return st;
+  if ((st = fk_ypass(g->k, true, s, nullptr))) return st;
   g->mark(6, s);
   // 5. velocity -> hfield update
   const bool in_place = (h_out[0] == h_in[0]);

@@ -28,6 +28,11 @@ awk_status hip_check(hipError_t e, const char *what) {
 
 constexpr int kBlock = 256;
 
+Tuning &tuning() {
+  static Tuning t;
+  return t;
+}
+
 static awk_status check_dims(int nx, int ny, int nz, const char *who) {
   if (nx < 1 || ny < 1 || nz < 1) return fail(AWK_ERR_INVALID, std::string(who) + ": sizes must be >= 1");
   if ((long long)nx * ny * nz >= (1LL << 31)) return fail(AWK_ERR_INVALID, std::string(who) + ": volume too large");
@@ -81,55 +86,84 @@ __global__ void k_inv0(float *__restrict__ ox, float *__restrict__ oy, float *__
 }
 
 // ------------------------------------------------------------ apply
-template <int BG>
+// VPT voxels per thread (strided by the block size, so every load instruction
+// stays coalesced): all h loads are issued before the gathers, raising the bytes
+// in flight per wave.
+template <int BG, int VPT>
 __global__ void __launch_bounds__(kBlock)
 k_apply(float *__restrict__ out, const float *__restrict__ img, const float *__restrict__ hx,
         const float *__restrict__ hy, const float *__restrict__ hz, int nx, int ny, int nz,
         float background) {
   const int N = nx * ny * nz;
-  const int i = (int)(xcd_block() * blockDim.x + threadIdx.x);
-  if (i >= N) return;
+  const int i0 = (int)(xcd_block() * (kBlock * VPT) + threadIdx.x);
   const Dims d{nx, ny, nz};
-  out[i] = trilerp1<BG>(img, d, hx[i], hy[i], hz[i], background);
+  float px[VPT], py[VPT], pz[VPT];
+#pragma unroll
+  for (int v = 0; v < VPT; ++v) {
+    const int i = i0 + v * kBlock;
+    if (i < N) { px[v] = hx[i]; py[v] = hy[i]; pz[v] = hz[i]; }
+  }
+  float r[VPT];
+#pragma unroll
+  for (int v = 0; v < VPT; ++v)
+    if (i0 + v * kBlock < N) r[v] = trilerp1<BG>(img, d, px[v], py[v], pz[v], background);
+#pragma unroll
+  for (int v = 0; v < VPT; ++v) {
+    const int i = i0 + v * kBlock;
+    if (i < N) out[i] = r[v];
+  }
 }
 
 // ------------------------------------------------------------ compose family
 // op 0: h = f(g(x))
 // op 1: h = g(x + s*delta*v(x)/sp)                 (composeHV / composeHVInv)
 // op 2: h = g(x) + s*v(g(x))/sp                    (composeVH / composeVHInv)
-template <int BG, int OP>
+template <int BG, int OP, int VPT>
 __global__ void __launch_bounds__(kBlock)
 k_compose(float *__restrict__ ox, float *__restrict__ oy, float *__restrict__ oz,
           const float *__restrict__ fx, const float *__restrict__ fy, const float *__restrict__ fz,
           const float *__restrict__ gx, const float *__restrict__ gy, const float *__restrict__ gz,
           int nx, int ny, int nz, float spx, float spy, float spz, float delta_h,
           const float *__restrict__ delta_dev, float sgn) {
-  int i, x, y, z;
-  if (!voxel(nx * ny * nz, nx, ny, i, x, y, z)) return;
+  const int N = nx * ny * nz;
+  const int i0 = (int)(xcd_block() * (kBlock * VPT) + threadIdx.x);
   const Dims d{nx, ny, nz};
   const float delta = delta_dev ? *delta_dev : delta_h;
-  if (OP == 0) {
-    const float3 r = trilerp3<BG>(fx, fy, fz, d, gx[i], gy[i], gz[i]);
-    ox[i] = r.x; oy[i] = r.y; oz[i] = r.z;
-  } else if (OP == 1) {
-    // f = g (field gathered), (gx,gy,gz) = v.  The reference scales v by the step
-    // first (GreedyIteratorCPU.cxx:84), then divides by the spacing.
-    const float vx = gx[i] * delta, vy = gy[i] * delta, vz = gz[i] * delta;
-    const float px = (float)x + sgn * (vx / spx);
-    const float py = (float)y + sgn * (vy / spy);
-    const float pz = (float)z + sgn * (vz / spz);
-    const float3 r = trilerp3<BG>(fx, fy, fz, d, px, py, pz);
-    ox[i] = r.x; oy[i] = r.y; oz[i] = r.z;
-  } else {
-    // f = v (field gathered), (gx,gy,gz) = g
-    const float ax = gx[i], ay = gy[i], az = gz[i];
-    float3 r = trilerp3<BG>(fx, fy, fz, d, ax, ay, az);
-    if (delta_dev || delta_h != 1.f) {  // v scaled by the step (greedy inverse update)
-      r.x *= delta; r.y *= delta; r.z *= delta;
+  float ax[VPT], ay[VPT], az[VPT];
+#pragma unroll
+  for (int v = 0; v < VPT; ++v) {
+    const int i = i0 + v * kBlock;
+    if (i < N) { ax[v] = gx[i]; ay[v] = gy[i]; az[v] = gz[i]; }
+  }
+  float3 r[VPT];
+#pragma unroll
+  for (int v = 0; v < VPT; ++v) {
+    const int i = i0 + v * kBlock;
+    if (i >= N) continue;
+    if (OP == 0) {
+      r[v] = trilerp3<BG>(fx, fy, fz, d, ax[v], ay[v], az[v]);
+    } else if (OP == 1) {
+      // f = g (field gathered), (gx,gy,gz) = v.  The reference scales v by the step
+      // first (GreedyIteratorCPU.cxx:84), then divides by the spacing.
+      const int x = i % nx, rr = i / nx, y = rr % ny, z = rr / ny;
+      const float vx = ax[v] * delta, vy = ay[v] * delta, vz = az[v] * delta;
+      const float px = (float)x + sgn * (vx / spx);
+      const float py = (float)y + sgn * (vy / spy);
+      const float pz = (float)z + sgn * (vz / spz);
+      r[v] = trilerp3<BG>(fx, fy, fz, d, px, py, pz);
+    } else {
+      // f = v (field gathered), (gx,gy,gz) = g
+      float3 q = trilerp3<BG>(fx, fy, fz, d, ax[v], ay[v], az[v]);
+      if (delta_dev || delta_h != 1.f) {  // v scaled by the step (greedy inverse update)
+        q.x *= delta; q.y *= delta; q.z *= delta;
+      }
+      r[v] = make_float3(q.x / (sgn * spx) + ax[v], q.y / (sgn * spy) + ay[v], q.z / (sgn * spz) + az[v]);
     }
-    ox[i] = r.x / (sgn * spx) + ax;
-    oy[i] = r.y / (sgn * spy) + ay;
-    oz[i] = r.z / (sgn * spz) + az;
+  }
+#pragma unroll
+  for (int v = 0; v < VPT; ++v) {
+    const int i = i0 + v * kBlock;
+    if (i < N) { ox[i] = r[v].x; oy[i] = r[v].y; oz[i] = r[v].z; }
   }
 }
 
@@ -158,50 +192,74 @@ k_gradient(float *__restrict__ gx, float *__restrict__ gy, float *__restrict__ g
   gz[i] = comp(z, nz, s2, spz);
 }
 
-template <int OP>
-static awk_status launch_compose(float *ox, float *oy, float *oz, const float *fx, const float *fy,
-                                 const float *fz, const float *gx, const float *gy,
-                                 const float *gz, int nx, int ny, int nz, float spx, float spy,
-                                 float spz, float delta, const float *delta_dev, float sgn, int bg,
-                                 hipStream_t s) {
-  const unsigned grid = grid_for((long long)nx * ny * nz, kBlock);
+template <int OP, int VPT>
+static void launch_compose_v(unsigned grid, float *ox, float *oy, float *oz, const float *fx,
+                             const float *fy, const float *fz, const float *gx, const float *gy,
+                             const float *gz, int nx, int ny, int nz, float spx, float spy,
+                             float spz, float delta, const float *delta_dev, float sgn, int bg,
+                             hipStream_t s) {
 #define AWK_CMP(B)                                                                            \
-  hipLaunchKernelGGL((k_compose<B, OP>), dim3(grid), dim3(kBlock), 0, s, ox, oy, oz, fx, fy, fz, \
-                     gx, gy, gz, nx, ny, nz, spx, spy, spz, delta, delta_dev, sgn)
+  hipLaunchKernelGGL((k_compose<B, OP, VPT>), dim3(grid), dim3(kBlock), 0, s, ox, oy, oz, fx, fy, \
+                     fz, gx, gy, gz, nx, ny, nz, spx, spy, spz, delta, delta_dev, sgn)
   switch (bg) {
     case AWK_BG_ID: AWK_CMP(AWK_BG_ID); break;
     case AWK_BG_ZERO: AWK_CMP(AWK_BG_ZERO); break;
     case AWK_BG_PARTIAL_ID: AWK_CMP(AWK_BG_PARTIAL_ID); break;
     case AWK_BG_PARTIAL_ZERO: AWK_CMP(AWK_BG_PARTIAL_ZERO); break;
     case AWK_BG_CLAMP: AWK_CMP(AWK_BG_CLAMP); break;
-    case AWK_BG_WRAP: AWK_CMP(AWK_BG_WRAP); break;
-    default: return fail(AWK_ERR_INVALID, "compose: invalid background strategy");
+    default: AWK_CMP(AWK_BG_WRAP); break;
   }
 #undef AWK_CMP
+}
+
+template <int OP>
+static awk_status launch_compose(float *ox, float *oy, float *oz, const float *fx, const float *fy,
+                                 const float *fz, const float *gx, const float *gy,
+                                 const float *gz, int nx, int ny, int nz, float spx, float spy,
+                                 float spz, float delta, const float *delta_dev, float sgn, int bg,
+                                 hipStream_t s) {
+  if (bg < AWK_BG_ID || bg > AWK_BG_WRAP) return fail(AWK_ERR_INVALID, "compose: invalid background strategy");
+  const long long N = (long long)nx * ny * nz;
+  const int vpt = tuning().compose_vpt;
+  if (vpt == 4)
+    launch_compose_v<OP, 4>(grid_for(N, kBlock * 4), ox, oy, oz, fx, fy, fz, gx, gy, gz, nx, ny, nz,
+                            spx, spy, spz, delta, delta_dev, sgn, bg, s);
+  else if (vpt == 2)
+    launch_compose_v<OP, 2>(grid_for(N, kBlock * 2), ox, oy, oz, fx, fy, fz, gx, gy, gz, nx, ny, nz,
+                            spx, spy, spz, delta, delta_dev, sgn, bg, s);
+  else
+    launch_compose_v<OP, 1>(grid_for(N, kBlock), ox, oy, oz, fx, fy, fz, gx, gy, gz, nx, ny, nz,
+                            spx, spy, spz, delta, delta_dev, sgn, bg, s);
   AWK_LAUNCH_CHECK("k_compose");
   return AWK_OK;
+}
+
+template <int VPT>
+static void launch_apply_v(float *out, const float *img, const float *hx, const float *hy,
+                           const float *hz, int nx, int ny, int nz, int bg, float background,
+                           hipStream_t s) {
+  const unsigned grid = grid_for((long long)nx * ny * nz, kBlock * VPT);
+#define AWK_AP(B)                                                                            \
+  hipLaunchKernelGGL((k_apply<B, VPT>), dim3(grid), dim3(kBlock), 0, s, out, img, hx, hy, hz, nx, \
+                     ny, nz, background)
+  switch (bg) {
+    case AWK_BG_VAL: AWK_AP(AWK_BG_VAL); break;
+    case AWK_BG_ZERO: AWK_AP(AWK_BG_ZERO); break;
+    case AWK_BG_CLAMP: AWK_AP(AWK_BG_CLAMP); break;
+    default: AWK_AP(AWK_BG_WRAP); break;
+  }
+#undef AWK_AP
 }
 
 awk_status launch_apply(float *out, const float *img, const float *hx, const float *hy,
                         const float *hz, int nx, int ny, int nz, int bg, float background,
                         hipStream_t s) {
-  const unsigned grid = grid_for((long long)nx * ny * nz, kBlock);
-  switch (bg) {
-    case AWK_BG_VAL:
-      hipLaunchKernelGGL(k_apply<AWK_BG_VAL>, dim3(grid), dim3(kBlock), 0, s, out, img, hx, hy, hz, nx, ny, nz, background);
-      break;
-    case AWK_BG_ZERO:
-      hipLaunchKernelGGL(k_apply<AWK_BG_ZERO>, dim3(grid), dim3(kBlock), 0, s, out, img, hx, hy, hz, nx, ny, nz, background);
-      break;
-    case AWK_BG_CLAMP:
-      hipLaunchKernelGGL(k_apply<AWK_BG_CLAMP>, dim3(grid), dim3(kBlock), 0, s, out, img, hx, hy, hz, nx, ny, nz, background);
-      break;
-    case AWK_BG_WRAP:
-      hipLaunchKernelGGL(k_apply<AWK_BG_WRAP>, dim3(grid), dim3(kBlock), 0, s, out, img, hx, hy, hz, nx, ny, nz, background);
-      break;
-    default:
-      return fail(AWK_ERR_INVALID, "apply: background must be VAL, ZERO, CLAMP or WRAP");
-  }
+  if (bg != AWK_BG_VAL && bg != AWK_BG_ZERO && bg != AWK_BG_CLAMP && bg != AWK_BG_WRAP)
+    return fail(AWK_ERR_INVALID, "apply: background must be VAL, ZERO, CLAMP or WRAP");
+  const int vpt = tuning().apply_vpt;
+  if (vpt == 4) launch_apply_v<4>(out, img, hx, hy, hz, nx, ny, nz, bg, background, s);
+  else if (vpt == 2) launch_apply_v<2>(out, img, hx, hy, hz, nx, ny, nz, bg, background, s);
+  else launch_apply_v<1>(out, img, hx, hy, hz, nx, ny, nz, bg, background, s);
   AWK_LAUNCH_CHECK("k_apply");
   return AWK_OK;
 }
@@ -231,6 +289,31 @@ extern "C" {
 const char *awk_last_error(void) { return g_err.c_str(); }
 
 int awk_version(void) { return 100; }
+
+awk_status awk_tune_set(const char *name, int value) {
+  if (!name) return fail(AWK_ERR_INVALID, "awk_tune_set: null name");
+  const std::string n(name);
+  Tuning &t = tuning();
+  if (n == "apply_vpt" || n == "compose_vpt" || n == "xinv_unroll") {
+    if (value != 1 && value != 2 && value != 4) return fail(AWK_ERR_INVALID, "awk_tune_set: value must be 1, 2 or 4");
+    if (n == "apply_vpt") t.apply_vpt = value;
+    else if (n == "compose_vpt") t.compose_vpt = value;
+    else t.xinv_unroll = value;
+    return AWK_OK;
+  }
+  return fail(AWK_ERR_INVALID, "awk_tune_set: unknown knob " + n);
+}
+
+awk_status awk_tune_get(const char *name, int *value) {
+  if (!name || !value) return fail(AWK_ERR_INVALID, "awk_tune_get: null argument");
+  const std::string n(name);
+  const Tuning &t = tuning();
+  if (n == "apply_vpt") *value = t.apply_vpt;
+  else if (n == "compose_vpt") *value = t.compose_vpt;
+  else if (n == "xinv_unroll") *value = t.xinv_unroll;
+  else return fail(AWK_ERR_INVALID, "awk_tune_get: unknown knob " + n);
+  return AWK_OK;
+}
 
 awk_status awk_hfield_set_to_identity(float *hx, float *hy, float *hz, int nx, int ny, int nz,
                                       awk_stream_t stream) {

@@ -60,6 +60,7 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-microbench", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-ref", action="store_true",
                     help="also run a plain 1 GiB device copy (counter scaling reference)")
     args = ap.parse_args()
@@ -93,6 +94,12 @@ def main():
     torch.cuda.synchronize()
     gen_s = time.time() - t0
 
+    tuning = None
+    if not args.no_autotune:
+        from atlaswerks_amd import autotune
+        tuning, tune_ms = autotune.tune((nx, ny, nz), reps=5)
+        tuning = {"chosen": tuning, "ms": {k: {str(a): round(b, 4) for a, b in v.items()}
+                                          for k, v in tune_ms.items()}}
     param = GreedyIteratorParam(max_pert=0.1, alpha=0.01, beta=0.01, gamma=0.001)
     it = GreedyIteratorGPU((nx, ny, nz), (1.0, 1.0, 1.0), param)
     data = GreedyDeformationDataGPU(I0, I1, max_iters=args.warmup + args.steps + 1)
@@ -159,6 +166,7 @@ def main():
         "stages_gbs": {k: round(sb[k] / (v * 1e-3) / 1e9, 1) for k, v in stage_avg.items() if sb[k]},
         "energy_first_last": [float(energies[0]), float(energies[-1])],
         "input_gen_s": round(gen_s, 2),
+        "tuning": tuning,
     }
 
     # ---- kernel microbenchmarks of the 256^3 workload (config-1 operations)

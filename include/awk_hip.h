@@ -48,6 +48,13 @@ enum {
   AWK_BG_VAL = 6
 };
 
+/* Launch-shape knobs chosen by the autotuner (atlaswerks_amd/autotune.py):
+ * "apply_vpt", "compose_vpt" (voxels per thread: 1, 2, 4), "xinv_unroll"
+ * (voxels per step of the fused inverse-x epilogue: 1, 2, 4).  Process-wide;
+ * results do not depend on them.  No counterpart in the reference. */
+awk_status awk_tune_set(const char *name, int value);
+awk_status awk_tune_get(const char *name, int *value);
+
 /* Last error message of the calling host thread ("" if none). */
 const char *awk_last_error(void);
 /* Library version, major*10000 + minor*100 + patch. */
@@ -205,7 +212,8 @@ awk_status awk_greedy_set_auto_step_reduce(awk_greedy_t g, int on);
 
 /* Per-stage HIP event timing of awk_greedy_iterate, for benchmarks: events are
  * recorded on the iteration's stream between its stages.  Stages:
- *   0 apply, 1 x pass + force + SSD partials, 2 energy reduction, 3 y pass,
+ *   0 apply, 1 x pass + force + SSD (its last block reduces the energy),
+ *   2 (empty, kept for a stable layout), 3 y pass,
  *   4 z pass + solve, 5 inverse y pass, 6 inverse x pass + hfield update.
  * No counterpart in the reference (its GreedyIteratorGPU has no timers). */
 #define AWK_GREEDY_NSTAGES 7
