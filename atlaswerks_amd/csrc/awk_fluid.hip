@@ -101,6 +101,71 @@ k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
   if (r0 >= NR) return;
   double ssd = 0.0;
   float *smf = reinterpret_cast<float *>(sm);
+  auto put = [&](int q, int x, float f0, float f1, float f2) {
+    const int p = q >> 1;
+    if ((q & 1) == 0) {
+      sm[x * LS + 3 * p] = make_float2(f0, f1);
+      smf[2 * (x * LS + 3 * p + 1)] = f2;
+    } else {
+      sm[x * LS + 3 * p + 2] = make_float2(f0, f1);
+      smf[2 * (x * LS + 3 * p + 1) + 1] = f2;
+    }
+  };
+  bool done = false;
+  if constexpr (NC != 0 && NC == BDC) {
+    // Column mode: thread x owns column x of the block's RT rows, which are RT
+    // consecutive y at one z.  Each Idef value is loaded once: the column with a
+    // one-row halo in y, the two neighbouring z rows, and x +- 1 from the
+    // neighbouring lanes (a load only at the wave edges).
+    if (a.mode == XLOAD_FORCE && ny % RTC == 0) {
+      done = true;
+      const int x = threadIdx.x;
+      const int z = r0 / ny, y0 = r0 - z * ny;
+      const size_t sxy = (size_t)NC * ny;
+      const float *I = a.idef;
+      const float *col0 = I + (size_t)z * sxy + (size_t)y0 * NC + x;
+      float col[RTC + 2], zm[RTC], zp[RTC], i1v[RTC], lft[RTC], rgt[RTC];
+#pragma unroll
+      for (int k = 0; k < RTC + 2; ++k) {
+        const int yy = y0 - 1 + k;
+        col[k] = (yy >= 0 && yy < ny) ? col0[(ptrdiff_t)(k - 1) * NC] : 0.f;
+      }
+      const int lane = threadIdx.x & 63;
+#pragma unroll
+      for (int q = 0; q < RTC; ++q) {
+        zm[q] = (z > 0) ? col0[(ptrdiff_t)q * NC - (ptrdiff_t)sxy] : 0.f;
+        zp[q] = (z < nz - 1) ? col0[(ptrdiff_t)q * NC + (ptrdiff_t)sxy] : 0.f;
+        i1v[q] = a.i1[(size_t)z * sxy + (size_t)(y0 + q) * NC + x];
+        lft[q] = (lane == 0 && x > 0) ? col0[(ptrdiff_t)q * NC - 1] : 0.f;
+        rgt[q] = (lane == 63 && x < NC - 1) ? col0[(ptrdiff_t)q * NC + 1] : 0.f;
+      }
+#pragma unroll
+      for (int q = 0; q < RTC; ++q) {
+        const float c = col[q + 1];
+        const float up = __shfl_up(c, 1, 64), dn = __shfl_down(c, 1, 64);
+        const float cl = (lane == 0) ? lft[q] : up;   // value at x - 1
+        const float cr = (lane == 63) ? rgt[q] : dn;  // value at x + 1
+        const int y = y0 + q;
+        // Array3DUtils::computeGradient (Array3DUtils.txx:783-901), wrap = false
+        float gx, gy, gz;
+        if (x > 0 && x < NC - 1) gx = (cr - cl) / (2.f * a.sp[0]);
+        else if (x == 0) gx = (cr - c) / a.sp[0];
+        else gx = (c - cl) / a.sp[0];
+        if (ny == 1) gy = 0.f;
+        else if (y > 0 && y < ny - 1) gy = (col[q + 2] - col[q]) / (2.f * a.sp[1]);
+        else if (y == 0) gy = (col[q + 2] - c) / a.sp[1];
+        else gy = (c - col[q]) / a.sp[1];
+        if (nz == 1) gz = 0.f;
+        else if (z > 0 && z < nz - 1) gz = (zp[q] - zm[q]) / (2.f * a.sp[2]);
+        else if (z == 0) gz = (zp[q] - c) / a.sp[2];
+        else gz = (c - zm[q]) / a.sp[2];
+        const float r = i1v[q] - c;
+        ssd += (double)r * (double)r;
+        put(q, x, gx * r, gy * r, gz * r);
+      }
+    }
+  }
+  if (!done) {
 #pragma unroll
   for (int w = threadIdx.x; w < RT * nx; w += bdim) {
     const int q = w / nx, x = w - q * nx;
@@ -143,14 +208,8 @@ k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
         f2 = gz * r;
       }
     }
-    const int p = q >> 1;
-    if ((q & 1) == 0) {
-      sm[x * LS + 3 * p] = make_float2(f0, f1);
-      smf[2 * (x * LS + 3 * p + 1)] = f2;
-    } else {
-      sm[x * LS + 3 * p + 2] = make_float2(f0, f1);
-      smf[2 * (x * LS + 3 * p + 1) + 1] = f2;
-    }
+    put(q, x, f0, f1, f2);
+  }
   }
   __syncthreads();
   if constexpr (NC != 0) ct_fft<NC, 3 * RTC / 2, 3 * RTC / 2 + 1, BDC>(sm, ax.tw, false);
