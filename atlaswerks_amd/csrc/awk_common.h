@@ -6,6 +6,7 @@
 //     as the reference cplVector3DArray, CUDACommon/src/cudaVector3DArray.cu:46-57).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -33,6 +34,23 @@ awk_status hip_check(hipError_t e, const char *what);
   } while (0)
 
 inline hipStream_t as_stream(awk_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Kernel-attached timing events: while set, launches go through
+// hipExtLaunchKernelGGL with these start/stop events, which the dispatch itself
+// timestamps (kernel duration only, the figure rocprof reports).
+struct LaunchEvents {
+  hipEvent_t start, stop;
+};
+LaunchEvents *&launch_events();
+
+#define AWK_LAUNCH(K, G, B, L, S, ...)                                                       \
+  do {                                                                                     \
+    if (::awk::launch_events())                                                            \
+      hipExtLaunchKernelGGL(K, G, B, L, S, ::awk::launch_events()->start,                  \
+                            ::awk::launch_events()->stop, 0, __VA_ARGS__);                 \
+    else                                                                                   \
+      hipLaunchKernelGGL(K, G, B, L, S, __VA_ARGS__);                                      \
+  } while (0)
 
 // Launch-shape knobs (set by the autotuner through awk_tune_set).
 struct Tuning {

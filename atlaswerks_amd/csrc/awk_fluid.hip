@@ -774,7 +774,7 @@ awk_status fk_prepare_eigen(FluidKernel &k, int dir, hipStream_t s) {
   }
   const double p = dir > 0 ? (double)k.lpow : -(double)k.lpow;
   unsigned grid = (unsigned)std::min<size_t>((nf + 255) / 256, 65536);
-  hipLaunchKernelGGL(k_eigen_table, dim3(grid), dim3(256), 0, s, k.eig, k.nx, k.ny, k.nz, k.nxh,
+  AWK_LAUNCH(k_eigen_table, dim3(grid), dim3(256), 0, s, k.eig, k.nx, k.ny, k.nz, k.nxh,
                      (double)k.sp[0], (double)k.sp[1], (double)k.sp[2], (double)k.alpha,
                      (double)k.beta, (double)k.gamma, p);
   AWK_LAUNCH_CHECK("k_eigen_table");
@@ -791,7 +791,7 @@ awk_status fk_xfwd(FluidKernel &k, const XFwdArgs &a, hipStream_t s, int *nblock
   const size_t lds = (size_t)k.nx * (3 * k.rt / 2 + 1) * sizeof(float2);
   const Axis ax = k.ax[0].dev();
 #define AWK_XF(NC, RTC, BDC)                                                                     \
-  hipLaunchKernelGGL((k_xfwd<NC, RTC, BDC>), dim3(grid), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \
+  AWK_LAUNCH((k_xfwd<NC, RTC, BDC>), dim3(grid), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \
                      k.nx, k.ny, k.nz, k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane())
   if (k.nx == 256 && k.rt == 8) AWK_XF(256, 8, 256);
   else if (k.nx == 128 && k.rt == 8) AWK_XF(128, 8, 256);
@@ -811,7 +811,7 @@ awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s, const ReduceArgs *r
   const int ntile16 = (k.nxh + 15) / 16;
   const unsigned g16 = grid_for((long long)3 * k.nz * ntile16, 1);
 #define AWK_YR(N, RA, RB)                                                                     \
-  hipLaunchKernelGGL((k_ypass_reg<N, RA, RB, 16>), dim3(g16), dim3(16 * (N / RA > N / RB ? N / RA : N / RB)), \
+  AWK_LAUNCH((k_ypass_reg<N, RA, RB, 16>), dim3(g16), dim3(16 * (N / RA > N / RB ? N / RA : N / RB)), \
                      0, s, tw, k.nz, k.nxh, k.nxh_pad, k.spec, k.spec_plane(), inv ? 1 : 0, ra)
   if (k.ny == 256) AWK_YR(256, 16, 16);
   else if (k.ny == 128) AWK_YR(128, 16, 8);
@@ -821,7 +821,7 @@ awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s, const ReduceArgs *r
     const int ntile = (k.nxh + k.tx_y - 1) / k.tx_y;
     const unsigned grid = grid_for((long long)3 * k.nz * ntile, 1);
     const size_t lds = std::max((size_t)k.ny * k.tx_y * sizeof(float2), (size_t)k.bd_y * sizeof(double));
-    hipLaunchKernelGGL(k_ypass, dim3(grid), dim3(k.bd_y), lds, s, k.ax[1].dev(), k.ny, k.nz, k.nxh,
+    AWK_LAUNCH(k_ypass, dim3(grid), dim3(k.bd_y), lds, s, k.ax[1].dev(), k.ny, k.nz, k.nxh,
                        k.nxh_pad, k.tx_y, k.spec, k.spec_plane(), inv ? 1 : 0, ra);
   }
 #undef AWK_YR
@@ -853,7 +853,7 @@ awk_status fk_zsolve(FluidKernel &k, bool inverse_op, hipStream_t s) {
   const int ntile16 = (k.nxh + 15) / 16;
   const unsigned g16 = grid_for((long long)k.ny * ntile16, 1);
 #define AWK_ZR(N, RA, RB)                                                                     \
-  hipLaunchKernelGGL((k_zsolve_reg<N, RA, RB, 16>), dim3(g16), dim3(16 * (N / RA > N / RB ? N / RA : N / RB)), \
+  AWK_LAUNCH((k_zsolve_reg<N, RA, RB, 16>), dim3(g16), dim3(16 * (N / RA > N / RB ? N / RA : N / RB)), \
                      0, s, tw, sa, k.ny, k.nxh, k.nxh_pad, k.spec, k.spec_plane())
   if (k.nz == 256) AWK_ZR(256, 16, 16);
   else if (k.nz == 128) AWK_ZR(128, 16, 8);
@@ -863,7 +863,7 @@ awk_status fk_zsolve(FluidKernel &k, bool inverse_op, hipStream_t s) {
     const int ntile = (k.nxh + k.tx_z - 1) / k.tx_z;
     const unsigned grid = grid_for((long long)k.ny * ntile, 1);
     const size_t lds = (size_t)k.nz * 3 * k.tx_z * sizeof(float2);
-    hipLaunchKernelGGL(k_zsolve, dim3(grid), dim3(k.bd_z), lds, s, k.ax[2].dev(), sa, k.ny, k.nz,
+    AWK_LAUNCH(k_zsolve, dim3(grid), dim3(k.bd_z), lds, s, k.ax[2].dev(), sa, k.ny, k.nz,
                        k.nxh, k.nxh_pad, k.tx_z, k.spec, k.spec_plane());
   }
 #undef AWK_ZR
@@ -876,7 +876,7 @@ awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblock
   const size_t lds = (size_t)k.nx * (3 * k.rt / 2 + 1) * sizeof(float2);
   const Axis ax = k.ax[0].dev();
 #define AWK_XI(NC, RTC, BDC, XU)                                                                 \
-  hipLaunchKernelGGL((k_xinv<NC, RTC, BDC, XU>), dim3(grid), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \
+  AWK_LAUNCH((k_xinv<NC, RTC, BDC, XU>), dim3(grid), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \
                      k.nx, k.ny, k.nz, k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane())
   const int xu = tuning().xinv_unroll;
   if (k.nx == 256 && k.rt == 8) {

@@ -14,6 +14,7 @@
 // and a separate composeHV applies it.  The same unfused tail serves the inverse
 // hfield update.
 #include <array>
+#include <utility>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -56,12 +57,13 @@ struct awk_greedy {
   double *prev = nullptr;
   int *have_prev = nullptr;
   int nblk = 0;
-  // optional per-stage HIP event timing (awk_greedy_set_profiling)
+  // optional per-kernel timing (awk_greedy_set_profiling): start/stop events
+  // attached to each stage's kernel dispatch
   bool prof = false;
-  std::vector<std::array<hipEvent_t, AWK_GREEDY_NSTAGES + 1>> ev;
+  std::vector<std::array<awk::LaunchEvents, AWK_GREEDY_NSTAGES>> ev;
   int ev_used = 0;
-  void mark(int i, hipStream_t s) {
-    if (prof) (void)hipEventRecord(ev[ev_used][i], s);
+  void stage(int i) {
+    if (prof) awk::launch_events() = &ev[ev_used][i];
   }
 };
 
@@ -69,7 +71,10 @@ using namespace awk;
 
 static void greedy_free(awk_greedy *g) {
   for (auto &set : g->ev)
-    for (hipEvent_t e : set) (void)hipEventDestroy(e);
+    for (LaunchEvents &e : set) {
+      (void)hipEventDestroy(e.start);
+      (void)hipEventDestroy(e.stop);
+    }
   g->ev.clear();
   fk_release(g->k);
   if (g->idef) (void)hipFree(g->idef);
@@ -183,15 +188,22 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
   awk_status st;
   if (g->prof) {
     if (g->ev_used == (int)g->ev.size()) {
-      std::array<hipEvent_t, AWK_GREEDY_NSTAGES + 1> set;
-      for (auto &e : set) AWK_HIP(hipEventCreate(&e));
+      std::array<LaunchEvents, AWK_GREEDY_NSTAGES> set;
+      for (auto &e : set) {
+        AWK_HIP(hipEventCreate(&e.start));
+        AWK_HIP(hipEventCreate(&e.stop));
+      }
       g->ev.push_back(set);
     }
   }
-  g->mark(0, s);
+  // the launches of this call are timed only while launch_events() is set
+  struct Clear {
+    ~Clear() { launch_events() = nullptr; }
+  } clear_events;
+  g->stage(0);
   // 1. deformed image (VAL background 0, HField3DUtils::apply defaults)
   if ((st = launch_apply(g->idef, I0, h_in[0], h_in[1], h_in[2], nx, ny, nz, AWK_BG_VAL, 0.f, s))) return st;
-  g->mark(1, s);
+  g->stage(1);
   XFwdArgs fa;
   std::memset(&fa, 0, sizeof(fa));
   fa.mode = XLOAD_FORCE;
@@ -201,8 +213,7 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
   fa.partials = g->partials;
   int nb = 0;
   if ((st = fk_xfwd(g->k, fa, s, &nb))) return st;
-  g->mark(2, s);
-  g->mark(3, s);
+  g->stage(2);
   // 3. solve; block 0 of the forward y pass also reduces the energy
   ReduceArgs ra;
   ra.partials = g->partials;
@@ -214,11 +225,11 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
   ra.step = g->step;
   ra.auto_reduce = g->auto_reduce;
   if ((st = fk_ypass(g->k, false, s, &ra))) return st;
-  g->mark(4, s);
+  g->stage(3);
   if ((st = fk_zsolve(g->k, true, s))) return st;
-  g->mark(5, s);
+  g->stage(4);
   if ((st = fk_ypass(g->k, true, s, nullptr))) return st;
-  g->mark(6, s);
+  g->stage(5);
   // 5. velocity -> hfield update
   const bool in_place = (h_out[0] == h_in[0]);
   float *dst[3];
@@ -232,7 +243,7 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
     ia.maxpart = g->maxpart;
     if ((st = fk_xinv(g->k, ia, s, &nb))) return st;
     if (g->update_step) {
-      hipLaunchKernelGGL(k_step_from_max, dim3(1), dim3(256), 0, s, g->maxpart, nb, g->max_pert, g->step);
+      AWK_LAUNCH(k_step_from_max, dim3(1), dim3(256), 0, s, g->maxpart, nb, g->max_pert, g->step);
       AWK_LAUNCH_CHECK("k_step_from_max");
       g->update_step = false;
     }
@@ -255,7 +266,6 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
         for (int c = 0; c < 3; ++c)
           AWK_HIP(hipMemcpyAsync(hinv_out[c], g->hbuf + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, s));
     }
-    g->mark(7, s);
     if (g->prof) ++g->ev_used;
     return AWK_OK;
   } else {
@@ -270,7 +280,6 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
   if (in_place)
     for (int c = 0; c < 3; ++c)
       AWK_HIP(hipMemcpyAsync(h_out[c], g->hbuf + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, s));
-  g->mark(7, s);
   if (g->prof) ++g->ev_used;
   return AWK_OK;
 }
@@ -286,10 +295,10 @@ awk_status awk_greedy_get_profile(awk_greedy_t g, double *stage_ms, int *iterati
   if (!g || !stage_ms || !iterations) return fail(AWK_ERR_INVALID, "awk_greedy_get_profile: null argument");
   for (int i = 0; i < AWK_GREEDY_NSTAGES; ++i) stage_ms[i] = 0.0;
   for (int it = 0; it < g->ev_used; ++it) {
-    AWK_HIP(hipEventSynchronize(g->ev[it][AWK_GREEDY_NSTAGES]));
     for (int i = 0; i < AWK_GREEDY_NSTAGES; ++i) {
+      AWK_HIP(hipEventSynchronize(g->ev[it][i].stop));
       float ms = 0.f;
-      AWK_HIP(hipEventElapsedTime(&ms, g->ev[it][i], g->ev[it][i + 1]));
+      AWK_HIP(hipEventElapsedTime(&ms, g->ev[it][i].start, g->ev[it][i].stop));
       stage_ms[i] += ms;
     }
   }

@@ -28,6 +28,11 @@ awk_status hip_check(hipError_t e, const char *what) {
 
 constexpr int kBlock = 256;
 
+LaunchEvents *&launch_events() {
+  static thread_local LaunchEvents *ev = nullptr;
+  return ev;
+}
+
 Tuning &tuning() {
   static Tuning t;
   return t;
@@ -199,7 +204,7 @@ static void launch_compose_v(unsigned grid, float *ox, float *oy, float *oz, con
                              float spz, float delta, const float *delta_dev, float sgn, int bg,
                              hipStream_t s) {
 #define AWK_CMP(B)                                                                            \
-  hipLaunchKernelGGL((k_compose<B, OP, VPT>), dim3(grid), dim3(kBlock), 0, s, ox, oy, oz, fx, fy, \
+  AWK_LAUNCH((k_compose<B, OP, VPT>), dim3(grid), dim3(kBlock), 0, s, ox, oy, oz, fx, fy, \
                      fz, gx, gy, gz, nx, ny, nz, spx, spy, spz, delta, delta_dev, sgn)
   switch (bg) {
     case AWK_BG_ID: AWK_CMP(AWK_BG_ID); break;
@@ -240,7 +245,7 @@ static void launch_apply_v(float *out, const float *img, const float *hx, const 
                            hipStream_t s) {
   const unsigned grid = grid_for((long long)nx * ny * nz, kBlock * VPT);
 #define AWK_AP(B)                                                                            \
-  hipLaunchKernelGGL((k_apply<B, VPT>), dim3(grid), dim3(kBlock), 0, s, out, img, hx, hy, hz, nx, \
+  AWK_LAUNCH((k_apply<B, VPT>), dim3(grid), dim3(kBlock), 0, s, out, img, hx, hy, hz, nx, \
                      ny, nz, background)
   switch (bg) {
     case AWK_BG_VAL: AWK_AP(AWK_BG_VAL); break;
@@ -319,7 +324,7 @@ awk_status awk_hfield_set_to_identity(float *hx, float *hy, float *hz, int nx, i
                                       awk_stream_t stream) {
   if (awk_status st = check_dims(nx, ny, nz, "awk_hfield_set_to_identity")) return st;
   if (!hx || !hy || !hz) return fail(AWK_ERR_INVALID, "awk_hfield_set_to_identity: null pointer");
-  hipLaunchKernelGGL(k_identity, dim3(grid_for((long long)nx * ny * nz, kBlock)), dim3(kBlock), 0,
+  AWK_LAUNCH(k_identity, dim3(grid_for((long long)nx * ny * nz, kBlock)), dim3(kBlock), 0,
                      as_stream(stream), hx, hy, hz, nx, ny, nz);
   AWK_LAUNCH_CHECK("k_identity");
   return AWK_OK;
@@ -329,7 +334,7 @@ awk_status awk_hfield_velocity_to_h(float *x, float *y, float *z, int nx, int ny
                                     float spy, float spz, awk_stream_t stream) {
   if (awk_status st = check_dims(nx, ny, nz, "awk_hfield_velocity_to_h")) return st;
   if (!x || !y || !z) return fail(AWK_ERR_INVALID, "awk_hfield_velocity_to_h: null pointer");
-  hipLaunchKernelGGL(k_hv_convert, dim3(grid_for((long long)nx * ny * nz, kBlock)), dim3(kBlock), 0,
+  AWK_LAUNCH(k_hv_convert, dim3(grid_for((long long)nx * ny * nz, kBlock)), dim3(kBlock), 0,
                      as_stream(stream), x, y, z, nx, ny, nz, spx, spy, spz, 0);
   AWK_LAUNCH_CHECK("k_hv_convert");
   return AWK_OK;
@@ -339,7 +344,7 @@ awk_status awk_hfield_h_to_velocity(float *x, float *y, float *z, int nx, int ny
                                     float spy, float spz, awk_stream_t stream) {
   if (awk_status st = check_dims(nx, ny, nz, "awk_hfield_h_to_velocity")) return st;
   if (!x || !y || !z) return fail(AWK_ERR_INVALID, "awk_hfield_h_to_velocity: null pointer");
-  hipLaunchKernelGGL(k_hv_convert, dim3(grid_for((long long)nx * ny * nz, kBlock)), dim3(kBlock), 0,
+  AWK_LAUNCH(k_hv_convert, dim3(grid_for((long long)nx * ny * nz, kBlock)), dim3(kBlock), 0,
                      as_stream(stream), x, y, z, nx, ny, nz, spx, spy, spz, 1);
   AWK_LAUNCH_CHECK("k_hv_convert");
   return AWK_OK;
@@ -351,7 +356,7 @@ awk_status awk_hfield_inverse_zeroth_order(float *ox, float *oy, float *oz, cons
   if (awk_status st = check_dims(nx, ny, nz, "awk_hfield_inverse_zeroth_order")) return st;
   if (!ox || !oy || !oz || !hx || !hy || !hz)
     return fail(AWK_ERR_INVALID, "awk_hfield_inverse_zeroth_order: null pointer");
-  hipLaunchKernelGGL(k_inv0, dim3(grid_for((long long)nx * ny * nz, kBlock)), dim3(kBlock), 0,
+  AWK_LAUNCH(k_inv0, dim3(grid_for((long long)nx * ny * nz, kBlock)), dim3(kBlock), 0,
                      as_stream(stream), ox, oy, oz, hx, hy, hz, nx, ny, nz);
   AWK_LAUNCH_CHECK("k_inv0");
   return AWK_OK;
@@ -406,7 +411,7 @@ awk_status awk_gradient(float *gx, float *gy, float *gz, const float *img, int n
                         float spx, float spy, float spz, int wrap, awk_stream_t stream) {
   if (awk_status st = check_dims(nx, ny, nz, "awk_gradient")) return st;
   if (!gx || !gy || !gz || !img) return fail(AWK_ERR_INVALID, "awk_gradient: null pointer");
-  hipLaunchKernelGGL(k_gradient, dim3(grid_for((long long)nx * ny * nz, kBlock)), dim3(kBlock), 0,
+  AWK_LAUNCH(k_gradient, dim3(grid_for((long long)nx * ny * nz, kBlock)), dim3(kBlock), 0,
                      as_stream(stream), gx, gy, gz, img, nx, ny, nz, spx, spy, spz, wrap);
   AWK_LAUNCH_CHECK("k_gradient");
   return AWK_OK;

@@ -67,25 +67,25 @@ awk_status launch_splat(float *dst, const float *src, const float *hx, const flo
     void *acc = nullptr;
     AWK_HIP(hipMallocAsync(&acc, (size_t)N * 8, s));
     AWK_HIP(hipMemsetAsync(acc, 0, (size_t)N * 8, s));
-    hipLaunchKernelGGL(k_splat<AWK_SPLAT_FIXED64>, dim3(grid), dim3(kSplatBlock), 0, s, acc, src, hx,
+    AWK_LAUNCH(k_splat<AWK_SPLAT_FIXED64>, dim3(grid), dim3(kSplatBlock), 0, s, acc, src, hx,
                        hy, hz, nx, ny, nz);
     AWK_LAUNCH_CHECK("k_splat<FIXED64>");
-    hipLaunchKernelGGL(k_fixed64_to_float, dim3(cgrid), dim3(256), 0, s, dst,
+    AWK_LAUNCH(k_fixed64_to_float, dim3(cgrid), dim3(256), 0, s, dst,
                        reinterpret_cast<const long long *>(acc), (int)N);
     AWK_LAUNCH_CHECK("k_fixed64_to_float");
     AWK_HIP(hipFreeAsync(acc, s));
   } else if (mode == AWK_SPLAT_FIXED20) {
     if ((const void *)dst == (const void *)src) return fail(AWK_ERR_INVALID, "awk_splat: dst aliases src");
     AWK_HIP(hipMemsetAsync(dst, 0, (size_t)N * 4, s));
-    hipLaunchKernelGGL(k_splat<AWK_SPLAT_FIXED20>, dim3(grid), dim3(kSplatBlock), 0, s, (void *)dst,
+    AWK_LAUNCH(k_splat<AWK_SPLAT_FIXED20>, dim3(grid), dim3(kSplatBlock), 0, s, (void *)dst,
                        src, hx, hy, hz, nx, ny, nz);
     AWK_LAUNCH_CHECK("k_splat<FIXED20>");
-    hipLaunchKernelGGL(k_fixed20_to_float, dim3(cgrid), dim3(256), 0, s, dst, (int)N);
+    AWK_LAUNCH(k_fixed20_to_float, dim3(cgrid), dim3(256), 0, s, dst, (int)N);
     AWK_LAUNCH_CHECK("k_fixed20_to_float");
   } else if (mode == AWK_SPLAT_FLOAT) {
     if ((const void *)dst == (const void *)src) return fail(AWK_ERR_INVALID, "awk_splat: dst aliases src");
     AWK_HIP(hipMemsetAsync(dst, 0, (size_t)N * 4, s));
-    hipLaunchKernelGGL(k_splat<AWK_SPLAT_FLOAT>, dim3(grid), dim3(kSplatBlock), 0, s, (void *)dst,
+    AWK_LAUNCH(k_splat<AWK_SPLAT_FLOAT>, dim3(grid), dim3(kSplatBlock), 0, s, (void *)dst,
                        src, hx, hy, hz, nx, ny, nz);
     AWK_LAUNCH_CHECK("k_splat<FLOAT>");
   } else {

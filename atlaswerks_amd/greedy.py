@@ -95,7 +95,7 @@ class GreedyIteratorGPU:
     def SetStepSize(self, step):
         _lib.call("awk_greedy_set_step", self._h, float(step))
 
-    STAGES = ("apply", "xfwd_force", "energy", "y_fwd", "z_solve", "y_inv", "xinv_compose")
+    STAGES = ("apply", "xfwd_force", "y_fwd", "z_solve", "y_inv", "xinv_compose")
 
     def SetProfiling(self, on):
         _lib.call("awk_greedy_set_profiling", self._h, int(bool(on)))

@@ -210,13 +210,13 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
  * this iteration's update (done on the device). */
 awk_status awk_greedy_set_auto_step_reduce(awk_greedy_t g, int on);
 
-/* Per-stage HIP event timing of awk_greedy_iterate, for benchmarks: events are
- * recorded on the iteration's stream between its stages.  Stages:
- *   0 apply, 1 x pass + force + SSD (its last block reduces the energy),
- *   2 (empty, kept for a stable layout), 3 y pass,
- *   4 z pass + solve, 5 inverse y pass, 6 inverse x pass + hfield update.
+/* Per-kernel timing of awk_greedy_iterate, for benchmarks: while on, each
+ * stage's kernel is launched with start/stop events attached to its dispatch
+ * (kernel duration only).  Stages of the fused steady-state iteration:
+ *   0 apply, 1 x pass + force + SSD partials, 2 y pass (+ energy reduction),
+ *   3 z pass + solve, 4 inverse y pass, 5 inverse x pass + hfield update.
  * No counterpart in the reference (its GreedyIteratorGPU has no timers). */
-#define AWK_GREEDY_NSTAGES 7
+#define AWK_GREEDY_NSTAGES 6
 awk_status awk_greedy_set_profiling(awk_greedy_t g, int on);
 /* Sums the recorded stage times (ms) over all iterations since profiling was
  * switched on or last read (synchronises), and resets the record. */
