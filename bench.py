@@ -128,11 +128,17 @@ def main():
     sb = stage_bytes(nx, ny, nz)
     dom = max(stage_avg, key=lambda k: stage_avg[k])
     achieved = sb[dom] / (stage_avg[dom] * 1e-3) / 1e9
+    # HBM bytes per launch of the dominant kernel, measured by rocprofv3 --pmc
+    # FETCH_SIZE / WRITE_SIZE (scaled by the copy calibration) and recorded in
+    # profiles/pmc_traffic_<n>.txt; null when no measurement exists for this size.
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "pmc_traffic_%d.json" % n)
+    tpath = os.path.join(ROOT, "profiles", "pmc_traffic_%d.txt" % n)
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get(dom)
+            for line in f:
+                parts = line.split()
+                if len(parts) == 4 and parts[0] == dom:
+                    traffic = int(parts[2]) + int(parts[3])
     yard = 196 * N  # fixed yardstick bytes of one greedy iteration
     out = {
         "metric": "greedy-fluid iter/s @256^3 (% HBM peak); time-to-SSD plain vs warm-start",
@@ -204,13 +210,24 @@ def main():
         for name, mode in (("splat_fixed64", _lib.SPLAT_FIXED64), ("splat_float", _lib.SPLAT_FLOAT)):
             ms = timeit(lambda: hfield.splat(outi, I0, g1, mode=mode))
             mb[name] = {"ms": round(ms, 4), "gbs_20N": round(20 * N / (ms * 1e-3) / 1e9, 1)}
-        if args.copy_ref:
-            a = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev)
-            b = torch.empty_like(a)
-            ms = timeit(lambda: b.copy_(a))
-            mb["copy_1GiB"] = {"ms": round(ms, 4), "gbs_rw": round(2 * a.numel() * 4 / (ms * 1e-3) / 1e9, 1)}
         out["microbench"] = mb
         del f, g1, g2, outi, outv
+
+    if args.copy_ref:
+        # plain 1 GiB device copy: the calibration point for the HBM byte counters
+        a = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 20
+        out["copy_1GiB"] = {"ms": round(ms, 4), "gbs_rw": round(2 * a.numel() * 4 / (ms * 1e-3) / 1e9, 1)}
+        del a, b
 
     # ---- CPU baseline: the oracle's FP32 restatement, one host thread, bounded sample
     if not args.no_cpu_baseline:

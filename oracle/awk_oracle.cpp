@@ -409,7 +409,7 @@ void trilerp_vec(const TF *h, const int *size, double x, double y, double z, dou
   } else {
     inside = (fX >= 0 && cX < sX && fY >= 0 && cY < sY && fZ >= 0 && cZ < sZ);
   }
-  // corner order v0..v7 (HField3DUtils.txx:2014-2020)
+  // corner order v0..v7 (HField3DUtils.txx:1998-2003)
   const int cx[8] = {fX, cX, cX, fX, fX, cX, cX, fX};
   const int cy[8] = {fY, fY, cY, cY, cY, cY, fY, fY};
   const int cz[8] = {fZ, fZ, fZ, fZ, cZ, cZ, cZ, cZ};
