@@ -89,6 +89,28 @@ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 
+// ---------------------------------------------------------------- loads
+// Gathers go through a loader: plain pointers, or raw buffer resources (32-bit
+// byte offsets, hardware range check: an offset past num_records reads 0).  The
+// buffer form saves the 64-bit address arithmetic of every gather.
+struct PtrLoad {
+  const float *__restrict__ p;
+  __device__ __forceinline__ float operator()(int i) const { return p[i]; }
+};
+struct BufLoad {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ float operator()(int i) const {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, i * 4, 0, 0));
+  }
+};
+// Buffer descriptor word 3 for gfx9-family raw buffers (32-bit data format).
+constexpr int kBufDword3 = 0x00020000;
+__device__ __forceinline__ BufLoad make_buf(const float *p, int n) {
+  return BufLoad{__builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), (short)0, n * 4, kBufDword3)};
+}
+// volumes whose byte size fits a 31-bit offset can use BufLoad
+constexpr long long kBufMaxElems = (1LL << 29);
+
 // ---------------------------------------------------------------- trilerp
 // Background strategies (numbering of include/awk_hip.h):
 //   ID, ZERO, PARTIAL_ID, PARTIAL_ZERO, CLAMP, WRAP, VAL(scalar)
@@ -105,11 +127,9 @@ __device__ __forceinline__ int wrap_mod(int r, int b) {
 // HField3DUtils::trilerp (UtilitiesDataTypes/HField3DUtils.txx:1969-2300):
 // corner order v0..v7, nested lerp, CPU semantics for PARTIAL_ID (out-of-volume
 // corners take their own integer coordinates).  FP32 weights.
-template <int BG>
-__device__ __forceinline__ float3 trilerp3(const float *__restrict__ fx,
-                                           const float *__restrict__ fy,
-                                           const float *__restrict__ fz, Dims d, float x,
-                                           float y, float z) {
+template <int BG, class L>
+__device__ __forceinline__ float3 trilerp3g(const L &fx, const L &fy, const L &fz, Dims d, float x,
+                                            float y, float z) {
   const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
   int x0 = (int)flx, y0 = (int)fly, z0 = (int)flz;
   int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
@@ -137,9 +157,9 @@ __device__ __forceinline__ float3 trilerp3(const float *__restrict__ fx,
     const int idx[8] = {b00 + x0, b00 + x1, b10 + x1, b10 + x0, b11 + x0, b11 + x1, b01 + x1, b01 + x0};
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      c[i][0] = fx[idx[i]];
-      c[i][1] = fy[idx[i]];
-      c[i][2] = fz[idx[i]];
+      c[i][0] = fx(idx[i]);
+      c[i][1] = fy(idx[i]);
+      c[i][2] = fz(idx[i]);
     }
   } else {
     if (BG == AWK_BG_ID) return make_float3(x, y, z);
@@ -157,9 +177,9 @@ __device__ __forceinline__ float3 trilerp3(const float *__restrict__ fx,
       const bool in = xin[ox[i]] & yin[oy[i]] & zin[oz[i]];
       if (in) {
         const int id = ((z0 + oz[i]) * d.ny + (y0 + oy[i])) * d.nx + (x0 + ox[i]);
-        c[i][0] = fx[id];
-        c[i][1] = fy[id];
-        c[i][2] = fz[id];
+        c[i][0] = fx(id);
+        c[i][1] = fy(id);
+        c[i][2] = fz(id);
       } else if (BG == AWK_BG_PARTIAL_ID) {
         // the corner's own integer coordinates (taken from the float floor so that
         // they stay exact for any finite coordinate)
@@ -179,11 +199,19 @@ __device__ __forceinline__ float3 trilerp3(const float *__restrict__ fx,
   return make_float3(r[0], r[1], r[2]);
 }
 
+template <int BG>
+__device__ __forceinline__ float3 trilerp3(const float *__restrict__ fx,
+                                           const float *__restrict__ fy,
+                                           const float *__restrict__ fz, Dims d, float x,
+                                           float y, float z) {
+  return trilerp3g<BG>(PtrLoad{fx}, PtrLoad{fy}, PtrLoad{fz}, d, x, y, z);
+}
+
 // Scalar trilerp (Array3DUtils::trilerp, UtilitiesDataTypes/Array3DUtils.txx:59-232).
 // BG: AWK_BG_VAL / AWK_BG_ZERO (background value), AWK_BG_CLAMP, AWK_BG_WRAP.
-template <int BG>
-__device__ __forceinline__ float trilerp1(const float *__restrict__ a, Dims d, float x, float y,
-                                          float z, float background) {
+template <int BG, class L>
+__device__ __forceinline__ float trilerp1g(const L &a, Dims d, float x, float y, float z,
+                                           float background) {
   const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
   int x0 = (int)flx, y0 = (int)fly, z0 = (int)flz;
   int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
@@ -203,8 +231,8 @@ __device__ __forceinline__ float trilerp1(const float *__restrict__ a, Dims d, f
     const int sxy = d.nx * d.ny;
     const int b00 = z0 * sxy + y0 * d.nx, b10 = z0 * sxy + y1 * d.nx;
     const int b01 = z1 * sxy + y0 * d.nx, b11 = z1 * sxy + y1 * d.nx;
-    c[0] = a[b00 + x0]; c[1] = a[b00 + x1]; c[2] = a[b10 + x1]; c[3] = a[b10 + x0];
-    c[4] = a[b11 + x0]; c[5] = a[b11 + x1]; c[6] = a[b01 + x1]; c[7] = a[b01 + x0];
+    c[0] = a(b00 + x0); c[1] = a(b00 + x1); c[2] = a(b10 + x1); c[3] = a(b10 + x0);
+    c[4] = a(b11 + x0); c[5] = a(b11 + x1); c[6] = a(b01 + x1); c[7] = a(b01 + x0);
   } else {
     const float bgv = (BG == AWK_BG_ZERO) ? 0.f : background;
     const bool inside = (x0 >= 0) & (x0 < d.nx - 1) & (y0 >= 0) & (y0 < d.ny - 1) & (z0 >= 0) &
@@ -213,8 +241,8 @@ __device__ __forceinline__ float trilerp1(const float *__restrict__ a, Dims d, f
       const int sxy = d.nx * d.ny;
       const int b00 = z0 * sxy + y0 * d.nx, b10 = z0 * sxy + y1 * d.nx;
       const int b01 = z1 * sxy + y0 * d.nx, b11 = z1 * sxy + y1 * d.nx;
-      c[0] = a[b00 + x0]; c[1] = a[b00 + x1]; c[2] = a[b10 + x1]; c[3] = a[b10 + x0];
-      c[4] = a[b11 + x0]; c[5] = a[b11 + x1]; c[6] = a[b01 + x1]; c[7] = a[b01 + x0];
+      c[0] = a(b00 + x0); c[1] = a(b00 + x1); c[2] = a(b10 + x1); c[3] = a(b10 + x0);
+      c[4] = a(b11 + x0); c[5] = a(b11 + x1); c[6] = a(b01 + x1); c[7] = a(b01 + x0);
     } else {
       const bool xin[2] = {(x0 >= 0) && (x0 < d.nx), (x0 >= -1) && (x0 < d.nx - 1)};
       const bool yin[2] = {(y0 >= 0) && (y0 < d.ny), (y0 >= -1) && (y0 < d.ny - 1)};
@@ -225,12 +253,18 @@ __device__ __forceinline__ float trilerp1(const float *__restrict__ a, Dims d, f
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
         const bool in = xin[ox[i]] & yin[oy[i]] & zin[oz[i]];
-        c[i] = in ? a[((z0 + oz[i]) * d.ny + (y0 + oy[i])) * d.nx + (x0 + ox[i])] : bgv;
+        c[i] = in ? a(((z0 + oz[i]) * d.ny + (y0 + oy[i])) * d.nx + (x0 + ox[i])) : bgv;
       }
     }
   }
   return omt * (omu * (c[0] * omv + c[7] * v) + u * (c[3] * omv + c[4] * v)) +
          t * (omu * (c[1] * omv + c[6] * v) + u * (c[2] * omv + c[5] * v));
+}
+
+template <int BG>
+__device__ __forceinline__ float trilerp1(const float *__restrict__ a, Dims d, float x, float y,
+                                          float z, float background) {
+  return trilerp1g<BG>(PtrLoad{a}, d, x, y, z, background);
 }
 
 }  // namespace awk

@@ -20,13 +20,17 @@ struct AxisPlan {
 enum XStore {
   XSTORE_PLAIN = 0,    // write v to the output planes
   XSTORE_MAXNORM = 1,  // write v and a per-block max |v|
-  XSTORE_COMPOSE = 2   // h_new(x) = h(x + step * v(x) / sp) (PARTIAL_ID); v not written
+  XSTORE_COMPOSE = 2,  // h_new(x) = h(x + step * v(x) / sp) (PARTIAL_ID); v not written
+  XSTORE_AXPBY = 3,    // out = (1 - 2 step) out + (2 step / sigma^2) v   (LDMM velocity update)
+  XSTORE_MAXAXPY = 4   // per-block max |2 out - (2 / sigma^2) v|; out not written
 };
 
 // Prologue of the forward x pass.
 enum XLoad {
   XLOAD_PLAIN = 0,  // read f from the input planes
-  XLOAD_FORCE = 1   // f = (I1 - Idef) * grad(Idef); per-block FP64 partial sums of (I1-Idef)^2
+  XLOAD_FORCE = 1,     // f = (I1 - Idef) * grad(Idef); per-block FP64 partial sums of (I1-Idef)^2
+  XLOAD_GRADMUL = 2,   // f = m * grad(Idef), m = the image i1
+  XLOAD_GRADMUL_FX = 3 // f = m * grad(Idef), m = acc * 2^-32 (int64 splat accumulator); acc zeroed
 };
 
 struct XFwdArgs {
@@ -36,6 +40,7 @@ struct XFwdArgs {
   const float *i1;      // FORCE: target image
   float sp[3];          // FORCE: spacing for the gradient
   double *partials;     // FORCE: one double per block (may be null)
+  long long *acc;       // GRADMUL_FX: fixed-point multiplier, cleared after reading
 };
 
 // Energy reduction of the greedy step (done by block 0 of the forward y pass).
@@ -56,8 +61,9 @@ struct XInvArgs {
   float *maxpart;           // MAXNORM: one float per block
   const float *h[3];        // COMPOSE: current hfield
   float *hnew[3];           // COMPOSE: updated hfield
-  const float *step;        // COMPOSE: step size (device scalar)
+  const float *step;        // COMPOSE / AXPBY: step size (device scalar)
   float sp[3];              // COMPOSE: spacing
+  float sigma2;             // AXPBY / MAXAXPY: sigma^2
 };
 
 struct FluidKernel {

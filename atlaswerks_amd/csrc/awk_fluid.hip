@@ -85,6 +85,18 @@ __device__ __forceinline__ void energy_block(const ReduceArgs &ra, double *tree,
 }
 
 
+// Force multiplier of the GRADMUL modes: the image i1, or an int64 fixed-point
+// splat accumulator (2^-32 units), which is cleared as it is read (every voxel is
+// read by exactly one thread of the x pass).
+__device__ __forceinline__ float load_mul(const XFwdArgs &a, size_t i) {
+  if (a.mode == XLOAD_GRADMUL_FX) {
+    const long long q = a.acc[i];
+    a.acc[i] = 0;
+    return (float)((double)q * (1.0 / 4294967296.0));
+  }
+  return a.i1[i];
+}
+
 // NC, RTC, BDC: compile-time length, rows per block and block size (0 = runtime).
 template <int NC, int RTC, int BDC>
 __global__ void __launch_bounds__(BDC ? BDC : 1024)
@@ -117,7 +129,7 @@ k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
     // consecutive y at one z.  Each Idef value is loaded once: the column with a
     // one-row halo in y, the two neighbouring z rows, and x +- 1 from the
     // neighbouring lanes (a load only at the wave edges).
-    if (a.mode == XLOAD_FORCE && ny % RTC == 0) {
+    if (a.mode != XLOAD_PLAIN && ny % RTC == 0) {
       done = true;
       const int x = threadIdx.x;
       const int z = r0 / ny, y0 = r0 - z * ny;
@@ -135,7 +147,7 @@ k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
       for (int q = 0; q < RTC; ++q) {
         zm[q] = (z > 0) ? col0[(ptrdiff_t)q * NC - (ptrdiff_t)sxy] : 0.f;
         zp[q] = (z < nz - 1) ? col0[(ptrdiff_t)q * NC + (ptrdiff_t)sxy] : 0.f;
-        i1v[q] = a.i1[(size_t)z * sxy + (size_t)(y0 + q) * NC + x];
+        i1v[q] = load_mul(a, (size_t)z * sxy + (size_t)(y0 + q) * NC + x);
         lft[q] = (lane == 0 && x > 0) ? col0[(ptrdiff_t)q * NC - 1] : 0.f;
         rgt[q] = (lane == 63 && x < NC - 1) ? col0[(ptrdiff_t)q * NC + 1] : 0.f;
       }
@@ -159,7 +171,7 @@ k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
         else if (z > 0 && z < nz - 1) gz = (zp[q] - zm[q]) / (2.f * a.sp[2]);
         else if (z == 0) gz = (zp[q] - c) / a.sp[2];
         else gz = (c - zm[q]) / a.sp[2];
-        const float r = i1v[q] - c;
+        const float r = (a.mode == XLOAD_FORCE) ? i1v[q] - c : i1v[q];
         ssd += (double)r * (double)r;
         put(q, x, gx * r, gy * r, gz * r);
       }
@@ -201,7 +213,7 @@ k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
           else if (z == 0) gz = (I[base + s] - c) / a.sp[2];
           else gz = (c - I[base - s]) / a.sp[2];
         }
-        const float r = a.i1[base] - c;
+        const float r = (a.mode == XLOAD_FORCE) ? a.i1[base] - c : load_mul(a, base);
         ssd += (double)r * (double)r;
         f0 = gx * r;
         f1 = gy * r;
@@ -256,7 +268,11 @@ k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
   const int NR = ny * nz;
   const int lb = (int)xcd_block();
   const int r0 = lb * RT;
-  if (r0 >= NR) return;
+  if (r0 >= NR) {
+    // a padding block of the grid: its max partial is still read
+    if ((a.mode == XSTORE_MAXNORM || a.mode == XSTORE_MAXAXPY) && threadIdx.x == 0) a.maxpart[lb] = 0.f;
+    return;
+  }
 #pragma unroll
   for (int w = threadIdx.x; w < NL * nxh; w += bdim) {
     const int line = w / nxh, k = w - line * nxh;
@@ -280,6 +296,11 @@ k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
     // XU voxels per step: their 24*XU gathers are in flight together.
     const float st = *a.step;
     const Dims d{nx, ny, nz};
+    // v * step / sp, with the division by the spacing taken as a reciprocal
+    // multiply (exact for unit spacing)
+    const float isx = 1.f / a.sp[0], isy = 1.f / a.sp[1], isz = 1.f / a.sp[2];
+    const int ntot = nx * ny * nz;
+    const BufLoad hx = make_buf(a.h[0], ntot), hy = make_buf(a.h[1], ntot), hz = make_buf(a.h[2], ntot);
 #pragma unroll
     for (int w0 = threadIdx.x; w0 < RT * nx; w0 += bdim * XU) {
       float px[XU], py[XU], pz[XU];
@@ -301,15 +322,15 @@ k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
             v0 = t.x; v1 = t.y; v2 = sm[x * LS + 3 * p + 1].y;
           }
           const int y = row_[u] % ny, z = row_[u] / ny;
-          px[u] = (float)x + (v0 * st) / a.sp[0];
-          py[u] = (float)y + (v1 * st) / a.sp[1];
-          pz[u] = (float)z + (v2 * st) / a.sp[2];
+          px[u] = (float)x + (v0 * st) * isx;
+          py[u] = (float)y + (v1 * st) * isy;
+          pz[u] = (float)z + (v2 * st) * isz;
         }
       }
       float3 hn[XU];
 #pragma unroll
       for (int u = 0; u < XU; ++u)
-        if (row_[u] < NR) hn[u] = trilerp3<AWK_BG_PARTIAL_ID>(a.h[0], a.h[1], a.h[2], d, px[u], py[u], pz[u]);
+        if (row_[u] < NR) hn[u] = trilerp3g<AWK_BG_PARTIAL_ID>(hx, hy, hz, d, px[u], py[u], pz[u]);
 #pragma unroll
       for (int u = 0; u < XU; ++u)
         if (row_[u] < NR) {
@@ -320,6 +341,15 @@ k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
         }
     }
   } else {
+    // LDMM coefficients, in float as the reference forms them
+    float cva = 0.f, cua = 0.f;
+    if (a.mode == XSTORE_AXPBY) {
+      const float st = *a.step;
+      cva = 1.f - 2.f * st;
+      cua = 2.f * st / a.sigma2;
+    } else if (a.mode == XSTORE_MAXAXPY) {
+      cua = -2.f / a.sigma2;
+    }
 #pragma unroll
     for (int w = threadIdx.x; w < RT * nx; w += bdim) {
       const int q = w / nx, x = w - q * nx;
@@ -335,13 +365,26 @@ k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
         v0 = t.x; v1 = t.y; v2 = sm[x * LS + 3 * p + 1].y;
       }
       const size_t base = (size_t)row * nx + x;
-      a.out[0][base] = v0;
-      a.out[1][base] = v1;
-      a.out[2][base] = v2;
-      if (a.mode == XSTORE_MAXNORM) mx = fmaxf(mx, sqrtf(v0 * v0 + v1 * v1 + v2 * v2));
+      if (a.mode == XSTORE_AXPBY) {
+        // updateVelocity (LDMMIteratorCPU.cxx:175-190): v = vFac v + uFac u
+        a.out[0][base] = cva * a.out[0][base] + cua * v0;
+        a.out[1][base] = cva * a.out[1][base] + cua * v1;
+        a.out[2][base] = cva * a.out[2][base] + cua * v2;
+      } else if (a.mode == XSTORE_MAXAXPY) {
+        // calcMaxDisplacement (LDMMIteratorCPU.cxx:192-212): |2 v - 2/sigma^2 u|
+        const float q0 = 2.f * a.out[0][base] + cua * v0;
+        const float q1 = 2.f * a.out[1][base] + cua * v1;
+        const float q2 = 2.f * a.out[2][base] + cua * v2;
+        mx = fmaxf(mx, sqrtf(q0 * q0 + q1 * q1 + q2 * q2));
+      } else {
+        a.out[0][base] = v0;
+        a.out[1][base] = v1;
+        a.out[2][base] = v2;
+        if (a.mode == XSTORE_MAXNORM) mx = fmaxf(mx, sqrtf(v0 * v0 + v1 * v1 + v2 * v2));
+      }
     }
   }
-  if (a.mode == XSTORE_MAXNORM) {
+  if (a.mode == XSTORE_MAXNORM || a.mode == XSTORE_MAXAXPY) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_down(mx, off, 64));
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -692,7 +735,7 @@ static int pick_bd(int n, int nl) {
 
 awk_status fk_init(FluidKernel &k, int nx, int ny, int nz, const float sp[3]) {
   if (nx < 1 || ny < 1 || nz < 1) return fail(AWK_ERR_INVALID, "fluid kernel: sizes must be >= 1");
-  if ((long long)nx * ny * nz >= (1LL << 31)) return fail(AWK_ERR_INVALID, "fluid kernel: volume too large");
+  if ((long long)nx * ny * nz > kBufMaxElems) return fail(AWK_ERR_INVALID, "fluid kernel: volume too large");
   k.nx = nx; k.ny = ny; k.nz = nz;
   k.nxh = nx / 2 + 1;
   k.nxh_pad = (k.nxh + 7) / 8 * 8;

@@ -40,7 +40,7 @@ Tuning &tuning() {
 
 static awk_status check_dims(int nx, int ny, int nz, const char *who) {
   if (nx < 1 || ny < 1 || nz < 1) return fail(AWK_ERR_INVALID, std::string(who) + ": sizes must be >= 1");
-  if ((long long)nx * ny * nz >= (1LL << 31)) return fail(AWK_ERR_INVALID, std::string(who) + ": volume too large");
+  if ((long long)nx * ny * nz > kBufMaxElems) return fail(AWK_ERR_INVALID, std::string(who) + ": volume too large (at most 2^29 voxels)");
   return AWK_OK;
 }
 
@@ -109,9 +109,10 @@ k_apply(float *__restrict__ out, const float *__restrict__ img, const float *__r
     if (i < N) { px[v] = hx[i]; py[v] = hy[i]; pz[v] = hz[i]; }
   }
   float r[VPT];
+  const BufLoad ib = make_buf(img, N);
 #pragma unroll
   for (int v = 0; v < VPT; ++v)
-    if (i0 + v * kBlock < N) r[v] = trilerp1<BG>(img, d, px[v], py[v], pz[v], background);
+    if (i0 + v * kBlock < N) r[v] = trilerp1g<BG>(ib, d, px[v], py[v], pz[v], background);
 #pragma unroll
   for (int v = 0; v < VPT; ++v) {
     const int i = i0 + v * kBlock;
@@ -134,6 +135,7 @@ k_compose(float *__restrict__ ox, float *__restrict__ oy, float *__restrict__ oz
   const int i0 = (int)(xcd_block() * (kBlock * VPT) + threadIdx.x);
   const Dims d{nx, ny, nz};
   const float delta = delta_dev ? *delta_dev : delta_h;
+  const BufLoad bx = make_buf(fx, N), by = make_buf(fy, N), bz = make_buf(fz, N);
   float ax[VPT], ay[VPT], az[VPT];
 #pragma unroll
   for (int v = 0; v < VPT; ++v) {
@@ -146,7 +148,7 @@ k_compose(float *__restrict__ ox, float *__restrict__ oy, float *__restrict__ oz
     const int i = i0 + v * kBlock;
     if (i >= N) continue;
     if (OP == 0) {
-      r[v] = trilerp3<BG>(fx, fy, fz, d, ax[v], ay[v], az[v]);
+      r[v] = trilerp3g<BG>(bx, by, bz, d, ax[v], ay[v], az[v]);
     } else if (OP == 1) {
       // f = g (field gathered), (gx,gy,gz) = v.  The reference scales v by the step
       // first (GreedyIteratorCPU.cxx:84), then divides by the spacing.
@@ -155,10 +157,10 @@ k_compose(float *__restrict__ ox, float *__restrict__ oy, float *__restrict__ oz
       const float px = (float)x + sgn * (vx / spx);
       const float py = (float)y + sgn * (vy / spy);
       const float pz = (float)z + sgn * (vz / spz);
-      r[v] = trilerp3<BG>(fx, fy, fz, d, px, py, pz);
+      r[v] = trilerp3g<BG>(bx, by, bz, d, px, py, pz);
     } else {
       // f = v (field gathered), (gx,gy,gz) = g
-      float3 q = trilerp3<BG>(fx, fy, fz, d, ax[v], ay[v], az[v]);
+      float3 q = trilerp3g<BG>(bx, by, bz, d, ax[v], ay[v], az[v]);
       if (delta_dev || delta_h != 1.f) {  // v scaled by the step (greedy inverse update)
         q.x *= delta; q.y *= delta; q.z *= delta;
       }

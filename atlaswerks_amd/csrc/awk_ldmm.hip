@@ -1,0 +1,438 @@
+// awk_ldmm.hip -- one LDMM iteration (adjoint gradient, IterateNew) on gfx950.
+//
+// LDMMIteratorGPU::IterateNew (Algorithms/LDMM/LDMMIteratorGPU.cxx:159-293) and
+// LDMMIteratorCPU::IterateNew (LDMMIteratorCPU.cxx:529-631).  For T time steps:
+//   forward   h <- h(x - v_t(x)/sp) (PARTIAL_ID) and J_t = I0(h), one fused kernel
+//             per t; the last one also forms diff = J_{T-1} - I1 and FP64 SSD partials
+//   backward  h <- h - v_t(h)/sp (PARTIAL_ZERO) and the splat of diff to h(x) into an
+//             int64 fixed-point accumulator, one fused kernel per t; then the
+//             Navier solve whose forward x pass forms grad(J_{t-1}) * alpha_t
+//             straight from the accumulator (and clears it), and whose inverse x
+//             pass updates v_t = (1 - 2 eps) v_t + (2 eps / sigma^2) u in place
+//             (or, on the step-size call, takes max |2 v_t - 2/sigma^2 u|).
+// No host synchronisation; energies stay on the device (FP64, fixed order).
+#include <cstring>
+#include <new>
+#include <utility>
+#include <vector>
+
+#include "awk_fluid.h"
+#include "awk_ops.h"
+
+namespace awk {
+
+constexpr int kLdmmBlock = 256;
+
+__device__ __forceinline__ void block_sum_store(double v, double *out, int lb) {
+  __shared__ double red[kLdmmBlock / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < kLdmmBlock / 64; ++i) s += red[i];
+    out[lb] = s;
+  }
+}
+
+__global__ void __launch_bounds__(kLdmmBlock)
+k_ldmm_identity(float *__restrict__ hx, float *__restrict__ hy, float *__restrict__ hz, Dims d) {
+  const int N = d.nx * d.ny * d.nz;
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= N) return;
+  const int x = i % d.nx, r = i / d.nx;
+  hx[i] = (float)x;
+  hy[i] = (float)(r % d.ny);
+  hz[i] = (float)(r / d.ny);
+}
+
+// Forward step t: composeHVInv(h, v_t, PARTIAL_ID) (HField3DUtils.txx:457-480) and
+// apply(I0, h) (VAL background 0); the last step also diff = J - I1.
+__global__ void __launch_bounds__(kLdmmBlock)
+k_ldmm_fwd(float *__restrict__ nx_, float *__restrict__ ny_, float *__restrict__ nz_,
+           const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
+           const float *__restrict__ vx, const float *__restrict__ vy, const float *__restrict__ vz,
+           const float *__restrict__ I0, float *__restrict__ J, const float *__restrict__ I1,
+           float *__restrict__ diff, double *__restrict__ partials, Dims d, float sx, float sy,
+           float sz) {
+  const int N = d.nx * d.ny * d.nz;
+  const int lb = (int)xcd_block();
+  const int i = lb * kLdmmBlock + (int)threadIdx.x;
+  double ssd = 0.0;
+  if (i < N) {
+    const int x = i % d.nx, r = i / d.nx, y = r % d.ny, z = r / d.ny;
+    const float px = (float)x - vx[i] / sx, py = (float)y - vy[i] / sy, pz = (float)z - vz[i] / sz;
+    const float3 hn = trilerp3g<AWK_BG_PARTIAL_ID>(make_buf(hx, N), make_buf(hy, N), make_buf(hz, N), d,
+                                                   px, py, pz);
+    nx_[i] = hn.x;
+    ny_[i] = hn.y;
+    nz_[i] = hn.z;
+    const float j = trilerp1g<AWK_BG_VAL>(make_buf(I0, N), d, hn.x, hn.y, hn.z, 0.f);
+    J[i] = j;
+    if (diff) {
+      const float dv = j - I1[i];
+      diff[i] = dv;
+      ssd = (double)dv * (double)dv;
+    }
+  }
+  if (partials) block_sum_store(ssd, partials, lb);
+}
+
+// Backward step t: composeVHInv(v_t, h, PARTIAL_ZERO) (HField3DUtils.txx:400-430)
+// and the splat of diff to h(x) (cplSplat3DH, cudaSplatAtomic.cu:185-247) as
+// int64 fixed point (2^-32), which is order-independent.
+__global__ void __launch_bounds__(kLdmmBlock)
+k_ldmm_bwd(float *__restrict__ nx_, float *__restrict__ ny_, float *__restrict__ nz_,
+           const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
+           const float *__restrict__ vx, const float *__restrict__ vy, const float *__restrict__ vz,
+           const float *__restrict__ diff, unsigned long long *__restrict__ acc, Dims d, float sx,
+           float sy, float sz) {
+  const int N = d.nx * d.ny * d.nz;
+  const int i = (int)(xcd_block() * kLdmmBlock + threadIdx.x);
+  if (i >= N) return;
+  const float gx = hx[i], gy = hy[i], gz = hz[i];
+  const float3 o = trilerp3g<AWK_BG_PARTIAL_ZERO>(make_buf(vx, N), make_buf(vy, N), make_buf(vz, N), d,
+                                                  gx, gy, gz);
+  const float x = o.x / -sx + gx, y = o.y / -sy + gy, z = o.z / -sz + gz;
+  nx_[i] = x;
+  ny_[i] = y;
+  nz_[i] = z;
+  const float mass = diff[i];
+  const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+  const int xi = (int)fx, yi = (int)fy, zi = (int)fz;
+  const float dx = 1.f - (x - fx), dy = 1.f - (y - fy), dz = 1.f - (z - fz);
+  const float wx[2] = {dx, 1.f - dx}, wy[2] = {dy, 1.f - dy}, wz[2] = {dz, 1.f - dz};
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int ox = c & 1, oy = (c >> 1) & 1, oz = c >> 2;
+    const int X = xi + ox, Y = yi + oy, Z = zi + oz;
+    if (X < 0 || X >= d.nx || Y < 0 || Y >= d.ny || Z < 0 || Z >= d.nz) continue;
+    const float w = mass * wx[ox] * wy[oy] * wz[oz];
+    atomicAdd(acc + (Z * d.ny + Y) * d.nx + X, (unsigned long long)llrint((double)w * 4294967296.0));
+  }
+}
+
+// alpha_t = splat / sigma^2 (LDMMIteratorGPU.cxx:197-204); clears the accumulator
+__global__ void __launch_bounds__(kLdmmBlock)
+k_ldmm_alpha(float *__restrict__ alpha, long long *__restrict__ acc, int N, float inv_sig2) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= N) return;
+  const long long q = acc[i];
+  acc[i] = 0;
+  alpha[i] = (float)((double)q * (1.0 / 4294967296.0)) * inv_sig2;
+}
+
+__global__ void __launch_bounds__(kLdmmBlock) k_ldmm_zero(long long *__restrict__ acc, int N) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < N) acc[i] = 0;
+}
+
+// FP64 dot product partials of two 3-plane fields: sum_c a_c . b_c
+__global__ void __launch_bounds__(kLdmmBlock)
+k_ldmm_dot(const float *__restrict__ a, const float *__restrict__ b0, const float *__restrict__ b1,
+           const float *__restrict__ b2, int N, double *__restrict__ partials) {
+  const int lb = (int)xcd_block();
+  const int i = lb * kLdmmBlock + (int)threadIdx.x;
+  double s = 0.0;
+  if (i < N) {
+    s = (double)a[i] * b0[i];
+    s += (double)a[N + i] * b1[i];
+    s += (double)a[2 * N + i] * b2[i];
+  }
+  block_sum_store(s, partials, lb);
+}
+
+// energy[0] = Prod(sp) * sum(diff^2) / sigma^2 (LDMMIteratorGPU.cxx:276-283,
+// LDMMEnergy.h:42); energy[1] = 0; energy[2] = energy[0].  Fixed order.
+__global__ void k_ldmm_image_energy(const double *__restrict__ partials, int n, double scale,
+                                    double *__restrict__ energy) {
+  __shared__ double s[256];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) a += partials[i];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    energy[0] = scale * s[0];
+    energy[1] = 0.0;
+    energy[2] = energy[0];
+  }
+}
+
+// energy[1] += Prod(sp) <L v_t, v_t> / T (LDMMIteratorGPU.cxx:250-256,
+// LDMMEnergy.cxx:57-64); energy[2] = energy[0] + energy[1].  Fixed order.
+__global__ void k_ldmm_vec_energy(const double *__restrict__ partials, int n, double scale,
+                                  double *__restrict__ energy) {
+  __shared__ double s[256];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) a += partials[i];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    energy[1] += scale * s[0];
+    energy[2] = energy[0] + energy[1];
+  }
+}
+
+// step = MaxPert / max over all time steps and blocks (LDMMIteratorGPU.cxx:286-289)
+__global__ void k_ldmm_step(const float *__restrict__ maxpart, int n, float max_pert,
+                            float *__restrict__ step) {
+  __shared__ float s[256];
+  float m = 0.f;
+  for (int i = threadIdx.x; i < n; i += 256) m = fmaxf(m, maxpart[i]);
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *step = max_pert / s[0];
+}
+
+}  // namespace awk
+
+struct awk_ldmm {
+  awk::FluidKernel k;
+  int nx = 0, ny = 0, nz = 0, nt = 0;
+  float sp[3] = {1, 1, 1};
+  float sigma = 0.05f, max_pert = 0.1f;
+  bool update_step = true;
+  float *h = nullptr, *hs = nullptr;  // 3 planes each
+  float *J = nullptr;                  // nt images
+  float *diff = nullptr;
+  long long *acc = nullptr;
+  double *partials = nullptr;  // max(forward grid, dot grid) doubles
+  float *maxpart = nullptr;    // nt * nblk
+  float *step = nullptr;
+  double *energy = nullptr;    // 3 doubles (internal when the caller passes none)
+  int nblk = 0, egrid = 0;
+};
+
+using namespace awk;
+
+static void ldmm_free(awk_ldmm *l) {
+  fk_release(l->k);
+  void *bufs[] = {l->h, l->hs, l->J, l->diff, l->acc, l->partials, l->maxpart, l->step, l->energy};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+}
+
+extern "C" {
+
+awk_status awk_ldmm_create(awk_ldmm_t *out, int nx, int ny, int nz, float spx, float spy, float spz,
+                           float alpha, float beta, float gamma, int n_time_steps, float sigma,
+                           float max_pert) {
+  if (!out) return fail(AWK_ERR_INVALID, "awk_ldmm_create: null handle pointer");
+  *out = nullptr;
+  if (!(spx > 0 && spy > 0 && spz > 0)) return fail(AWK_ERR_INVALID, "awk_ldmm_create: spacing must be > 0");
+  if (n_time_steps < 1) return fail(AWK_ERR_INVALID, "awk_ldmm_create: need at least one time step");
+  if (!(sigma > 0) || !(max_pert > 0)) return fail(AWK_ERR_INVALID, "awk_ldmm_create: sigma and MaxPert must be > 0");
+  if ((long long)nx * ny * nz > (1LL << 28)) return fail(AWK_ERR_INVALID, "awk_ldmm_create: at most 2^28 voxels");
+  awk_ldmm *l = new (std::nothrow) awk_ldmm();
+  if (!l) return fail(AWK_ERR_ALLOC, "awk_ldmm_create: host allocation failed");
+  const float sp[3] = {spx, spy, spz};
+  awk_status st = fk_init(l->k, nx, ny, nz, sp);
+  if (st) {
+    ldmm_free(l);
+    delete l;
+    return st;
+  }
+  l->k.alpha = alpha;
+  l->k.beta = beta;
+  l->k.gamma = gamma;
+  l->k.lpow = 1.f;
+  l->k.flags = 0;
+  l->nx = nx;
+  l->ny = ny;
+  l->nz = nz;
+  l->nt = n_time_steps;
+  for (int i = 0; i < 3; ++i) l->sp[i] = sp[i];
+  l->sigma = sigma;
+  l->max_pert = max_pert;
+  const size_t N = (size_t)nx * ny * nz;
+  l->nblk = fk_xblocks(l->k);
+  l->egrid = (int)grid_for((long long)N, kLdmmBlock);
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = hipMalloc(&l->h, 3 * N * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&l->hs, 3 * N * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&l->J, (size_t)l->nt * N * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&l->diff, N * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&l->acc, N * sizeof(long long));
+  if (e == hipSuccess) e = hipMalloc(&l->partials, (size_t)l->egrid * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&l->maxpart, (size_t)l->nt * l->nblk * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&l->step, sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&l->energy, 3 * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(l->acc, 0, N * sizeof(long long));
+  if (e == hipSuccess) e = hipMemset(l->step, 0, sizeof(float));
+  if (e != hipSuccess) {
+    st = hip_check(e, "awk_ldmm_create: allocation");
+    ldmm_free(l);
+    delete l;
+    return st;
+  }
+  *out = l;
+  return AWK_OK;
+}
+
+awk_status awk_ldmm_destroy(awk_ldmm_t l) {
+  if (!l) return AWK_OK;
+  ldmm_free(l);
+  delete l;
+  return AWK_OK;
+}
+
+awk_status awk_ldmm_update_step_next(awk_ldmm_t l) {
+  if (!l) return fail(AWK_ERR_INVALID, "awk_ldmm_update_step_next: null handle");
+  l->update_step = true;
+  return AWK_OK;
+}
+
+awk_status awk_ldmm_set_step(awk_ldmm_t l, float step) {
+  if (!l) return fail(AWK_ERR_INVALID, "awk_ldmm_set_step: null handle");
+  AWK_HIP(hipMemcpy(l->step, &step, sizeof(float), hipMemcpyHostToDevice));
+  l->update_step = false;
+  return AWK_OK;
+}
+
+awk_status awk_ldmm_get_step(awk_ldmm_t l, float *step) {
+  if (!l || !step) return fail(AWK_ERR_INVALID, "awk_ldmm_get_step: null argument");
+  AWK_HIP(hipDeviceSynchronize());
+  AWK_HIP(hipMemcpy(step, l->step, sizeof(float), hipMemcpyDeviceToHost));
+  return AWK_OK;
+}
+
+static awk_status ldmm_run(awk_ldmm *l, const float *I0, const float *I1, float *const v[],
+                           float *const alpha[], double *energy_dev, int vec_energy, hipStream_t s) {
+  const int nx = l->nx, ny = l->ny, nz = l->nz, T = l->nt;
+  const int N = nx * ny * nz;
+  const Dims d{nx, ny, nz};
+  const float sx = l->sp[0], sy = l->sp[1], sz = l->sp[2];
+  const unsigned egrid = (unsigned)l->egrid;
+  const unsigned lgrid = (unsigned)((N + kLdmmBlock - 1) / kLdmmBlock);
+  const double vol = (double)sx * sy * sz;
+  const bool step_call = !alpha && l->update_step;
+  double *energy = energy_dev ? energy_dev : l->energy;
+  awk_status st;
+  float *h = l->h, *hs = l->hs;
+  // ---- forward: J_t = I0(h_t), h_t = h_{t-1}(x - v_t / sp)
+  AWK_LAUNCH(k_ldmm_identity, dim3(lgrid), dim3(kLdmmBlock), 0, s, h, h + N, h + 2 * N, d);
+  for (int t = 0; t < T; ++t) {
+    const bool last = (t == T - 1);
+    AWK_LAUNCH(k_ldmm_fwd, dim3(egrid), dim3(kLdmmBlock), 0, s, hs, hs + N, hs + 2 * N, h, h + N,
+               h + 2 * N, v[3 * t], v[3 * t + 1], v[3 * t + 2], I0, l->J + (size_t)t * N,
+               last ? I1 : nullptr, last ? l->diff : nullptr, last ? l->partials : nullptr, d, sx, sy,
+               sz);
+    AWK_LAUNCH_CHECK("k_ldmm_fwd");
+    std::swap(h, hs);
+  }
+  AWK_LAUNCH(k_ldmm_image_energy, dim3(1), dim3(256), 0, s, l->partials, (int)egrid,
+             vol / ((double)l->sigma * l->sigma), energy);
+  AWK_LAUNCH_CHECK("k_ldmm_image_energy");
+  // ---- backward: h_t = h_{t+1} - v_t(h_{t+1}) / sp; alpha_t = splat(diff, h_t)
+  AWK_LAUNCH(k_ldmm_zero, dim3(lgrid), dim3(kLdmmBlock), 0, s, l->acc, N);
+  AWK_LAUNCH(k_ldmm_identity, dim3(lgrid), dim3(kLdmmBlock), 0, s, h, h + N, h + 2 * N, d);
+  int nb = 0;
+  for (int t = T - 1; t >= 0; --t) {
+    AWK_LAUNCH(k_ldmm_bwd, dim3(egrid), dim3(kLdmmBlock), 0, s, hs, hs + N, hs + 2 * N, h, h + N,
+               h + 2 * N, v[3 * t], v[3 * t + 1], v[3 * t + 2], l->diff,
+               reinterpret_cast<unsigned long long *>(l->acc), d, sx, sy, sz);
+    AWK_LAUNCH_CHECK("k_ldmm_bwd");
+    std::swap(h, hs);
+    if (alpha) {
+      AWK_LAUNCH(k_ldmm_alpha, dim3(lgrid), dim3(kLdmmBlock), 0, s, alpha[t], l->acc, N,
+                 1.f / (l->sigma * l->sigma));
+      AWK_LAUNCH_CHECK("k_ldmm_alpha");
+      continue;
+    }
+    // u = L^-1 (grad(J_{t-1}) * alpha_t), grad(I0) at t = 0
+    XFwdArgs fa;
+    std::memset(&fa, 0, sizeof(fa));
+    fa.mode = XLOAD_GRADMUL_FX;
+    fa.idef = t > 0 ? l->J + (size_t)(t - 1) * N : I0;
+    fa.acc = l->acc;
+    fa.sp[0] = sx;
+    fa.sp[1] = sy;
+    fa.sp[2] = sz;
+    if ((st = fk_xfwd(l->k, fa, s, &nb))) return st;
+    if ((st = fk_ypass(l->k, false, s, nullptr))) return st;
+    if ((st = fk_zsolve(l->k, true, s))) return st;
+    if ((st = fk_ypass(l->k, true, s, nullptr))) return st;
+    XInvArgs ia;
+    std::memset(&ia, 0, sizeof(ia));
+    ia.mode = step_call ? XSTORE_MAXAXPY : XSTORE_AXPBY;
+    for (int c = 0; c < 3; ++c) ia.out[c] = v[3 * t + c];
+    ia.maxpart = l->maxpart + (size_t)t * l->nblk;
+    ia.step = l->step;
+    ia.sigma2 = l->sigma * l->sigma;
+    if ((st = fk_xinv(l->k, ia, s, &nb))) return st;
+    if (!step_call && vec_energy) {
+      // <L v_t, v_t> with L applied to a copy of v_t in the free hfield buffer
+      float *lv = hs;
+      for (int c = 0; c < 3; ++c)
+        AWK_HIP(hipMemcpyAsync(lv + (size_t)c * N, v[3 * t + c], (size_t)N * sizeof(float),
+                               hipMemcpyDeviceToDevice, s));
+      XFwdArgs pa;
+      std::memset(&pa, 0, sizeof(pa));
+      pa.mode = XLOAD_PLAIN;
+      for (int c = 0; c < 3; ++c) pa.in[c] = lv + (size_t)c * N;
+      if ((st = fk_xfwd(l->k, pa, s, &nb))) return st;
+      if ((st = fk_ypass(l->k, false, s, nullptr))) return st;
+      if ((st = fk_zsolve(l->k, false, s))) return st;
+      if ((st = fk_ypass(l->k, true, s, nullptr))) return st;
+      XInvArgs pi;
+      std::memset(&pi, 0, sizeof(pi));
+      pi.mode = XSTORE_PLAIN;
+      for (int c = 0; c < 3; ++c) pi.out[c] = lv + (size_t)c * N;
+      if ((st = fk_xinv(l->k, pi, s, &nb))) return st;
+      AWK_LAUNCH(k_ldmm_dot, dim3(egrid), dim3(kLdmmBlock), 0, s, lv, v[3 * t], v[3 * t + 1],
+                 v[3 * t + 2], N, l->partials);
+      AWK_LAUNCH(k_ldmm_vec_energy, dim3(1), dim3(256), 0, s, l->partials, (int)egrid, vol / T,
+                 energy);
+      AWK_LAUNCH_CHECK("vector energy");
+    }
+  }
+  if (step_call) {
+    AWK_LAUNCH(k_ldmm_step, dim3(1), dim3(256), 0, s, l->maxpart, T * l->nblk, l->max_pert, l->step);
+    AWK_LAUNCH_CHECK("k_ldmm_step");
+    l->update_step = false;
+  }
+  return AWK_OK;
+}
+
+awk_status awk_ldmm_iterate(awk_ldmm_t l, const float *I0, const float *I1, float *const v[],
+                            double *energy_dev, int vec_energy, awk_stream_t stream) {
+  if (!l || !I0 || !I1 || !v) return fail(AWK_ERR_INVALID, "awk_ldmm_iterate: null argument");
+  for (int i = 0; i < 3 * l->nt; ++i)
+    if (!v[i]) return fail(AWK_ERR_INVALID, "awk_ldmm_iterate: null velocity plane");
+  return ldmm_run(l, I0, I1, v, nullptr, energy_dev, vec_energy, as_stream(stream));
+}
+
+awk_status awk_ldmm_alphas(awk_ldmm_t l, const float *I0, const float *I1, float *const v[],
+                           float *const alpha[], awk_stream_t stream) {
+  if (!l || !I0 || !I1 || !v || !alpha) return fail(AWK_ERR_INVALID, "awk_ldmm_alphas: null argument");
+  for (int i = 0; i < 3 * l->nt; ++i)
+    if (!v[i]) return fail(AWK_ERR_INVALID, "awk_ldmm_alphas: null velocity plane");
+  for (int t = 0; t < l->nt; ++t)
+    if (!alpha[t]) return fail(AWK_ERR_INVALID, "awk_ldmm_alphas: null alpha image");
+  return ldmm_run(l, I0, I1, v, alpha, nullptr, 0, as_stream(stream));
+}
+
+awk_status awk_ldmm_deformed_image(awk_ldmm_t l, int t, float *out, awk_stream_t stream) {
+  if (!l || !out) return fail(AWK_ERR_INVALID, "awk_ldmm_deformed_image: null argument");
+  if (t < 0 || t >= l->nt) return fail(AWK_ERR_INVALID, "awk_ldmm_deformed_image: time step out of range");
+  const size_t N = (size_t)l->nx * l->ny * l->nz;
+  AWK_HIP(hipMemcpyAsync(out, l->J + (size_t)t * N, N * sizeof(float), hipMemcpyDeviceToDevice,
+                         as_stream(stream)));
+  return AWK_OK;
+}
+
+}  // extern "C"

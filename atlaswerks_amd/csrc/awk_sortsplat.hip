@@ -1,0 +1,286 @@
+// awk_sortsplat.hip -- deterministic sort/scan splatting on gfx950.
+//
+// cplSplating (CUDACommon/src/cudaSplat.cu:175-238): every source voxel emits
+// 8 (cell, weight*mass) pairs in the createIndex order (cudaSplat.cu:31-117,
+// sentinel cell = N for corners outside the volume); the pairs are stably
+// sorted by cell; each cell's contributions are summed.  Here:
+//   1. k_create_index  keys/values, two uint4/float4 stores per voxel
+//   2. stable LSD radix sort, 8-bit digits, ceil(log2(N+1)/8) passes:
+//        k_radix_hist  per-tile digit counts (LDS), digit-major layout
+//        k_scan_*      exclusive scan of the counts (3 kernels, integer: exact)
+//        k_radix_scatter  stable ranks from wave ballots + per-wave digit counts
+//   3. k_segment_sum   one thread per segment head, sequential sum in sorted
+//      order (float): the result is a pure function of the inputs, identical
+//      run to run and to a CPU sequential sum in stable-sorted order.
+// The reference sums with a parallel CUDPP segmented scan; the keys are the
+// bit-exact target, the sums match to rounding.
+#include <algorithm>
+#include <cstring>
+
+#include "awk_ops.h"
+
+namespace awk {
+
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 16;                     // elements per thread per tile
+constexpr int kTile = kSortBlock * kSortItems;     // 4096 elements per tile
+constexpr int kScanChunk = kSortBlock * kSortItems;
+
+// ---------------------------------------------------------------- keys
+__global__ void __launch_bounds__(256)
+k_create_index(uint4 *__restrict__ keys, float4 *__restrict__ vals, const float *__restrict__ src,
+               const float *__restrict__ vx, const float *__restrict__ vy,
+               const float *__restrict__ vz, int w, int h, int l, int truncate) {
+  const int N = w * h * l;
+  const int id = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (id >= N) return;
+  const int i = id % w, rest = id / w, j = rest % h, k = rest / h;
+  const float mass = src[id];
+  const float x = (float)i + vx[id], y = (float)j + vy[id], z = (float)k + vz[id];
+  int xi = (int)x, yi = (int)y, zi = (int)z;  // the reference truncates (cudaSplat.cu:50-52)
+  if (!truncate) {
+    if (x < 0.f && x != (float)xi) --xi;
+    if (y < 0.f && y != (float)yi) --yi;
+    if (z < 0.f && z != (float)zi) --zi;
+  }
+  const float dx = 1.f - (x - xi), dy = 1.f - (y - yi), dz = 1.f - (z - zi);
+  const unsigned sent = (unsigned)N;
+  auto in = [&](int a, int b, int c) { return a >= 0 && a < w && b >= 0 && b < h && c >= 0 && c < l; };
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    const int zz = zi + half;
+    const float wz = half ? (1.f - dz) : dz;
+    const unsigned base = (unsigned)((zz * h + yi) * w + xi);
+    uint4 c;
+    float4 d;
+    // corner order x, x+1, x+w, x+w+1 and weight products as cudaSplat.cu:64-112
+    c.x = in(xi, yi, zz) ? base : sent;
+    d.x = in(xi, yi, zz) ? mass * dx * dy * wz : 0.f;
+    c.y = in(xi + 1, yi, zz) ? base + 1 : sent;
+    d.y = in(xi + 1, yi, zz) ? mass * (1.f - dx) * dy * wz : 0.f;
+    c.z = in(xi, yi + 1, zz) ? base + w : sent;
+    d.z = in(xi, yi + 1, zz) ? mass * dx * (1.f - dy) * wz : 0.f;
+    c.w = in(xi + 1, yi + 1, zz) ? base + w + 1 : sent;
+    d.w = in(xi + 1, yi + 1, zz) ? mass * (1.f - dx) * (1.f - dy) * wz : 0.f;
+    keys[(size_t)half * N + id] = c;
+    vals[(size_t)half * N + id] = d;
+  }
+}
+
+// ---------------------------------------------------------------- radix sort
+__global__ void __launch_bounds__(kSortBlock)
+k_radix_hist(const unsigned *__restrict__ keys, long long M, int shift, int ntiles,
+             unsigned *__restrict__ hist) {
+  __shared__ unsigned cnt[kRadix];
+  const int t = blockIdx.x;
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const long long base = (long long)t * kTile;
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const long long e = base + r * kSortBlock + threadIdx.x;
+    if (e < M) atomicAdd(&cnt[(keys[e] >> shift) & (kRadix - 1)], 1u);
+  }
+  __syncthreads();
+  hist[(size_t)threadIdx.x * ntiles + t] = cnt[threadIdx.x];
+}
+
+// exclusive scan of n unsigned values, in three steps (exact integer arithmetic)
+__global__ void __launch_bounds__(kSortBlock)
+k_scan_reduce(const unsigned *__restrict__ a, long long n, unsigned *__restrict__ sums) {
+  __shared__ unsigned s[kSortBlock];
+  const long long base = (long long)blockIdx.x * kScanChunk;
+  unsigned acc = 0;
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const long long e = base + r * kSortBlock + threadIdx.x;
+    if (e < n) acc += a[e];
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kSortBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = s[0];
+}
+
+__device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned *s, unsigned &total) {
+  // Hillis-Steele over the block (kSortBlock threads)
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < kSortBlock; off <<= 1) {
+    const unsigned add = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += add;
+    __syncthreads();
+  }
+  total = s[kSortBlock - 1];
+  const unsigned incl = s[threadIdx.x];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ void __launch_bounds__(kSortBlock)
+k_scan_top(unsigned *__restrict__ sums, int n) {
+  __shared__ unsigned s[kSortBlock];
+  unsigned carry = 0;
+  for (int base = 0; base < n; base += kSortBlock) {
+    const int i = base + threadIdx.x;
+    const unsigned v = i < n ? sums[i] : 0u;
+    unsigned total;
+    const unsigned ex = block_exclusive_scan(v, s, total);
+    if (i < n) sums[i] = carry + ex;
+    carry += total;
+  }
+}
+
+__global__ void __launch_bounds__(kSortBlock)
+k_scan_down(unsigned *__restrict__ a, long long n, const unsigned *__restrict__ sums) {
+  __shared__ unsigned s[kSortBlock];
+  // thread t owns kSortItems consecutive entries
+  const long long base = (long long)blockIdx.x * kScanChunk + (long long)threadIdx.x * kSortItems;
+  unsigned v[kSortItems];
+  unsigned acc = 0;
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    v[r] = (base + r < n) ? a[base + r] : 0u;
+    acc += v[r];
+  }
+  unsigned total;
+  unsigned run = sums[blockIdx.x] + block_exclusive_scan(acc, s, total);
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    if (base + r < n) a[base + r] = run;
+    run += v[r];
+  }
+}
+
+// Stable scatter of one pass.  Elements are visited in their original order
+// (round r, thread t <-> element r*256 + t of the tile); within a wave the rank
+// among equal digits comes from 8 ballots, across waves from LDS counts.
+__global__ void __launch_bounds__(kSortBlock)
+k_radix_scatter(const unsigned *__restrict__ kin, const float *__restrict__ vin,
+                unsigned *__restrict__ kout, float *__restrict__ vout, long long M, int shift,
+                int ntiles, const unsigned *__restrict__ offs) {
+  constexpr int kWaves = kSortBlock / 64;
+  __shared__ unsigned base_off[kRadix];
+  __shared__ unsigned wcnt[kWaves][kRadix];
+  const int t = blockIdx.x;
+  base_off[threadIdx.x] = offs[(size_t)threadIdx.x * ntiles + t];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const long long tile0 = (long long)t * kTile;
+  for (int r = 0; r < kSortItems; ++r) {
+    for (int q = threadIdx.x; q < kWaves * kRadix; q += kSortBlock) (&wcnt[0][0])[q] = 0u;
+    __syncthreads();
+    const long long e = tile0 + r * kSortBlock + threadIdx.x;
+    const bool valid = e < M;
+    const unsigned key = valid ? kin[e] : 0u;
+    const float val = valid ? vin[e] : 0.f;
+    const unsigned d = (key >> shift) & (kRadix - 1);
+    unsigned long long same = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < kRadixBits; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const unsigned long long m = __ballot(bit);
+      same &= bit ? m : ~m;
+    }
+    const unsigned rank = (unsigned)__popcll(same & lt_mask);
+    // the lowest lane of each digit group publishes the group size
+    if (valid && (same & lt_mask) == 0ull) wcnt[wave][d] = (unsigned)__popcll(same);
+    __syncthreads();
+    if (valid) {
+      unsigned before = base_off[d];
+      for (int w2 = 0; w2 < wave; ++w2) before += wcnt[w2][d];
+      const unsigned pos = before + rank;
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    // advance the per-digit running offsets by this round's counts
+    unsigned add = 0;
+#pragma unroll
+    for (int w2 = 0; w2 < kWaves; ++w2) add += wcnt[w2][threadIdx.x];
+    base_off[threadIdx.x] += add;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- segment sums
+__global__ void __launch_bounds__(256)
+k_segment_sum(const unsigned *__restrict__ keys, const float *__restrict__ vals, long long M,
+              unsigned N, float *__restrict__ dst) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  const unsigned k = keys[i];
+  if (k >= N) return;                       // sentinel segment: corners outside the volume
+  if (i > 0 && keys[i - 1] == k) return;    // not a segment head
+  float acc = 0.f;
+  for (long long j = i; j < M && keys[j] == k; ++j) acc += vals[j];
+  dst[k] = acc;
+}
+
+awk_status launch_splat_sorted(float *dst, const float *src, const float *vx, const float *vy,
+                               const float *vz, int nx, int ny, int nz, int truncate,
+                               unsigned *keys_out, float *vals_out, hipStream_t s) {
+  const long long N = (long long)nx * ny * nz;
+  const long long M = 8 * N;
+  int bits = 1;
+  while ((1LL << bits) < N + 1) ++bits;
+  const int passes = (bits + kRadixBits - 1) / kRadixBits;
+  const int ntiles = (int)((M + kTile - 1) / kTile);
+  const long long nh = (long long)kRadix * ntiles;
+  const int nchunks = (int)((nh + kScanChunk - 1) / kScanChunk);
+  // scratch: two key/value buffers, histogram, chunk sums (stream-ordered)
+  const size_t bytes = 2 * (size_t)M * 4 + 2 * (size_t)M * 4 + (size_t)nh * 4 + (size_t)nchunks * 4 + 256;
+  char *scratch = nullptr;
+  AWK_HIP(hipMallocAsync((void **)&scratch, bytes, s));
+  unsigned *ka = reinterpret_cast<unsigned *>(scratch);
+  unsigned *kb = ka + M;
+  float *va = reinterpret_cast<float *>(kb + M);
+  float *vb = va + M;
+  unsigned *hist = reinterpret_cast<unsigned *>(vb + M);
+  unsigned *sums = hist + nh;
+  AWK_LAUNCH(k_create_index, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s,
+             reinterpret_cast<uint4 *>(ka), reinterpret_cast<float4 *>(va), src, vx, vy, vz, nx, ny,
+             nz, truncate);
+  AWK_LAUNCH_CHECK("k_create_index");
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * kRadixBits;
+    AWK_LAUNCH(k_radix_hist, dim3(ntiles), dim3(kSortBlock), 0, s, ka, M, shift, ntiles, hist);
+    AWK_LAUNCH(k_scan_reduce, dim3(nchunks), dim3(kSortBlock), 0, s, hist, nh, sums);
+    AWK_LAUNCH(k_scan_top, dim3(1), dim3(kSortBlock), 0, s, sums, nchunks);
+    AWK_LAUNCH(k_scan_down, dim3(nchunks), dim3(kSortBlock), 0, s, hist, nh, sums);
+    AWK_LAUNCH(k_radix_scatter, dim3(ntiles), dim3(kSortBlock), 0, s, ka, va, kb, vb, M, shift,
+               ntiles, hist);
+    AWK_LAUNCH_CHECK("radix pass");
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  AWK_HIP(hipMemsetAsync(dst, 0, (size_t)N * 4, s));
+  AWK_LAUNCH(k_segment_sum, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, ka, va, M,
+             (unsigned)N, dst);
+  AWK_LAUNCH_CHECK("k_segment_sum");
+  if (keys_out) AWK_HIP(hipMemcpyAsync(keys_out, ka, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
+  if (vals_out) AWK_HIP(hipMemcpyAsync(vals_out, va, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
+  AWK_HIP(hipFreeAsync(scratch, s));
+  return AWK_OK;
+}
+
+}  // namespace awk
+
+using namespace awk;
+
+extern "C" awk_status awk_splat_sorted(float *dst, const float *src, const float *vx,
+                                       const float *vy, const float *vz, int nx, int ny, int nz,
+                                       int reference_truncation, unsigned int *keys_out,
+                                       float *vals_out, awk_stream_t stream) {
+  if (nx < 1 || ny < 1 || nz < 1 || (long long)nx * ny * nz * 8 > (1LL << 31) - 1)
+    return fail(AWK_ERR_INVALID, "awk_splat_sorted: bad sizes (at most 2^28 voxels)");
+  if (!dst || !src || !vx || !vy || !vz) return fail(AWK_ERR_INVALID, "awk_splat_sorted: null pointer");
+  return launch_splat_sorted(dst, src, vx, vy, vz, nx, ny, nz, reference_truncation ? 1 : 0, keys_out,
+                             vals_out, as_stream(stream));
+}

@@ -101,7 +101,7 @@ using namespace awk;
 extern "C" awk_status awk_splat(float *dst, const float *src, const float *hx, const float *hy,
                                 const float *hz, int nx, int ny, int nz, int mode,
                                 awk_stream_t stream) {
-  if (nx < 1 || ny < 1 || nz < 1 || (long long)nx * ny * nz >= (1LL << 31))
+  if (nx < 1 || ny < 1 || nz < 1 || (long long)nx * ny * nz > kBufMaxElems)
     return fail(AWK_ERR_INVALID, "awk_splat: bad sizes");
   if (!dst || !src || !hx || !hy || !hz) return fail(AWK_ERR_INVALID, "awk_splat: null pointer");
   return launch_splat(dst, src, hx, hy, hz, nx, ny, nz, mode, as_stream(stream));

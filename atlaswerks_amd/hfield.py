@@ -5,6 +5,10 @@ current torch stream and write into caller-provided tensors.
 Background strategies: BACKGROUND_STRATEGY_{ID, ZERO, PARTIAL_ID, PARTIAL_ZERO,
 CLAMP, WRAP} for vector interpolation, VAL/ZERO/CLAMP/WRAP for images.
 """
+import ctypes
+
+import torch
+
 from . import _lib
 from .device import dims, planes, ptr, stream_handle
 
@@ -108,3 +112,28 @@ def splat(dst, src, h, mode=_lib.SPLAT_FIXED64, stream=None):
 
 
 cplSplat3DH = splat
+
+
+def splat_sorted(dst, src, v, reference_truncation=True, keys=None, vals=None, stream=None):
+    """cplSplating (cudaSplat.cu:175-238): sort/scan splat of src to voxel + v.
+
+    Deterministic.  ``reference_truncation`` keeps the reference's int()
+    truncation of negative positions (bit-exact keys); False floors them.
+    ``keys`` (int32, 8*N, holding uint32 cells) / ``vals`` (float32, 8*N) receive
+    the sorted (cell, contribution) pairs when given."""
+    nx, ny, nz = _same(dst, src, v)
+    n8 = 8 * nx * ny * nz
+    kp = vp = None
+    if keys is not None:
+        if not keys.is_cuda or keys.dtype != torch.int32 or keys.numel() != n8 or not keys.is_contiguous():
+            raise ValueError("keys must be a contiguous int32 GPU tensor of 8*N elements")
+        kp = ctypes.c_void_p(keys.data_ptr())
+    if vals is not None:
+        if vals.numel() != n8:
+            raise ValueError("vals must hold 8*N elements")
+        vp = ptr(vals)
+    _lib.call("awk_splat_sorted", ptr(dst), ptr(src), *planes(v), nx, ny, nz,
+              int(bool(reference_truncation)), kp, vp, stream_handle(stream))
+
+
+cplSplating = splat_sorted

@@ -1,0 +1,153 @@
+"""LDMM registration (adjoint-gradient ``IterateNew``) on the GPU.
+
+Mirrors ``LDMMIteratorGPU`` (Algorithms/LDMM/LDMMIteratorGPU.cxx:159-293), the
+per-registration state of ``LDMMDeformationDataGPU`` (T velocity fields, I0, I1)
+and the single-scale loop of ``LDMMWarp::RunWarp`` (Algorithms/LDMM/LDMMWarp.txx:
+69-125).  Each ``Iterate`` is one ``awk_ldmm_iterate`` call: 2T fused gather
+kernels, T Navier solves with the force and the velocity update fused into the
+FFT x passes, no host synchronisation.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from .device import ptr, stream_handle
+
+
+class LDMMIteratorParam:
+    """LDMMIteratorParam (LDMMIteratorParam.h) + DiffOperParam."""
+
+    def __init__(self, sigma=0.05, max_pert=0.1, alpha=0.01, beta=0.01, gamma=0.001,
+                 use_adaptive_step_size=True, step_size=0.005):
+        self.Sigma = sigma
+        self.MaxPert = max_pert
+        self.alpha, self.beta, self.gamma = alpha, beta, gamma
+        self.UseAdaptiveStepSize = use_adaptive_step_size
+        self.StepSize = step_size
+
+
+class LDMMDeformationDataGPU:
+    """I0 (moving), I1 (fixed) and the velocity fields v_t, each [3, nz, ny, nx]."""
+
+    def __init__(self, I0, I1, n_time_steps, spacing=(1.0, 1.0, 1.0), max_iters=4096):
+        if I0.shape != I1.shape:
+            raise ValueError("I0 and I1 must have the same size")
+        self.I0, self.I1 = I0.contiguous(), I1.contiguous()
+        self.spacing = tuple(float(s) for s in spacing)
+        nz, ny, nx = I0.shape
+        self.v = torch.zeros((n_time_steps, 3, nz, ny, nx), dtype=torch.float32, device=I0.device)
+        self._energy = torch.zeros((max_iters, 3), dtype=torch.float64, device=I0.device)
+        self.cur_iter = 0
+
+    def planes(self):
+        T = self.v.shape[0]
+        arr = (ctypes.c_void_p * (3 * T))()
+        for t in range(T):
+            for c in range(3):
+                arr[3 * t + c] = self.v[t, c].data_ptr()
+        return arr
+
+    def energy_slot(self):
+        if self.cur_iter >= self._energy.shape[0]:
+            grown = torch.zeros((2 * self._energy.shape[0], 3), dtype=torch.float64,
+                                device=self._energy.device)
+            grown[: self._energy.shape[0]] = self._energy
+            self._energy = grown
+        return self._energy[self.cur_iter]
+
+    def energies(self):
+        """[iterations, 3]: image, vector and total energy (host numpy, synchronises)."""
+        return self._energy[: self.cur_iter].cpu().numpy()
+
+
+class LDMMIteratorGPU:
+    def __init__(self, size, spacing, n_time_steps, param=None):
+        self.param = param or LDMMIteratorParam()
+        self._h = None
+        self.SetScaleLevel(size, spacing, n_time_steps, self.param)
+
+    def SetScaleLevel(self, size, spacing, n_time_steps, param=None):
+        if param is not None:
+            self.param = param
+        self._destroy()
+        self.size = tuple(int(s) for s in size)
+        self.spacing = tuple(float(s) for s in spacing)
+        self.n_time_steps = int(n_time_steps)
+        h = ctypes.c_void_p()
+        p = self.param
+        _lib.call("awk_ldmm_create", ctypes.byref(h), *self.size, *self.spacing, p.alpha, p.beta,
+                  p.gamma, self.n_time_steps, p.Sigma, p.MaxPert)
+        self._h = h
+        if not p.UseAdaptiveStepSize:
+            _lib.call("awk_ldmm_set_step", self._h, float(p.StepSize))
+
+    def UpdateStepSizeNextIteration(self):
+        _lib.call("awk_ldmm_update_step_next", self._h)
+
+    def StepSize(self):
+        out = ctypes.c_float()
+        _lib.call("awk_ldmm_get_step", self._h, ctypes.byref(out))
+        return out.value
+
+    def _check(self, data):
+        nz, ny, nx = data.I0.shape
+        if (nx, ny, nz) != self.size or data.v.shape[0] != self.n_time_steps:
+            raise ValueError("deformation data does not match the iterator's grid")
+
+    def Iterate(self, data, vec_energy=False, record_energy=True, stream=None):
+        self._check(data)
+        e = data.energy_slot() if record_energy else None
+        _lib.call("awk_ldmm_iterate", self._h, ptr(data.I0), ptr(data.I1), data.planes(),
+                  ctypes.c_void_p(e.data_ptr()) if e is not None else None, int(bool(vec_energy)),
+                  stream_handle(stream))
+        if record_energy:
+            data.cur_iter += 1
+
+    def ComputeAlphas(self, data, stream=None):
+        """IterateNew(computeAlphasOnly=true): [T, nz, ny, nx] alpha images."""
+        self._check(data)
+        alpha = torch.empty((self.n_time_steps,) + tuple(data.I0.shape), dtype=torch.float32,
+                            device=data.I0.device)
+        arr = (ctypes.c_void_p * self.n_time_steps)(*[alpha[t].data_ptr() for t in range(self.n_time_steps)])
+        _lib.call("awk_ldmm_alphas", self._h, ptr(data.I0), ptr(data.I1), data.planes(), arr,
+                  stream_handle(stream))
+        return alpha
+
+    def DeformedImage(self, t, stream=None):
+        """J_t = I0 o h_t of the last iterate."""
+        nx, ny, nz = self.size
+        out = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda")
+        _lib.call("awk_ldmm_deformed_image", self._h, int(t), ptr(out), stream_handle(stream))
+        return out
+
+    def _destroy(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.call("awk_ldmm_destroy", self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self._destroy()
+        except Exception:
+            pass
+
+
+class LDMMWarp:
+    """LDMMWarp::RunWarp for one scale level (LDMMWarp.txx:94-125): an extra
+    step-size call when the step is adaptive, then n_iter iterations."""
+
+    def __init__(self, I0, I1, n_time_steps=10, n_iter=40, spacing=(1.0, 1.0, 1.0), param=None):
+        self.data = LDMMDeformationDataGPU(I0, I1, n_time_steps, spacing)
+        nz, ny, nx = I0.shape
+        self.param = param or LDMMIteratorParam()
+        self.iterator = LDMMIteratorGPU((nx, ny, nz), spacing, n_time_steps, self.param)
+        self.n_iter = n_iter
+
+    def RunWarp(self, vec_energy=True):
+        if self.param.UseAdaptiveStepSize:
+            self.iterator.UpdateStepSizeNextIteration()
+            self.iterator.Iterate(self.data, record_energy=False)
+        for _ in range(self.n_iter):
+            self.iterator.Iterate(self.data, vec_energy=vec_energy)
+        return self.data

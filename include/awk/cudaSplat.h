@@ -17,4 +17,12 @@ inline void cplSplat3DH(float *d_dst, float *d_src, float *d_px, float *d_py, fl
   AWK_CHECK(awk_splat(d_dst, d_src, d_px, d_py, d_pz, (int)sizeX, (int)sizeY, (int)sizeZ, mode, s));
 }
 
+// cplSplating (CUDACommon/inc/cudaSplat.h:32-34): sort/scan splat to voxel + v,
+// keeping the reference's truncation of negative positions.
+inline void cplSplating(float *d_dst, float *d_src, float *d_vx, float *d_vy, float *d_vz,
+                        unsigned int w, unsigned int h, unsigned int l, awkStream_t s = nullptr) {
+  AWK_CHECK(awk_splat_sorted(d_dst, d_src, d_vx, d_vy, d_vz, (int)w, (int)h, (int)l, 1, nullptr,
+                             nullptr, s));
+}
+
 #endif

@@ -168,6 +168,20 @@ enum {
 awk_status awk_splat(float *dst, const float *src, const float *hx, const float *hy,
                      const float *hz, int nx, int ny, int nz, int mode, awk_stream_t stream);
 
+/* Sort/scan splatting, cplSplating (CUDACommon/inc/cudaSplat.h:32-34,
+ * cudaSplat.cu:175-238): positions are voxel + (vx, vy, vz) (a velocity in
+ * voxels).  Every voxel emits 8 (cell, weight * src) pairs in the createIndex
+ * order (cudaSplat.cu:31-117; sentinel cell nx*ny*nz for corners outside the
+ * volume), a stable radix sort orders them by cell, and each cell's pairs are
+ * summed in sorted order.  Deterministic.  reference_truncation != 0 keeps the
+ * reference's int() truncation of negative positions (cudaSplat.cu:50-52,
+ * bit-exact keys); 0 floors them (the trilinear weights the other splats use).
+ * keys_out / vals_out (8*nx*ny*nz each, device, may be NULL) receive the sorted
+ * pairs.  At most 2^28 voxels. */
+awk_status awk_splat_sorted(float *dst, const float *src, const float *vx, const float *vy,
+                            const float *vz, int nx, int ny, int nz, int reference_truncation,
+                            unsigned int *keys_out, float *vals_out, awk_stream_t stream);
+
 /* ====================================================================== */
 /* Greedy-fluid iteration                                                   */
 /* ====================================================================== */
@@ -221,6 +235,43 @@ awk_status awk_greedy_set_profiling(awk_greedy_t g, int on);
 /* Sums the recorded stage times (ms) over all iterations since profiling was
  * switched on or last read (synchronises), and resets the record. */
 awk_status awk_greedy_get_profile(awk_greedy_t g, double *stage_ms, int *iterations);
+
+/* ====================================================================== */
+/* LDMM (adjoint-gradient iteration, IterateNew)                            */
+/* ====================================================================== */
+
+typedef struct awk_ldmm *awk_ldmm_t;
+
+/* LDMMIteratorGPU (Algorithms/LDMM/LDMMIteratorGPU.h, LDMMIteratorGPU.cxx:159-293)
+ * for one nx*ny*nz grid with n_time_steps velocity fields, the Navier operator
+ * (alpha, beta, gamma), sigma and MaxPert.  The first iterate computes the step
+ * size (UseAdaptiveStepSize, LDMMWarp.txx:97-103) unless awk_ldmm_set_step is
+ * called first.  At most 2^28 voxels. */
+awk_status awk_ldmm_create(awk_ldmm_t *out, int nx, int ny, int nz, float spx, float spy,
+                           float spz, float alpha, float beta, float gamma, int n_time_steps,
+                           float sigma, float max_pert);
+awk_status awk_ldmm_destroy(awk_ldmm_t l);
+/* LDMMIteratorGPU::UpdateStepSizeNextIteration: the next iterate only sets the step. */
+awk_status awk_ldmm_update_step_next(awk_ldmm_t l);
+awk_status awk_ldmm_set_step(awk_ldmm_t l, float step);
+awk_status awk_ldmm_get_step(awk_ldmm_t l, float *step); /* synchronises */
+
+/* One IterateNew(computeAlphasOnly = false) (LDMMIteratorCPU.cxx:529-631).
+ * v: 3 * n_time_steps device planes, v[3t + c] = component c of v_t, updated in
+ * place (v_t = (1 - 2 eps) v_t + (2 eps / sigma^2) L^-1(grad(J_{t-1}) alpha_t)).
+ * energy_dev (3 device doubles, may be NULL): image energy Prod(sp) sum(diff^2) /
+ * sigma^2 before the update, vector energy mean_t Prod(sp) <L v_t, v_t> after it
+ * (only when vec_energy != 0: one extra operator application per time step),
+ * and their sum (LDMMEnergy.cxx:48-64).  FP64, fixed summation order.  No host
+ * synchronisation. */
+awk_status awk_ldmm_iterate(awk_ldmm_t l, const float *I0, const float *I1, float *const v[],
+                            double *energy_dev, int vec_energy, awk_stream_t stream);
+/* IterateNew(computeAlphasOnly = true): alpha[t] = splat(J_{T-1} - I1, h_t) / sigma^2
+ * (LDMMIteratorGPU.cxx:197-204) for every t; v is not changed. */
+awk_status awk_ldmm_alphas(awk_ldmm_t l, const float *I0, const float *I1, float *const v[],
+                           float *const alpha[], awk_stream_t stream);
+/* Copies J_t = I0 o h_t of the last iterate (the reference's mdJ0t[t]). */
+awk_status awk_ldmm_deformed_image(awk_ldmm_t l, int t, float *out, awk_stream_t stream);
 
 #ifdef __cplusplus
 }

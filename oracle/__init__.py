@@ -185,6 +185,21 @@ def splat_index(src, v, floor_fix=True):
     return keys, vals
 
 
+def splat_sorted(src, v, floor_fix=False):
+    """cplSplating restated: (dst, sorted_keys, sorted_vals); positions voxel + v (AoS v)."""
+    src = _f32(src)
+    vx = np.ascontiguousarray(v[..., 0], dtype=np.float32)
+    vy = np.ascontiguousarray(v[..., 1], dtype=np.float32)
+    vz = np.ascontiguousarray(v[..., 2], dtype=np.float32)
+    n = src.size
+    dst = np.zeros(src.shape, dtype=np.float32)
+    keys = np.zeros(8 * n, dtype=np.uint32)
+    vals = np.zeros(8 * n, dtype=np.float32)
+    lib().orc_splat_sorted(_p(src), _p(vx), _p(vy), _p(vz), _size(src.shape), int(floor_fix),
+                           _p(dst), _p(keys), _p(vals))
+    return dst, keys, vals
+
+
 def gradient(img, sp=(1, 1, 1), wrap=False, dtype=np.float64):
     img = _f32(img)
     out = np.zeros(img.shape + (3,), dtype=dtype)
@@ -230,6 +245,58 @@ def greedy_run(I0, I1, n_iter, sp=(1, 1, 1), alpha=0.01, beta=0.01, gamma=0.001,
                                ctypes.byref(step))
         ef = ed.copy()
     return h, hinv, ef, ed, step.value
+
+
+class LdmmParams(ctypes.Structure):
+    _fields_ = [
+        ("size", ctypes.c_int * 3),
+        ("sp", ctypes.c_double * 3),
+        ("alpha", ctypes.c_double),
+        ("beta", ctypes.c_double),
+        ("gamma", ctypes.c_double),
+        ("sigma", ctypes.c_double),
+        ("max_pert", ctypes.c_double),
+        ("n_time_steps", ctypes.c_int),
+        ("n_iter", ctypes.c_int),
+    ]
+
+
+def _ldmm_params(shape3, n_t, n_iter, sp, alpha, beta, gamma, sigma, max_pert):
+    p = LdmmParams()
+    nz, ny, nx = shape3
+    p.size[:] = [nx, ny, nz]
+    p.sp[:] = [float(s) for s in sp]
+    p.alpha, p.beta, p.gamma, p.sigma, p.max_pert = alpha, beta, gamma, sigma, max_pert
+    p.n_time_steps, p.n_iter = int(n_t), int(n_iter)
+    return p
+
+
+def ldmm_run(I0, I1, n_t, n_iter, v=None, step=0.0, sp=(1, 1, 1), alpha=0.01, beta=0.01,
+             gamma=0.001, sigma=0.05, max_pert=0.1, dtype=np.float64):
+    """LDMMWarp single scale: IterateNew x n_iter.  v: [T, z, y, x, 3] AoS (zeros if
+    None).  step <= 0: one extra call first sets the step size.  Returns (v, e_img,
+    e_vec, step)."""
+    I0, I1 = _f32(I0), _f32(I1)
+    p = _ldmm_params(I0.shape, n_t, n_iter, sp, alpha, beta, gamma, sigma, max_pert)
+    v = (np.zeros((n_t,) + I0.shape + (3,), dtype) if v is None
+         else np.ascontiguousarray(v, dtype=dtype).copy())
+    ei = np.zeros(n_iter)
+    ev = np.zeros(n_iter)
+    st = ctypes.c_double(step)
+    fn = lib().orc_ldmm_run_d if dtype == np.float64 else lib().orc_ldmm_run_f
+    fn(ctypes.byref(p), _p(I0), _p(I1), _p(v), _p(ei), _p(ev), ctypes.byref(st))
+    return v, ei, ev, st.value
+
+
+def ldmm_alphas(I0, I1, v, sp=(1, 1, 1), sigma=0.05):
+    """IterateNew(computeAlphasOnly): alpha_t images [T, z, y, x]."""
+    I0, I1 = _f32(I0), _f32(I1)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    n_t = v.shape[0]
+    p = _ldmm_params(I0.shape, n_t, 1, sp, 0.01, 0.01, 0.001, sigma, 0.1)
+    a = np.zeros((n_t,) + I0.shape, np.float32)
+    lib().orc_ldmm_alphas_d(ctypes.byref(p), _p(I0), _p(I1), _p(v), _p(a))
+    return a
 
 
 def greedy_time(shape3, n_iter=1, seed=0):

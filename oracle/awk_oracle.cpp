@@ -494,8 +494,8 @@ void apply_t(const float *img, const float *h, const int *size, TO *out, int bg,
 }
 
 /* Array3DUtils::computeGradient (Array3DUtils.txx:783-901) -> AoS */
-template <class TO>
-void gradient_t(const float *a, const int *size, const double *sp, int wrap, TO *g) {
+template <class TO, class TI>
+void gradient_t(const TI *a, const int *size, const double *sp, int wrap, TO *g) {
   const int sx = size[0], sy = size[1], sz = size[2];
   auto A = [&](int x, int y, int z) -> double { return (double)a[((size_t)z * sy + y) * sx + x]; };
   auto G = [&](int x, int y, int z, int c) -> TO & { return g[3 * (((size_t)z * sy + y) * sx + x) + c]; };
@@ -675,6 +675,144 @@ int greedy_run(const orc_greedy_params *p, const float *I0, const float *I1, T *
     if (it > 0 && p->auto_step_reduce && e - prev > 0) step = step / 2.0;
     prev = e;
   }
+  if (step_out) *step_out = step;
+  return 0;
+}
+
+/* ====================================================================== */
+/* LDMM                                                                    */
+/* ====================================================================== */
+
+/* cpuSplat3D semantics (cudaSplatAtomic.cu:602-663) for positions and masses of
+ * type T, accumulated in double. */
+template <class T>
+void splat_t(const T *m, const T *h, const int *size, double *out) {
+  const int w = size[0], hh = size[1], l = size[2];
+  const size_t N = (size_t)w * hh * l;
+  for (size_t i = 0; i < N; ++i) out[i] = 0.0;
+  for (size_t i = 0; i < N; ++i) {
+    const double x = h[3 * i], y = h[3 * i + 1], z = h[3 * i + 2];
+    const int xi = ref_floor(x), yi = ref_floor(y), zi = ref_floor(z);
+    const double dx = 1.0 - (x - xi), dy = 1.0 - (y - yi), dz = 1.0 - (z - zi);
+    const double wx[2] = {dx, 1.0 - dx}, wy[2] = {dy, 1.0 - dy}, wz[2] = {dz, 1.0 - dz};
+    for (int c = 0; c < 8; ++c) {
+      const int ox = c & 1, oy = (c >> 1) & 1, oz = c >> 2;
+      const int X = xi + ox, Y = yi + oy, Z = zi + oz;
+      if (X >= 0 && X < w && Y >= 0 && Y < hh && Z >= 0 && Z < l)
+        out[((size_t)Z * hh + Y) * w + X] += (double)m[i] * wx[ox] * wy[oy] * wz[oz];
+    }
+  }
+}
+
+/* LDMMIteratorCPU::IterateNew (LDMMIteratorCPU.cxx:529-631) with its helpers
+ * updateVelocity (:175-190) and calcMaxDisplacement (:192-212); energies as
+ * LDMMEnergy (LDMMEnergy.cxx:48-64, LDMMEnergy.h:42): image energy
+ * Prod(sp) * sum(diff^2) / sigma^2, vector energy mean over t of
+ * Prod(sp) * <L v_t, v_t> (LDMMIteratorGPU.cxx:250-256).  v: T fields, AoS. */
+template <class T>
+void ldmm_iterate(const orc_ldmm_params *p, const float *I0, const float *I1, T *v,
+                  bool update_step, double &step, double *e_img, double *e_vec, float *alpha) {
+  const int *size = p->size;
+  const int sx = size[0], sy = size[1], sz = size[2];
+  const size_t N = (size_t)sx * sy * sz;
+  const int nt = p->n_time_steps;
+  const double sig2 = p->sigma * p->sigma;
+  const double vol = p->sp[0] * p->sp[1] * p->sp[2];
+  std::vector<T> h(3 * N), hs(3 * N), J((size_t)nt * N), diff(N), f(3 * N), lv;
+  std::vector<double> si(N);
+  auto identity = [&](std::vector<T> &g) {
+    size_t i = 0;
+    for (int z = 0; z < sz; ++z)
+      for (int y = 0; y < sy; ++y)
+        for (int x = 0; x < sx; ++x, ++i) {
+          g[3 * i] = (T)x; g[3 * i + 1] = (T)y; g[3 * i + 2] = (T)z;
+        }
+  };
+  // forward: h <- composeHVInv(h, v_t) (PARTIAL_ID); J_t = I0(h)
+  identity(h);
+  for (int t = 0; t < nt; ++t) {
+    const T *vt = v + (size_t)t * 3 * N;
+    size_t i = 0;
+    for (int z = 0; z < sz; ++z)
+      for (int y = 0; y < sy; ++y)
+        for (int x = 0; x < sx; ++x, ++i) {
+          double o[3];
+          trilerp_vec<T>(h.data(), size, x - (double)vt[3 * i] / p->sp[0],
+                         y - (double)vt[3 * i + 1] / p->sp[1], z - (double)vt[3 * i + 2] / p->sp[2],
+                         o, ORC_BG_PARTIAL_ID);
+          hs[3 * i] = (T)o[0]; hs[3 * i + 1] = (T)o[1]; hs[3 * i + 2] = (T)o[2];
+        }
+    std::swap(h, hs);
+    T *Jt = J.data() + (size_t)t * N;
+    for (size_t j = 0; j < N; ++j)
+      Jt[j] = (T)trilerp_scalar<float>(I0, size, (double)h[3 * j], (double)h[3 * j + 1],
+                                       (double)h[3 * j + 2], ORC_BG_VAL, 0.0);
+  }
+  double ssd = 0;
+  const T *JT = J.data() + (size_t)(nt - 1) * N;
+  for (size_t j = 0; j < N; ++j) {
+    diff[j] = (T)(JT[j] - (T)I1[j]);
+    ssd += (double)diff[j] * (double)diff[j];
+  }
+  if (e_img) *e_img = vol * ssd / sig2;
+  // backward: h <- composeVHInv(v_t, h) (PARTIAL_ZERO); alpha_t = splat(diff, h)
+  identity(h);
+  double maxdisp = 0, vec = 0;
+  for (int t = nt - 1; t >= 0; --t) {
+    T *vt = v + (size_t)t * 3 * N;
+    for (size_t j = 0; j < N; ++j) {
+      double o[3];
+      trilerp_vec<T>(vt, size, (double)h[3 * j], (double)h[3 * j + 1], (double)h[3 * j + 2], o,
+                     ORC_BG_PARTIAL_ZERO);
+      for (int c = 0; c < 3; ++c) hs[3 * j + c] = (T)(o[c] / -p->sp[c] + (double)h[3 * j + c]);
+    }
+    std::swap(h, hs);
+    splat_t<T>(diff.data(), h.data(), size, si.data());
+    if (alpha) {
+      for (size_t j = 0; j < N; ++j) alpha[(size_t)t * N + j] = (float)(si[j] / sig2);
+      continue;
+    }
+    // force = grad(J_{t-1}) * alpha_t (grad(I0) at t = 0)
+    if (t > 0) gradient_t<T>(J.data() + (size_t)(t - 1) * N, size, p->sp, 0, f.data());
+    else gradient_t<T>(I0, size, p->sp, 0, f.data());
+    for (size_t j = 0; j < N; ++j)
+      for (int c = 0; c < 3; ++c) f[3 * j + c] = (T)(f[3 * j + c] * (T)si[j]);
+    diffoper<T>(size, p->sp, p->alpha, p->beta, p->gamma, 1.0, 0, f.data(), true);
+    if (update_step) {
+      for (size_t j = 0; j < N; ++j) {
+        double l2 = 0;
+        for (int c = 0; c < 3; ++c) {
+          const double q = 2.0 * (double)vt[3 * j + c] - (2.0 / sig2) * (double)f[3 * j + c];
+          l2 += q * q;
+        }
+        maxdisp = std::max(maxdisp, std::sqrt(l2));
+      }
+    } else {
+      const T a = (T)(1.0 - 2.0 * step), b = (T)(2.0 * step / sig2);
+      for (size_t j = 0; j < 3 * N; ++j) vt[j] = (T)(a * vt[j] + b * f[j]);
+      if (e_vec) {
+        lv.assign(vt, vt + 3 * N);
+        diffoper<T>(size, p->sp, p->alpha, p->beta, p->gamma, 1.0, 0, lv.data(), false);
+        double d = 0;
+        for (size_t j = 0; j < 3 * N; ++j) d += (double)lv[j] * (double)vt[j];
+        vec += vol * d;
+      }
+    }
+  }
+  if (update_step && !alpha) step = p->max_pert / maxdisp;
+  if (e_vec) *e_vec = update_step ? 0.0 : vec / nt;
+}
+
+template <class T>
+int ldmm_run(const orc_ldmm_params *p, const float *I0, const float *I1, T *v, double *e_img,
+             double *e_vec, double *step_out) {
+  // LDMMWarp::RunWarp (LDMMWarp.txx:97-110): with an adaptive step, one extra
+  // IterateNew call before the first iteration only sets the step size.
+  double step = step_out ? *step_out : 0.0;
+  if (step <= 0.0) ldmm_iterate<T>(p, I0, I1, v, true, step, nullptr, nullptr, nullptr);
+  for (int it = 0; it < p->n_iter; ++it)
+    ldmm_iterate<T>(p, I0, I1, v, false, step, e_img ? e_img + it : nullptr,
+                    e_vec ? e_vec + it : nullptr, nullptr);
   if (step_out) *step_out = step;
   return 0;
 }
@@ -878,6 +1016,34 @@ void orc_splat_index(const float *src, const float *vx, const float *vy, const f
       }
 }
 
+void orc_splat_sorted(const float *src, const float *vx, const float *vy, const float *vz,
+                      const int *size, int floor_fix, float *dst, unsigned int *sorted_keys,
+                      float *sorted_vals) {
+  const size_t N = (size_t)size[0] * size[1] * size[2];
+  std::vector<unsigned int> keys(8 * N);
+  std::vector<float> vals(8 * N);
+  orc_splat_index(src, vx, vy, vz, size, floor_fix, keys.data(), vals.data());
+  std::vector<unsigned int> order(8 * N);
+  for (size_t i = 0; i < 8 * N; ++i) order[i] = (unsigned int)i;
+  // stable sort by key: equal keys keep the flattened createIndex order
+  std::stable_sort(order.begin(), order.end(),
+                   [&](unsigned int a, unsigned int b) { return keys[a] < keys[b]; });
+  for (size_t i = 0; i < N; ++i) dst[i] = 0.f;
+  size_t i = 0;
+  while (i < 8 * N) {
+    const unsigned int k = keys[order[i]];
+    float acc = 0.f;
+    size_t j = i;
+    for (; j < 8 * N && keys[order[j]] == k; ++j) acc += vals[order[j]];
+    if (k < N) dst[k] = acc;  // the sentinel segment (k == N) is dropped
+    i = j;
+  }
+  if (sorted_keys)
+    for (size_t t = 0; t < 8 * N; ++t) sorted_keys[t] = keys[order[t]];
+  if (sorted_vals)
+    for (size_t t = 0; t < 8 * N; ++t) sorted_vals[t] = vals[order[t]];
+}
+
 void orc_gradient_d(const float *img, const int *size, const double *sp, int wrap, double *grad) {
   gradient_t<double>(img, size, sp, wrap, grad);
 }
@@ -894,6 +1060,23 @@ int orc_greedy_run_f(const orc_greedy_params *p, const float *I0, const float *I
 int orc_greedy_run_d(const orc_greedy_params *p, const float *I0, const float *I1, double *h,
                      double *hinv, double *energy_d, double *step_out) {
   return greedy_run<double>(p, I0, I1, h, hinv, nullptr, energy_d, step_out);
+}
+
+int orc_ldmm_run_d(const orc_ldmm_params *p, const float *I0, const float *I1, double *v,
+                   double *e_img, double *e_vec, double *step) {
+  return ldmm_run<double>(p, I0, I1, v, e_img, e_vec, step);
+}
+
+int orc_ldmm_run_f(const orc_ldmm_params *p, const float *I0, const float *I1, float *v,
+                   double *e_img, double *e_vec, double *step) {
+  return ldmm_run<float>(p, I0, I1, v, e_img, e_vec, step);
+}
+
+void orc_ldmm_alphas_d(const orc_ldmm_params *p, const float *I0, const float *I1,
+                       const double *v, float *alpha) {
+  std::vector<double> vc(v, v + (size_t)3 * p->n_time_steps * p->size[0] * p->size[1] * p->size[2]);
+  double step = 0;
+  ldmm_iterate<double>(p, I0, I1, vc.data(), false, step, nullptr, nullptr, alpha);
 }
 
 double orc_greedy_time_f(const int *size, int n_iter, int seed) {

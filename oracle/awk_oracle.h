@@ -86,6 +86,13 @@ void orc_splat_d(const float *src, const float *px, const float *py, const float
 void orc_splat_index(const float *src, const float *vx, const float *vy, const float *vz,
                      const int *size, int floor_fix, unsigned int *keys, float *vals);
 
+/* cplSplating (cudaSplat.cu:175-238) restated: createIndex keys/values, a stable
+ * sort by key (std::stable_sort), then per-cell sums taken sequentially in the
+ * sorted order (float).  sorted_keys/sorted_vals (8N each) may be NULL. */
+void orc_splat_sorted(const float *src, const float *vx, const float *vy, const float *vz,
+                      const int *size, int floor_fix, float *dst, unsigned int *sorted_keys,
+                      float *sorted_vals);
+
 /* ---- gradient / residual / energy ---- */
 /* computeGradient (Array3DUtils.txx:783-901), output AoS */
 void orc_gradient_d(const float *img, const int *size, const double *sp, int wrap,
@@ -116,6 +123,31 @@ int orc_greedy_run_d(const orc_greedy_params *p, const float *I0, const float *I
 /* One FP32 iteration timed for the CPU baseline (same as orc_greedy_run_f with
  * n_iter=1 but with preallocated state; returns seconds per iteration). */
 double orc_greedy_time_f(const int *size, int n_iter, int seed);
+
+/* ---- LDMM (LDMMIteratorCPU.cxx:175-212, 529-631; LDMMEnergy.cxx:48-64) ---- */
+typedef struct {
+  int size[3];
+  double sp[3];
+  double alpha, beta, gamma;
+  double sigma;
+  double max_pert;
+  int n_time_steps;
+  int n_iter;
+} orc_ldmm_params;
+
+/* n_iter IterateNew updates of velocity fields v (n_time_steps AoS fields, in/out).
+ * *step <= 0 on entry: one extra call first computes the step size (MaxPert /
+ * max over t of |2 v_t - 2/sigma^2 L^-1 f_t|) and leaves v unchanged, as
+ * mUpdateStepSizeNextIter does.  e_img / e_vec (n_iter each, may be NULL): image
+ * energy (before the update) and vector energy (after it).  *step: in/out. */
+int orc_ldmm_run_d(const orc_ldmm_params *p, const float *I0, const float *I1, double *v,
+                   double *e_img, double *e_vec, double *step);
+int orc_ldmm_run_f(const orc_ldmm_params *p, const float *I0, const float *I1, float *v,
+                   double *e_img, double *e_vec, double *step);
+/* IterateNew(computeAlphasOnly = true): alpha_t = splat(J0t_T - I1, h_t) / sigma^2
+ * for every t (n_time_steps images, float). */
+void orc_ldmm_alphas_d(const orc_ldmm_params *p, const float *I0, const float *I1,
+                       const double *v, float *alpha);
 
 /* ---- FluidWarp::_computeHFieldAsymmetric (FluidWarp.cxx:925-1175) ---- */
 int orc_fluidwarp_asym_f(const float *fixed, const float *moving, const int *size,

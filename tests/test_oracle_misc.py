@@ -140,3 +140,37 @@ def test_metaimage_roundtrip(tmp_path, golden_dir):
     nr = read_nrrd(os.path.join(golden_dir, "BullseyeTestBlur00.nhdr"))
     assert nr.data.shape == (36, 31, 32)
     np.testing.assert_allclose(nr.spacing, (2.0, 2.06452, 1.77778))
+
+
+def test_splat_sorted_pinned():
+    """cplSplating (cudaSplat.cu:175-238) restated: stable sort + sequential sums.
+
+    Pinned three ways: (1) with floored positions it equals cpuSplat3D's double
+    accumulation to float rounding; (2) its sorted pairs equal numpy's stable
+    argsort of the createIndex pairs; (3) for positions >= 0 truncation and floor
+    agree bit for bit, and a negative position gives the extrapolated weights
+    (1.5, -0.5) that int() truncation implies (by hand)."""
+    rng = np.random.default_rng(0)
+    shape = (12, 10, 14)
+    src = rng.random(shape).astype(np.float32)
+    v = (rng.random(shape + (3,)) * 3 - 1.5).astype(np.float32)
+    ref = oracle.splat(src, oracle.identity(shape) + v)
+    d, k, vals = oracle.splat_sorted(src, v, floor_fix=True)
+    assert np.abs(d - ref).max() < 1e-5 * np.abs(ref).max()
+    keys, vv = oracle.splat_index(src, v, floor_fix=True)
+    o = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(keys[o], k)
+    np.testing.assert_array_equal(vv[o], vals)
+    vpos = np.abs(v)  # every position >= 0
+    a = oracle.splat_sorted(src, vpos, floor_fix=True)
+    b = oracle.splat_sorted(src, vpos, floor_fix=False)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    one = np.zeros((1, 1, 2), np.float32)
+    one[0, 0, 0] = 2.0
+    v1 = np.zeros((1, 1, 2, 3), np.float32)
+    v1[0, 0, 0, 0] = -0.5
+    d_t, _, _ = oracle.splat_sorted(one, v1, floor_fix=False)
+    d_f, _, _ = oracle.splat_sorted(one, v1, floor_fix=True)
+    np.testing.assert_allclose(d_t[0, 0], [3.0, -1.0])   # int(-0.5) = 0: dx = 1.5
+    np.testing.assert_allclose(d_f[0, 0], [1.0, 0.0])    # floor: corner -1 dropped
