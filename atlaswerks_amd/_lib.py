@@ -34,6 +34,8 @@ SIGNATURES = {
     "awk_hfield_h_to_velocity": [_P, _P, _P, _I, _I, _I, _F, _F, _F, _P],
     "awk_hfield_inverse_zeroth_order": [_P, _P, _P, _P, _P, _P, _I, _I, _I, _P],
     "awk_hfield_apply": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _F, _P],
+    "awk_hfield_apply_grid": [_P, _P, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _F, _P],
+    "awk_hfield_init_affine": [_P, _P, _P, _I, _I, _I, _P, _P],
     "awk_hfield_compose": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _P],
     "awk_hfield_compose_hv": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _F, _F, _F, _F, _I,
                               _I, _P],
@@ -44,6 +46,15 @@ SIGNATURES = {
     "awk_splat_sorted": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _P, _P, _P],
     "awk_greedy_create": [ctypes.POINTER(_P), _I, _I, _I, _F, _F, _F, _F, _F, _F, _F],
     "awk_greedy_destroy": [_P],
+    "awk_atlas_create": [ctypes.POINTER(_P), _I, _I, _I, _F, _F, _F, _F, _F, _F, _F, _I],
+    "awk_atlas_destroy": [_P],
+    "awk_atlas_set_subject": [_P, _I, _P, _I, _I, _I, _F, ctypes.POINTER(_P3), _P],
+    "awk_atlas_get_hfield": [_P, _I, ctypes.POINTER(_P3), _P],
+    "awk_atlas_update_step_next": [_P],
+    "awk_atlas_set_graph": [_P, _I],
+    "awk_atlas_iterate": [_P, _I, _P, _P],
+    "awk_atlas_mean": [_P, _P, _P],
+    "awk_atlas_deformed_image": [_P, _I, _P, _P],
     "awk_ldmm_create": [ctypes.POINTER(_P), _I, _I, _I, _F, _F, _F, _F, _F, _F, _I, _F, _F],
     "awk_ldmm_destroy": [_P],
     "awk_ldmm_update_step_next": [_P],

@@ -1,0 +1,437 @@
+// awk_atlas.hip -- greedy atlas building on one MI355X.
+//
+// GreedyAtlasBuilder + AtlasThread + GreedyAtlasThreadGPU (Algorithms/Common/
+// AtlasThread.txx:95-181, Algorithms/Greedy/GreedyAtlasBuilder.txx:236-301,
+// GreedyAtlasThreadGPU.cxx:161-206).  Every subject's hfield lives on the template
+// (mean) grid and holds coordinates in that subject's own image grid, so subject
+// images may have different sizes.  One atlas iteration:
+//   A  Idef_s = I0_s(h_s) for every subject                    (gather, per subject)
+//   M  mean = sum_s w_s Idef_s in subject order                 (one kernel)
+//   B  per subject the greedy step against I1 = mean: force + FFT passes + Navier
+//      solve + composeHV in the inverse x pass, energy E_s on the device
+//   E  energy = sum_s w_s E_s (FP64, subject order) into a device history
+// The mean that the reference forms after iteration k from the updated hfields
+// (ComputeThreadMean) is the same sum of the Idef_s that iteration k+1 needs, so
+// stage A serves both.  Steady-state iterations (step sizes known) are captured
+// once into a hipGraph per hfield parity and replayed: one graph launch per atlas
+// iteration for all subjects.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "awk_fluid.h"
+#include "awk_ops.h"
+
+namespace awk {
+
+__global__ void __launch_bounds__(256)
+k_atlas_mean(float *__restrict__ mean, const float *const *__restrict__ idef,
+             const float *__restrict__ w, int ns, int N) {
+  const int i = (int)(xcd_block() * 256 + threadIdx.x);
+  if (i >= N) return;
+  // GreedyAtlasThreadGPU::ComputeThreadMean: mean = 0; mean += Idef_s * w_s, s in order
+  float m = 0.f;
+  for (int s = 0; s < ns; ++s) m = m + idef[s][i] * w[s];
+  mean[i] = m;
+}
+
+__global__ void k_atlas_energy(const double *__restrict__ esub, const float *__restrict__ w, int ns,
+                               double *__restrict__ hist, int *__restrict__ cursor, int cap) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double e = 0.0;
+  for (int s = 0; s < ns; ++s) e += esub[s] * (double)w[s];
+  const int c = *cursor;
+  hist[c % cap] = e;
+  *cursor = c + 1;
+}
+
+__global__ void k_atlas_step(const float *__restrict__ maxpart, int n, float max_pert,
+                             float *__restrict__ step) {
+  __shared__ float sm[256];
+  float m = 0.f;
+  for (int i = threadIdx.x; i < n; i += 256) m = fmaxf(m, maxpart[i]);
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sm[threadIdx.x] = fmaxf(sm[threadIdx.x], sm[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *step = max_pert / sm[0];
+}
+
+}  // namespace awk
+
+namespace {
+
+struct Subject {
+  const float *I0 = nullptr;
+  int dims[3] = {0, 0, 0};
+  float w = 0.f;
+  float *h[2] = {nullptr, nullptr};  // 3 planes each, ping-pong
+  float *idef = nullptr;
+  float *step = nullptr;
+  double *prev = nullptr;
+  int *have_prev = nullptr;
+  bool update_step = true;
+  bool set = false;
+};
+
+constexpr int kHistCap = 1 << 16;
+
+}  // namespace
+
+struct awk_atlas {
+  awk::FluidKernel k;
+  int nx = 0, ny = 0, nz = 0, ns = 0;
+  float sp[3] = {1, 1, 1};
+  float max_pert = 0.1f;
+  std::vector<Subject> subj;
+  int cur = 0;  // which hfield buffer holds the current hfields
+  float *mean = nullptr;
+  const float **idef_tab = nullptr;  // device table of Idef pointers
+  float *w_dev = nullptr;
+  double *esub = nullptr;
+  double *partials = nullptr;
+  float *maxpart = nullptr;
+  double *hist = nullptr;
+  int *cursor = nullptr;
+  long long host_cursor = 0;
+  float *vbuf = nullptr;  // 3 planes: velocity of a step-size iteration
+  hipStream_t as = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  bool use_graph = true;
+  bool tables_dirty = true;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+};
+
+using namespace awk;
+
+static void atlas_drop_graphs(awk_atlas *a) {
+  for (int i = 0; i < 2; ++i)
+    if (a->exec[i]) {
+      (void)hipGraphExecDestroy(a->exec[i]);
+      a->exec[i] = nullptr;
+    }
+}
+
+static void atlas_free(awk_atlas *a) {
+  atlas_drop_graphs(a);
+  fk_release(a->k);
+  for (Subject &s : a->subj) {
+    void *b[] = {s.h[0], s.h[1], s.idef, s.step, s.prev, s.have_prev};
+    for (void *p : b)
+      if (p) (void)hipFree(p);
+  }
+  void *b[] = {a->mean, (void *)a->idef_tab, a->w_dev, a->esub, a->partials, a->maxpart,
+               a->hist, a->cursor, a->vbuf};
+  for (void *p : b)
+    if (p) (void)hipFree(p);
+  if (a->ev_in) (void)hipEventDestroy(a->ev_in);
+  if (a->ev_out) (void)hipEventDestroy(a->ev_out);
+  if (a->as) (void)hipStreamDestroy(a->as);
+}
+
+// Stage A + M: Idef_s = I0_s(h_s), mean = sum_s w_s Idef_s.
+static awk_status atlas_mean_stage(awk_atlas *a, hipStream_t s) {
+  awk_status st;
+  const int N = a->nx * a->ny * a->nz;
+  for (Subject &sb : a->subj) {
+    const float *h = sb.h[a->cur];
+    if ((st = launch_apply_grid(sb.idef, sb.I0, sb.dims[0], sb.dims[1], sb.dims[2], h, h + N,
+                                h + 2 * (size_t)N, a->nx, a->ny, a->nz, AWK_BG_VAL, 0.f, s)))
+      return st;
+  }
+  AWK_LAUNCH(k_atlas_mean, dim3(grid_for(N, 256)), dim3(256), 0, s, a->mean, a->idef_tab, a->w_dev,
+             a->ns, N);
+  AWK_LAUNCH_CHECK("k_atlas_mean");
+  return AWK_OK;
+}
+
+// One atlas iteration on stream s (stages A, M, B, E).  Leaves the new hfields in
+// buffer 1 - cur; the caller flips cur.
+static awk_status atlas_iteration(awk_atlas *a, hipStream_t s) {
+  awk_status st;
+  const int N = a->nx * a->ny * a->nz;
+  const int nxt = 1 - a->cur;
+  if ((st = atlas_mean_stage(a, s))) return st;
+  for (int si = 0; si < a->ns; ++si) {
+    Subject &sb = a->subj[si];
+    const float *h = sb.h[a->cur];
+    float *hn = sb.h[nxt];
+    XFwdArgs fa;
+    std::memset(&fa, 0, sizeof(fa));
+    fa.mode = XLOAD_FORCE;
+    fa.idef = sb.idef;
+    fa.i1 = a->mean;
+    for (int c = 0; c < 3; ++c) fa.sp[c] = a->sp[c];
+    fa.partials = a->partials;
+    int nb = 0;
+    if ((st = fk_xfwd(a->k, fa, s, &nb))) return st;
+    ReduceArgs ra;
+    ra.partials = a->partials;
+    ra.n = (a->ny * a->nz + a->k.rt - 1) / a->k.rt;
+    ra.scale = (double)a->sp[0] * a->sp[1] * a->sp[2];
+    ra.energy_out = a->esub + si;
+    ra.prev = sb.prev;
+    ra.have_prev = sb.have_prev;
+    ra.step = sb.step;
+    ra.auto_reduce = 0;
+    if ((st = fk_ypass(a->k, false, s, &ra))) return st;
+    if ((st = fk_zsolve(a->k, true, s))) return st;
+    if ((st = fk_ypass(a->k, true, s, nullptr))) return st;
+    XInvArgs ia;
+    std::memset(&ia, 0, sizeof(ia));
+    for (int c = 0; c < 3; ++c) ia.sp[c] = a->sp[c];
+    if (sb.update_step) {
+      // iteration 0 of a scale level: v, step = MaxPert / max|v|, then composeHV
+      ia.mode = XSTORE_MAXNORM;
+      for (int c = 0; c < 3; ++c) ia.out[c] = a->vbuf + (size_t)c * N;
+      ia.maxpart = a->maxpart;
+      if ((st = fk_xinv(a->k, ia, s, &nb))) return st;
+      AWK_LAUNCH(k_atlas_step, dim3(1), dim3(256), 0, s, a->maxpart, nb, a->max_pert, sb.step);
+      AWK_LAUNCH_CHECK("k_atlas_step");
+      if ((st = launch_compose_hv(hn, hn + N, hn + 2 * (size_t)N, h, h + N, h + 2 * (size_t)N,
+                                  a->vbuf, a->vbuf + N, a->vbuf + 2 * (size_t)N, a->nx, a->ny,
+                                  a->nz, a->sp, 1.f, sb.step, 0, AWK_BG_PARTIAL_ID, s)))
+        return st;
+    } else {
+      ia.mode = XSTORE_COMPOSE;
+      for (int c = 0; c < 3; ++c) {
+        ia.h[c] = h + (size_t)c * N;
+        ia.hnew[c] = hn + (size_t)c * N;
+      }
+      ia.step = sb.step;
+      if ((st = fk_xinv(a->k, ia, s, &nb))) return st;
+    }
+  }
+  AWK_LAUNCH(k_atlas_energy, dim3(1), dim3(64), 0, s, a->esub, a->w_dev, a->ns, a->hist, a->cursor,
+             kHistCap);
+  AWK_LAUNCH_CHECK("k_atlas_energy");
+  return AWK_OK;
+}
+
+static awk_status atlas_sync_tables(awk_atlas *a) {
+  if (!a->tables_dirty) return AWK_OK;
+  for (int i = 0; i < a->ns; ++i)
+    if (!a->subj[i].set) return fail(AWK_ERR_INVALID, "awk_atlas: subject " + std::to_string(i) + " not set");
+  std::vector<const float *> tab(a->ns);
+  std::vector<float> w(a->ns);
+  for (int i = 0; i < a->ns; ++i) {
+    tab[i] = a->subj[i].idef;
+    w[i] = a->subj[i].w;
+  }
+  AWK_HIP(hipMemcpyAsync((void *)a->idef_tab, tab.data(), a->ns * sizeof(float *), hipMemcpyHostToDevice, a->as));
+  AWK_HIP(hipMemcpyAsync(a->w_dev, w.data(), a->ns * sizeof(float), hipMemcpyHostToDevice, a->as));
+  AWK_HIP(hipStreamSynchronize(a->as));
+  a->tables_dirty = false;
+  return AWK_OK;
+}
+
+extern "C" {
+
+awk_status awk_atlas_create(awk_atlas_t *out, int nx, int ny, int nz, float spx, float spy, float spz,
+                            float alpha, float beta, float gamma, float max_pert, int n_subjects) {
+  if (!out) return fail(AWK_ERR_INVALID, "awk_atlas_create: null handle pointer");
+  *out = nullptr;
+  if (!(spx > 0 && spy > 0 && spz > 0)) return fail(AWK_ERR_INVALID, "awk_atlas_create: spacing must be > 0");
+  if (!(max_pert > 0)) return fail(AWK_ERR_INVALID, "awk_atlas_create: MaxPert must be > 0");
+  if (n_subjects < 1 || n_subjects > 65536) return fail(AWK_ERR_INVALID, "awk_atlas_create: bad subject count");
+  awk_atlas *a = new (std::nothrow) awk_atlas();
+  if (!a) return fail(AWK_ERR_ALLOC, "awk_atlas_create: host allocation failed");
+  const float sp[3] = {spx, spy, spz};
+  awk_status st = fk_init(a->k, nx, ny, nz, sp);
+  if (st) {
+    atlas_free(a);
+    delete a;
+    return st;
+  }
+  a->k.alpha = alpha;
+  a->k.beta = beta;
+  a->k.gamma = gamma;
+  a->k.lpow = 1.f;
+  a->k.flags = 0;
+  a->nx = nx;
+  a->ny = ny;
+  a->nz = nz;
+  a->ns = n_subjects;
+  for (int i = 0; i < 3; ++i) a->sp[i] = sp[i];
+  a->max_pert = max_pert;
+  a->subj.resize(n_subjects);
+  const size_t N = (size_t)nx * ny * nz;
+  const int nblk = fk_xblocks(a->k);
+  hipError_t e = hipSuccess;
+  for (Subject &s : a->subj) {
+    if (e == hipSuccess) e = hipMalloc(&s.h[0], 3 * N * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&s.h[1], 3 * N * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&s.idef, N * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&s.step, sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&s.prev, sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&s.have_prev, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(s.step, 0, sizeof(float));
+    if (e == hipSuccess) e = hipMemset(s.have_prev, 0, sizeof(int));
+  }
+  if (e == hipSuccess) e = hipMalloc(&a->mean, N * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void **)&a->idef_tab, n_subjects * sizeof(float *));
+  if (e == hipSuccess) e = hipMalloc(&a->w_dev, n_subjects * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&a->esub, n_subjects * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&a->partials, (size_t)nblk * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&a->maxpart, (size_t)nblk * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&a->hist, kHistCap * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&a->cursor, sizeof(int));
+  if (e == hipSuccess) e = hipMemset(a->cursor, 0, sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&a->vbuf, 3 * N * sizeof(float));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&a->as, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&a->ev_in, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&a->ev_out, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    st = hip_check(e, "awk_atlas_create: allocation");
+    atlas_free(a);
+    delete a;
+    return st;
+  }
+  *out = a;
+  return AWK_OK;
+}
+
+awk_status awk_atlas_destroy(awk_atlas_t a) {
+  if (!a) return AWK_OK;
+  (void)hipStreamSynchronize(a->as);
+  atlas_free(a);
+  delete a;
+  return AWK_OK;
+}
+
+awk_status awk_atlas_set_subject(awk_atlas_t a, int s, const float *I0, int nx_img, int ny_img,
+                                 int nz_img, float weight, const float *const h0[3],
+                                 awk_stream_t stream) {
+  if (!a || !I0 || !h0 || !h0[0] || !h0[1] || !h0[2]) return fail(AWK_ERR_INVALID, "awk_atlas_set_subject: null argument");
+  if (s < 0 || s >= a->ns) return fail(AWK_ERR_INVALID, "awk_atlas_set_subject: subject index out of range");
+  if (nx_img < 1 || ny_img < 1 || nz_img < 1 || (long long)nx_img * ny_img * nz_img > kBufMaxElems)
+    return fail(AWK_ERR_INVALID, "awk_atlas_set_subject: bad image size");
+  Subject &sb = a->subj[s];
+  sb.I0 = I0;
+  sb.dims[0] = nx_img;
+  sb.dims[1] = ny_img;
+  sb.dims[2] = nz_img;
+  sb.w = weight;
+  sb.update_step = true;
+  const size_t N = (size_t)a->nx * a->ny * a->nz;
+  hipStream_t cs = as_stream(stream);
+  for (int c = 0; c < 3; ++c)
+    AWK_HIP(hipMemcpyAsync(sb.h[a->cur] + c * N, h0[c], N * sizeof(float), hipMemcpyDeviceToDevice, cs));
+  sb.set = true;
+  a->tables_dirty = true;
+  atlas_drop_graphs(a);  // the image pointer is baked into captured graphs
+  return AWK_OK;
+}
+
+awk_status awk_atlas_get_hfield(awk_atlas_t a, int s, float *const h_out[3], awk_stream_t stream) {
+  if (!a || !h_out || !h_out[0] || !h_out[1] || !h_out[2]) return fail(AWK_ERR_INVALID, "awk_atlas_get_hfield: null argument");
+  if (s < 0 || s >= a->ns) return fail(AWK_ERR_INVALID, "awk_atlas_get_hfield: subject index out of range");
+  const size_t N = (size_t)a->nx * a->ny * a->nz;
+  hipStream_t cs = as_stream(stream);
+  AWK_HIP(hipEventRecord(a->ev_out, a->as));
+  AWK_HIP(hipStreamWaitEvent(cs, a->ev_out, 0));
+  for (int c = 0; c < 3; ++c)
+    AWK_HIP(hipMemcpyAsync(h_out[c], a->subj[s].h[a->cur] + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, cs));
+  return AWK_OK;
+}
+
+awk_status awk_atlas_update_step_next(awk_atlas_t a) {
+  if (!a) return fail(AWK_ERR_INVALID, "awk_atlas_update_step_next: null handle");
+  for (Subject &s : a->subj) s.update_step = true;
+  return AWK_OK;
+}
+
+awk_status awk_atlas_set_graph(awk_atlas_t a, int on) {
+  if (!a) return fail(AWK_ERR_INVALID, "awk_atlas_set_graph: null handle");
+  a->use_graph = on != 0;
+  if (!a->use_graph) atlas_drop_graphs(a);
+  return AWK_OK;
+}
+
+awk_status awk_atlas_iterate(awk_atlas_t a, int n_iter, double *energy_dev, awk_stream_t stream) {
+  if (!a) return fail(AWK_ERR_INVALID, "awk_atlas_iterate: null handle");
+  if (n_iter < 0 || n_iter > kHistCap) return fail(AWK_ERR_INVALID, "awk_atlas_iterate: bad iteration count");
+  awk_status st;
+  if ((st = atlas_sync_tables(a))) return st;
+  hipStream_t cs = as_stream(stream);
+  AWK_HIP(hipEventRecord(a->ev_in, cs));
+  AWK_HIP(hipStreamWaitEvent(a->as, a->ev_in, 0));
+  const long long first = a->host_cursor;
+  for (int it = 0; it < n_iter; ++it) {
+    bool any_step = false;
+    for (const Subject &s : a->subj) any_step = any_step || s.update_step;
+    if (any_step || !a->use_graph) {
+      if ((st = atlas_iteration(a, a->as))) return st;
+      for (Subject &s : a->subj) s.update_step = false;
+    } else {
+      hipGraphExec_t &ex = a->exec[a->cur];
+      if (!ex) {
+        hipGraph_t g = nullptr;
+        AWK_HIP(hipStreamBeginCapture(a->as, hipStreamCaptureModeThreadLocal));
+        st = atlas_iteration(a, a->as);
+        hipError_t ce = hipStreamEndCapture(a->as, &g);
+        if (st) {
+          if (g) (void)hipGraphDestroy(g);
+          return st;
+        }
+        if (ce != hipSuccess) return hip_check(ce, "awk_atlas_iterate: graph capture");
+        hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) {
+          ex = nullptr;
+          return hip_check(ie, "awk_atlas_iterate: graph instantiate");
+        }
+      }
+      AWK_HIP(hipGraphLaunch(ex, a->as));
+    }
+    a->cur = 1 - a->cur;
+    ++a->host_cursor;
+  }
+  if (energy_dev && n_iter > 0) {
+    // history slots [first, first + n_iter) of the ring
+    const int c0 = (int)(first % kHistCap);
+    const int n1 = std::min(n_iter, kHistCap - c0);
+    AWK_HIP(hipMemcpyAsync(energy_dev, a->hist + c0, n1 * sizeof(double), hipMemcpyDeviceToDevice, a->as));
+    if (n1 < n_iter)
+      AWK_HIP(hipMemcpyAsync(energy_dev + n1, a->hist, (n_iter - n1) * sizeof(double),
+                             hipMemcpyDeviceToDevice, a->as));
+  }
+  AWK_HIP(hipEventRecord(a->ev_out, a->as));
+  AWK_HIP(hipStreamWaitEvent(cs, a->ev_out, 0));
+  return AWK_OK;
+}
+
+awk_status awk_atlas_mean(awk_atlas_t a, float *mean_out, awk_stream_t stream) {
+  if (!a || !mean_out) return fail(AWK_ERR_INVALID, "awk_atlas_mean: null argument");
+  awk_status st;
+  if ((st = atlas_sync_tables(a))) return st;
+  hipStream_t cs = as_stream(stream);
+  AWK_HIP(hipEventRecord(a->ev_in, cs));
+  AWK_HIP(hipStreamWaitEvent(a->as, a->ev_in, 0));
+  if ((st = atlas_mean_stage(a, a->as))) return st;
+  const size_t N = (size_t)a->nx * a->ny * a->nz;
+  AWK_HIP(hipMemcpyAsync(mean_out, a->mean, N * sizeof(float), hipMemcpyDeviceToDevice, a->as));
+  AWK_HIP(hipEventRecord(a->ev_out, a->as));
+  AWK_HIP(hipStreamWaitEvent(cs, a->ev_out, 0));
+  return AWK_OK;
+}
+
+awk_status awk_atlas_deformed_image(awk_atlas_t a, int s, float *out, awk_stream_t stream) {
+  if (!a || !out) return fail(AWK_ERR_INVALID, "awk_atlas_deformed_image: null argument");
+  if (s < 0 || s >= a->ns || !a->subj[s].set) return fail(AWK_ERR_INVALID, "awk_atlas_deformed_image: bad subject");
+  const Subject &sb = a->subj[s];
+  const size_t N = (size_t)a->nx * a->ny * a->nz;
+  hipStream_t cs = as_stream(stream);
+  AWK_HIP(hipEventRecord(a->ev_out, a->as));
+  AWK_HIP(hipStreamWaitEvent(cs, a->ev_out, 0));
+  const float *h = sb.h[a->cur];
+  return launch_apply_grid(out, sb.I0, sb.dims[0], sb.dims[1], sb.dims[2], h, h + N, h + 2 * N,
+                           a->nx, a->ny, a->nz, AWK_BG_VAL, 0.f, cs);
+}
+
+}  // extern "C"

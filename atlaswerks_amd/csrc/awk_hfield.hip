@@ -64,6 +64,20 @@ __global__ void k_identity(float *__restrict__ hx, float *__restrict__ hy, float
   hz[i] = (float)z;
 }
 
+// h(x) = M [x y z 1]^T, M a 3x4 voxel-to-voxel affine (row major)
+struct Affine34 {
+  float m[12];
+};
+__global__ void k_affine(float *__restrict__ hx, float *__restrict__ hy, float *__restrict__ hz,
+                         int nx, int ny, int nz, Affine34 a) {
+  int i, x, y, z;
+  if (!voxel(nx * ny * nz, nx, ny, i, x, y, z)) return;
+  const float fx = (float)x, fy = (float)y, fz = (float)z;
+  hx[i] = a.m[0] * fx + a.m[1] * fy + a.m[2] * fz + a.m[3];
+  hy[i] = a.m[4] * fx + a.m[5] * fy + a.m[6] * fz + a.m[7];
+  hz[i] = a.m[8] * fx + a.m[9] * fy + a.m[10] * fz + a.m[11];
+}
+
 // mode 0: v -> h (x + v/sp), 1: h -> v ((h - x)*sp)
 __global__ void k_hv_convert(float *__restrict__ a, float *__restrict__ b, float *__restrict__ c,
                              int nx, int ny, int nz, float spx, float spy, float spz, int mode) {
@@ -97,11 +111,10 @@ __global__ void k_inv0(float *__restrict__ ox, float *__restrict__ oy, float *__
 template <int BG, int VPT>
 __global__ void __launch_bounds__(kBlock)
 k_apply(float *__restrict__ out, const float *__restrict__ img, const float *__restrict__ hx,
-        const float *__restrict__ hy, const float *__restrict__ hz, int nx, int ny, int nz,
+        const float *__restrict__ hy, const float *__restrict__ hz, int N, Dims d,
         float background) {
-  const int N = nx * ny * nz;
+  // N: voxels of the hfield / output; d: the image's own grid
   const int i0 = (int)(xcd_block() * (kBlock * VPT) + threadIdx.x);
-  const Dims d{nx, ny, nz};
   float px[VPT], py[VPT], pz[VPT];
 #pragma unroll
   for (int v = 0; v < VPT; ++v) {
@@ -109,7 +122,7 @@ k_apply(float *__restrict__ out, const float *__restrict__ img, const float *__r
     if (i < N) { px[v] = hx[i]; py[v] = hy[i]; pz[v] = hz[i]; }
   }
   float r[VPT];
-  const BufLoad ib = make_buf(img, N);
+  const BufLoad ib = make_buf(img, d.nx * d.ny * d.nz);
 #pragma unroll
   for (int v = 0; v < VPT; ++v)
     if (i0 + v * kBlock < N) r[v] = trilerp1g<BG>(ib, d, px[v], py[v], pz[v], background);
@@ -242,13 +255,13 @@ static awk_status launch_compose(float *ox, float *oy, float *oz, const float *f
 }
 
 template <int VPT>
-static void launch_apply_v(float *out, const float *img, const float *hx, const float *hy,
-                           const float *hz, int nx, int ny, int nz, int bg, float background,
+static void launch_apply_v(float *out, const float *img, Dims di, const float *hx,
+                           const float *hy, const float *hz, int N, int bg, float background,
                            hipStream_t s) {
-  const unsigned grid = grid_for((long long)nx * ny * nz, kBlock * VPT);
+  const unsigned grid = grid_for((long long)N, kBlock * VPT);
 #define AWK_AP(B)                                                                            \
-  AWK_LAUNCH((k_apply<B, VPT>), dim3(grid), dim3(kBlock), 0, s, out, img, hx, hy, hz, nx, \
-                     ny, nz, background)
+  AWK_LAUNCH((k_apply<B, VPT>), dim3(grid), dim3(kBlock), 0, s, out, img, hx, hy, hz, N, di, \
+                     background)
   switch (bg) {
     case AWK_BG_VAL: AWK_AP(AWK_BG_VAL); break;
     case AWK_BG_ZERO: AWK_AP(AWK_BG_ZERO); break;
@@ -258,17 +271,25 @@ static void launch_apply_v(float *out, const float *img, const float *hx, const 
 #undef AWK_AP
 }
 
-awk_status launch_apply(float *out, const float *img, const float *hx, const float *hy,
-                        const float *hz, int nx, int ny, int nz, int bg, float background,
-                        hipStream_t s) {
+awk_status launch_apply_grid(float *out, const float *img, int inx, int iny, int inz,
+                             const float *hx, const float *hy, const float *hz, int nx, int ny,
+                             int nz, int bg, float background, hipStream_t s) {
   if (bg != AWK_BG_VAL && bg != AWK_BG_ZERO && bg != AWK_BG_CLAMP && bg != AWK_BG_WRAP)
     return fail(AWK_ERR_INVALID, "apply: background must be VAL, ZERO, CLAMP or WRAP");
   const int vpt = tuning().apply_vpt;
-  if (vpt == 4) launch_apply_v<4>(out, img, hx, hy, hz, nx, ny, nz, bg, background, s);
-  else if (vpt == 2) launch_apply_v<2>(out, img, hx, hy, hz, nx, ny, nz, bg, background, s);
-  else launch_apply_v<1>(out, img, hx, hy, hz, nx, ny, nz, bg, background, s);
+  const Dims di{inx, iny, inz};
+  const int N = nx * ny * nz;
+  if (vpt == 4) launch_apply_v<4>(out, img, di, hx, hy, hz, N, bg, background, s);
+  else if (vpt == 2) launch_apply_v<2>(out, img, di, hx, hy, hz, N, bg, background, s);
+  else launch_apply_v<1>(out, img, di, hx, hy, hz, N, bg, background, s);
   AWK_LAUNCH_CHECK("k_apply");
   return AWK_OK;
+}
+
+awk_status launch_apply(float *out, const float *img, const float *hx, const float *hy,
+                        const float *hz, int nx, int ny, int nz, int bg, float background,
+                        hipStream_t s) {
+  return launch_apply_grid(out, img, nx, ny, nz, hx, hy, hz, nx, ny, nz, bg, background, s);
 }
 
 awk_status launch_compose_hv(float *ox, float *oy, float *oz, const float *gx, const float *gy,
@@ -370,6 +391,28 @@ awk_status awk_hfield_apply(float *out, const float *img, const float *hx, const
   if (awk_status st = check_dims(nx, ny, nz, "awk_hfield_apply")) return st;
   if (!out || !img || !hx || !hy || !hz) return fail(AWK_ERR_INVALID, "awk_hfield_apply: null pointer");
   return launch_apply(out, img, hx, hy, hz, nx, ny, nz, bg, background, as_stream(stream));
+}
+
+awk_status awk_hfield_apply_grid(float *out, const float *img, int inx, int iny, int inz,
+                                 const float *hx, const float *hy, const float *hz, int nx, int ny,
+                                 int nz, int bg, float background, awk_stream_t stream) {
+  if (awk_status st = check_dims(nx, ny, nz, "awk_hfield_apply_grid")) return st;
+  if (awk_status st = check_dims(inx, iny, inz, "awk_hfield_apply_grid (image)")) return st;
+  if (!out || !img || !hx || !hy || !hz) return fail(AWK_ERR_INVALID, "awk_hfield_apply_grid: null pointer");
+  return launch_apply_grid(out, img, inx, iny, inz, hx, hy, hz, nx, ny, nz, bg, background,
+                           as_stream(stream));
+}
+
+awk_status awk_hfield_init_affine(float *hx, float *hy, float *hz, int nx, int ny, int nz,
+                                  const float m[12], awk_stream_t stream) {
+  if (awk_status st = check_dims(nx, ny, nz, "awk_hfield_init_affine")) return st;
+  if (!hx || !hy || !hz || !m) return fail(AWK_ERR_INVALID, "awk_hfield_init_affine: null pointer");
+  Affine34 a;
+  for (int i = 0; i < 12; ++i) a.m[i] = m[i];
+  AWK_LAUNCH(k_affine, dim3(grid_for((long long)nx * ny * nz, kBlock)), dim3(kBlock), 0,
+             as_stream(stream), hx, hy, hz, nx, ny, nz, a);
+  AWK_LAUNCH_CHECK("k_affine");
+  return AWK_OK;
 }
 
 awk_status awk_hfield_compose(float *ox, float *oy, float *oz, const float *fx, const float *fy,

@@ -53,10 +53,39 @@ def computeInverseZerothOrder(h, hinv, stream=None):
 
 
 def apply(out, image, h, bg=BACKGROUND_STRATEGY_VAL, background=0.0, stream=None):
-    """out(x) = image(h(x))."""
-    nx, ny, nz = _same(out, image, h)
-    _lib.call("awk_hfield_apply", ptr(out), ptr(image), *planes(h), nx, ny, nz, int(bg),
-              float(background), stream_handle(stream))
+    """out(x) = image(h(x)).  The image may live on its own grid (h then holds
+    coordinates in the image's voxels), as HField3DUtils::apply allows."""
+    nx, ny, nz = _same(out, h)
+    inx, iny, inz = dims(image)
+    if (inx, iny, inz) == (nx, ny, nz):
+        _lib.call("awk_hfield_apply", ptr(out), ptr(image), *planes(h), nx, ny, nz, int(bg),
+                  float(background), stream_handle(stream))
+    else:
+        _lib.call("awk_hfield_apply_grid", ptr(out), ptr(image), inx, iny, inz, *planes(h), nx, ny,
+                  nz, int(bg), float(background), stream_handle(stream))
+
+
+def initializeFromAffine(h, m, stream=None):
+    """h(x) = M [x, y, z, 1] for a 3x4 voxel-to-voxel affine M
+    (HField3DUtils::initializeFromAffine with the grid conversions folded in)."""
+    import numpy as np
+    nx, ny, nz = dims(h)
+    arr = (ctypes.c_float * 12)(*[float(v) for v in np.asarray(m, dtype=np.float64).reshape(12)])
+    _lib.call("awk_hfield_init_affine", *planes(h), nx, ny, nz, arr, stream_handle(stream))
+
+
+def grid_affine(src_origin, src_spacing, dst_origin, dst_spacing, world=None):
+    """3x4 map from voxels of a source grid to voxels of a destination grid through
+    world space: p = o_s + sp_s x; q = W p (W a 3x4 world affine, identity if None);
+    result (q - o_d) / sp_d."""
+    import numpy as np
+    so, ss = np.asarray(src_origin, float), np.asarray(src_spacing, float)
+    do, ds = np.asarray(dst_origin, float), np.asarray(dst_spacing, float)
+    W = np.eye(4)[:3] if world is None else np.asarray(world, float).reshape(3, 4)
+    A = np.zeros((3, 4))
+    A[:, :3] = (W[:, :3] * ss[None, :]) / ds[:, None]
+    A[:, 3] = (W[:, :3] @ so + W[:, 3] - do) / ds
+    return A
 
 
 def compose(out, f, g, bg=BACKGROUND_STRATEGY_ID, stream=None):

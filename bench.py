@@ -51,6 +51,78 @@ def stage_bytes(nx, ny, nz):
     }
 
 
+def _events_ms(fn, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def bench_ldmm(dev, n=128, T=10, subjects=8, reps=5):
+    """IterateNew at n^3 with T velocity fields; subjects run back to back."""
+    import torch
+    from atlaswerks_amd import hfield, ldmm, synth
+    shape = (n, n, n)
+    t = torch.from_numpy(synth.blobs(shape, seed=3)).to(dev)
+    warps = []
+    for s in range(subjects):
+        u = torch.from_numpy(synth.identity(shape) + synth.sinusoidal_displacement(shape, 3.0, seed=300 + s)).to(dev)
+        tgt = torch.empty_like(t)
+        hfield.apply(tgt, t, u)
+        w = ldmm.LDMMWarp(t, tgt, n_time_steps=T, n_iter=0, param=ldmm.LDMMIteratorParam(sigma=0.05))
+        w.RunWarp()  # step-size call
+        warps.append(w)
+
+    def one():
+        for w in warps:
+            w.iterator.Iterate(w.data, record_energy=False)
+    ms = _events_ms(one, reps)
+    N = n ** 3
+    return {"ms_per_iteration_all_subjects": round(ms, 3), "subjects": subjects, "T": T,
+            "subject_iter_per_s": round(1000.0 * subjects / ms, 1),
+            "ms_per_subject_iteration": round(ms / subjects, 3),
+            "size": [n, n, n], "N": N}
+
+
+def bench_atlas(dev, reps=10):
+    """16 subjects (6 x 96^3, 6 x 128^3, 4 x 160x192x144) of one world-space blob
+    template, each on its own grid, atlas on a 128^3 template grid."""
+    import torch
+    from atlaswerks_amd import atlas, hfield, synth
+    world = 128.0
+    grids = [(96, 96, 96)] * 6 + [(128, 128, 128)] * 6 + [(144, 192, 160)] * 4  # (z, y, x)
+    base = torch.from_numpy(synth.blobs((128, 128, 128), seed=2)).to(dev)
+    subs = []
+    for k, g in enumerate(grids):
+        nz, ny, nx = g
+        sp = (world / nx, world / ny, world / nz)
+        h = torch.empty((3, nz, ny, nx), dtype=torch.float32, device=dev)
+        hfield.initializeFromAffine(h, hfield.grid_affine((0, 0, 0), sp, (0, 0, 0), (1, 1, 1)))
+        u = torch.from_numpy(synth.sinusoidal_displacement(g, 4.0, seed=2000 + k)).to(dev)
+        h += u
+        im = torch.empty(g, dtype=torch.float32, device=dev)
+        hfield.apply(im, base, h)
+        subs.append(atlas.AtlasSubject(im, spacing=sp))
+    out = {}
+    for use_graph in (False, True):
+        b = atlas.GreedyAtlasBuilderGPU(subs, (128, 128, 128), max_pert=0.1, use_graph=use_graph)
+        b.RunScaleLevel(2)  # step sizes + graph capture
+        ms = _events_ms(lambda: b.Iterate(1), reps)
+        out["graph" if use_graph else "eager"] = {"ms_per_atlas_iteration": round(ms, 3)}
+        del b
+    out["subjects"] = len(grids)
+    out["template"] = [128, 128, 128]
+    out["subject_iter_per_s_graph"] = round(1000.0 * len(grids) / out["graph"]["ms_per_atlas_iteration"], 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +132,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-microbench", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--copy-ref", action="store_true",
                     help="also run a plain 1 GiB device copy (counter scaling reference)")
     args = ap.parse_args()
@@ -210,8 +283,17 @@ def main():
         for name, mode in (("splat_fixed64", _lib.SPLAT_FIXED64), ("splat_float", _lib.SPLAT_FLOAT)):
             ms = timeit(lambda: hfield.splat(outi, I0, g1, mode=mode))
             mb[name] = {"ms": round(ms, 4), "gbs_20N": round(20 * N / (ms * 1e-3) / 1e9, 1)}
+        vel = torch.from_numpy(synth.sinusoidal_displacement(shape, 8.0, seed=3)).to(dev)
+        ms = timeit(lambda: hfield.splat_sorted(outi, I0, vel), reps=5)
+        mb["splat_sorted"] = {"ms": round(ms, 4), "gbs_20N": round(20 * N / (ms * 1e-3) / 1e9, 1)}
         out["microbench"] = mb
-        del f, g1, g2, outi, outv
+        del f, g1, g2, outi, outv, vel
+
+    # ---- secondary workloads: LDMM (8 subjects x 128^3, T = 10) and a 16-subject
+    #      greedy atlas over uneven grids (template 128^3); not part of the headline
+    if not args.no_secondary:
+        out["ldmm_128_T10"] = bench_ldmm(dev)
+        out["atlas_16_uneven"] = bench_atlas(dev)
 
     if args.copy_ref:
         # plain 1 GiB device copy: the calibration point for the HBM byte counters

@@ -119,6 +119,21 @@ awk_status awk_hfield_apply(float *out, const float *img, const float *hx, const
                             const float *hz, int nx, int ny, int nz, int bg, float background,
                             awk_stream_t stream);
 
+/* The same with an image on its own nx_img*ny_img*nz_img grid: the output and
+ * the hfield have nx*ny*nz voxels and h holds coordinates in the image's voxels.
+ * HField3DUtils::apply takes the output size from defImage and interpolates the
+ * image on its own size (HField3DUtils.txx:728-752). */
+awk_status awk_hfield_apply_grid(float *out, const float *img, int nx_img, int ny_img, int nz_img,
+                                 const float *hx, const float *hy, const float *hz, int nx, int ny,
+                                 int nz, int bg, float background, awk_stream_t stream);
+
+/* h(x) = M [x y z 1]^T with M a row-major 3x4 voxel-to-voxel affine map:
+ * HField3DUtils::initializeFromAffine (HField3DUtils.txx:249-287), with the
+ * world <-> voxel conversions of both grids folded into M by the caller, so the
+ * target grid may differ from the hfield's own. */
+awk_status awk_hfield_init_affine(float *hx, float *hy, float *hz, int nx, int ny, int nz,
+                                  const float m[12], awk_stream_t stream);
+
 /* h(x) = f(g(x)): HField3DUtils::compose (HField3DUtils.txx:290-310),
  * cudaHField3DUtils::compose (cudaHField3DUtils.cu:1163-1226).  out may not
  * alias f. */
@@ -235,6 +250,44 @@ awk_status awk_greedy_set_profiling(awk_greedy_t g, int on);
 /* Sums the recorded stage times (ms) over all iterations since profiling was
  * switched on or last read (synchronises), and resets the record. */
 awk_status awk_greedy_get_profile(awk_greedy_t g, double *stage_ms, int *iterations);
+
+/* ====================================================================== */
+/* Greedy atlas                                                             */
+/* ====================================================================== */
+
+typedef struct awk_atlas *awk_atlas_t;
+
+/* GreedyAtlasBuilder + AtlasThread + GreedyAtlasThreadGPU (Algorithms/Common/
+ * AtlasThread.txx:95-181, Algorithms/Greedy/GreedyAtlasBuilder.txx:236-301,
+ * GreedyAtlasThreadGPU.cxx:85-206) on one device for n_subjects subjects.  The
+ * mean and every hfield live on the nx*ny*nz template grid (spacing sp*); each
+ * hfield holds coordinates in its subject's own image grid, so images of
+ * different sizes can share one atlas (the reference requires equal sizes,
+ * WeightedImageSet.cxx:316-318). */
+awk_status awk_atlas_create(awk_atlas_t *out, int nx, int ny, int nz, float spx, float spy,
+                            float spz, float alpha, float beta, float gamma, float max_pert,
+                            int n_subjects);
+awk_status awk_atlas_destroy(awk_atlas_t a);
+/* Subject s: image I0 (nx_img*ny_img*nz_img, device, kept by pointer), weight,
+ * initial hfield h0 (3 template-grid planes, copied).  Resets its step size. */
+awk_status awk_atlas_set_subject(awk_atlas_t a, int s, const float *I0, int nx_img, int ny_img,
+                                 int nz_img, float weight, const float *const h0[3],
+                                 awk_stream_t stream);
+awk_status awk_atlas_get_hfield(awk_atlas_t a, int s, float *const h_out[3], awk_stream_t stream);
+/* Every subject's next iteration sets its step size (iteration 0 of a scale
+ * level, GreedyAtlasThreadGPU::BeginImageIteration). */
+awk_status awk_atlas_update_step_next(awk_atlas_t a);
+/* Steady-state iterations are captured into hipGraphs and replayed (default on). */
+awk_status awk_atlas_set_graph(awk_atlas_t a, int on);
+/* n_iter atlas iterations: each subject's greedy step against the current
+ * weighted mean, then the new mean (AtlasThreadMain's loop).  energy_dev (n_iter
+ * device doubles, may be NULL): sum_s w_s E_s per iteration (FP64, subject
+ * order).  No host synchronisation. */
+awk_status awk_atlas_iterate(awk_atlas_t a, int n_iter, double *energy_dev, awk_stream_t stream);
+/* The weighted mean sum_s w_s I0_s(h_s) of the current hfields (ComputeThreadMean). */
+awk_status awk_atlas_mean(awk_atlas_t a, float *mean_out, awk_stream_t stream);
+/* I0_s(h_s) on the template grid (DefToMean). */
+awk_status awk_atlas_deformed_image(awk_atlas_t a, int s, float *out, awk_stream_t stream);
 
 /* ====================================================================== */
 /* LDMM (adjoint-gradient iteration, IterateNew)                            */

@@ -299,6 +299,51 @@ def ldmm_alphas(I0, I1, v, sp=(1, 1, 1), sigma=0.05):
     return a
 
 
+class AtlasParams(ctypes.Structure):
+    _fields_ = [
+        ("size", ctypes.c_int * 3),
+        ("sp", ctypes.c_double * 3),
+        ("alpha", ctypes.c_double),
+        ("beta", ctypes.c_double),
+        ("gamma", ctypes.c_double),
+        ("max_pert", ctypes.c_double),
+        ("n_iter", ctypes.c_int),
+        ("n_subjects", ctypes.c_int),
+    ]
+
+
+def apply_grid(img, h, background=0.0):
+    """apply with the image on its own grid: out has h's grid; h in image voxels."""
+    img = _f32(img)
+    out = np.zeros(h.shape[:3])
+    for i, pt in enumerate(np.asarray(h, np.float64).reshape(-1, 3)):
+        out.flat[i] = trilerp_scalar(img, *pt, bg=BG_VAL, background=background)
+    return out
+
+
+def atlas_run(images, h0, weights, n_iter, template_shape, sp=(1, 1, 1), alpha=0.01, beta=0.01,
+              gamma=0.001, max_pert=0.1):
+    """GreedyAtlasBuilder, one scale level, FP64.  images: list of [z,y,x] arrays
+    (any sizes); h0: list of AoS hfields on the template grid in each image's
+    voxels.  Returns (h list, mean, energies)."""
+    p = AtlasParams()
+    nz, ny, nx = template_shape
+    p.size[:] = [nx, ny, nz]
+    p.sp[:] = [float(s) for s in sp]
+    p.alpha, p.beta, p.gamma, p.max_pert = alpha, beta, gamma, max_pert
+    p.n_iter, p.n_subjects = int(n_iter), len(images)
+    ims = [_f32(im) for im in images]
+    hs = [np.ascontiguousarray(h, dtype=np.float64).copy() for h in h0]
+    I0 = (ctypes.c_void_p * len(ims))(*[im.ctypes.data for im in ims])
+    H = (ctypes.c_void_p * len(hs))(*[h.ctypes.data for h in hs])
+    isz = np.array([[im.shape[2], im.shape[1], im.shape[0]] for im in ims], dtype=np.int32)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    mean = np.zeros(template_shape)
+    energy = np.zeros(n_iter)
+    lib().orc_atlas_run_d(ctypes.byref(p), I0, _p(isz), _p(w), H, _p(mean), _p(energy))
+    return hs, mean, energy
+
+
 def greedy_time(shape3, n_iter=1, seed=0):
     """Seconds per FP32 greedy iteration of the oracle on one host thread."""
     nz, ny, nx = shape3

@@ -555,16 +555,18 @@ void trilerp_vec_T(const T *h, const int *size, double x, double y, double z, do
   trilerp_vec<T>(h, size, x, y, z, out, bg);
 }
 
-template <class T>
+template <class T, class TI1 = float>
 double greedy_iterate(GreedyState<T> &st, const orc_greedy_params *p, const float *I0,
-                      const float *I1, T *h, T *hinv, bool update_step, double &step,
-                      double *energy_float_sum) {
+                      const TI1 *I1, T *h, T *hinv, bool update_step, double &step,
+                      double *energy_float_sum, const int *isize = nullptr) {
   const int *size = st.size;
   const size_t N = st.N;
   const int sx = size[0], sy = size[1];
-  // 1. deformed image (HField3DUtils::apply, VAL background 0)
+  // 1. deformed image (HField3DUtils::apply, VAL background 0); I0 may live on
+  //    its own grid isize (HField3DUtils.txx:728-752 interpolates on the image's size)
+  const int *img_size = isize ? isize : size;
   for (size_t i = 0; i < N; ++i)
-    st.Idef[i] = (T)trilerp_scalar(I0, size, (double)h[3 * i], (double)h[3 * i + 1],
+    st.Idef[i] = (T)trilerp_scalar(I0, img_size, (double)h[3 * i], (double)h[3 * i + 1],
                                    (double)h[3 * i + 2], ORC_BG_VAL, 0.0);
   // 2. gradient into the operator's field
   {
@@ -676,6 +678,46 @@ int greedy_run(const orc_greedy_params *p, const float *I0, const float *I1, T *
     prev = e;
   }
   if (step_out) *step_out = step;
+  return 0;
+}
+
+/* ====================================================================== */
+/* Greedy atlas                                                            */
+/* ====================================================================== */
+
+/* GreedyAtlasBuilder / AtlasThread (AtlasThread.txx:95-181, GreedyAtlasBuilder.txx:
+ * 236-301, GreedyAtlasThreadGPU.cxx:161-206) at one scale level, FP64: the mean
+ * sum_i w_i I0_i(h_i) is formed at the start and after every iteration; each
+ * iteration runs one greedy Iterate per subject against the current mean (the
+ * step size set at iteration 0); the iteration energy is sum_i w_i E_i. */
+int atlas_run_d(const orc_atlas_params *p, const float *const *I0, const int *isizes,
+                const double *w, double *const *h, double *mean, double *energy) {
+  const int *size = p->size;
+  const size_t N = (size_t)size[0] * size[1] * size[2];
+  const int ns = p->n_subjects;
+  orc_greedy_params gp;
+  for (int i = 0; i < 3; ++i) { gp.size[i] = size[i]; gp.sp[i] = p->sp[i]; }
+  gp.alpha = p->alpha; gp.beta = p->beta; gp.gamma = p->gamma; gp.max_pert = p->max_pert;
+  gp.n_iter = 1; gp.compute_inverse = 0; gp.auto_step_reduce = 0;
+  GreedyState<double> st(size, p->sp);
+  std::vector<double> steps(ns, 0.0), m(N);
+  auto compute_mean = [&]() {
+    std::fill(m.begin(), m.end(), 0.0);
+    for (int s = 0; s < ns; ++s)
+      for (size_t i = 0; i < N; ++i)
+        m[i] += w[s] * trilerp_scalar(I0[s], isizes + 3 * s, h[s][3 * i], h[s][3 * i + 1],
+                                      h[s][3 * i + 2], ORC_BG_VAL, 0.0);
+  };
+  compute_mean();
+  for (int it = 0; it < p->n_iter; ++it) {
+    double e = 0;
+    for (int s = 0; s < ns; ++s)
+      e += w[s] * greedy_iterate<double, double>(st, &gp, I0[s], m.data(), h[s], nullptr, it == 0,
+                                                 steps[s], nullptr, isizes + 3 * s);
+    if (energy) energy[it] = e;
+    compute_mean();
+  }
+  if (mean) std::memcpy(mean, m.data(), N * sizeof(double));
   return 0;
 }
 
@@ -1060,6 +1102,11 @@ int orc_greedy_run_f(const orc_greedy_params *p, const float *I0, const float *I
 int orc_greedy_run_d(const orc_greedy_params *p, const float *I0, const float *I1, double *h,
                      double *hinv, double *energy_d, double *step_out) {
   return greedy_run<double>(p, I0, I1, h, hinv, nullptr, energy_d, step_out);
+}
+
+int orc_atlas_run_d(const orc_atlas_params *p, const float *const *I0, const int *isizes,
+                    const double *weights, double *const *h, double *mean, double *energy) {
+  return atlas_run_d(p, I0, isizes, weights, h, mean, energy);
 }
 
 int orc_ldmm_run_d(const orc_ldmm_params *p, const float *I0, const float *I1, double *v,

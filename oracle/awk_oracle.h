@@ -124,6 +124,23 @@ int orc_greedy_run_d(const orc_greedy_params *p, const float *I0, const float *I
  * n_iter=1 but with preallocated state; returns seconds per iteration). */
 double orc_greedy_time_f(const int *size, int n_iter, int seed);
 
+/* ---- Greedy atlas (AtlasThread.txx:95-181, GreedyAtlasBuilder.txx:236-301) ---- */
+typedef struct {
+  int size[3];       /* the template (mean) grid; every hfield lives on it */
+  double sp[3];
+  double alpha, beta, gamma;
+  double max_pert;
+  int n_iter;
+  int n_subjects;
+} orc_atlas_params;
+
+/* One scale level of GreedyAtlasBuilder in FP64.  I0[s]: subject image on its own
+ * grid isizes[3s..3s+2]; h[s]: AoS hfield on the template grid holding voxel
+ * coordinates of subject s (in/out); weights w_s.  mean (N, may be NULL): the
+ * final weighted mean; energy (n_iter, may be NULL): sum_s w_s E_s per iteration. */
+int orc_atlas_run_d(const orc_atlas_params *p, const float *const *I0, const int *isizes,
+                    const double *weights, double *const *h, double *mean, double *energy);
+
 /* ---- LDMM (LDMMIteratorCPU.cxx:175-212, 529-631; LDMMEnergy.cxx:48-64) ---- */
 typedef struct {
   int size[3];
