@@ -135,22 +135,17 @@ __device__ __forceinline__ float3 trilerp3g(const L &fx, const L &fy, const L &f
   int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
   const float t = x - flx, u = y - fly, v = z - flz;
   const float omt = 1.f - t, omu = 1.f - u, omv = 1.f - v;
-  bool inside = true;
-  if (BG == AWK_BG_WRAP) {
-    x0 = wrap_mod(x0, d.nx); x1 = wrap_mod(x1, d.nx);
-    y0 = wrap_mod(y0, d.ny); y1 = wrap_mod(y1, d.ny);
-    z0 = wrap_mod(z0, d.nz); z1 = wrap_mod(z1, d.nz);
-  } else if (BG == AWK_BG_CLAMP) {
-    x0 = min(max(x0, 0), d.nx - 1); x1 = min(max(x1, 0), d.nx - 1);
-    y0 = min(max(y0, 0), d.ny - 1); y1 = min(max(y1, 0), d.ny - 1);
-    z0 = min(max(z0, 0), d.nz - 1); z1 = min(max(z1, 0), d.nz - 1);
-  } else {
-    // x0 < n-1 rather than x1 < n: stays correct when a huge coordinate saturates
-    // the float->int conversion (x1 = x0 + 1 would overflow).
-    inside = (x0 >= 0) & (x0 < d.nx - 1) & (y0 >= 0) & (y0 < d.ny - 1) & (z0 >= 0) & (z0 < d.nz - 1);
-  }
   float c[8][3];
-  if (inside) {
+  if (BG == AWK_BG_WRAP || BG == AWK_BG_CLAMP) {
+    if (BG == AWK_BG_WRAP) {
+      x0 = wrap_mod(x0, d.nx); x1 = wrap_mod(x1, d.nx);
+      y0 = wrap_mod(y0, d.ny); y1 = wrap_mod(y1, d.ny);
+      z0 = wrap_mod(z0, d.nz); z1 = wrap_mod(z1, d.nz);
+    } else {
+      x0 = min(max(x0, 0), d.nx - 1); x1 = min(max(x1, 0), d.nx - 1);
+      y0 = min(max(y0, 0), d.ny - 1); y1 = min(max(y1, 0), d.ny - 1);
+      z0 = min(max(z0, 0), d.nz - 1); z1 = min(max(z1, 0), d.nz - 1);
+    }
     const int sxy = d.nx * d.ny;
     const int b00 = z0 * sxy + y0 * d.nx, b10 = z0 * sxy + y1 * d.nx;
     const int b01 = z1 * sxy + y0 * d.nx, b11 = z1 * sxy + y1 * d.nx;
@@ -162,32 +157,49 @@ __device__ __forceinline__ float3 trilerp3g(const L &fx, const L &fy, const L &f
       c[i][2] = fz(idx[i]);
     }
   } else {
-    if (BG == AWK_BG_ID) return make_float3(x, y, z);
-    if (BG == AWK_BG_ZERO) return make_float3(0.f, 0.f, 0.f);
-    // PARTIAL_ID / PARTIAL_ZERO
-    // corner in-range flags per axis: floor corner in [0, n), ceil corner in [0, n)
-    const bool xin[2] = {(x0 >= 0) && (x0 < d.nx), (x0 >= -1) && (x0 < d.nx - 1)};
-    const bool yin[2] = {(y0 >= 0) && (y0 < d.ny), (y0 >= -1) && (y0 < d.ny - 1)};
-    const bool zin[2] = {(z0 >= 0) && (z0 < d.nz), (z0 >= -1) && (z0 < d.nz - 1)};
-    const int ox[8] = {0, 1, 1, 0, 0, 1, 1, 0};
-    const int oy[8] = {0, 0, 1, 1, 1, 1, 0, 0};
-    const int oz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    // ID, ZERO, PARTIAL_ID, PARTIAL_ZERO, branch-free: every corner is loaded from a
+    // clamped (always valid) index, so all 24 loads are issued together; corners
+    // outside the volume are replaced afterwards.  Per-axis in-range flags of the
+    // floor and ceil corners (x0 < n - 1 rather than x1 < n: stays correct when a
+    // huge coordinate saturates the float->int conversion).
+    const bool xin0 = (x0 >= 0) && (x0 < d.nx), xin1 = (x0 >= -1) && (x0 < d.nx - 1);
+    const bool yin0 = (y0 >= 0) && (y0 < d.ny), yin1 = (y0 >= -1) && (y0 < d.ny - 1);
+    const bool zin0 = (z0 >= 0) && (z0 < d.nz), zin1 = (z0 >= -1) && (z0 < d.nz - 1);
+    const int xc0 = min(max(x0, 0), d.nx - 1), xc1 = xin1 ? x0 + 1 : xc0;
+    const int yc0 = min(max(y0, 0), d.ny - 1), yc1 = yin1 ? y0 + 1 : yc0;
+    const int zc0 = min(max(z0, 0), d.nz - 1), zc1 = zin1 ? z0 + 1 : zc0;
+    const int sxy = d.nx * d.ny;
+    const int b00 = zc0 * sxy + yc0 * d.nx, b10 = zc0 * sxy + yc1 * d.nx;
+    const int b01 = zc1 * sxy + yc0 * d.nx, b11 = zc1 * sxy + yc1 * d.nx;
+    const int idx[8] = {b00 + xc0, b00 + xc1, b10 + xc1, b10 + xc0, b11 + xc0, b11 + xc1, b01 + xc1, b01 + xc0};
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      const bool in = xin[ox[i]] & yin[oy[i]] & zin[oz[i]];
-      if (in) {
-        const int id = ((z0 + oz[i]) * d.ny + (y0 + oy[i])) * d.nx + (x0 + ox[i]);
-        c[i][0] = fx(id);
-        c[i][1] = fy(id);
-        c[i][2] = fz(id);
-      } else if (BG == AWK_BG_PARTIAL_ID) {
-        // the corner's own integer coordinates (taken from the float floor so that
-        // they stay exact for any finite coordinate)
-        c[i][0] = flx + (float)ox[i];
-        c[i][1] = fly + (float)oy[i];
-        c[i][2] = flz + (float)oz[i];
-      } else {
-        c[i][0] = c[i][1] = c[i][2] = 0.f;
+      c[i][0] = fx(idx[i]);
+      c[i][1] = fy(idx[i]);
+      c[i][2] = fz(idx[i]);
+    }
+    const bool inside = xin0 && xin1 && yin0 && yin1 && zin0 && zin1;
+    if (BG == AWK_BG_ID || BG == AWK_BG_ZERO) {
+      if (!inside) return BG == AWK_BG_ID ? make_float3(x, y, z) : make_float3(0.f, 0.f, 0.f);
+    } else {
+      const bool xin[2] = {xin0, xin1}, yin[2] = {yin0, yin1}, zin[2] = {zin0, zin1};
+      const int ox[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+      const int oy[8] = {0, 0, 1, 1, 1, 1, 0, 0};
+      const int oz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const bool in = xin[ox[i]] && yin[oy[i]] && zin[oz[i]];
+        if (BG == AWK_BG_PARTIAL_ID) {
+          // the corner's own integer coordinates (taken from the float floor so
+          // that they stay exact for any finite coordinate)
+          c[i][0] = in ? c[i][0] : flx + (float)ox[i];
+          c[i][1] = in ? c[i][1] : fly + (float)oy[i];
+          c[i][2] = in ? c[i][2] : flz + (float)oz[i];
+        } else {
+          c[i][0] = in ? c[i][0] : 0.f;
+          c[i][1] = in ? c[i][1] : 0.f;
+          c[i][2] = in ? c[i][2] : 0.f;
+        }
       }
     }
   }
