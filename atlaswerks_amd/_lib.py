@@ -46,6 +46,10 @@ SIGNATURES = {
     "awk_splat_sorted": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _P, _P, _P],
     "awk_greedy_create": [ctypes.POINTER(_P), _I, _I, _I, _F, _F, _F, _F, _F, _F, _F],
     "awk_greedy_destroy": [_P],
+    "awk_gaussian_filter": [_P, _P, _I, _I, _I, _F, _F, _F, _I, _I, _I, _P],
+    "awk_gaussian_downsample": [_P, _I, _I, _I, _P, _I, _I, _I, _P],
+    "awk_hfield_resample": [_P, _P, _P, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _I, _P],
+    "awk_hfield_add_affine": [_P, _P, _P, _I, _I, _I, _P, _F, _P],
     "awk_atlas_create": [ctypes.POINTER(_P), _I, _I, _I, _F, _F, _F, _F, _F, _F, _F, _I],
     "awk_atlas_destroy": [_P],
     "awk_atlas_set_subject": [_P, _I, _P, _I, _I, _I, _F, ctypes.POINTER(_P3), _P],

@@ -7,6 +7,12 @@ subjects held by one device.  Subject images may have different sizes and
 spacings: every hfield lives on the template (mean) grid and maps it into its
 subject's voxels; ``initial_hfield`` builds that map from the grids' origins and
 spacings (and an optional world affine).
+
+Scale levels follow MultiscaleManager (Algorithms/Common/MultiscaleManager.cxx:
+104-111, 300-366): at factor f the template grid is size // f with spacing * f,
+each subject image is Gaussian-downsampled by the same factor, and the hfields are
+carried to the next level as displacements from the grid-to-grid map, resampled
+with CLAMP and rescaled (ScaleVectorFields, GreedyAtlasBuilder.txx:34).
 """
 import ctypes
 
@@ -28,39 +34,105 @@ class AtlasSubject:
         self.affine = affine  # optional 3x4 world-space map template -> subject
 
 
+class _AtlasLevel:
+    """One awk_atlas handle: template grid, subjects' (level) images, hfields."""
+
+    def __init__(self, images, weights, size, spacing, max_pert, alpha, beta, gamma, use_graph,
+                 h0s):
+        self.size = tuple(int(v) for v in size)
+        self.images = images  # kept alive: the library holds the pointers
+        h = ctypes.c_void_p()
+        _lib.call("awk_atlas_create", ctypes.byref(h), *self.size, *[float(s) for s in spacing],
+                  alpha, beta, gamma, max_pert, len(images))
+        self._h = h
+        _lib.call("awk_atlas_set_graph", self._h, int(bool(use_graph)))
+        for i, (im, w, h0) in enumerate(zip(images, weights, h0s)):
+            iz, iy, ix = im.shape
+            _lib.call("awk_atlas_set_subject", self._h, i, ptr(im), ix, iy, iz, float(w),
+                      planes_arr(h0), stream_handle(None))
+        torch.cuda.current_stream().synchronize()
+
+    def field(self, i, stream=None):
+        nx, ny, nz = self.size
+        out = torch.empty((3, nz, ny, nx), dtype=torch.float32, device="cuda")
+        _lib.call("awk_atlas_get_hfield", self._h, int(i), planes_arr(out), stream_handle(stream))
+        return out
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            _lib.call("awk_atlas_destroy", self._h)
+        self._h = None
+
+
 class GreedyAtlasBuilderGPU:
     def __init__(self, subjects, template_size, template_origin=(0.0, 0.0, 0.0),
                  template_spacing=(1.0, 1.0, 1.0), max_pert=0.1, alpha=0.01, beta=0.01,
-                 gamma=0.001, use_graph=True):
+                 gamma=0.001, use_graph=True, scale_levels=None):
         if not subjects:
             raise ValueError("an atlas needs at least one subject")
         self.subjects = list(subjects)
         self.size = tuple(int(v) for v in template_size)  # (nx, ny, nz)
         self.origin = tuple(float(v) for v in template_origin)
         self.spacing = tuple(float(v) for v in template_spacing)
+        self.prm = (max_pert, alpha, beta, gamma, use_graph)
         ns = len(self.subjects)
-        h = ctypes.c_void_p()
-        _lib.call("awk_atlas_create", ctypes.byref(h), *self.size, *self.spacing, alpha, beta,
-                  gamma, max_pert, ns)
-        self._h = h
-        _lib.call("awk_atlas_set_graph", self._h, int(bool(use_graph)))
-        nx, ny, nz = self.size
+        self.weights = [s.weight if s.weight is not None else 1.0 / ns for s in self.subjects]
+        self.scale_levels = list(scale_levels) if scale_levels else None
         self._energy = []
-        dev = self.subjects[0].image.device
-        h0 = torch.empty((3, nz, ny, nx), dtype=torch.float32, device=dev)
-        for i, s in enumerate(self.subjects):
-            w = s.weight if s.weight is not None else 1.0 / ns
-            self.initial_hfield(s, h0)
-            iz, iy, ix = s.image.shape
-            _lib.call("awk_atlas_set_subject", self._h, i, ptr(s.image), ix, iy, iz, float(w),
-                      planes_arr(h0), stream_handle(None))
-        torch.cuda.current_stream().synchronize()
+        self._level = None
+        self._factor = None
+        if self.scale_levels is None:
+            self._enter_level(1)
 
-    def initial_hfield(self, subject, out):
-        m = hfield.grid_affine(self.origin, self.spacing, subject.origin, subject.spacing,
-                               subject.affine)
-        hfield.initializeFromAffine(out, m)
+    # ---- grids of a level
+    def _template_grid(self, f):
+        size = tuple(n // f for n in self.size)
+        return size, tuple(s * f for s in self.spacing)
+
+    def _subject_grid(self, s, f):
+        iz, iy, ix = s.image.shape
+        return (ix // f, iy // f, iz // f), tuple(v * f for v in s.spacing)
+
+    def _base_map(self, s, f):
+        _, tsp = self._template_grid(f)
+        _, ssp = self._subject_grid(s, f)
+        return hfield.grid_affine(self.origin, tsp, s.origin, ssp, s.affine)
+
+    def initial_hfield(self, subject, out, f=1):
+        hfield.initializeFromAffine(out, self._base_map(subject, f))
         return out
+
+    def _enter_level(self, f):
+        size, sp = self._template_grid(f)
+        nx, ny, nz = size
+        images, h0s = [], []
+        for s in self.subjects:
+            if f == 1:
+                im = s.image
+            else:
+                (sx, sy, sz), _ = self._subject_grid(s, f)
+                im = torch.empty((sz, sy, sx), dtype=torch.float32, device=s.image.device)
+                hfield.gaussianDownsample(im, s.image)
+            images.append(im)
+        for i, s in enumerate(self.subjects):
+            if self._level is None:
+                h0 = torch.empty((3, nz, ny, nx), dtype=torch.float32, device="cuda")
+                self.initial_hfield(s, h0, f)
+            else:
+                h0 = hfield.upsampleHField(self._level.field(i), size, rescale=True,
+                                           base_in=self._base_map(s, self._factor),
+                                           base_out=self._base_map(s, f))
+            h0s.append(h0)
+        if self._level is not None:
+            self._level.close()
+        max_pert, alpha, beta, gamma, use_graph = self.prm
+        self._level = _AtlasLevel(images, self.weights, size, sp, max_pert, alpha, beta, gamma,
+                                  use_graph, h0s)
+        self._factor = f
+
+    @property
+    def _h(self):
+        return self._level._h
 
     def UpdateStepSizeNextIteration(self):
         _lib.call("awk_atlas_update_step_next", self._h)
@@ -68,7 +140,8 @@ class GreedyAtlasBuilderGPU:
     def Iterate(self, n_iter=1, stream=None):
         """n_iter atlas iterations; energies are kept on the device."""
         e = torch.empty(max(n_iter, 1), dtype=torch.float64, device="cuda")
-        _lib.call("awk_atlas_iterate", self._h, int(n_iter), ptr_f64(e), stream_handle(stream))
+        _lib.call("awk_atlas_iterate", self._h, int(n_iter), ctypes.c_void_p(e.data_ptr()),
+                  stream_handle(stream))
         self._energy.append(e[:n_iter])
 
     def RunScaleLevel(self, n_iter):
@@ -76,39 +149,42 @@ class GreedyAtlasBuilderGPU:
         self.UpdateStepSizeNextIteration()
         self.Iterate(n_iter)
 
+    def Run(self):
+        """All scale levels (AtlasThread::AtlasThreadMain)."""
+        if self.scale_levels is None:
+            raise ValueError("no scale levels given; use RunScaleLevel")
+        for f, n_iter in self.scale_levels:
+            if self._factor != f or self._level is None:
+                self._enter_level(int(f))
+            self.RunScaleLevel(int(n_iter))
+
     def energies(self):
         if not self._energy:
             return torch.empty(0, dtype=torch.float64).numpy()
         return torch.cat(self._energy).cpu().numpy()
 
+    def level_size(self):
+        return self._level.size
+
     def Mean(self, stream=None):
-        nx, ny, nz = self.size
+        nx, ny, nz = self._level.size
         out = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda")
         _lib.call("awk_atlas_mean", self._h, ptr(out), stream_handle(stream))
         return out
 
     def HField(self, i, stream=None):
-        nx, ny, nz = self.size
-        out = torch.empty((3, nz, ny, nx), dtype=torch.float32, device="cuda")
-        _lib.call("awk_atlas_get_hfield", self._h, int(i), planes_arr(out), stream_handle(stream))
-        return out
+        return self._level.field(i, stream)
 
     def DeformedImage(self, i, stream=None):
-        nx, ny, nz = self.size
+        nx, ny, nz = self._level.size
         out = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda")
         _lib.call("awk_atlas_deformed_image", self._h, int(i), ptr(out), stream_handle(stream))
         return out
 
     def __del__(self):
         try:
-            if self._h is not None and self._h.value:
-                _lib.call("awk_atlas_destroy", self._h)
-            self._h = None
+            if self._level is not None:
+                self._level.close()
+            self._level = None
         except Exception:
             pass
-
-
-def ptr_f64(t):
-    if not t.is_cuda or t.dtype != torch.float64:
-        raise ValueError("expected a float64 GPU tensor")
-    return ctypes.c_void_p(t.data_ptr())

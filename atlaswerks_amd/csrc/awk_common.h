@@ -111,6 +111,11 @@ __device__ __forceinline__ BufLoad make_buf(const float *p, int n) {
 // volumes whose byte size fits a 31-bit offset can use BufLoad
 constexpr long long kBufMaxElems = (1LL << 29);
 
+// A row-major 3x4 affine map of voxel coordinates (kernel argument).
+struct Affine34f {
+  float m[12];
+};
+
 // ---------------------------------------------------------------- trilerp
 // Background strategies (numbering of include/awk_hip.h):
 //   ID, ZERO, PARTIAL_ID, PARTIAL_ZERO, CLAMP, WRAP, VAL(scalar)

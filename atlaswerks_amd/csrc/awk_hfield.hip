@@ -65,11 +65,8 @@ __global__ void k_identity(float *__restrict__ hx, float *__restrict__ hy, float
 }
 
 // h(x) = M [x y z 1]^T, M a 3x4 voxel-to-voxel affine (row major)
-struct Affine34 {
-  float m[12];
-};
 __global__ void k_affine(float *__restrict__ hx, float *__restrict__ hy, float *__restrict__ hz,
-                         int nx, int ny, int nz, Affine34 a) {
+                         int nx, int ny, int nz, Affine34f a) {
   int i, x, y, z;
   if (!voxel(nx * ny * nz, nx, ny, i, x, y, z)) return;
   const float fx = (float)x, fy = (float)y, fz = (float)z;
@@ -407,7 +404,7 @@ awk_status awk_hfield_init_affine(float *hx, float *hy, float *hz, int nx, int n
                                   const float m[12], awk_stream_t stream) {
   if (awk_status st = check_dims(nx, ny, nz, "awk_hfield_init_affine")) return st;
   if (!hx || !hy || !hz || !m) return fail(AWK_ERR_INVALID, "awk_hfield_init_affine: null pointer");
-  Affine34 a;
+  Affine34f a;
   for (int i = 0; i < 12; ++i) a.m[i] = m[i];
   AWK_LAUNCH(k_affine, dim3(grid_for((long long)nx * ny * nz, kBlock)), dim3(kBlock), 0,
              as_stream(stream), hx, hy, hz, nx, ny, nz, a);

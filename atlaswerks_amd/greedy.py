@@ -135,19 +135,58 @@ class GreedyIteratorGPU:
 
 
 class GreedyWarp:
-    """Single-scale GreedyWarp::RunWarp (GreedyWarp.txx:69-125)."""
+    """GreedyWarp::RunWarp (GreedyWarp.txx:69-125).
+
+    ``scale_levels``: list of (downsample factor, iterations), coarsest first, as
+    MultiscaleManager builds them (size /= f, spacing *= f); None = one level at full
+    size with ``n_iterations``.  Between levels the hfield is upsampled as
+    GreedyDeformationData::SetScaleLevel does (hfield -> displacement, resampleNew
+    with CLAMP and vector rescaling, displacement -> hfield), and the images of every
+    level but the finest are Gaussian-downsampled from the originals."""
 
     def __init__(self, I0, I1, n_iterations=50, spacing=(1.0, 1.0, 1.0), param=None,
-                 compute_inverse=False, auto_step_reduce=False, h0=None):
-        self.data = GreedyDeformationDataGPU(I0, I1, spacing, compute_inverse,
-                                             max_iters=max(n_iterations, 1), h0=h0)
-        self.iterator = GreedyIteratorGPU(dims(I0), spacing, param)
-        self.iterator.SetAutoStepReduce(auto_step_reduce)
+                 compute_inverse=False, auto_step_reduce=False, h0=None, scale_levels=None):
+        self.I0orig, self.I1orig = I0, I1
+        self.spacing = tuple(float(v) for v in spacing)
+        self.param = param
+        self.compute_inverse = compute_inverse
+        self.auto_step_reduce = auto_step_reduce
+        self.h0 = h0
+        self.levels = list(scale_levels) if scale_levels else [(1, n_iterations)]
         self.n_iterations = n_iterations
+        self.energy_history = []
+        self.data = None
+
+    def _level_images(self, f):
+        if f == 1:
+            return self.I0orig, self.I1orig
+        nz, ny, nx = self.I0orig.shape
+        out = []
+        for im in (self.I0orig, self.I1orig):
+            d = torch.empty((nz // f, ny // f, nx // f), dtype=torch.float32, device=im.device)
+            hfield.gaussianDownsample(d, im)
+            out.append(d)
+        return out[0], out[1]
 
     def RunWarp(self, stream=None):
-        for it in range(self.n_iterations):
-            if it == 0:
-                self.iterator.UpdateStepSizeNextIteration()
-            self.iterator.Iterate(self.data, stream)
-        return self.data
+        prev = None
+        for level, (f, n_iter) in enumerate(self.levels):
+            I0, I1 = self._level_images(int(f))
+            nz, ny, nx = I0.shape
+            sp = tuple(s * f for s in self.spacing)
+            h0 = self.h0 if level == 0 else hfield.upsampleHField(prev.Def1To0, (nx, ny, nz))
+            data = GreedyDeformationDataGPU(I0, I1, sp, self.compute_inverse,
+                                            max_iters=max(n_iter, 1), h0=h0)
+            if level > 0 and self.compute_inverse:
+                data.Def0To1.copy_(hfield.upsampleHField(prev.Def0To1, (nx, ny, nz)))
+            it = GreedyIteratorGPU((nx, ny, nz), sp, self.param)
+            it.SetAutoStepReduce(self.auto_step_reduce)
+            for k in range(n_iter):
+                if k == 0:
+                    it.UpdateStepSizeNextIteration()
+                it.Iterate(data, stream)
+            self.energy_history.append(data.energies())
+            prev = data
+            self.iterator = it
+        self.data = prev
+        return prev

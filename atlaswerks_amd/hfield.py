@@ -166,3 +166,52 @@ def splat_sorted(dst, src, v, reference_truncation=True, keys=None, vals=None, s
 
 
 cplSplating = splat_sorted
+
+
+def gaussianFilter(out, image, sigma, half, stream=None):
+    """GaussianFilter3D: separable Gaussian (sigma, half sizes per axis x, y, z)."""
+    nx, ny, nz = _same(out, image)
+    _lib.call("awk_gaussian_filter", ptr(out), ptr(image), nx, ny, nz, *[float(v) for v in sigma],
+              *[int(v) for v in half], stream_handle(stream))
+
+
+def gaussianDownsample(out, image, stream=None):
+    """ImageUtils::gaussianDownsample from image's grid to out's grid."""
+    onx, ony, onz = dims(out)
+    nx, ny, nz = dims(image)
+    _lib.call("awk_gaussian_downsample", ptr(out), onx, ony, onz, ptr(image), nx, ny, nz,
+              stream_handle(stream))
+
+
+def resampleNew(out, field, bg=BACKGROUND_STRATEGY_CLAMP, rescale=False, stream=None):
+    """HField3DUtils::resampleNew of a vector field onto out's grid."""
+    onx, ony, onz = dims(out)
+    nx, ny, nz = dims(field)
+    _lib.call("awk_hfield_resample", *planes(out), onx, ony, onz, *planes(field), nx, ny, nz,
+              int(bg), int(bool(rescale)), stream_handle(stream))
+
+
+def addAffine(h, m, sign=1.0, stream=None):
+    """h += sign * M [x, y, z, 1] (M a 3x4 voxel affine)."""
+    import numpy as np
+    nx, ny, nz = dims(h)
+    arr = (ctypes.c_float * 12)(*[float(v) for v in np.asarray(m, dtype=np.float64).reshape(12)])
+    _lib.call("awk_hfield_add_affine", *planes(h), nx, ny, nz, arr, float(sign), stream_handle(stream))
+
+
+IDENTITY_AFFINE = ((1.0, 0.0, 0.0, 0.0), (0.0, 1.0, 0.0, 0.0), (0.0, 0.0, 1.0, 0.0))
+
+
+def upsampleHField(h, new_size, rescale=True, base_in=IDENTITY_AFFINE, base_out=IDENTITY_AFFINE,
+                   stream=None):
+    """Scale-level change of an hfield (GreedyDeformationData::SetScaleLevel,
+    GreedyDeformationData.cxx:208-219): v = h - base_in(x); v resampled to new_size
+    with CLAMP (MultiscaleManager::UpsampleToLevel, scaled by 1/r when rescale);
+    h' = base_out(x) + v.  The base maps are the identity in the reference."""
+    nx, ny, nz = (int(v) for v in new_size)
+    v = h.clone()
+    addAffine(v, base_in, -1.0, stream)
+    out = torch.empty((3, nz, ny, nx), dtype=torch.float32, device=h.device)
+    resampleNew(out, v, BACKGROUND_STRATEGY_CLAMP, rescale, stream)
+    addAffine(out, base_out, 1.0, stream)
+    return out

@@ -252,6 +252,33 @@ awk_status awk_greedy_set_profiling(awk_greedy_t g, int on);
 awk_status awk_greedy_get_profile(awk_greedy_t g, double *stage_ms, int *iterations);
 
 /* ====================================================================== */
+/* Scale levels                                                             */
+/* ====================================================================== */
+
+/* GaussianFilter3D (Utilities/DownsampleFilter/GaussianFilter3D.cxx:159-330):
+ * separable Gaussian with half kernel sizes half_* (clipped to n/2 - 1, at most
+ * 15); within half_* of an edge the one-sided kernel is renormalised. */
+awk_status awk_gaussian_filter(float *out, const float *in, int nx, int ny, int nz, float sigma_x,
+                               float sigma_y, float sigma_z, int half_x, int half_y, int half_z,
+                               awk_stream_t stream);
+/* ImageUtils::gaussianDownsample (ImageUtils.txx:255-290): sigma = sqrt(f/2),
+ * half size 2 ceil(sigma) per axis (f = n / n_out), then resampleNew at the
+ * centred points f x + (f - 1)/2 (trilinear, background 0). */
+awk_status awk_gaussian_downsample(float *out, int nx_out, int ny_out, int nz_out, const float *in,
+                                   int nx, int ny, int nz, awk_stream_t stream);
+/* HField3DUtils::resampleNew (HField3DUtils.txx:1814-1885): output voxel x samples
+ * the input at (r - 1)/2 + x r (r = n_in / n_out) with background strategy bg;
+ * rescale != 0 scales the vectors by 1/r (MultiscaleManager::UpsampleToLevel with
+ * ScaleVectorFields). */
+awk_status awk_hfield_resample(float *ox, float *oy, float *oz, int nx_out, int ny_out, int nz_out,
+                               const float *ix, const float *iy, const float *iz, int nx, int ny,
+                               int nz, int bg, int rescale, awk_stream_t stream);
+/* h += sign * M [x y z 1]^T: hToVelocity / velocityToH (HField3DUtils.txx:19-128)
+ * for M = identity, generalised to an affine base map. */
+awk_status awk_hfield_add_affine(float *hx, float *hy, float *hz, int nx, int ny, int nz,
+                                 const float m[12], float sign, awk_stream_t stream);
+
+/* ====================================================================== */
 /* Greedy atlas                                                             */
 /* ====================================================================== */
 

@@ -299,6 +299,31 @@ def ldmm_alphas(I0, I1, v, sp=(1, 1, 1), sigma=0.05):
     return a
 
 
+def gaussian_filter(img, sigma, ksize):
+    img = _f32(img)
+    out = np.zeros(img.shape, np.float32)
+    lib().orc_gaussian_filter(_p(img), _size(img.shape), _sp(sigma), (ctypes.c_int * 3)(*ksize),
+                              _p(out))
+    return out
+
+
+def gaussian_downsample(img, new_shape):
+    """ImageUtils::gaussianDownsample; new_shape as [z, y, x]."""
+    img = _f32(img)
+    out = np.zeros(new_shape, np.float32)
+    lib().orc_gaussian_downsample(_p(img), _size(img.shape), _size(new_shape), _p(out))
+    return out
+
+
+def resample_hfield(h, new_shape, bg=BG_CLAMP, rescale=False):
+    """HField3DUtils::resampleNew of an AoS field [z, y, x, 3]."""
+    h = _f32(h)
+    out = np.zeros(tuple(new_shape) + (3,))
+    lib().orc_resample_hfield_d(_p(h), _size(h.shape[:3]), _size(new_shape), int(bg),
+                                int(rescale), _p(out))
+    return out
+
+
 class AtlasParams(ctypes.Structure):
     _fields_ = [
         ("size", ctypes.c_int * 3),

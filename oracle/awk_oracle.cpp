@@ -682,6 +682,67 @@ int greedy_run(const orc_greedy_params *p, const float *I0, const float *I1, T *
 }
 
 /* ====================================================================== */
+/* Multiscale: Gaussian downsample, hfield resample                        */
+/* ====================================================================== */
+
+/* filter_generate_gauss_full (Utilities/DownsampleFilter/GaussianFilter3D.cxx:
+ * 92-140): row 0 the normalised half kernel, row j the kernel renormalised for
+ * an element j taps short of a full window (float arithmetic as there). */
+std::vector<std::vector<float>> gauss_table(float sigma, int k) {
+  std::vector<std::vector<float>> f(k + 1, std::vector<float>(k + 1));
+  const float es = (float)(1.0 / (2.0 * sigma * sigma));
+  float sc = 1.0f;
+  f[0][0] = 1.0f;
+  for (int i = 1; i <= k; ++i) {
+    f[0][i] = std::exp(-(float)(i * i) * es);
+    sc += 2.0f * f[0][i];
+  }
+  sc = 1.0f / sc;
+  for (int i = 0; i <= k; ++i) f[0][i] *= sc;
+  float miss = 0.0f;
+  for (int j = 1; j <= k; ++j) {
+    miss += f[0][k + 1 - j];
+    for (int i = 0; i <= k; ++i) f[j][i] = (float)(f[0][i] / (1. - miss));
+  }
+  return f;
+}
+
+/* One axis of filter_symmetric_full (GaussianFilter3D.cxx:159-330): interior
+ * elements take the full window; the k elements at either edge take the
+ * one-sided window with the renormalised kernel. */
+void gauss_axis(const float *in, float *out, const int *size, int axis, float sigma, int k) {
+  const int n = size[axis];
+  const auto f = gauss_table(sigma, k);
+  const size_t stride = axis == 0 ? 1 : axis == 1 ? (size_t)size[0] : (size_t)size[0] * size[1];
+  const int o1 = axis == 0 ? 1 : 0, o2 = axis == 2 ? 1 : 2;
+  for (int b = 0; b < size[o2]; ++b)
+    for (int a = 0; a < size[o1]; ++a) {
+      size_t base;
+      if (axis == 0) base = ((size_t)b * size[1] + a) * size[0];
+      else if (axis == 1) base = (size_t)b * size[0] * size[1] + a;
+      else base = (size_t)b * size[0] + a;
+      auto I = [&](int i) { return in[base + (size_t)i * stride]; };
+      for (int x = k; x < n - k; ++x) {
+        float acc = I(x) * f[0][0];
+        for (int i = 1; i <= k; ++i) acc += (I(x - i) + I(x + i)) * f[0][i];
+        out[base + (size_t)x * stride] = acc;
+      }
+      for (int x = k - 1; x >= 0; --x) {
+        const auto &g = f[k - x];
+        float acc = I(x) * g[0];
+        for (int i = 1; i <= x; ++i) acc += (I(x - i) + I(x + i)) * g[i];
+        for (int i = x + 1; i <= k; ++i) acc += I(x + i) * g[i];
+        out[base + (size_t)x * stride] = acc;
+        const int xr = n - 1 - x;
+        acc = I(xr) * g[0];
+        for (int i = 1; i <= x; ++i) acc += (I(xr + i) + I(xr - i)) * g[i];
+        for (int i = x + 1; i <= k; ++i) acc += I(xr - i) * g[i];
+        out[base + (size_t)xr * stride] = acc;
+      }
+    }
+}
+
+/* ====================================================================== */
 /* Greedy atlas                                                            */
 /* ====================================================================== */
 
@@ -1102,6 +1163,65 @@ int orc_greedy_run_f(const orc_greedy_params *p, const float *I0, const float *I
 int orc_greedy_run_d(const orc_greedy_params *p, const float *I0, const float *I1, double *h,
                      double *hinv, double *energy_d, double *step_out) {
   return greedy_run<double>(p, I0, I1, h, hinv, nullptr, energy_d, step_out);
+}
+
+void orc_gaussian_filter(const float *in, const int *size, const double *sigma, const int *ksize,
+                         float *out) {
+  // GaussianFilter3D::filter_symmetric_full: x, then y, then z; kernel half sizes
+  // clipped to size/2 - 1 (GaussianFilter3D.cxx:178-184)
+  const size_t N = (size_t)size[0] * size[1] * size[2];
+  std::vector<float> a(in, in + N), b(N, 0.f);
+  for (int ax = 0; ax < 3; ++ax) {
+    // below 2 voxels (where the reference's unsigned arithmetic skips the clip) no filtering
+    const int kmax = std::max(size[ax] / 2 - 1, 0);
+    const int k = std::min(std::max(ksize[ax], 0), kmax);
+    gauss_axis(a.data(), b.data(), size, ax, (float)sigma[ax], k);
+    std::swap(a, b);
+  }
+  std::memcpy(out, a.data(), N * sizeof(float));
+}
+
+void orc_gaussian_downsample(const float *in, const int *size, const int *new_size, float *out) {
+  // ImageUtils::gaussianDownsample (ImageUtils.txx:255-290): sigma = sqrt(f/2),
+  // kernel 2 ceil(sigma), then resampleNew (ImageUtils.txx:320-347) to the new
+  // grid: source index = f x + (f - 1)/2, trilinear, background 0.
+  double f[3], sg[3];
+  int k[3];
+  for (int i = 0; i < 3; ++i) {
+    f[i] = (double)size[i] / new_size[i];
+    sg[i] = std::sqrt(f[i] / 2.0);
+    k[i] = 2 * (int)std::ceil(sg[i]);
+  }
+  const size_t N = (size_t)size[0] * size[1] * size[2];
+  std::vector<float> g(N);
+  orc_gaussian_filter(in, size, sg, k, g.data());
+  size_t o = 0;
+  for (int z = 0; z < new_size[2]; ++z)
+    for (int y = 0; y < new_size[1]; ++y)
+      for (int x = 0; x < new_size[0]; ++x, ++o)
+        out[o] = (float)trilerp_scalar(g.data(), size, f[0] * x + (f[0] - 1) / 2,
+                                       f[1] * y + (f[1] - 1) / 2, f[2] * z + (f[2] - 1) / 2,
+                                       ORC_BG_VAL, 0.0);
+}
+
+void orc_resample_hfield_d(const float *in, const int *in_size, const int *out_size, int bg,
+                           int rescale, double *out) {
+  // HField3DUtils::resampleNew (HField3DUtils.txx:1814-1885): output index x samples
+  // the input at (r - 1)/2 + x r with r = in/out; vectors optionally scaled by 1/r
+  double r[3], o[3];
+  for (int i = 0; i < 3; ++i) {
+    r[i] = (double)in_size[i] / out_size[i];
+    o[i] = (r[i] - 1.0) / 2.0;
+  }
+  size_t q = 0;
+  for (int z = 0; z < out_size[2]; ++z)
+    for (int y = 0; y < out_size[1]; ++y)
+      for (int x = 0; x < out_size[0]; ++x, ++q) {
+        double v[3];
+        trilerp_vec<float>(in, in_size, (float)(o[0] + x * r[0]), (float)(o[1] + y * r[1]),
+                           (float)(o[2] + z * r[2]), v, bg);
+        for (int c = 0; c < 3; ++c) out[3 * q + c] = rescale ? v[c] * (1.0 / r[c]) : v[c];
+      }
 }
 
 int orc_atlas_run_d(const orc_atlas_params *p, const float *const *I0, const int *isizes,

@@ -124,6 +124,16 @@ int orc_greedy_run_d(const orc_greedy_params *p, const float *I0, const float *I
  * n_iter=1 but with preallocated state; returns seconds per iteration). */
 double orc_greedy_time_f(const int *size, int n_iter, int seed);
 
+/* ---- Multiscale (ImageUtils.txx:255-347, GaussianFilter3D.cxx, HField3DUtils.txx:1814-1885) ---- */
+/* GaussianFilter3D with sigma[3] and half kernel sizes ksize[3] (float). */
+void orc_gaussian_filter(const float *in, const int *size, const double *sigma, const int *ksize,
+                         float *out);
+/* ImageUtils::gaussianDownsample to new_size (VAL background 0, trilinear). */
+void orc_gaussian_downsample(const float *in, const int *size, const int *new_size, float *out);
+/* HField3DUtils::resampleNew of an AoS vector field (bg: vector strategy). */
+void orc_resample_hfield_d(const float *in, const int *in_size, const int *out_size, int bg,
+                           int rescale, double *out);
+
 /* ---- Greedy atlas (AtlasThread.txx:95-181, GreedyAtlasBuilder.txx:236-301) ---- */
 typedef struct {
   int size[3];       /* the template (mean) grid; every hfield lives on it */
