@@ -57,6 +57,7 @@ struct Tuning {
   int apply_vpt = 1;    // voxels per thread, apply
   int compose_vpt = 1;  // voxels per thread, compose family
   int xinv_unroll = 1;  // voxels per epilogue step of the inverse x pass
+  int splat_tile = 1;   // fixed-point splat through per-block LDS windows (0: direct atomics)
 };
 Tuning &tuning();
 

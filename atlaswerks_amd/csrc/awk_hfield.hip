@@ -326,6 +326,11 @@ awk_status awk_tune_set(const char *name, int value) {
     else t.xinv_unroll = value;
     return AWK_OK;
   }
+  if (n == "splat_tile") {
+    if (value != 0 && value != 1) return fail(AWK_ERR_INVALID, "awk_tune_set: splat_tile must be 0 or 1");
+    t.splat_tile = value;
+    return AWK_OK;
+  }
   return fail(AWK_ERR_INVALID, "awk_tune_set: unknown knob " + n);
 }
 
@@ -336,6 +341,7 @@ awk_status awk_tune_get(const char *name, int *value) {
   if (n == "apply_vpt") *value = t.apply_vpt;
   else if (n == "compose_vpt") *value = t.compose_vpt;
   else if (n == "xinv_unroll") *value = t.xinv_unroll;
+  else if (n == "splat_tile") *value = t.splat_tile;
   else return fail(AWK_ERR_INVALID, "awk_tune_get: unknown knob " + n);
   return AWK_OK;
 }

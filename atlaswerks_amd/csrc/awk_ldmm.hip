@@ -18,6 +18,7 @@
 
 #include "awk_fluid.h"
 #include "awk_ops.h"
+#include "awk_splat_tile.h"
 
 namespace awk {
 
@@ -82,36 +83,18 @@ k_ldmm_fwd(float *__restrict__ nx_, float *__restrict__ ny_, float *__restrict__
 
 // Backward step t: composeVHInv(v_t, h, PARTIAL_ZERO) (HField3DUtils.txx:400-430)
 // and the splat of diff to h(x) (cplSplat3DH, cudaSplatAtomic.cu:185-247) as
-// int64 fixed point (2^-32), which is order-independent.
-__global__ void __launch_bounds__(kLdmmBlock)
+// int64 fixed point (2^-32), which is order-independent; tiled through LDS
+// windows (awk_splat_tile.h).
+__global__ void __launch_bounds__(kSplatTileThreads)
 k_ldmm_bwd(float *__restrict__ nx_, float *__restrict__ ny_, float *__restrict__ nz_,
            const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
            const float *__restrict__ vx, const float *__restrict__ vy, const float *__restrict__ vz,
            const float *__restrict__ diff, unsigned long long *__restrict__ acc, Dims d, float sx,
            float sy, float sz) {
   const int N = d.nx * d.ny * d.nz;
-  const int i = (int)(xcd_block() * kLdmmBlock + threadIdx.x);
-  if (i >= N) return;
-  const float gx = hx[i], gy = hy[i], gz = hz[i];
-  const float3 o = trilerp3g<AWK_BG_PARTIAL_ZERO>(make_buf(vx, N), make_buf(vy, N), make_buf(vz, N), d,
-                                                  gx, gy, gz);
-  const float x = o.x / -sx + gx, y = o.y / -sy + gy, z = o.z / -sz + gz;
-  nx_[i] = x;
-  ny_[i] = y;
-  nz_[i] = z;
-  const float mass = diff[i];
-  const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
-  const int xi = (int)fx, yi = (int)fy, zi = (int)fz;
-  const float dx = 1.f - (x - fx), dy = 1.f - (y - fy), dz = 1.f - (z - fz);
-  const float wx[2] = {dx, 1.f - dx}, wy[2] = {dy, 1.f - dy}, wz[2] = {dz, 1.f - dz};
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const int ox = c & 1, oy = (c >> 1) & 1, oz = c >> 2;
-    const int X = xi + ox, Y = yi + oy, Z = zi + oz;
-    if (X < 0 || X >= d.nx || Y < 0 || Y >= d.ny || Z < 0 || Z >= d.nz) continue;
-    const float w = mass * wx[ox] * wy[oy] * wz[oz];
-    atomicAdd(acc + (Z * d.ny + Y) * d.nx + X, (unsigned long long)llrint((double)w * 4294967296.0));
-  }
+  SplatPosLdmm p{hx, hy, hz, make_buf(vx, N), make_buf(vy, N), make_buf(vz, N), d, sx, sy, sz,
+                 nx_, ny_, nz_};
+  splat_tile(p, diff, acc, d);
 }
 
 // alpha_t = splat / sigma^2 (LDMMIteratorGPU.cxx:197-204); clears the accumulator
@@ -342,7 +325,7 @@ static awk_status ldmm_run(awk_ldmm *l, const float *I0, const float *I1, float 
   AWK_LAUNCH(k_ldmm_identity, dim3(lgrid), dim3(kLdmmBlock), 0, s, h, h + N, h + 2 * N, d);
   int nb = 0;
   for (int t = T - 1; t >= 0; --t) {
-    AWK_LAUNCH(k_ldmm_bwd, dim3(egrid), dim3(kLdmmBlock), 0, s, hs, hs + N, hs + 2 * N, h, h + N,
+    AWK_LAUNCH(k_ldmm_bwd, dim3(splat_tile_grid(nx, ny, nz)), dim3(kSplatTileThreads), 0, s, hs, hs + N, hs + 2 * N, h, h + N,
                h + 2 * N, v[3 * t], v[3 * t + 1], v[3 * t + 2], l->diff,
                reinterpret_cast<unsigned long long *>(l->acc), d, sx, sy, sz);
     AWK_LAUNCH_CHECK("k_ldmm_bwd");

@@ -5,12 +5,14 @@
 // floor fix ADJUST_INT of cudaSplatAtomic.cu:27).  Three accumulation modes:
 //   FIXED64  int64 fixed point with 32 fractional bits: integer adds commute, so
 //            the result is bit-identical for any scheduling (deterministic);
-//            per-contribution rounding error <= 2^-33, |sum| < 2^31.
+//            per-contribution rounding error <= 2^-33, |sum| < 2^31.  By default
+//            accumulated per block in LDS windows first (awk_splat_tile.h).
 //   FIXED20  the reference's int32 fixed point S2p20 = int(a * 2^20 + 0.5f)
 //            (CUDACommon/inc/cplMacro.h:17, 67-91), accumulated in the dst buffer
 //            itself and converted in place (FixedToFloating); |sum| < 2048.
 //   FLOAT    float atomics (fast; summation order, hence rounding, varies).
 #include "awk_ops.h"
+#include "awk_splat_tile.h"
 
 namespace awk {
 
@@ -48,6 +50,13 @@ k_splat(void *__restrict__ acc, const float *__restrict__ src, const float *__re
   }
 }
 
+__global__ void __launch_bounds__(kSplatTileThreads)
+k_splat_tiled(unsigned long long *__restrict__ acc, const float *__restrict__ src,
+              const float *__restrict__ hx, const float *__restrict__ hy,
+              const float *__restrict__ hz, Dims d) {
+  splat_tile(SplatPosH{hx, hy, hz}, src, acc, d);
+}
+
 __global__ void k_fixed64_to_float(float *__restrict__ dst, const long long *__restrict__ acc, int N) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i < N) dst[i] = (float)((double)acc[i] * (1.0 / 4294967296.0));
@@ -67,8 +76,12 @@ awk_status launch_splat(float *dst, const float *src, const float *hx, const flo
     void *acc = nullptr;
     AWK_HIP(hipMallocAsync(&acc, (size_t)N * 8, s));
     AWK_HIP(hipMemsetAsync(acc, 0, (size_t)N * 8, s));
-    AWK_LAUNCH(k_splat<AWK_SPLAT_FIXED64>, dim3(grid), dim3(kSplatBlock), 0, s, acc, src, hx,
-                       hy, hz, nx, ny, nz);
+    if (tuning().splat_tile)
+      AWK_LAUNCH(k_splat_tiled, dim3(splat_tile_grid(nx, ny, nz)), dim3(kSplatTileThreads), 0, s,
+                 reinterpret_cast<unsigned long long *>(acc), src, hx, hy, hz, Dims{nx, ny, nz});
+    else
+      AWK_LAUNCH(k_splat<AWK_SPLAT_FIXED64>, dim3(grid), dim3(kSplatBlock), 0, s, acc, src, hx,
+                 hy, hz, nx, ny, nz);
     AWK_LAUNCH_CHECK("k_splat<FIXED64>");
     AWK_LAUNCH(k_fixed64_to_float, dim3(cgrid), dim3(256), 0, s, dst,
                        reinterpret_cast<const long long *>(acc), (int)N);

@@ -160,3 +160,30 @@ def test_splat_sorted_bit_exact(shape, trunc):
     np.testing.assert_array_equal(d, ref_d)
     hfield.splat_sorted(out, S, V, reference_truncation=trunc)
     np.testing.assert_array_equal(host(out), d)  # deterministic, also without key export
+
+
+@pytest.mark.parametrize("shape,amp", [((17, 9, 5), 3.0), ((64, 48, 40), 3.0), ((40, 36, 100), 1.0),
+                                       ((32, 32, 32), 40.0)])
+def test_splat_tiled_equals_direct(shape, amp):
+    """The LDS-window splat and the direct atomic splat add the same int64
+    contributions, so their FIXED64 results are bit-identical (40-voxel
+    displacements overflow the window and take the direct path inside the tile)."""
+    import torch
+    from atlaswerks_amd import _lib, hfield
+    from atlaswerks_amd.autotune import get_knob, set_knob
+    from gpu_util import host, to_dev
+    print("splat tiled", shape, amp, flush=True)
+    src = synth.blobs(shape, seed=12) - 0.5
+    h, _ = _hfield(shape, 13, amp=amp)
+    S, H = to_dev(src), to_dev(h)
+    out = torch.empty(shape, dtype=torch.float32, device="cuda")
+    old = get_knob("splat_tile")
+    try:
+        set_knob("splat_tile", 0)
+        hfield.splat(out, S, H, mode=_lib.SPLAT_FIXED64)
+        direct = host(out).copy()
+        set_knob("splat_tile", 1)
+        hfield.splat(out, S, H, mode=_lib.SPLAT_FIXED64)
+        np.testing.assert_array_equal(host(out), direct)
+    finally:
+        set_knob("splat_tile", old)
