@@ -59,6 +59,11 @@ SIGNATURES = {
     "awk_atlas_iterate": [_P, _I, _P, _P],
     "awk_atlas_mean": [_P, _P, _P],
     "awk_atlas_deformed_image": [_P, _I, _P, _P],
+    "awk_unet_weight_count": [],
+    "awk_unet_create": [ctypes.POINTER(_P), _I, _I, _I, _P, ctypes.c_size_t],
+    "awk_unet_destroy": [_P],
+    "awk_unet_forward": [_P, _P, _P, ctypes.POINTER(_P3), _P],
+    "awk_unet_mfma_probe": [_P, _P],
     "awk_ldmm_create": [ctypes.POINTER(_P), _I, _I, _I, _F, _F, _F, _F, _F, _F, _I, _F, _F],
     "awk_ldmm_destroy": [_P],
     "awk_ldmm_update_step_next": [_P],
@@ -76,7 +81,8 @@ SIGNATURES = {
     "awk_greedy_iterate": [_P, _P, _P, ctypes.POINTER(_P3), ctypes.POINTER(_P3),
                            ctypes.POINTER(_P3), ctypes.POINTER(_P3), _P, _P],
 }
-_RESTYPE = {"awk_last_error": ctypes.c_char_p, "awk_version": ctypes.c_int}
+_RESTYPE = {"awk_last_error": ctypes.c_char_p, "awk_version": ctypes.c_int,
+            "awk_unet_weight_count": ctypes.c_size_t}
 
 _lib = None
 

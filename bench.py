@@ -123,6 +123,26 @@ def bench_atlas(dev, reps=10):
     return out
 
 
+def bench_unet(dev, n=128, reps=10):
+    """U-Net warm start forward at n^3 (2 input channels -> 3-channel displacement)."""
+    import torch
+    from atlaswerks_amd import synth
+    from atlaswerks_amd.unet import LAYERS, UNetWarmStart
+    shape = (n, n, n)
+    t = torch.from_numpy(synth.blobs(shape, seed=1)).to(dev)
+    s = torch.from_numpy(synth.blobs(shape, seed=2)).to(dev)
+    net = UNetWarmStart((n, n, n), seed=0)
+    out = torch.empty((3, n, n, n), dtype=torch.float32, device=dev)
+    ms = _events_ms(lambda: net.forward(t, s, out=out), reps)
+    N0 = n ** 3
+    res = [N0, N0, N0 // 8, N0 // 8, N0 // 64, N0 // 64, N0 // 512, N0 // 512,
+           N0 // 64, N0 // 64, N0 // 8, N0 // 8, N0, N0]
+    flops = sum(2.0 * N * 27 * cin * cout for N, (cin, cout) in zip(res, LAYERS)) + 2.0 * N0 * 16 * 3
+    return {"ms": round(ms, 3), "gflop": round(flops / 1e9, 1),
+            "tflops": round(flops / (ms * 1e-3) / 1e12, 1), "peak_bf16_tflops": 2500.0,
+            "frac": round(flops / (ms * 1e-3) / 2.5e15, 4), "size": [n, n, n]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -294,6 +314,7 @@ def main():
     if not args.no_secondary:
         out["ldmm_128_T10"] = bench_ldmm(dev)
         out["atlas_16_uneven"] = bench_atlas(dev)
+        out["unet_128"] = bench_unet(dev)
 
     if args.copy_ref:
         # plain 1 GiB device copy: the calibration point for the HBM byte counters

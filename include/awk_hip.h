@@ -353,6 +353,31 @@ awk_status awk_ldmm_alphas(awk_ldmm_t l, const float *I0, const float *I1, float
 /* Copies J_t = I0 o h_t of the last iterate (the reference's mdJ0t[t]). */
 awk_status awk_ldmm_deformed_image(awk_ldmm_t l, int t, float *out, awk_stream_t stream);
 
+/* ====================================================================== */
+/* U-Net warm start (no reference counterpart)                              */
+/* ====================================================================== */
+
+typedef struct awk_unet *awk_unet_t;
+
+/* Number of fp32 weights the network takes.  Blob order, per 3x3x3 convolution
+ * l = 0..13 (encoder 2->16, 16->16, 16->32, 32->32, 32->64, 64->64, 64->128,
+ * 128->128; decoder 192->64, 64->64, 96->32, 32->32, 48->16, 16->16, where the
+ * decoder inputs are [upsampled coarser level, skip] concatenated): weights in
+ * PyTorch OIDHW order, then the bias; finally the 1x1x1 head W[3][16], b[3]. */
+size_t awk_unet_weight_count(void);
+/* Network for an nx*ny*nz grid (each a multiple of 8, at most 2^27 voxels).
+ * Weights are rounded to bf16 on upload. */
+awk_status awk_unet_create(awk_unet_t *out, int nx, int ny, int nz, const float *weights,
+                           size_t n_weights);
+awk_status awk_unet_destroy(awk_unet_t u);
+/* disp = U-Net(template, subject): bf16 activations, fp32 accumulation (MFMA),
+ * LeakyReLU(0.2), 2x2x2 average pooling, nearest x2 upsampling; fp32 output. */
+awk_status awk_unet_forward(awk_unet_t u, const float *templ, const float *subject,
+                            float *const disp[3], awk_stream_t stream);
+/* Diagnostic: 64 x 4 floats, lane l's accumulator registers of one
+ * v_mfma_f32_16x16x32_bf16 on fixed small-integer operands. */
+awk_status awk_unet_mfma_probe(float *out, awk_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

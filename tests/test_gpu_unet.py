@@ -1,0 +1,74 @@
+"""U-Net warm start on MFMA vs a float32 PyTorch reference of the same network.
+
+The reference runs on the CPU in float32 and rounds to bf16 exactly where the
+kernels store bf16 (inputs, weights, every activation), so the two differ only in
+fp32 summation order and the rare bf16 rounding it flips.  Bar: relative L2 of the
+displacement <= 1e-2.  The MFMA lane layout the kernels assume is checked
+exactly on small integers."""
+import numpy as np
+import pytest
+
+from atlaswerks_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mfma_lane_layout():
+    import torch
+    from atlaswerks_amd import _lib
+    from atlaswerks_amd.device import ptr
+    out = torch.zeros(256, dtype=torch.float32, device="cuda")
+    _lib.call("awk_unet_mfma_probe", ptr(out), None)
+    got = out.cpu().numpy().reshape(64, 4)
+    m, k, n = np.arange(16), np.arange(32), np.arange(16)
+    A = ((3 * m[:, None] + k[None, :]) % 5 - 2).astype(np.float64)
+    B = ((7 * k[:, None] + n[None, :]) % 5 - 2).astype(np.float64)
+    D = A @ B
+    for lane in range(64):
+        for i in range(4):
+            assert got[lane, i] == D[4 * (lane // 16) + i, lane % 16], (lane, i)
+
+
+def reference_forward(blob, t, s):
+    import torch
+    import torch.nn.functional as F
+    from atlaswerks_amd.unet import split_weights
+    bf = lambda x: x.to(torch.bfloat16).to(torch.float32)
+    convs, (hw, hb) = split_weights(blob)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a))
+
+    def conv(x, l):
+        w, b = convs[l]
+        return bf(F.leaky_relu(F.conv3d(x, bf(T(w)), T(b), padding=1), 0.2))
+
+    pool = lambda x: bf(F.avg_pool3d(x, 2))
+    up = lambda x: F.interpolate(x, scale_factor=2, mode="nearest")
+    x = bf(torch.stack([T(t), T(s)])[None])
+    s0 = conv(conv(x, 0), 1)
+    s1 = conv(conv(pool(s0), 2), 3)
+    s2 = conv(conv(pool(s1), 4), 5)
+    s3 = conv(conv(pool(s2), 6), 7)
+    d2 = conv(conv(torch.cat([up(s3), s2], 1), 8), 9)
+    d1 = conv(conv(torch.cat([up(d2), s1], 1), 10), 11)
+    d0 = conv(conv(torch.cat([up(d1), s0], 1), 12), 13)
+    u = F.conv3d(d0, T(hw).view(3, 16, 1, 1, 1), T(hb))
+    return u[0].numpy()
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (16, 24, 40), (8, 8, 16)])
+def test_unet_forward_vs_torch(shape):
+    import torch
+    from atlaswerks_amd.unet import UNetWarmStart, init_weights
+    from gpu_util import host, rel_l2, to_dev
+    print("unet", shape, flush=True)
+    t = synth.blobs(shape, seed=1)
+    s = synth.blobs(shape, seed=2)
+    blob = init_weights(seed=3, head_std=0.5)
+    net = UNetWarmStart(shape[::-1], blob)
+    u = host(net.forward(to_dev(t), to_dev(s)))
+    ref = reference_forward(blob, t, s)
+    err = rel_l2(u, ref)
+    print("unet rel L2", err)
+    assert err <= 1e-2
+    u2 = host(net.forward(to_dev(t), to_dev(s)))
+    np.testing.assert_array_equal(u, u2)  # deterministic
