@@ -4,18 +4,20 @@
 // subject) pair; the greedy or LDMM loop then starts from h = x + u.  No
 // reference counterpart: the network is fixed at 4 levels with 16-32-64-128
 // channels, two 3x3x3 convolutions per level, 2x2x2 average pooling, nearest x2
-// upsampling with skip concatenation, LeakyReLU(0.2), and a 1x1x1 head.
+// upsampling concatenated with the skip tensor, LeakyReLU(0.2), a 1x1x1 head.
 //
-// Activations are channels-last bf16 (NDHWC: act[((z*ny + y)*nx + x)*C + c]).  A
-// 3x3x3 convolution is an implicit GEMM: D[co][p] = sum_k W[co][k] X[k][p] with
-// k = tap * Cin + ci, tap = kz*9 + ky*3 + kx, on v_mfma_f32_16x16x32_bf16 with
-// fp32 accumulation.  A wave owns 16 output positions and all output channels;
-// lane l supplies, per 32-wide K step, 8 consecutive k (one tap, 8 channels: a
-// single 16-byte load) of position l%16 and of output channel l%16 of every
-// 16-channel tile.  The decoder's nearest upsampling and skip concatenation are
-// folded into the B-operand load (source A read at half resolution, channels
-// past A's count read from source B), and bias + LeakyReLU + bf16 rounding into
-// the epilogue, which stores 4 consecutive channels per lane.
+// Activations are channels-last bf16 (NDHWC: act[((z*ny + y)*nx + x)*C + c]), C a
+// multiple of 8.  A 3x3x3 convolution is an implicit GEMM D[co][p] = sum_k W[co][k]
+// X[k][p] on v_mfma_f32_16x16x32_bf16 with fp32 accumulation.  K is tap-major,
+// k = tap*C + ci, in chunks of 8 channels, so a lane's 8 consecutive k are 8
+// channels of one tap (one 16-byte load at the position's base address plus the
+// tap's constant offset, guarded by a 27-bit in-volume mask computed once per
+// position) and the 4 lane groups of a K step read neighbouring channels of the
+// same position wherever C >= 32.  A wave owns MT 16-channel output tiles x PT
+// 16-position tiles; bias, LeakyReLU (or, for gradients, the derivative of the
+// layer's activation) and bf16 rounding are fused into the epilogue, which stores
+// 4 consecutive channels per lane.
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -34,14 +36,15 @@ constexpr int kConvBlock = 64 * kConvWaves;
 constexpr int kPosPerWave = 16;
 
 struct ConvArgs {
-  const bf16 *a;  // source A: ca channels, at half resolution when a_half
-  const bf16 *b;  // source B (skip connection): cb channels, full resolution; may be null
-  const bf16 *w;  // [cout][kpad]
-  const float *bias;
-  bf16 *out;      // [N][cout]
-  int ca, cb, a_half, cout, kpad;
+  const bf16 *x;      // input [N][C]
+  const bf16 *w;      // [cout][kpad], k = tap*C + ci
+  const float *bias;  // [cout] or null
+  const bf16 *dact;   // gradient mode: the activation whose derivative scales the output; else null
+  bf16 *out;          // [N][cout]
+  int C, cout, kpad, nchunks;
+  unsigned c8magic;   // floor(2^32 / (C/8)) + 1 for C > 8: chunk / (C/8) as a multiply-high
   int nx, ny, nz;
-  float slope;
+  float slope;        // LeakyReLU slope (forward: applied; gradient mode: derivative)
 };
 
 __device__ __forceinline__ bf16x8 zero8() {
@@ -51,61 +54,137 @@ __device__ __forceinline__ bf16x8 zero8() {
   return z;
 }
 
-// MT: 16-channel output tiles (cout / 16)
-template <int MT>
+// blockIdx.y selects the wave's group of MT output-channel tiles.  SK = 1: the 4
+// waves of a block take consecutive position groups; SK = 4: they share one
+// position group and split the K steps (ks = wave mod 4), and wave 0 adds the
+// other waves' partial sums from LDS before the epilogue; there the fragments of
+// the next step are loaded before the MFMAs of the current one.
+template <int MT, int PT, int SK>
 __global__ void __launch_bounds__(kConvBlock)
 k_conv3(ConvArgs g) {
+  __shared__ f32x4 red[SK > 1 ? (SK - 1) * MT * PT * 64 : 1];
+  const int mbase = (int)blockIdx.y * MT * 16;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int N = g.nx * g.ny * g.nz;
-  const int p0 = (blockIdx.x * kConvWaves + wave) * kPosPerWave;
-  if (p0 >= N) return;
+  const int slot = SK > 1 ? (int)blockIdx.x : (int)blockIdx.x * kConvWaves + wave;
+  const int p0 = slot * (kPosPerWave * PT);
+  if (p0 >= N) return;  // uniform over the block when SK > 1
   const int col = lane & 15, grp = lane >> 4;
-  // this lane's position (B-operand column)
-  const int p = p0 + col;
-  const bool valid = p < N;  // grids smaller than 16 voxels leave lanes idle
-  const int px = p % g.nx, pr = p / g.nx, py = pr % g.ny, pz = pr / g.ny;
-  const int cin = g.ca + g.cb;
-  const int hx = g.nx >> 1;
-  f32x4 acc[MT];
+  // per position tile: base address and the mask of taps inside the volume
+  long long base[PT];
+  unsigned mask[PT];
 #pragma unroll
-  for (int m = 0; m < MT; ++m) acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int t = 0; t < PT; ++t) {
+    const int p = p0 + t * 16 + col;
+    unsigned m = 0u;
+    if (p < N) {
+      const int px = p % g.nx, pr = p / g.nx, py = pr % g.ny, pz = pr / g.ny;
+      const unsigned xm = (px > 0 ? 1u : 0u) | 2u | (px < g.nx - 1 ? 4u : 0u);
+      const unsigned ym = (py > 0 ? 1u : 0u) | 2u | (py < g.ny - 1 ? 4u : 0u);
+      const unsigned zm = (pz > 0 ? 1u : 0u) | 2u | (pz < g.nz - 1 ? 4u : 0u);
+#pragma unroll
+      for (int tap = 0; tap < 27; ++tap)
+        if (((xm >> (tap % 3)) & (ym >> ((tap / 3) % 3)) & (zm >> (tap / 9))) & 1u) m |= 1u << tap;
+    }
+    mask[t] = m;
+    base[t] = (long long)p * g.C;
+  }
+  const int sy = g.nx, sz = g.nx * g.ny;
+  f32x4 acc[MT][PT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int t = 0; t < PT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int ksteps = g.kpad >> 5;
-  for (int ks = 0; ks < ksteps; ++ks) {
-    const int k0 = ks * 32 + grp * 8;
-    const int tap = k0 / cin, ci = k0 - tap * cin;
-    bf16x8 bv = zero8();
-    if (valid && tap < 27) {
-      const int qx = px + (tap % 3) - 1, qy = py + (tap / 3) % 3 - 1, qz = pz + tap / 9 - 1;
-      if (qx >= 0 && qx < g.nx && qy >= 0 && qy < g.ny && qz >= 0 && qz < g.nz) {
-        if (ci < g.ca) {
-          const long long q = g.a_half ? ((long long)((qz >> 1) * (g.ny >> 1) + (qy >> 1)) * hx + (qx >> 1))
-                                       : ((long long)(qz * g.ny + qy) * g.nx + qx);
-          bv = *reinterpret_cast<const bf16x8 *>(g.a + q * g.ca + ci);
-        } else {
-          const long long q = (long long)(qz * g.ny + qy) * g.nx + qx;
-          bv = *reinterpret_cast<const bf16x8 *>(g.b + q * g.cb + (ci - g.ca));
-        }
+  bf16x8 bv[PT], av[MT];
+  auto load = [&](int ks, bf16x8 *b, bf16x8 *a) {
+    const int c = ks * 4 + grp;
+    const int tap = g.C == 8 ? c : (int)__umulhi((unsigned)c, g.c8magic), cg = c - tap * (g.C >> 3);
+    const int tz = tap / 9, tr = tap - tz * 9, ty = tr / 3, tx = tr - ty * 3;
+    const long long off = (long long)((tz - 1) * sz + (ty - 1) * sy + (tx - 1)) * g.C + cg * 8;
+    const bool live = c < g.nchunks;
+#pragma unroll
+    for (int t = 0; t < PT; ++t) {
+      b[t] = zero8();
+      if (live && ((mask[t] >> tap) & 1u)) b[t] = *reinterpret_cast<const bf16x8 *>(g.x + base[t] + off);
+    }
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+      a[m] = *reinterpret_cast<const bf16x8 *>(g.w + (size_t)(mbase + m * 16 + col) * g.kpad + ks * 32 + grp * 8);
+  };
+  if constexpr (SK > 1) {
+    // split K: few waves, so the next step's loads are issued before these MFMAs
+    if (wave < ksteps) load(wave, bv, av);
+    for (int ks = wave; ks < ksteps; ks += SK) {
+      bf16x8 bn[PT], an[MT];
+      const bool more = ks + SK < ksteps;
+      if (more) load(ks + SK, bn, an);
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int t = 0; t < PT; ++t)
+          acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[t], acc[m][t], 0, 0, 0);
+      if (more) {
+#pragma unroll
+        for (int t = 0; t < PT; ++t) bv[t] = bn[t];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) av[m] = an[m];
       }
     }
+  } else {
+    // full grids: occupancy hides the load latency
+    for (int ks = 0; ks < ksteps; ++ks) {
+      load(ks, bv, av);
 #pragma unroll
-    for (int m = 0; m < MT; ++m) {
-      const bf16x8 av = *reinterpret_cast<const bf16x8 *>(g.w + (size_t)(m * 16 + col) * g.kpad + k0);
-      acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[m], 0, 0, 0);
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int t = 0; t < PT; ++t)
+          acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[t], acc[m][t], 0, 0, 0);
     }
   }
-  // epilogue: lane holds channels m*16 + 4*grp + 0..3 of position p
-  if (!valid) return;
+  if (SK > 1) {
+    if (wave > 0) {
 #pragma unroll
-  for (int m = 0; m < MT; ++m) {
-    const int c0 = m * 16 + grp * 4;
-    bf16x4 o;
+      for (int m = 0; m < MT; ++m)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      float v = acc[m][i] + g.bias[c0 + i];
-      v = v >= 0.f ? v : v * g.slope;
-      o[i] = (bf16)v;
+        for (int t = 0; t < PT; ++t) red[(((wave - 1) * MT + m) * PT + t) * 64 + lane] = acc[m][t];
     }
-    *reinterpret_cast<bf16x4 *>(g.out + (size_t)p * g.cout + c0) = o;
+    __syncthreads();
+    if (wave > 0) return;
+#pragma unroll
+    for (int w = 0; w < SK - 1; ++w)
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int t = 0; t < PT; ++t) acc[m][t] += red[((w * MT + m) * PT + t) * 64 + lane];
+  }
+  // epilogue: lane holds channels m*16 + 4*grp + 0..3 of its position in tile t
+#pragma unroll
+  for (int t = 0; t < PT; ++t) {
+    const int p = p0 + t * 16 + col;
+    if (p >= N) continue;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const int c0 = mbase + m * 16 + grp * 4;
+      const size_t o0 = (size_t)p * g.cout + c0;
+      bf16x4 o;
+      if (g.dact) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4 *>(g.dact + o0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float v = acc[m][t][i];
+          o[i] = (bf16)((float)a[i] > 0.f ? v : v * g.slope);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          float v = acc[m][t][i] + (g.bias ? g.bias[c0 + i] : 0.f);
+          v = v >= 0.f ? v : v * g.slope;
+          o[i] = (bf16)v;
+        }
+      }
+      *reinterpret_cast<bf16x4 *>(g.out + o0) = o;
+    }
   }
 }
 
@@ -145,6 +224,28 @@ k_pool2(bf16 *__restrict__ out, const bf16 *__restrict__ in, int nx, int ny, int
   *reinterpret_cast<bf16x8 *>(out + (size_t)q * C + gi * 8) = o;
 }
 
+// out = [nearest x2 upsample of a (ca channels, half resolution), b (cb channels)]
+__global__ void __launch_bounds__(256)
+k_upcat(bf16 *__restrict__ out, const bf16 *__restrict__ a, int ca, const bf16 *__restrict__ b,
+        int cb, int nx, int ny, int nz) {
+  const int C = ca + cb, groups = C >> 3;
+  const long long total = (long long)nx * ny * nz * groups;
+  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= total) return;
+  const int gi = (int)(id % groups);
+  const long long p = id / groups;
+  const int x = (int)(p % nx), r = (int)(p / nx), y = r % ny, z = r / ny;
+  const int c = gi * 8;
+  bf16x8 v;
+  if (c < ca) {
+    const long long q = ((long long)(z >> 1) * (ny >> 1) + (y >> 1)) * (nx >> 1) + (x >> 1);
+    v = *reinterpret_cast<const bf16x8 *>(a + q * ca + c);
+  } else {
+    v = *reinterpret_cast<const bf16x8 *>(b + p * cb + (c - ca));
+  }
+  *reinterpret_cast<bf16x8 *>(out + p * C + c) = v;
+}
+
 // 1x1x1 head: u_c = b_c + sum_ci W[c][ci] d[ci] (fp32), written as 3 planes
 __global__ void __launch_bounds__(256)
 k_head(float *__restrict__ ux, float *__restrict__ uy, float *__restrict__ uz,
@@ -180,23 +281,58 @@ __global__ void k_mfma_probe(float *__restrict__ out) {
   for (int i = 0; i < 4; ++i) out[l * 4 + i] = d[i];
 }
 
+// Launch one 3x3x3 convolution.  Full grids: the most reuse per wave (MT * PT <=
+// 16) that still gives at least 8192 waves.  Grids too small for that keep all
+// output channels per wave (MT up to 4) and split K over the block's 4 waves,
+// splitting the channels over blockIdx.y only if that is still too few.
+awk_status launch_conv3(const ConvArgs &g, hipStream_t s) {
+  const long long N = (long long)g.nx * g.ny * g.nz;
+  const int mt_all = g.cout / 16;
+  if (mt_all < 1 || mt_all > 8 || (mt_all & (mt_all - 1)) || g.cout % 16)
+    return fail(AWK_ERR_INVALID, "unet: output channels must be 16, 32, 64 or 128");
+  const long long target = 8192;
+  int mt = mt_all, pt = mt_all >= 8 ? 2 : 4, sk = 1;
+  auto tiles = [&](int p) { return (N + kPosPerWave * p - 1) / (kPosPerWave * p); };
+  while (tiles(pt) * (mt_all / mt) < target && pt > 1) pt >>= 1;
+  if (tiles(pt) * (mt_all / mt) < target) {
+    sk = 4;
+    mt = mt_all > 4 ? 4 : mt_all;
+    while (tiles(1) * 4 * (mt_all / mt) < target && mt > 1) mt >>= 1;
+  }
+  const long long per_block = sk > 1 ? (long long)kPosPerWave * pt : (long long)kConvWaves * kPosPerWave * pt;
+  const dim3 grid((unsigned)((N + per_block - 1) / per_block), (unsigned)(mt_all / mt));
+#define AWK_CV(M, P, S) AWK_LAUNCH((k_conv3<M, P, S>), grid, dim3(kConvBlock), 0, s, g)
+  if (sk == 1) {
+    if (mt == 1) { if (pt == 4) AWK_CV(1, 4, 1); else if (pt == 2) AWK_CV(1, 2, 1); else AWK_CV(1, 1, 1); }
+    else if (mt == 2) { if (pt == 4) AWK_CV(2, 4, 1); else if (pt == 2) AWK_CV(2, 2, 1); else AWK_CV(2, 1, 1); }
+    else if (mt == 4) { if (pt == 4) AWK_CV(4, 4, 1); else if (pt == 2) AWK_CV(4, 2, 1); else AWK_CV(4, 1, 1); }
+    else { if (pt == 2) AWK_CV(8, 2, 1); else AWK_CV(8, 1, 1); }
+  } else {
+    if (mt == 1) AWK_CV(1, 1, 4);
+    else if (mt == 2) AWK_CV(2, 1, 4);
+    else AWK_CV(4, 1, 4);
+  }
+#undef AWK_CV
+  AWK_LAUNCH_CHECK("k_conv3");
+  return AWK_OK;
+}
+
 }  // namespace awk
 
 namespace {
 
-// layer table: (cin of source A, cin of source B, cout, A at half resolution)
+// input channels (as stored: the first layer's 2 are padded to 8) and outputs
 struct LayerSpec {
-  int ca, cb, cout, a_half;
+  int cin, cout;
 };
-// encoder: 2 convs per level at 16/32/64/128; decoder: up + skip concat, 2 convs
 constexpr LayerSpec kLayers[14] = {
-    {8, 0, 16, 0},   {16, 0, 16, 0},                     // level 0 (input padded 2 -> 8)
-    {16, 0, 32, 0},  {32, 0, 32, 0},                     // level 1
-    {32, 0, 64, 0},  {64, 0, 64, 0},                     // level 2
-    {64, 0, 128, 0}, {128, 0, 128, 0},                   // level 3 (bottom)
-    {128, 64, 64, 1}, {64, 0, 64, 0},                    // decoder level 2
-    {64, 32, 32, 1},  {32, 0, 32, 0},                    // decoder level 1
-    {32, 16, 16, 1},  {16, 0, 16, 0},                    // decoder level 0
+    {8, 16},   {16, 16},   // level 0
+    {16, 32},  {32, 32},   // level 1
+    {32, 64},  {64, 64},   // level 2
+    {64, 128}, {128, 128}, // level 3 (bottom)
+    {192, 64}, {64, 64},   // decoder level 2: [up(level 3), level 2]
+    {96, 32},  {32, 32},   // decoder level 1
+    {48, 16},  {16, 16},   // decoder level 0
 };
 // input channels the weight blob carries per layer (the first layer has 2)
 constexpr int kBlobCin[14] = {2, 16, 16, 32, 32, 64, 64, 128, 192, 64, 96, 32, 48, 16};
@@ -214,8 +350,9 @@ struct awk_unet {
   awk::bf16 *t1 = nullptr, *s1 = nullptr, *p1 = nullptr;
   awk::bf16 *t2 = nullptr, *s2 = nullptr, *p2 = nullptr;
   awk::bf16 *t3 = nullptr, *s3 = nullptr;
-  awk::bf16 *d2a = nullptr, *d2 = nullptr, *d1a = nullptr, *d1 = nullptr, *d0a = nullptr,
-            *d0 = nullptr;
+  awk::bf16 *c2 = nullptr, *d2a = nullptr, *d2 = nullptr;
+  awk::bf16 *c1 = nullptr, *d1a = nullptr, *d1 = nullptr;
+  awk::bf16 *c0 = nullptr, *d0a = nullptr, *d0 = nullptr;
 };
 
 using namespace awk;
@@ -225,47 +362,55 @@ static void unet_free(awk_unet *u) {
     if (u->w[l]) (void)hipFree(u->w[l]);
     if (u->bias[l]) (void)hipFree(u->bias[l]);
   }
-  void *b[] = {u->head_w, u->head_b, u->x0, u->t0, u->s0, u->p0, u->t1, u->s1, u->p1, u->t2,
-               u->s2, u->p2, u->t3, u->s3, u->d2a, u->d2, u->d1a, u->d1, u->d0a, u->d0};
+  void *b[] = {u->head_w, u->head_b, u->x0, u->t0, u->s0, u->p0, u->t1, u->s1, u->p1, u->t2, u->s2,
+               u->p2, u->t3, u->s3, u->c2, u->d2a, u->d2, u->c1, u->d1a, u->d1, u->c0, u->d0a, u->d0};
   for (void *p : b)
     if (p) (void)hipFree(p);
 }
 
-static awk_status conv(const awk_unet *u, int l, const bf16 *a, const bf16 *b, bf16 *out, int nx,
-                       int ny, int nz, hipStream_t s) {
-  const LayerSpec &L = kLayers[l];
+// repack OIDHW fp32 (blob_cin inputs) into [cout][kpad] bf16, k = tap*cin + ci
+static std::vector<bf16> pack_weights(const float *w, int cout, int cin, int blob_cin, int kpad) {
+  std::vector<bf16> wt((size_t)cout * kpad, (bf16)0.f);
+  for (int co = 0; co < cout; ++co)
+    for (int ci = 0; ci < blob_cin; ++ci)
+      for (int tap = 0; tap < 27; ++tap)
+        wt[(size_t)co * kpad + tap * cin + ci] = (bf16)w[((size_t)co * blob_cin + ci) * 27 + tap];
+  return wt;
+}
+
+static awk_status conv(const awk_unet *u, int l, const bf16 *x, bf16 *out, int nx, int ny, int nz,
+                       hipStream_t s) {
   ConvArgs g;
-  g.a = a;
-  g.b = b;
+  g.x = x;
   g.w = u->w[l];
   g.bias = u->bias[l];
+  g.dact = nullptr;
   g.out = out;
-  g.ca = L.ca;
-  g.cb = L.cb;
-  g.a_half = L.a_half;
-  g.cout = L.cout;
+  g.C = kLayers[l].cin;
+  g.cout = kLayers[l].cout;
   g.kpad = u->kpad[l];
+  g.nchunks = 27 * (g.C / 8);
+  g.c8magic = g.C > 8 ? (unsigned)(4294967296ull / (unsigned)(g.C / 8)) + 1u : 0u;
   g.nx = nx;
   g.ny = ny;
   g.nz = nz;
   g.slope = 0.2f;
-  const long long N = (long long)nx * ny * nz;
-  const unsigned grid = (unsigned)((N + kConvWaves * kPosPerWave - 1) / (kConvWaves * kPosPerWave));
-  switch (L.cout / 16) {
-    case 1: AWK_LAUNCH(k_conv3<1>, dim3(grid), dim3(kConvBlock), 0, s, g); break;
-    case 2: AWK_LAUNCH(k_conv3<2>, dim3(grid), dim3(kConvBlock), 0, s, g); break;
-    case 4: AWK_LAUNCH(k_conv3<4>, dim3(grid), dim3(kConvBlock), 0, s, g); break;
-    case 8: AWK_LAUNCH(k_conv3<8>, dim3(grid), dim3(kConvBlock), 0, s, g); break;
-    default: return fail(AWK_ERR_INVALID, "unet: unsupported channel count");
-  }
-  AWK_LAUNCH_CHECK("k_conv3");
-  return AWK_OK;
+  return launch_conv3(g, s);
 }
 
 static awk_status pool(const bf16 *in, bf16 *out, int nx, int ny, int nz, int C, hipStream_t s) {
   const long long total = (long long)(nx / 2) * (ny / 2) * (nz / 2) * (C / 8);
   AWK_LAUNCH(k_pool2, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, out, in, nx, ny, nz, C);
   AWK_LAUNCH_CHECK("k_pool2");
+  return AWK_OK;
+}
+
+static awk_status upcat(bf16 *out, const bf16 *a, int ca, const bf16 *b, int cb, int nx, int ny,
+                        int nz, hipStream_t s) {
+  const long long total = (long long)nx * ny * nz * ((ca + cb) / 8);
+  AWK_LAUNCH(k_upcat, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, out, a, ca, b, cb,
+             nx, ny, nz);
+  AWK_LAUNCH_CHECK("k_upcat");
   return AWK_OK;
 }
 
@@ -283,7 +428,7 @@ awk_status awk_unet_create(awk_unet_t *out, int nx, int ny, int nz, const float 
   *out = nullptr;
   if (nx < 8 || ny < 8 || nz < 8 || nx % 8 || ny % 8 || nz % 8)
     return fail(AWK_ERR_INVALID, "awk_unet_create: every axis must be a multiple of 8 (three poolings)");
-  if ((long long)nx * ny * nz > (1LL << 27)) return fail(AWK_ERR_INVALID, "awk_unet_create: at most 2^27 voxels");
+  if ((long long)nx * ny * nz > (1LL << 24)) return fail(AWK_ERR_INVALID, "awk_unet_create: at most 2^24 voxels");
   if (n_weights != awk_unet_weight_count())
     return fail(AWK_ERR_INVALID, "awk_unet_create: weight count does not match the network");
   awk_unet *u = new (std::nothrow) awk_unet();
@@ -295,16 +440,9 @@ awk_status awk_unet_create(awk_unet_t *out, int nx, int ny, int nz, const float 
   size_t off = 0;
   for (int l = 0; l < 14 && e == hipSuccess; ++l) {
     const LayerSpec &L = kLayers[l];
-    const int cin = L.ca + L.cb, bc = kBlobCin[l];
-    const int K = 27 * cin;
-    u->kpad[l] = (K + 31) / 32 * 32;
-    // repack OIDHW fp32 -> [cout][kpad] bf16 with k = tap*cin + ci (zeros for padding)
-    std::vector<bf16> wt((size_t)L.cout * u->kpad[l], (bf16)0.f);
-    for (int co = 0; co < L.cout; ++co)
-      for (int ci = 0; ci < bc; ++ci)
-        for (int tap = 0; tap < 27; ++tap)
-          wt[(size_t)co * u->kpad[l] + tap * cin + ci] =
-              (bf16)weights[off + ((size_t)co * bc + ci) * 27 + tap];
+    const int bc = kBlobCin[l];
+    u->kpad[l] = (27 * L.cin + 31) / 32 * 32;
+    const std::vector<bf16> wt = pack_weights(weights + off, L.cout, L.cin, bc, u->kpad[l]);
     off += (size_t)L.cout * bc * 27;
     if (e == hipSuccess) e = hipMalloc(&u->w[l], wt.size() * sizeof(bf16));
     if (e == hipSuccess) e = hipMemcpy(u->w[l], wt.data(), wt.size() * sizeof(bf16), hipMemcpyHostToDevice);
@@ -320,11 +458,12 @@ awk_status awk_unet_create(awk_unet_t *out, int nx, int ny, int nz, const float 
   struct {
     bf16 **p;
     size_t n;
-  } bufs[] = {{&u->x0, N0 * 8},  {&u->t0, N0 * 16},  {&u->s0, N0 * 16},  {&u->p0, N1 * 16},
-              {&u->t1, N1 * 32}, {&u->s1, N1 * 32},  {&u->p1, N2 * 32},  {&u->t2, N2 * 64},
-              {&u->s2, N2 * 64}, {&u->p2, N3 * 64},  {&u->t3, N3 * 128}, {&u->s3, N3 * 128},
-              {&u->d2a, N2 * 64}, {&u->d2, N2 * 64}, {&u->d1a, N1 * 32}, {&u->d1, N1 * 32},
-              {&u->d0a, N0 * 16}, {&u->d0, N0 * 16}};
+  } bufs[] = {{&u->x0, N0 * 8},   {&u->t0, N0 * 16},  {&u->s0, N0 * 16},  {&u->p0, N1 * 16},
+              {&u->t1, N1 * 32},  {&u->s1, N1 * 32},  {&u->p1, N2 * 32},  {&u->t2, N2 * 64},
+              {&u->s2, N2 * 64},  {&u->p2, N3 * 64},  {&u->t3, N3 * 128}, {&u->s3, N3 * 128},
+              {&u->c2, N2 * 192}, {&u->d2a, N2 * 64}, {&u->d2, N2 * 64},  {&u->c1, N1 * 96},
+              {&u->d1a, N1 * 32}, {&u->d1, N1 * 32},  {&u->c0, N0 * 48},  {&u->d0a, N0 * 16},
+              {&u->d0, N0 * 16}};
   for (auto &b : bufs)
     if (e == hipSuccess) e = hipMalloc((void **)b.p, b.n * sizeof(bf16));
   if (e != hipSuccess) {
@@ -354,25 +493,28 @@ awk_status awk_unet_forward(awk_unet_t u, const float *templ, const float *subje
   awk_status st;
   AWK_LAUNCH(k_pack_input, dim3((N0 + 255) / 256), dim3(256), 0, s, u->x0, templ, subject, N0);
   AWK_LAUNCH_CHECK("k_pack_input");
-  const int n1[3] = {nx / 2, ny / 2, nz / 2}, n2[3] = {nx / 4, ny / 4, nz / 4},
-            n3[3] = {nx / 8, ny / 8, nz / 8};
-  if ((st = conv(u, 0, u->x0, nullptr, u->t0, nx, ny, nz, s))) return st;
-  if ((st = conv(u, 1, u->t0, nullptr, u->s0, nx, ny, nz, s))) return st;
+  const int a1 = nx / 2, b1 = ny / 2, c1 = nz / 2, a2 = nx / 4, b2 = ny / 4, c2 = nz / 4,
+            a3 = nx / 8, b3 = ny / 8, c3 = nz / 8;
+  if ((st = conv(u, 0, u->x0, u->t0, nx, ny, nz, s))) return st;
+  if ((st = conv(u, 1, u->t0, u->s0, nx, ny, nz, s))) return st;
   if ((st = pool(u->s0, u->p0, nx, ny, nz, 16, s))) return st;
-  if ((st = conv(u, 2, u->p0, nullptr, u->t1, n1[0], n1[1], n1[2], s))) return st;
-  if ((st = conv(u, 3, u->t1, nullptr, u->s1, n1[0], n1[1], n1[2], s))) return st;
-  if ((st = pool(u->s1, u->p1, n1[0], n1[1], n1[2], 32, s))) return st;
-  if ((st = conv(u, 4, u->p1, nullptr, u->t2, n2[0], n2[1], n2[2], s))) return st;
-  if ((st = conv(u, 5, u->t2, nullptr, u->s2, n2[0], n2[1], n2[2], s))) return st;
-  if ((st = pool(u->s2, u->p2, n2[0], n2[1], n2[2], 64, s))) return st;
-  if ((st = conv(u, 6, u->p2, nullptr, u->t3, n3[0], n3[1], n3[2], s))) return st;
-  if ((st = conv(u, 7, u->t3, nullptr, u->s3, n3[0], n3[1], n3[2], s))) return st;
-  if ((st = conv(u, 8, u->s3, u->s2, u->d2a, n2[0], n2[1], n2[2], s))) return st;
-  if ((st = conv(u, 9, u->d2a, nullptr, u->d2, n2[0], n2[1], n2[2], s))) return st;
-  if ((st = conv(u, 10, u->d2, u->s1, u->d1a, n1[0], n1[1], n1[2], s))) return st;
-  if ((st = conv(u, 11, u->d1a, nullptr, u->d1, n1[0], n1[1], n1[2], s))) return st;
-  if ((st = conv(u, 12, u->d1, u->s0, u->d0a, nx, ny, nz, s))) return st;
-  if ((st = conv(u, 13, u->d0a, nullptr, u->d0, nx, ny, nz, s))) return st;
+  if ((st = conv(u, 2, u->p0, u->t1, a1, b1, c1, s))) return st;
+  if ((st = conv(u, 3, u->t1, u->s1, a1, b1, c1, s))) return st;
+  if ((st = pool(u->s1, u->p1, a1, b1, c1, 32, s))) return st;
+  if ((st = conv(u, 4, u->p1, u->t2, a2, b2, c2, s))) return st;
+  if ((st = conv(u, 5, u->t2, u->s2, a2, b2, c2, s))) return st;
+  if ((st = pool(u->s2, u->p2, a2, b2, c2, 64, s))) return st;
+  if ((st = conv(u, 6, u->p2, u->t3, a3, b3, c3, s))) return st;
+  if ((st = conv(u, 7, u->t3, u->s3, a3, b3, c3, s))) return st;
+  if ((st = upcat(u->c2, u->s3, 128, u->s2, 64, a2, b2, c2, s))) return st;
+  if ((st = conv(u, 8, u->c2, u->d2a, a2, b2, c2, s))) return st;
+  if ((st = conv(u, 9, u->d2a, u->d2, a2, b2, c2, s))) return st;
+  if ((st = upcat(u->c1, u->d2, 64, u->s1, 32, a1, b1, c1, s))) return st;
+  if ((st = conv(u, 10, u->c1, u->d1a, a1, b1, c1, s))) return st;
+  if ((st = conv(u, 11, u->d1a, u->d1, a1, b1, c1, s))) return st;
+  if ((st = upcat(u->c0, u->d1, 32, u->s0, 16, nx, ny, nz, s))) return st;
+  if ((st = conv(u, 12, u->c0, u->d0a, nx, ny, nz, s))) return st;
+  if ((st = conv(u, 13, u->d0a, u->d0, nx, ny, nz, s))) return st;
   AWK_LAUNCH(k_head, dim3((N0 + 255) / 256), dim3(256), 0, s, disp[0], disp[1], disp[2], u->d0,
              u->head_w, u->head_b, N0, 16);
   AWK_LAUNCH_CHECK("k_head");

@@ -365,7 +365,7 @@ typedef struct awk_unet *awk_unet_t;
  * decoder inputs are [upsampled coarser level, skip] concatenated): weights in
  * PyTorch OIDHW order, then the bias; finally the 1x1x1 head W[3][16], b[3]. */
 size_t awk_unet_weight_count(void);
-/* Network for an nx*ny*nz grid (each a multiple of 8, at most 2^27 voxels).
+/* Network for an nx*ny*nz grid (each a multiple of 8, at most 2^24 voxels).
  * Weights are rounded to bf16 on upload. */
 awk_status awk_unet_create(awk_unet_t *out, int nx, int ny, int nz, const float *weights,
                            size_t n_weights);
