@@ -64,6 +64,11 @@ SIGNATURES = {
     "awk_unet_destroy": [_P],
     "awk_unet_forward": [_P, _P, _P, ctypes.POINTER(_P3), _P],
     "awk_unet_mfma_probe": [_P, _P],
+    "awk_unet_trainer_create": [ctypes.POINTER(_P), _P, _P, ctypes.c_size_t],
+    "awk_unet_trainer_destroy": [_P],
+    "awk_unet_trainer_step": [_P, _P, _P, ctypes.POINTER(_P3), _F, _P, _P],
+    "awk_unet_trainer_gradients": [_P, _P, _P, ctypes.POINTER(_P3), _P, ctypes.POINTER(ctypes.c_double)],
+    "awk_unet_trainer_get_weights": [_P, _P, ctypes.c_size_t],
     "awk_ldmm_create": [ctypes.POINTER(_P), _I, _I, _I, _F, _F, _F, _F, _F, _F, _I, _F, _F],
     "awk_ldmm_destroy": [_P],
     "awk_ldmm_update_step_next": [_P],

@@ -22,37 +22,9 @@
 #include <new>
 #include <vector>
 
-#include "awk_ops.h"
+#include "awk_unet.h"
 
 namespace awk {
-
-typedef __bf16 bf16;
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-constexpr int kConvWaves = 4;
-constexpr int kConvBlock = 64 * kConvWaves;
-constexpr int kPosPerWave = 16;
-
-struct ConvArgs {
-  const bf16 *x;      // input [N][C]
-  const bf16 *w;      // [cout][kpad], k = tap*C + ci
-  const float *bias;  // [cout] or null
-  const bf16 *dact;   // gradient mode: the activation whose derivative scales the output; else null
-  bf16 *out;          // [N][cout]
-  int C, cout, kpad, nchunks;
-  unsigned c8magic;   // floor(2^32 / (C/8)) + 1 for C > 8: chunk / (C/8) as a multiply-high
-  int nx, ny, nz;
-  float slope;        // LeakyReLU slope (forward: applied; gradient mode: derivative)
-};
-
-__device__ __forceinline__ bf16x8 zero8() {
-  bf16x8 z;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) z[i] = (bf16)0.f;
-  return z;
-}
 
 // blockIdx.y selects the wave's group of MT output-channel tiles.  SK = 1: the 4
 // waves of a block take consecutive position groups; SK = 4: they share one
@@ -288,15 +260,18 @@ __global__ void k_mfma_probe(float *__restrict__ out) {
 awk_status launch_conv3(const ConvArgs &g, hipStream_t s) {
   const long long N = (long long)g.nx * g.ny * g.nz;
   const int mt_all = g.cout / 16;
-  if (mt_all < 1 || mt_all > 8 || (mt_all & (mt_all - 1)) || g.cout % 16)
-    return fail(AWK_ERR_INVALID, "unet: output channels must be 16, 32, 64 or 128");
+  if (g.cout % 16 || mt_all < 1 || mt_all > 16 || g.C % 8 || g.C < 8)
+    return fail(AWK_ERR_INVALID, "unet: channels must be multiples of 16 (outputs, at most 256) and 8 (inputs)");
+  // channel tiles per wave: the largest power of two (<= 8) dividing the tile count
+  int mt_max = 1;
+  while (mt_max < 8 && mt_all % (mt_max * 2) == 0) mt_max *= 2;
   const long long target = 8192;
-  int mt = mt_all, pt = mt_all >= 8 ? 2 : 4, sk = 1;
+  int mt = mt_max, pt = mt_max >= 8 ? 2 : 4, sk = 1;
   auto tiles = [&](int p) { return (N + kPosPerWave * p - 1) / (kPosPerWave * p); };
   while (tiles(pt) * (mt_all / mt) < target && pt > 1) pt >>= 1;
   if (tiles(pt) * (mt_all / mt) < target) {
     sk = 4;
-    mt = mt_all > 4 ? 4 : mt_all;
+    mt = mt_max > 4 ? 4 : mt_max;
     while (tiles(1) * 4 * (mt_all / mt) < target && mt > 1) mt >>= 1;
   }
   const long long per_block = sk > 1 ? (long long)kPosPerWave * pt : (long long)kConvWaves * kPosPerWave * pt;
@@ -318,42 +293,6 @@ awk_status launch_conv3(const ConvArgs &g, hipStream_t s) {
 }
 
 }  // namespace awk
-
-namespace {
-
-// input channels (as stored: the first layer's 2 are padded to 8) and outputs
-struct LayerSpec {
-  int cin, cout;
-};
-constexpr LayerSpec kLayers[14] = {
-    {8, 16},   {16, 16},   // level 0
-    {16, 32},  {32, 32},   // level 1
-    {32, 64},  {64, 64},   // level 2
-    {64, 128}, {128, 128}, // level 3 (bottom)
-    {192, 64}, {64, 64},   // decoder level 2: [up(level 3), level 2]
-    {96, 32},  {32, 32},   // decoder level 1
-    {48, 16},  {16, 16},   // decoder level 0
-};
-// input channels the weight blob carries per layer (the first layer has 2)
-constexpr int kBlobCin[14] = {2, 16, 16, 32, 32, 64, 64, 128, 192, 64, 96, 32, 48, 16};
-
-}  // namespace
-
-struct awk_unet {
-  int nx = 0, ny = 0, nz = 0;
-  awk::bf16 *w[14] = {};
-  float *bias[14] = {};
-  int kpad[14] = {};
-  float *head_w = nullptr, *head_b = nullptr;
-  // activations
-  awk::bf16 *x0 = nullptr, *t0 = nullptr, *s0 = nullptr, *p0 = nullptr;
-  awk::bf16 *t1 = nullptr, *s1 = nullptr, *p1 = nullptr;
-  awk::bf16 *t2 = nullptr, *s2 = nullptr, *p2 = nullptr;
-  awk::bf16 *t3 = nullptr, *s3 = nullptr;
-  awk::bf16 *c2 = nullptr, *d2a = nullptr, *d2 = nullptr;
-  awk::bf16 *c1 = nullptr, *d1a = nullptr, *d1 = nullptr;
-  awk::bf16 *c0 = nullptr, *d0a = nullptr, *d0 = nullptr;
-};
 
 using namespace awk;
 
@@ -386,27 +325,27 @@ static awk_status conv(const awk_unet *u, int l, const bf16 *x, bf16 *out, int n
   g.bias = u->bias[l];
   g.dact = nullptr;
   g.out = out;
-  g.C = kLayers[l].cin;
-  g.cout = kLayers[l].cout;
+  g.C = kUnetLayers[l].cin;
+  g.cout = kUnetLayers[l].cout;
   g.kpad = u->kpad[l];
   g.nchunks = 27 * (g.C / 8);
   g.c8magic = g.C > 8 ? (unsigned)(4294967296ull / (unsigned)(g.C / 8)) + 1u : 0u;
   g.nx = nx;
   g.ny = ny;
   g.nz = nz;
-  g.slope = 0.2f;
+  g.slope = kUnetSlope;
   return launch_conv3(g, s);
 }
 
-static awk_status pool(const bf16 *in, bf16 *out, int nx, int ny, int nz, int C, hipStream_t s) {
+awk_status awk::unet_pool(const bf16 *in, bf16 *out, int nx, int ny, int nz, int C, hipStream_t s) {
   const long long total = (long long)(nx / 2) * (ny / 2) * (nz / 2) * (C / 8);
   AWK_LAUNCH(k_pool2, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, out, in, nx, ny, nz, C);
   AWK_LAUNCH_CHECK("k_pool2");
   return AWK_OK;
 }
 
-static awk_status upcat(bf16 *out, const bf16 *a, int ca, const bf16 *b, int cb, int nx, int ny,
-                        int nz, hipStream_t s) {
+awk_status awk::unet_upcat(bf16 *out, const bf16 *a, int ca, const bf16 *b, int cb, int nx, int ny,
+                           int nz, hipStream_t s) {
   const long long total = (long long)nx * ny * nz * ((ca + cb) / 8);
   AWK_LAUNCH(k_upcat, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, out, a, ca, b, cb,
              nx, ny, nz);
@@ -414,11 +353,47 @@ static awk_status upcat(bf16 *out, const bf16 *a, int ca, const bf16 *b, int cb,
   return AWK_OK;
 }
 
+awk_status awk::unet_forward(awk_unet *u, const float *templ, const float *subject,
+                             float *const disp[3], hipStream_t s) {
+  const int nx = u->nx, ny = u->ny, nz = u->nz;
+  const int N0 = nx * ny * nz;
+  awk_status st;
+  AWK_LAUNCH(k_pack_input, dim3((N0 + 255) / 256), dim3(256), 0, s, u->x0, templ, subject, N0);
+  AWK_LAUNCH_CHECK("k_pack_input");
+  const int a1 = nx / 2, b1 = ny / 2, c1 = nz / 2, a2 = nx / 4, b2 = ny / 4, c2 = nz / 4,
+            a3 = nx / 8, b3 = ny / 8, c3 = nz / 8;
+  if ((st = conv(u, 0, u->x0, u->t0, nx, ny, nz, s))) return st;
+  if ((st = conv(u, 1, u->t0, u->s0, nx, ny, nz, s))) return st;
+  if ((st = unet_pool(u->s0, u->p0, nx, ny, nz, 16, s))) return st;
+  if ((st = conv(u, 2, u->p0, u->t1, a1, b1, c1, s))) return st;
+  if ((st = conv(u, 3, u->t1, u->s1, a1, b1, c1, s))) return st;
+  if ((st = unet_pool(u->s1, u->p1, a1, b1, c1, 32, s))) return st;
+  if ((st = conv(u, 4, u->p1, u->t2, a2, b2, c2, s))) return st;
+  if ((st = conv(u, 5, u->t2, u->s2, a2, b2, c2, s))) return st;
+  if ((st = unet_pool(u->s2, u->p2, a2, b2, c2, 64, s))) return st;
+  if ((st = conv(u, 6, u->p2, u->t3, a3, b3, c3, s))) return st;
+  if ((st = conv(u, 7, u->t3, u->s3, a3, b3, c3, s))) return st;
+  if ((st = unet_upcat(u->c2, u->s3, 128, u->s2, 64, a2, b2, c2, s))) return st;
+  if ((st = conv(u, 8, u->c2, u->d2a, a2, b2, c2, s))) return st;
+  if ((st = conv(u, 9, u->d2a, u->d2, a2, b2, c2, s))) return st;
+  if ((st = unet_upcat(u->c1, u->d2, 64, u->s1, 32, a1, b1, c1, s))) return st;
+  if ((st = conv(u, 10, u->c1, u->d1a, a1, b1, c1, s))) return st;
+  if ((st = conv(u, 11, u->d1a, u->d1, a1, b1, c1, s))) return st;
+  if ((st = unet_upcat(u->c0, u->d1, 32, u->s0, 16, nx, ny, nz, s))) return st;
+  if ((st = conv(u, 12, u->c0, u->d0a, nx, ny, nz, s))) return st;
+  if ((st = conv(u, 13, u->d0a, u->d0, nx, ny, nz, s))) return st;
+  AWK_LAUNCH(k_head, dim3((N0 + 255) / 256), dim3(256), 0, s, disp[0], disp[1], disp[2], u->d0,
+             u->head_w, u->head_b, N0, 16);
+  AWK_LAUNCH_CHECK("k_head");
+  return AWK_OK;
+}
+
+
 extern "C" {
 
 size_t awk_unet_weight_count(void) {
   size_t n = 0;
-  for (int l = 0; l < 14; ++l) n += (size_t)kLayers[l].cout * kBlobCin[l] * 27 + kLayers[l].cout;
+  for (int l = 0; l < 14; ++l) n += (size_t)kUnetLayers[l].cout * kUnetBlobCin[l] * 27 + kUnetLayers[l].cout;
   return n + 3 * 16 + 3;
 }
 
@@ -439,8 +414,8 @@ awk_status awk_unet_create(awk_unet_t *out, int nx, int ny, int nz, const float 
   hipError_t e = hipSuccess;
   size_t off = 0;
   for (int l = 0; l < 14 && e == hipSuccess; ++l) {
-    const LayerSpec &L = kLayers[l];
-    const int bc = kBlobCin[l];
+    const LayerSpec &L = kUnetLayers[l];
+    const int bc = kUnetBlobCin[l];
     u->kpad[l] = (27 * L.cin + 31) / 32 * 32;
     const std::vector<bf16> wt = pack_weights(weights + off, L.cout, L.cin, bc, u->kpad[l]);
     off += (size_t)L.cout * bc * 27;
@@ -487,38 +462,7 @@ awk_status awk_unet_forward(awk_unet_t u, const float *templ, const float *subje
                             float *const disp[3], awk_stream_t stream) {
   if (!u || !templ || !subject || !disp || !disp[0] || !disp[1] || !disp[2])
     return fail(AWK_ERR_INVALID, "awk_unet_forward: null argument");
-  hipStream_t s = as_stream(stream);
-  const int nx = u->nx, ny = u->ny, nz = u->nz;
-  const int N0 = nx * ny * nz;
-  awk_status st;
-  AWK_LAUNCH(k_pack_input, dim3((N0 + 255) / 256), dim3(256), 0, s, u->x0, templ, subject, N0);
-  AWK_LAUNCH_CHECK("k_pack_input");
-  const int a1 = nx / 2, b1 = ny / 2, c1 = nz / 2, a2 = nx / 4, b2 = ny / 4, c2 = nz / 4,
-            a3 = nx / 8, b3 = ny / 8, c3 = nz / 8;
-  if ((st = conv(u, 0, u->x0, u->t0, nx, ny, nz, s))) return st;
-  if ((st = conv(u, 1, u->t0, u->s0, nx, ny, nz, s))) return st;
-  if ((st = pool(u->s0, u->p0, nx, ny, nz, 16, s))) return st;
-  if ((st = conv(u, 2, u->p0, u->t1, a1, b1, c1, s))) return st;
-  if ((st = conv(u, 3, u->t1, u->s1, a1, b1, c1, s))) return st;
-  if ((st = pool(u->s1, u->p1, a1, b1, c1, 32, s))) return st;
-  if ((st = conv(u, 4, u->p1, u->t2, a2, b2, c2, s))) return st;
-  if ((st = conv(u, 5, u->t2, u->s2, a2, b2, c2, s))) return st;
-  if ((st = pool(u->s2, u->p2, a2, b2, c2, 64, s))) return st;
-  if ((st = conv(u, 6, u->p2, u->t3, a3, b3, c3, s))) return st;
-  if ((st = conv(u, 7, u->t3, u->s3, a3, b3, c3, s))) return st;
-  if ((st = upcat(u->c2, u->s3, 128, u->s2, 64, a2, b2, c2, s))) return st;
-  if ((st = conv(u, 8, u->c2, u->d2a, a2, b2, c2, s))) return st;
-  if ((st = conv(u, 9, u->d2a, u->d2, a2, b2, c2, s))) return st;
-  if ((st = upcat(u->c1, u->d2, 64, u->s1, 32, a1, b1, c1, s))) return st;
-  if ((st = conv(u, 10, u->c1, u->d1a, a1, b1, c1, s))) return st;
-  if ((st = conv(u, 11, u->d1a, u->d1, a1, b1, c1, s))) return st;
-  if ((st = upcat(u->c0, u->d1, 32, u->s0, 16, nx, ny, nz, s))) return st;
-  if ((st = conv(u, 12, u->c0, u->d0a, nx, ny, nz, s))) return st;
-  if ((st = conv(u, 13, u->d0a, u->d0, nx, ny, nz, s))) return st;
-  AWK_LAUNCH(k_head, dim3((N0 + 255) / 256), dim3(256), 0, s, disp[0], disp[1], disp[2], u->d0,
-             u->head_w, u->head_b, N0, 16);
-  AWK_LAUNCH_CHECK("k_head");
-  return AWK_OK;
+  return unet_forward(u, templ, subject, disp, as_stream(stream));
 }
 
 awk_status awk_unet_mfma_probe(float *out, awk_stream_t stream) {

@@ -83,3 +83,44 @@ class UNetWarmStart:
             self._h = None
         except Exception:
             pass
+
+
+class UNetTrainer:
+    """Adam training of a ``UNetWarmStart`` on (template, subject, target
+    displacement) triples, all on the device (``awk_unet_trainer_*``)."""
+
+    def __init__(self, net):
+        self.net = net
+        h = ctypes.c_void_p()
+        w = net.weights
+        _lib.call("awk_unet_trainer_create", ctypes.byref(h), net._h, w.ctypes.data_as(ctypes.c_void_p),
+                  ctypes.c_size_t(w.size))
+        self._h = h
+        self.loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+
+    def step(self, template, subject, target, lr=1e-3, stream=None):
+        """One Adam step; the step's loss is added to ``self.loss`` on the device."""
+        _lib.call("awk_unet_trainer_step", self._h, ptr(template), ptr(subject), planes_arr(target),
+                  float(lr), ctypes.c_void_p(self.loss.data_ptr()), stream_handle(stream))
+
+    def gradients(self, template, subject, target):
+        """(gradient blob, loss) of one pair without an update (synchronises)."""
+        g = np.zeros(self.net.weights.size, np.float32)
+        loss = ctypes.c_double()
+        _lib.call("awk_unet_trainer_gradients", self._h, ptr(template), ptr(subject),
+                  planes_arr(target), g.ctypes.data_as(ctypes.c_void_p), ctypes.byref(loss))
+        return g, loss.value
+
+    def weights(self):
+        w = np.zeros(self.net.weights.size, np.float32)
+        _lib.call("awk_unet_trainer_get_weights", self._h, w.ctypes.data_as(ctypes.c_void_p),
+                  ctypes.c_size_t(w.size))
+        return w
+
+    def __del__(self):
+        try:
+            if self._h is not None and self._h.value:
+                _lib.call("awk_unet_trainer_destroy", self._h)
+            self._h = None
+        except Exception:
+            pass

@@ -143,6 +143,107 @@ def bench_unet(dev, n=128, reps=10):
             "frac": round(flops / (ms * 1e-3) / 2.5e15, 4), "size": [n, n, n]}
 
 
+def _rand_disp(shape, max_disp, gen, dev):
+    """Smooth random displacement [3, z, y, x]: 4 sinusoids per component with
+    wave numbers in {1, 2, 3}^3 and random phases, scaled to max |u| = max_disp."""
+    import math
+    import torch
+    nz, ny, nx = shape
+    z, y, x = torch.meshgrid(torch.arange(nz, device=dev, dtype=torch.float32),
+                             torch.arange(ny, device=dev, dtype=torch.float32),
+                             torch.arange(nx, device=dev, dtype=torch.float32), indexing="ij")
+    u = torch.zeros((3,) + tuple(shape), device=dev)
+    for c in range(3):
+        for _ in range(4):
+            k = torch.randint(1, 4, (3,), generator=gen).tolist()
+            ph = float(torch.rand(1, generator=gen)) * 2 * math.pi
+            a = float(torch.rand(1, generator=gen)) + 0.5
+            u[c] += a * torch.sin(2 * math.pi * (k[0] * x / nx + k[1] * y / ny + k[2] * z / nz) + ph)
+    return u * (max_disp / u.norm(dim=0).max())
+
+
+def bench_warm_start(dev, n=128, n_train=96, n_eval=4, steps=480, plain_iters=60, seed=4):
+    """Time-to-SSD, plain greedy vs U-Net warm start, on synthetic pairs at n^3.
+
+    Pairs: subject(x) = template(x + u(x)) with a smooth random u (max 5 voxels);
+    training targets are plain greedy's final displacements (plain_iters
+    iterations, MaxPert 0.1), as the U-Net would be trained on real data.  The
+    U-Net trains on the device (Adam, lr 3e-4, one pair per step), then on held-out
+    pairs: plain greedy from the identity for plain_iters iterations sets the
+    target SSD 1.01 x E_plain; the warm start runs greedy from h0 = x + U-Net(pair)
+    until its SSD reaches it.  Times: per-iteration and forward times measured
+    with events, times the iteration counts."""
+    import time
+    import torch
+    from atlaswerks_amd import hfield, synth
+    from atlaswerks_amd.greedy import GreedyIteratorParam, GreedyWarp
+    from atlaswerks_amd.unet import UNetTrainer, UNetWarmStart
+    shape = (n, n, n)
+    gen = torch.Generator().manual_seed(seed)
+    templ = torch.from_numpy(synth.blobs(shape, seed=seed)).to(dev)
+    ident = torch.from_numpy(synth.identity(shape)).to(dev)
+    prm = GreedyIteratorParam(max_pert=0.1)
+
+    def make_pair():
+        u = _rand_disp(shape, 5.0, gen, dev)
+        subj = torch.empty_like(templ)
+        hfield.apply(subj, templ, ident + u)
+        return subj
+
+    def plain(subj, iters, h0=None):
+        w = GreedyWarp(templ, subj, iters, param=prm, h0=h0)
+        d = w.RunWarp()
+        return d
+    t0 = time.time()
+    train = []
+    for _ in range(n_train):
+        subj = make_pair()
+        d = plain(subj, plain_iters)
+        train.append((subj, (d.Def1To0 - ident).contiguous()))
+    torch.cuda.synchronize()
+    gen_s = time.time() - t0
+    net = UNetWarmStart((n, n, n), seed=seed)
+    tr = UNetTrainer(net)
+    t0 = time.time()
+    for k in range(steps):
+        subj, tgt = train[k % n_train]
+        tr.step(templ, subj, tgt, lr=3e-4)
+    torch.cuda.synchronize()
+    train_s = time.time() - t0
+    # per-iteration and forward times
+    d = plain(train[0][0], 2)
+    it = GreedyWarp(templ, train[0][0], 1, param=prm)
+    it.RunWarp()
+    iter_ms = _events_ms(lambda: it.iterator.Iterate(it.data), 20)
+    fwd_ms = _events_ms(lambda: net.forward(templ, train[0][0]), 5)
+    res = []
+    for _ in range(n_eval):
+        subj = make_pair()
+        e_plain = plain(subj, plain_iters).energies()
+        thr = 1.01 * e_plain[-1]
+        k_plain = int(next(i for i, e in enumerate(e_plain) if e <= thr))
+        h0 = net.forward(templ, subj)
+        h0 += ident
+        e_warm = plain(subj, plain_iters, h0=h0).energies()
+        hit = [i for i, e in enumerate(e_warm) if e <= thr]
+        k_warm = int(hit[0]) if hit else None
+        res.append({"plain_iters": k_plain, "warm_iters": k_warm,
+                    "e0_plain": round(float(e_plain[0]), 2), "e0_warm": round(float(e_warm[0]), 2),
+                    "e_final_plain": round(float(e_plain[-1]), 2),
+                    "e_final_warm": round(float(e_warm[-1]), 2)})
+    ok = [r for r in res if r["warm_iters"] is not None]
+    t_plain = sum(r["plain_iters"] for r in res) / len(res) * iter_ms
+    t_warm = (sum(r["warm_iters"] for r in ok) / len(ok) * iter_ms + fwd_ms) if ok else None
+    return {"size": [n, n, n], "train_pairs": n_train, "train_steps": steps,
+            "pair_generation_s": round(gen_s, 2), "train_s": round(train_s, 2),
+            "ms_per_train_step": round(1000 * train_s / steps, 2),
+            "iter_ms": round(iter_ms, 4), "unet_forward_ms": round(fwd_ms, 3),
+            "time_to_ssd_plain_ms": round(t_plain, 3),
+            "time_to_ssd_warm_ms": round(t_warm, 3) if t_warm is not None else None,
+            "warm_reached": len(ok), "pairs": res,
+            "warm_over_plain": round(t_warm / t_plain, 3) if t_warm is not None and t_plain else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -315,6 +416,7 @@ def main():
         out["ldmm_128_T10"] = bench_ldmm(dev)
         out["atlas_16_uneven"] = bench_atlas(dev)
         out["unet_128"] = bench_unet(dev)
+        out["warm_start_128"] = bench_warm_start(dev)
 
     if args.copy_ref:
         # plain 1 GiB device copy: the calibration point for the HBM byte counters

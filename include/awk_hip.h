@@ -374,6 +374,27 @@ awk_status awk_unet_destroy(awk_unet_t u);
  * LeakyReLU(0.2), 2x2x2 average pooling, nearest x2 upsampling; fp32 output. */
 awk_status awk_unet_forward(awk_unet_t u, const float *templ, const float *subject,
                             float *const disp[3], awk_stream_t stream);
+/* Training on the device (no reference counterpart): MSE between the predicted
+ * and a target displacement, backward on MFMA, Adam (beta 0.9 / 0.999, eps 1e-8)
+ * on fp32 master weights initialised from `weights`; every step refreshes the
+ * bf16 weights of `net`.  Gradient sums use float atomics: steps are not
+ * bit-reproducible. */
+typedef struct awk_unet_trainer *awk_unet_trainer_t;
+awk_status awk_unet_trainer_create(awk_unet_trainer_t *out, awk_unet_t net, const float *weights,
+                                   size_t n_weights);
+awk_status awk_unet_trainer_destroy(awk_unet_trainer_t t);
+/* One step on one (template, subject, target displacement) triple; *loss_dev
+ * (device double) is incremented by the step's loss. */
+awk_status awk_unet_trainer_step(awk_unet_trainer_t t, const float *templ, const float *subject,
+                                 const float *const target[3], float lr, double *loss_dev,
+                                 awk_stream_t stream);
+/* Gradient of the loss with respect to the weight blob (host copy; synchronises),
+ * without an update. */
+awk_status awk_unet_trainer_gradients(awk_unet_trainer_t t, const float *templ, const float *subject,
+                                      const float *const target[3], float *grad_host,
+                                      double *loss_host);
+/* The fp32 master weights, in blob order (synchronises). */
+awk_status awk_unet_trainer_get_weights(awk_unet_trainer_t t, float *host_out, size_t n);
 /* Diagnostic: 64 x 4 floats, lane l's accumulator registers of one
  * v_mfma_f32_16x16x32_bf16 on fixed small-integer operands. */
 awk_status awk_unet_mfma_probe(float *out, awk_stream_t stream);

@@ -72,3 +72,87 @@ def test_unet_forward_vs_torch(shape):
     assert err <= 1e-2
     u2 = host(net.forward(to_dev(t), to_dev(s)))
     np.testing.assert_array_equal(u, u2)  # deterministic
+
+
+def reference_grads(blob, t, s, target):
+    """Gradient of mean((u - target)^2) through the reference network (float32,
+    bf16 rounding where the kernels store bf16; the casts pass gradients through,
+    rounded to bf16 on the way back as the kernels' gradient tensors are)."""
+    import torch
+    import torch.nn.functional as F
+    from atlaswerks_amd.unet import split_weights
+    bf = lambda x: x.to(torch.bfloat16).to(torch.float32)
+    convs, (hw, hb) = split_weights(blob)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a))
+    Ws = [bf(T(w)).requires_grad_() for w, _ in convs]
+    Bs = [T(b).clone().requires_grad_() for _, b in convs]
+    HW = T(hw).clone().requires_grad_()
+    HB = T(hb).clone().requires_grad_()
+
+    def conv(x, l):
+        return bf(F.leaky_relu(F.conv3d(x, Ws[l], Bs[l], padding=1), 0.2))
+
+    pool = lambda x: bf(F.avg_pool3d(x, 2))
+    up = lambda x: F.interpolate(x, scale_factor=2, mode="nearest")
+    x = bf(torch.stack([T(t), T(s)])[None])
+    s0 = conv(conv(x, 0), 1)
+    s1 = conv(conv(pool(s0), 2), 3)
+    s2 = conv(conv(pool(s1), 4), 5)
+    s3 = conv(conv(pool(s2), 6), 7)
+    d2 = conv(conv(torch.cat([up(s3), s2], 1), 8), 9)
+    d1 = conv(conv(torch.cat([up(d2), s1], 1), 10), 11)
+    d0 = conv(conv(torch.cat([up(d1), s0], 1), 12), 13)
+    u = F.conv3d(d0, HW.view(3, 16, 1, 1, 1), HB)[0]
+    loss = ((u - T(target)) ** 2).mean()
+    loss.backward()
+    parts = []
+    for w, b in zip(Ws, Bs):
+        parts += [w.grad.numpy().ravel(), b.grad.numpy().ravel()]
+    parts += [HW.grad.numpy().ravel(), HB.grad.numpy().ravel()]
+    return np.concatenate(parts), float(loss)
+
+
+def test_unet_gradients_vs_torch():
+    """Weight gradients of one pair against autograd through the reference: every
+    layer's gradient within 5 % relative L2 (bf16 gradient tensors both sides)."""
+    import torch
+    from atlaswerks_amd.unet import LAYERS, UNetTrainer, UNetWarmStart, init_weights
+    from gpu_util import to_dev
+    shape = (16, 16, 32)
+    t = synth.blobs(shape, seed=4)
+    s = synth.blobs(shape, seed=5)
+    target = synth.sinusoidal_displacement(shape, max_disp=2.0, seed=6)
+    blob = init_weights(seed=7, head_std=0.5)
+    net = UNetWarmStart(shape[::-1], blob)
+    tr = UNetTrainer(net)
+    g, loss = tr.gradients(to_dev(t), to_dev(s), to_dev(target))
+    gr, lr_ = reference_grads(blob, t, s, target)
+    print("loss", loss, lr_, flush=True)
+    assert abs(loss / lr_ - 1) < 1e-2
+    off = 0
+    for l, (cin, cout) in enumerate(LAYERS + ((16, 3),)):
+        n = cout * cin * 27 + cout if l < len(LAYERS) else 51
+        a, b = g[off:off + n], gr[off:off + n]
+        err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        print("layer", l, "grad rel L2", err, flush=True)
+        assert err <= 5e-2, (l, err)
+        off += n
+
+
+def test_unet_training_reduces_loss():
+    import torch
+    from atlaswerks_amd.unet import UNetTrainer, UNetWarmStart, init_weights
+    from gpu_util import to_dev
+    shape = (32, 32, 32)
+    t = to_dev(synth.blobs(shape, seed=8))
+    s = to_dev(synth.blobs(shape, seed=9))
+    target = to_dev(synth.sinusoidal_displacement(shape, max_disp=2.0, seed=10))
+    net = UNetWarmStart(shape[::-1], init_weights(seed=11))
+    tr = UNetTrainer(net)
+    losses = []
+    for it in range(60):
+        tr.loss.zero_()
+        tr.step(t, s, target, lr=3e-4)
+        losses.append(float(tr.loss.item()))
+    print("losses", np.array2string(np.array(losses), precision=4), flush=True)
+    assert losses[-1] < 0.5 * losses[0]
