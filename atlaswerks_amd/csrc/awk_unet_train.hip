@@ -25,8 +25,8 @@
 
 namespace awk {
 
-constexpr int kWgStage = 64;      // positions staged per LDS round
-constexpr int kWgPitch = 72;      // LDS row pitch (bf16): 144 B keeps 16-byte reads aligned
+constexpr int kWgStage = 128;     // positions staged per LDS round (4 K steps of 32)
+constexpr int kWgPitch = 136;     // LDS row pitch (bf16): 272 B keeps 16-byte reads aligned
 constexpr int kWgMaxCi = 192;
 constexpr int kWgMaxCo = 128;
 constexpr int kWgMaxTiles = 16;   // accumulator tiles per wave
@@ -35,15 +35,20 @@ struct WgradArgs {
   const bf16 *dz;  // [N][CO]
   const bf16 *x;   // [N][CI]
   float *dw;       // [CO][CIb][27] (blob OIDHW), accumulated
+  float *db;       // [CO], accumulated by the centre-tap blocks; may be null
   int CO, CI, CIb;
   int nx, ny, nz;
   int chunk;       // positions per block (multiple of kWgStage)
 };
 
+// Work split: with at least 4 output tiles, wave w owns tiles w, w+4, ... over all
+// K steps; with fewer, every wave owns all tiles and the K step ks = w of each
+// staged round, and the waves' partial sums are added through LDS at the end.
 __global__ void __launch_bounds__(256)
 k_wgrad(WgradArgs g) {
-  __shared__ bf16 zt[kWgMaxCo * kWgPitch];
-  __shared__ bf16 xt[kWgMaxCi * kWgPitch];
+  extern __shared__ bf16 lds[];
+  bf16 *zt = lds;
+  bf16 *xt = lds + g.CO * kWgPitch;
   const int tap = (int)blockIdx.y;
   const int N = g.nx * g.ny * g.nz;
   const int pbeg = (int)blockIdx.x * g.chunk;
@@ -53,6 +58,9 @@ k_wgrad(WgradArgs g) {
   const int col = lane & 15, grp = lane >> 4;
   const int CIt = (g.CI + 15) / 16, COt = g.CO / 16;
   const int ntiles = COt * CIt;
+  const bool split_k = ntiles < 4;
+  const bool do_bias = g.db && tap == 13;
+  float bsum = 0.f;
   f32x4 acc[kWgMaxTiles];
 #pragma unroll
   for (int j = 0; j < kWgMaxTiles; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -81,24 +89,54 @@ k_wgrad(WgradArgs g) {
       for (int j = 0; j < 8; ++j) xt[(c8 * 8 + j) * kWgPitch + q] = v[j];
     }
     __syncthreads();
+    if (do_bias && (int)threadIdx.x < g.CO)
+      for (int q = 0; q < kWgStage; ++q) bsum += (float)zt[threadIdx.x * kWgPitch + q];
+    if (split_k) {
+      const int ks = wave;
 #pragma unroll
-    for (int ks = 0; ks < kWgStage / 32; ++ks) {
-#pragma unroll
-      for (int j = 0; j < kWgMaxTiles; ++j) {
-        const int tile = wave + 4 * j;
-        if (tile >= ntiles) break;
-        const int m = tile / CIt, n = tile - m * CIt;
+      for (int j = 0; j < 3; ++j) {
+        if (j >= ntiles) break;
+        const int m = j / CIt, n = j - m * CIt;
         const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&zt[(m * 16 + col) * kWgPitch + ks * 32 + grp * 8]);
         const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&xt[(n * 16 + col) * kWgPitch + ks * 32 + grp * 8]);
         acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
       }
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < kWgStage / 32; ++ks) {
+#pragma unroll
+        for (int j = 0; j < kWgMaxTiles; ++j) {
+          const int tile = wave + 4 * j;
+          if (tile >= ntiles) break;
+          const int m = tile / CIt, n = tile - m * CIt;
+          const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&zt[(m * 16 + col) * kWgPitch + ks * 32 + grp * 8]);
+          const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&xt[(n * 16 + col) * kWgPitch + ks * 32 + grp * 8]);
+          acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+        }
+      }
     }
     __syncthreads();
+  }
+  if (do_bias && (int)threadIdx.x < g.CO) atomicAdd(&g.db[threadIdx.x], bsum);
+  if (split_k) {
+    // add the 4 waves' partial tiles (the staging space is free after the last sync)
+    f32x4 *red = reinterpret_cast<f32x4 *>(lds);
+    if (wave > 0)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        if (j < ntiles) red[((wave - 1) * 3 + j) * 64 + lane] = acc[j];
+    __syncthreads();
+    if (wave > 0) return;
+#pragma unroll
+    for (int w = 0; w < 3; ++w)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        if (j < ntiles) acc[j] += red[(w * 3 + j) * 64 + lane];
   }
   // D[co][ci]: lane holds co = m*16 + 4*grp + i, ci = n*16 + col
 #pragma unroll
   for (int j = 0; j < kWgMaxTiles; ++j) {
-    const int tile = wave + 4 * j;
+    const int tile = split_k ? j : wave + 4 * j;
     if (tile >= ntiles) break;
     const int m = tile / CIt, n = tile - m * CIt;
     const int ci = n * 16 + col;
@@ -111,20 +149,9 @@ k_wgrad(WgradArgs g) {
   }
 }
 
-// db[c] += sum_p dz[p][c]
-__global__ void __launch_bounds__(256)
-k_bias_grad(float *__restrict__ db, const bf16 *__restrict__ dz, int N, int C, int per_block) {
-  const int c = threadIdx.x % C, lanes = 256 / C;
-  if (threadIdx.x >= lanes * C) return;
-  const int p0 = (int)blockIdx.x * per_block, p1 = min(p0 + per_block, N);
-  float s = 0.f;
-  for (int p = p0 + (int)threadIdx.x / C; p < p1; p += lanes) s += (float)dz[(size_t)p * C + c];
-  atomicAdd(&db[c], s);
-}
-
-// Loss and head backward: du = 2 (u - t) / (3N); loss += sum (u - t)^2 / (3N);
-// dZ13 = (Wh^T du) * f'(d0); dWh += du d0^T; dbh += du.  The 51 weight sums are
-// reduced over each wave by shuffles and added once per wave.
+// Loss and head backward: du = 2 (u - t) / (3N); dZ13 = (Wh^T du) * f'(d0).  The
+// 48 + 3 head-gradient sums and the loss are reduced per block (shuffles, then
+// LDS) into part[block][52]; k_head_reduce adds the blocks in order.
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -132,20 +159,20 @@ __device__ __forceinline__ float wave_sum(float v) {
 }
 
 __global__ void __launch_bounds__(256)
-k_head_bwd(bf16 *__restrict__ dz13, float *__restrict__ dwh, float *__restrict__ dbh,
-           double *__restrict__ loss, const float *__restrict__ u0, const float *__restrict__ u1,
-           const float *__restrict__ u2, const float *__restrict__ t0, const float *__restrict__ t1,
-           const float *__restrict__ t2, const bf16 *__restrict__ d0, const float *__restrict__ wh,
-           int N, float slope) {
+k_head_bwd(bf16 *__restrict__ dz13, float *__restrict__ part, const float *__restrict__ u0,
+           const float *__restrict__ u1, const float *__restrict__ u2, const float *__restrict__ t0,
+           const float *__restrict__ t1, const float *__restrict__ t2, const bf16 *__restrict__ d0,
+           const float *__restrict__ wh, int N, float slope) {
+  __shared__ float sred[4][52];
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const bool live = i < N;
-  double l = 0.0;
+  float l = 0.f;
   float g0 = 0.f, g1 = 0.f, g2 = 0.f;
   float a[16];
   if (live) {
     const float inv = 2.f / (3.f * (float)N);
     const float e0 = u0[i] - t0[i], e1 = u1[i] - t1[i], e2 = u2[i] - t2[i];
-    l = ((double)e0 * e0 + (double)e1 * e1 + (double)e2 * e2) / (3.0 * N);
+    l = (e0 * e0 + e1 * e1 + e2 * e2) / (3.f * (float)N);
     g0 = e0 * inv;
     g1 = e1 * inv;
     g2 = e2 * inv;
@@ -166,25 +193,49 @@ k_head_bwd(bf16 *__restrict__ dz13, float *__restrict__ dwh, float *__restrict__
 #pragma unroll
     for (int c = 0; c < 16; ++c) a[c] = 0.f;
   }
+  const int wave = threadIdx.x >> 6;
   const bool lead = (threadIdx.x & 63) == 0;
 #pragma unroll
   for (int c = 0; c < 16; ++c) {
     const float s0 = wave_sum(g0 * a[c]), s1 = wave_sum(g1 * a[c]), s2 = wave_sum(g2 * a[c]);
     if (lead) {
-      atomicAdd(&dwh[c], s0);
-      atomicAdd(&dwh[16 + c], s1);
-      atomicAdd(&dwh[32 + c], s2);
+      sred[wave][c] = s0;
+      sred[wave][16 + c] = s1;
+      sred[wave][32 + c] = s2;
     }
   }
-  const float b0 = wave_sum(g0), b1 = wave_sum(g1), b2 = wave_sum(g2);
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o, 64);
+  const float b0 = wave_sum(g0), b1 = wave_sum(g1), b2 = wave_sum(g2), ls = wave_sum(l);
   if (lead) {
-    atomicAdd(&dbh[0], b0);
-    atomicAdd(&dbh[1], b1);
-    atomicAdd(&dbh[2], b2);
-    atomicAdd(loss, l);
+    sred[wave][48] = b0;
+    sred[wave][49] = b1;
+    sred[wave][50] = b2;
+    sred[wave][51] = ls;
   }
+  __syncthreads();
+  if (threadIdx.x < 52)
+    part[(size_t)blockIdx.x * 52 + threadIdx.x] =
+        sred[0][threadIdx.x] + sred[1][threadIdx.x] + sred[2][threadIdx.x] + sred[3][threadIdx.x];
+}
+
+// One block per column c of the partials: a fixed-order FP64 tree over the
+// blocks; dWh, dbh into the gradient, the loss into *loss.
+__global__ void __launch_bounds__(256)
+k_head_reduce(float *__restrict__ dwh, float *__restrict__ dbh, double *__restrict__ loss,
+              const float *__restrict__ part, int nblocks) {
+  __shared__ double sm[256];
+  const int c = (int)blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += 256) s += part[(size_t)b * 52 + c];
+  sm[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (c < 48) dwh[c] += (float)sm[0];
+  else if (c < 51) dbh[c - 48] += (float)sm[0];
+  else *loss += sm[0];
 }
 
 // Backward of [up(A), B]: dA[q][c] = (sum over the 8 children of dCat[child][c]) *
@@ -286,13 +337,14 @@ struct awk_unet_trainer {
        *dc1 = nullptr, *dz9 = nullptr, *dz8 = nullptr, *dc2 = nullptr, *dz7 = nullptr,
        *dz6 = nullptr, *dp2 = nullptr, *dz5 = nullptr, *dz4 = nullptr, *dp1 = nullptr,
        *dz3 = nullptr, *dz2 = nullptr, *dp0 = nullptr, *dz1 = nullptr, *dz0 = nullptr;
-  float *u = nullptr;  // 3 planes: the forward output
+  float *u = nullptr;     // 3 planes: the forward output
+  float *hpart = nullptr; // head-gradient block partials [blocks][52]
 };
 
 static void trainer_free(awk_unet_trainer *t) {
   void *b[] = {t->w, t->g, t->m, t->v, t->dz13, t->dz12, t->dc0, t->dz11, t->dz10, t->dc1, t->dz9,
                t->dz8, t->dc2, t->dz7, t->dz6, t->dp2, t->dz5, t->dz4, t->dp1, t->dz3, t->dz2,
-               t->dp0, t->dz1, t->dz0, t->u};
+               t->dp0, t->dz1, t->dz0, t->u, t->hpart};
   for (void *p : b)
     if (p) (void)hipFree(p);
   for (int l = 0; l < 14; ++l)
@@ -350,17 +402,22 @@ static awk_status wgrad(awk_unet_trainer *t, int l, const bf16 *dz, const bf16 *
   g.nx = nx;
   g.ny = ny;
   g.nz = nz;
+  g.db = t->g + t->boff[l];
   const long long N = (long long)nx * ny * nz;
-  long long chunk = (N + 255) / 256;
+  // blocks per tap: up to 256, but few enough that the layer's partial-sum
+  // atomics (blocks x cout x cin) stay around 4M
+  long long per_tap = (4LL << 20) / (27LL * L.cout * kUnetBlobCin[l]);
+  per_tap = per_tap < 1 ? 1 : (per_tap > 256 ? 256 : per_tap);
+  long long chunk = (N + per_tap - 1) / per_tap;
   chunk = (chunk + kWgStage - 1) / kWgStage * kWgStage;
   g.chunk = (int)chunk;
   const dim3 grid((unsigned)((N + chunk - 1) / chunk), 27);
-  AWK_LAUNCH(k_wgrad, grid, dim3(256), 0, s, g);
+  const int CIt = (L.cin + 15) / 16;
+  size_t lds = (size_t)(L.cout + CIt * 16) * kWgPitch * sizeof(bf16);
+  const size_t red = (size_t)3 * 3 * 64 * sizeof(f32x4);  // split-K partials reuse the staging space
+  if (lds < red) lds = red;
+  AWK_LAUNCH(k_wgrad, grid, dim3(256), lds, s, g);
   AWK_LAUNCH_CHECK("k_wgrad");
-  const int per_block = 4096;
-  AWK_LAUNCH(k_bias_grad, dim3((unsigned)((N + per_block - 1) / per_block)), dim3(256), 0, s,
-             t->g + t->boff[l], dz, (int)N, L.cout, per_block);
-  AWK_LAUNCH_CHECK("k_bias_grad");
   return AWK_OK;
 }
 
@@ -377,9 +434,11 @@ static awk_status train_backward(awk_unet_trainer *t, const float *templ, const 
   float *up[3] = {t->u, t->u + n0, t->u + 2 * n0};
   awk_status st;
   if ((st = unet_forward(u, templ, subject, up, s))) return st;
-  AWK_LAUNCH(k_head_bwd, dim3((N0 + 255) / 256), dim3(256), 0, s, t->dz13, t->g + t->hoff,
-             t->g + t->hoff + 48, loss_dev, up[0], up[1], up[2], target[0], target[1], target[2],
-             u->d0, u->head_w, N0, kUnetSlope);
+  const int hb = (N0 + 255) / 256;
+  AWK_LAUNCH(k_head_bwd, dim3(hb), dim3(256), 0, s, t->dz13, t->hpart, up[0], up[1], up[2],
+             target[0], target[1], target[2], u->d0, u->head_w, N0, kUnetSlope);
+  AWK_LAUNCH(k_head_reduce, dim3(52), dim3(256), 0, s, t->g + t->hoff, t->g + t->hoff + 48, loss_dev,
+             t->hpart, hb);
   AWK_LAUNCH_CHECK("k_head_bwd");
   const float sl = kUnetSlope;
   auto blocks = [](long long n) { return dim3((unsigned)((n + 255) / 256)); };
@@ -445,6 +504,10 @@ awk_status awk_unet_trainer_create(awk_unet_trainer_t *out, awk_unet_t net, cons
     if (kUnetLayers[l].cout > kWgMaxCo || kUnetLayers[l].cin > kWgMaxCi ||
         (kUnetLayers[l].cout / 16) * ((kUnetLayers[l].cin + 15) / 16) > 4 * kWgMaxTiles)
       return fail(AWK_ERR_UNSUPPORTED, "awk_unet_trainer_create: layer too wide for the gradient kernel");
+  // the widest layers stage (128 + 128) x 136 bf16 = 68 KiB of LDS per wgrad block
+  AWK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_wgrad),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (kWgMaxCo + kWgMaxCi) * kWgPitch * (int)sizeof(bf16)));
   awk_unet_trainer *t = new (std::nothrow) awk_unet_trainer();
   if (!t) return fail(AWK_ERR_ALLOC, "awk_unet_trainer_create: host allocation failed");
   t->net = net;
@@ -484,6 +547,7 @@ awk_status awk_unet_trainer_create(awk_unet_trainer_t *out, awk_unet_t net, cons
   for (auto &b : bufs)
     if (e == hipSuccess) e = hipMalloc((void **)b.p, b.n * sizeof(bf16));
   if (e == hipSuccess) e = hipMalloc(&t->u, 3 * N0 * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&t->hpart, (N0 + 255) / 256 * 52 * sizeof(float));
   awk_status st = AWK_OK;
   if (e != hipSuccess) st = hip_check(e, "awk_unet_trainer_create: allocation");
   if (!st) st = repack_all(t, nullptr);
