@@ -71,6 +71,7 @@ SIGNATURES = {
     "awk_unet_trainer_get_weights": [_P, _P, ctypes.c_size_t],
     "awk_ldmm_create": [ctypes.POINTER(_P), _I, _I, _I, _F, _F, _F, _F, _F, _F, _I, _F, _F],
     "awk_ldmm_destroy": [_P],
+    "awk_ldmm_set_gradient": [_P, _I],
     "awk_ldmm_update_step_next": [_P],
     "awk_ldmm_set_step": [_P, _F],
     "awk_ldmm_get_step": [_P, ctypes.POINTER(_F)],

@@ -1,7 +1,15 @@
-// awk_ldmm.hip -- one LDMM iteration (adjoint gradient, IterateNew) on gfx950.
+// awk_ldmm.hip -- one LDMM iteration on gfx950, in both gradient forms of the
+// reference (LDMMIteratorGPU.cxx / LDMMIteratorCPU.cxx; mUseOrigGradientCalc,
+// true by default, selects IterateOrig).
 //
-// LDMMIteratorGPU::IterateNew (Algorithms/LDMM/LDMMIteratorGPU.cxx:159-293) and
-// LDMMIteratorCPU::IterateNew (LDMMIteratorCPU.cxx:529-631).  For T time steps:
+// IterateOrig (LDMMIteratorGPU.cxx:295-441, LDMMIteratorCPU.cxx:379-527):
+//   backward  t = T-1..0, one fused gather kernel per t: h <- h(x + v_t/sp),
+//             JT_t = I1(h), JacDet_t = JacDet_{t+1}(x + v_t/sp) |D(x sp + v_t)|
+//   forward   t = 0..T-1: the Navier solve whose x pass forms grad(J_t) m_t,
+//             m_t = (J_t - JT_t) JacDet_t, and whose inverse x pass updates v_t
+//             in place; then one fused kernel h <- h(x - v_t/sp), J_{t+1} = I0(h)
+//             and m_{t+1}; the last one forms the FP64 SSD partials.
+// IterateNew (LDMMIteratorGPU.cxx:159-293, LDMMIteratorCPU.cxx:529-631):
 //   forward   h <- h(x - v_t(x)/sp) (PARTIAL_ID) and J_t = I0(h), one fused kernel
 //             per t; the last one also forms diff = J_{T-1} - I1 and FP64 SSD partials
 //   backward  h <- h - v_t(h)/sp (PARTIAL_ZERO) and the splat of diff to h(x) into an
@@ -181,6 +189,145 @@ __global__ void k_ldmm_step(const float *__restrict__ maxpart, int n, float max_
   if (threadIdx.x == 0) *step = max_pert / s[0];
 }
 
+// ---- IterateOrig (LDMMIteratorGPU.cxx:295-441, LDMMIteratorCPU.cxx:379-527) ----
+
+// Gradient along one axis of f(q) = b(q) + a[q] (computeGradient, wrap = false;
+// Array3DUtils.txx:783-901); bm, b0, bp = the identity part at p - 1, p, p + 1.
+__device__ __forceinline__ float hgrad(const float *__restrict__ a, int i, int p, int n, int stride,
+                                       float sp, float bm, float b0, float bp) {
+  if (n == 1) return 0.f;
+  if (p > 0 && p < n - 1) return ((bp + a[i + stride]) - (bm + a[i - stride])) / (2.f * sp);
+  if (p == 0) return ((bp + a[i + stride]) - (b0 + a[i])) / sp;
+  return ((b0 + a[i]) - (bm + a[i - stride])) / sp;
+}
+
+// |D(x sp + v)| at voxel i: HField3DUtils::jacobian (HField3DUtils.txx:2370-2420) of
+// velocityToH_US (cudaHField3DUtilsSp.cu:137-153), one row per component.
+__device__ __forceinline__ float jac_det_xsp_v(const float *__restrict__ vx,
+                                               const float *__restrict__ vy,
+                                               const float *__restrict__ vz, int i, int x, int y,
+                                               int z, Dims d, float sx, float sy, float sz) {
+  const int sxy = d.nx * d.ny;
+  const float *vc[3] = {vx, vy, vz};
+  const int pos[3] = {x, y, z}, n[3] = {d.nx, d.ny, d.nz}, stride[3] = {1, d.nx, sxy};
+  const float sp[3] = {sx, sy, sz};
+  float g[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float own = (float)pos[c] * sp[c];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      float bm = own, b0 = own, bp = own;
+      if (c == e) {
+        bm = (float)(pos[c] - 1) * sp[c];
+        bp = (float)(pos[c] + 1) * sp[c];
+      }
+      g[c][e] = hgrad(vc[c], i, pos[e], n[e], stride[e], sp[e], bm, b0, bp);
+    }
+  }
+  const float t1 = g[0][0] * (g[1][1] * g[2][2] - g[1][2] * g[2][1]);
+  const float t2 = g[0][1] * (g[1][0] * g[2][2] - g[1][2] * g[2][0]);
+  const float t3 = g[0][2] * (g[1][0] * g[2][1] - g[1][1] * g[2][0]);
+  return t1 - t2 + t3;
+}
+
+// Backward step t: h <- h(x + v_t/sp) (composeHV, PARTIAL_ID), JT_t = I1(h) (VAL 0),
+// JacDet_t = JacDet_{t+1}(x + v_t/sp) (applyU, VAL 0) * |D(x sp + v_t)|, or the
+// determinant alone at t = T-1 (jdn == nullptr).
+__global__ void __launch_bounds__(kLdmmBlock)
+k_orig_bwd(float *__restrict__ nx_, float *__restrict__ ny_, float *__restrict__ nz_,
+           const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
+           const float *__restrict__ vx, const float *__restrict__ vy, const float *__restrict__ vz,
+           const float *__restrict__ I1, const float *__restrict__ jdn, float *__restrict__ jt,
+           float *__restrict__ jd, Dims d, float sx, float sy, float sz) {
+  const int N = d.nx * d.ny * d.nz;
+  const int i = (int)xcd_block() * kLdmmBlock + (int)threadIdx.x;
+  if (i >= N) return;
+  const int x = i % d.nx, r = i / d.nx, y = r % d.ny, z = r / d.ny;
+  const float px = (float)x + vx[i] / sx, py = (float)y + vy[i] / sy, pz = (float)z + vz[i] / sz;
+  const float3 hn = trilerp3g<AWK_BG_PARTIAL_ID>(make_buf(hx, N), make_buf(hy, N), make_buf(hz, N), d,
+                                                 px, py, pz);
+  nx_[i] = hn.x;
+  ny_[i] = hn.y;
+  nz_[i] = hn.z;
+  jt[i] = trilerp1g<AWK_BG_VAL>(make_buf(I1, N), d, hn.x, hn.y, hn.z, 0.f);
+  const float det = jac_det_xsp_v(vx, vy, vz, i, x, y, z, d, sx, sy, sz);
+  jd[i] = jdn ? trilerp1g<AWK_BG_VAL>(make_buf(jdn, N), d, px, py, pz, 0.f) * det : det;
+}
+
+// Forward start: h = identity and m_0 = (I0 - JT_0) JacDet_0 * mscale.
+__global__ void __launch_bounds__(kLdmmBlock)
+k_orig_init(float *__restrict__ hx, float *__restrict__ hy, float *__restrict__ hz,
+            const float *__restrict__ I0, const float *__restrict__ jt, const float *__restrict__ jd,
+            float *__restrict__ m, float mscale, Dims d) {
+  const int N = d.nx * d.ny * d.nz;
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= N) return;
+  const int x = i % d.nx, r = i / d.nx;
+  hx[i] = (float)x;
+  hy[i] = (float)(r % d.ny);
+  hz[i] = (float)(r / d.ny);
+  m[i] = (I0[i] - jt[i]) * jd[i] * mscale;
+}
+
+// Forward step t after the update of v_t: h <- h(x - v_t/sp) (composeHVInv,
+// PARTIAL_ID), J = I0(h) (VAL 0) into J (may be null); with the next step's terms,
+// m = (J - JT_{t+1}) JacDet_{t+1} * mscale; at the last step diff = J - I1 into
+// FP64 SSD partials.
+__global__ void __launch_bounds__(kLdmmBlock)
+k_orig_fwd(float *__restrict__ nx_, float *__restrict__ ny_, float *__restrict__ nz_,
+           const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
+           const float *__restrict__ vx, const float *__restrict__ vy, const float *__restrict__ vz,
+           const float *__restrict__ I0, float *__restrict__ J, const float *__restrict__ jt,
+           const float *__restrict__ jd, float *__restrict__ m, float mscale,
+           const float *__restrict__ I1, double *__restrict__ partials, Dims d, float sx, float sy,
+           float sz) {
+  const int N = d.nx * d.ny * d.nz;
+  const int lb = (int)xcd_block();
+  const int i = lb * kLdmmBlock + (int)threadIdx.x;
+  double ssd = 0.0;
+  if (i < N) {
+    const int x = i % d.nx, r = i / d.nx, y = r % d.ny, z = r / d.ny;
+    const float px = (float)x - vx[i] / sx, py = (float)y - vy[i] / sy, pz = (float)z - vz[i] / sz;
+    const float3 hn = trilerp3g<AWK_BG_PARTIAL_ID>(make_buf(hx, N), make_buf(hy, N), make_buf(hz, N), d,
+                                                   px, py, pz);
+    nx_[i] = hn.x;
+    ny_[i] = hn.y;
+    nz_[i] = hn.z;
+    const float j = trilerp1g<AWK_BG_VAL>(make_buf(I0, N), d, hn.x, hn.y, hn.z, 0.f);
+    if (J) J[i] = j;
+    if (jt) m[i] = (j - jt[i]) * jd[i] * mscale;
+    if (I1) {
+      const float dv = j - I1[i];
+      ssd = (double)dv * (double)dv;
+    }
+  }
+  if (partials) block_sum_store(ssd, partials, lb);
+}
+
+__global__ void k_orig_energy_zero(double *__restrict__ energy) {
+  if (threadIdx.x < 3) energy[threadIdx.x] = 0.0;
+}
+
+// energy[0] = Prod(sp) sum(diff^2) / sigma^2 measured after the sweep
+// (LDMMIteratorGPU.cxx:430-441); energy[2] = energy[0] + energy[1].  Fixed order.
+__global__ void k_orig_image_energy(const double *__restrict__ partials, int n, double scale,
+                                    double *__restrict__ energy) {
+  __shared__ double s[256];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) a += partials[i];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    energy[0] = scale * s[0];
+    energy[2] = energy[0] + energy[1];
+  }
+}
+
 }  // namespace awk
 
 struct awk_ldmm {
@@ -191,7 +338,9 @@ struct awk_ldmm {
   bool update_step = true;
   float *h = nullptr, *hs = nullptr;  // 3 planes each
   float *J = nullptr;                  // nt images
-  float *diff = nullptr;
+  float *diff = nullptr;               // IterateOrig: the force multiplier m
+  bool orig = true;                    // mUseOrigGradientCalc (LDMMIteratorGPU.cxx:37)
+  float *jt = nullptr, *jd = nullptr;  // IterateOrig: JT_t, JacDet_t, nt images each
   long long *acc = nullptr;
   double *partials = nullptr;  // max(forward grid, dot grid) doubles
   float *maxpart = nullptr;    // nt * nblk
@@ -204,7 +353,8 @@ using namespace awk;
 
 static void ldmm_free(awk_ldmm *l) {
   fk_release(l->k);
-  void *bufs[] = {l->h, l->hs, l->J, l->diff, l->acc, l->partials, l->maxpart, l->step, l->energy};
+  void *bufs[] = {l->h,        l->hs,      l->J,    l->diff,   l->acc, l->partials,
+                  l->maxpart, l->step, l->energy, l->jt, l->jd};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
 }
@@ -254,6 +404,8 @@ awk_status awk_ldmm_create(awk_ldmm_t *out, int nx, int ny, int nz, float spx, f
   if (e == hipSuccess) e = hipMalloc(&l->maxpart, (size_t)l->nt * l->nblk * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&l->step, sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&l->energy, 3 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&l->jt, (size_t)l->nt * N * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&l->jd, (size_t)l->nt * N * sizeof(float));
   if (e == hipSuccess) e = hipMemset(l->acc, 0, N * sizeof(long long));
   if (e == hipSuccess) e = hipMemset(l->step, 0, sizeof(float));
   if (e != hipSuccess) {
@@ -270,6 +422,12 @@ awk_status awk_ldmm_destroy(awk_ldmm_t l) {
   if (!l) return AWK_OK;
   ldmm_free(l);
   delete l;
+  return AWK_OK;
+}
+
+awk_status awk_ldmm_set_gradient(awk_ldmm_t l, int use_orig) {
+  if (!l) return fail(AWK_ERR_INVALID, "awk_ldmm_set_gradient: null handle");
+  l->orig = use_orig != 0;
   return AWK_OK;
 }
 
@@ -290,6 +448,128 @@ awk_status awk_ldmm_get_step(awk_ldmm_t l, float *step) {
   if (!l || !step) return fail(AWK_ERR_INVALID, "awk_ldmm_get_step: null argument");
   AWK_HIP(hipDeviceSynchronize());
   AWK_HIP(hipMemcpy(step, l->step, sizeof(float), hipMemcpyDeviceToHost));
+  return AWK_OK;
+}
+
+// energy[1] += Prod(sp) <L v_t, v_t> / T, with L applied to a copy of v_t in lv (3N).
+static awk_status vec_energy_step(awk_ldmm *l, float *const v[], int t, float *lv, double *energy,
+                                  hipStream_t s) {
+  const int N = l->nx * l->ny * l->nz;
+  const double vol = (double)l->sp[0] * l->sp[1] * l->sp[2];
+  awk_status st;
+  int nb = 0;
+  for (int c = 0; c < 3; ++c)
+    AWK_HIP(hipMemcpyAsync(lv + (size_t)c * N, v[3 * t + c], (size_t)N * sizeof(float),
+                           hipMemcpyDeviceToDevice, s));
+  XFwdArgs pa;
+  std::memset(&pa, 0, sizeof(pa));
+  pa.mode = XLOAD_PLAIN;
+  for (int c = 0; c < 3; ++c) pa.in[c] = lv + (size_t)c * N;
+  if ((st = fk_xfwd(l->k, pa, s, &nb))) return st;
+  if ((st = fk_ypass(l->k, false, s, nullptr))) return st;
+  if ((st = fk_zsolve(l->k, false, s))) return st;
+  if ((st = fk_ypass(l->k, true, s, nullptr))) return st;
+  XInvArgs pi;
+  std::memset(&pi, 0, sizeof(pi));
+  pi.mode = XSTORE_PLAIN;
+  for (int c = 0; c < 3; ++c) pi.out[c] = lv + (size_t)c * N;
+  if ((st = fk_xinv(l->k, pi, s, &nb))) return st;
+  AWK_LAUNCH(k_ldmm_dot, dim3(l->egrid), dim3(kLdmmBlock), 0, s, lv, v[3 * t], v[3 * t + 1],
+             v[3 * t + 2], N, l->partials);
+  AWK_LAUNCH(k_ldmm_vec_energy, dim3(1), dim3(256), 0, s, l->partials, l->egrid, vol / l->nt,
+             energy);
+  AWK_LAUNCH_CHECK("vector energy");
+  return AWK_OK;
+}
+
+// u_t = L^-1 (grad(src) * m) and the update of v_t (or the step-size maximum),
+// the force and the update fused into the FFT x passes.
+static awk_status solve_update(awk_ldmm *l, float *const v[], int t, const float *src,
+                               const float *m, const long long *acc, bool step_call,
+                               hipStream_t s) {
+  awk_status st;
+  int nb = 0;
+  XFwdArgs fa;
+  std::memset(&fa, 0, sizeof(fa));
+  fa.mode = acc ? XLOAD_GRADMUL_FX : XLOAD_GRADMUL;
+  fa.idef = src;
+  fa.i1 = m;
+  fa.acc = const_cast<long long *>(acc);
+  for (int c = 0; c < 3; ++c) fa.sp[c] = l->sp[c];
+  if ((st = fk_xfwd(l->k, fa, s, &nb))) return st;
+  if ((st = fk_ypass(l->k, false, s, nullptr))) return st;
+  if ((st = fk_zsolve(l->k, true, s))) return st;
+  if ((st = fk_ypass(l->k, true, s, nullptr))) return st;
+  XInvArgs ia;
+  std::memset(&ia, 0, sizeof(ia));
+  ia.mode = step_call ? XSTORE_MAXAXPY : XSTORE_AXPBY;
+  for (int c = 0; c < 3; ++c) ia.out[c] = v[3 * t + c];
+  ia.maxpart = l->maxpart + (size_t)t * l->nblk;
+  ia.step = l->step;
+  ia.sigma2 = l->sigma * l->sigma;
+  return fk_xinv(l->k, ia, s, &nb);
+}
+
+// IterateOrig: backward sweep for JT_t and JacDet_t, then the forward sweep that
+// updates each v_t and composes h with the updated field.
+static awk_status ldmm_run_orig(awk_ldmm *l, const float *I0, const float *I1, float *const v[],
+                                float *const alpha[], double *energy_dev, int vec_energy,
+                                hipStream_t s) {
+  const int nx = l->nx, ny = l->ny, nz = l->nz, T = l->nt;
+  const int N = nx * ny * nz;
+  const Dims d{nx, ny, nz};
+  const float sx = l->sp[0], sy = l->sp[1], sz = l->sp[2];
+  const unsigned egrid = (unsigned)l->egrid;
+  const unsigned lgrid = (unsigned)((N + kLdmmBlock - 1) / kLdmmBlock);
+  const double vol = (double)sx * sy * sz;
+  const bool step_call = !alpha && l->update_step;
+  const float inv_sig2 = 1.f / (l->sigma * l->sigma);
+  double *energy = energy_dev ? energy_dev : l->energy;
+  awk_status st;
+  float *h = l->h, *hs = l->hs;
+  AWK_LAUNCH(k_ldmm_identity, dim3(lgrid), dim3(kLdmmBlock), 0, s, h, h + N, h + 2 * N, d);
+  for (int t = T - 1; t >= 0; --t) {
+    AWK_LAUNCH(k_orig_bwd, dim3(egrid), dim3(kLdmmBlock), 0, s, hs, hs + N, hs + 2 * N, h, h + N,
+               h + 2 * N, v[3 * t], v[3 * t + 1], v[3 * t + 2], I1,
+               t == T - 1 ? nullptr : l->jd + (size_t)(t + 1) * N, l->jt + (size_t)t * N,
+               l->jd + (size_t)t * N, d, sx, sy, sz);
+    AWK_LAUNCH_CHECK("k_orig_bwd");
+    std::swap(h, hs);
+  }
+  if (!alpha) AWK_LAUNCH(k_orig_energy_zero, dim3(1), dim3(64), 0, s, energy);
+  AWK_LAUNCH(k_orig_init, dim3(lgrid), dim3(kLdmmBlock), 0, s, h, h + N, h + 2 * N, I0, l->jt,
+             l->jd, alpha ? alpha[0] : l->diff, alpha ? inv_sig2 : 1.f, d);
+  AWK_LAUNCH_CHECK("k_orig_init");
+  for (int t = 0; t < T; ++t) {
+    const bool last = (t == T - 1);
+    if (!alpha &&
+        (st = solve_update(l, v, t, t > 0 ? l->J + (size_t)(t - 1) * N : I0, l->diff, nullptr,
+                           step_call, s)))
+      return st;
+    AWK_LAUNCH(k_orig_fwd, dim3(egrid), dim3(kLdmmBlock), 0, s, hs, hs + N, hs + 2 * N, h, h + N,
+               h + 2 * N, v[3 * t], v[3 * t + 1], v[3 * t + 2], I0,
+               alpha ? nullptr : l->J + (size_t)t * N, last ? nullptr : l->jt + (size_t)(t + 1) * N,
+               last ? nullptr : l->jd + (size_t)(t + 1) * N,
+               alpha ? (last ? nullptr : alpha[t + 1]) : l->diff, alpha ? inv_sig2 : 1.f,
+               (last && !alpha) ? I1 : nullptr, (last && !alpha) ? l->partials : nullptr, d, sx, sy,
+               sz);
+    AWK_LAUNCH_CHECK("k_orig_fwd");
+    std::swap(h, hs);
+    if (last && !alpha) {
+      // reduce the SSD partials before the vector energy reuses the buffer
+      AWK_LAUNCH(k_orig_image_energy, dim3(1), dim3(256), 0, s, l->partials, (int)egrid,
+                 vol * inv_sig2, energy);
+      AWK_LAUNCH_CHECK("k_orig_image_energy");
+    }
+    if (!alpha && !step_call && vec_energy && (st = vec_energy_step(l, v, t, hs, energy, s)))
+      return st;
+  }
+  if (alpha) return AWK_OK;
+  if (step_call) {
+    AWK_LAUNCH(k_ldmm_step, dim3(1), dim3(256), 0, s, l->maxpart, T * l->nblk, l->max_pert, l->step);
+    AWK_LAUNCH_CHECK("k_ldmm_step");
+    l->update_step = false;
+  }
   return AWK_OK;
 }
 
@@ -323,7 +603,6 @@ static awk_status ldmm_run(awk_ldmm *l, const float *I0, const float *I1, float 
   // ---- backward: h_t = h_{t+1} - v_t(h_{t+1}) / sp; alpha_t = splat(diff, h_t)
   AWK_LAUNCH(k_ldmm_zero, dim3(lgrid), dim3(kLdmmBlock), 0, s, l->acc, N);
   AWK_LAUNCH(k_ldmm_identity, dim3(lgrid), dim3(kLdmmBlock), 0, s, h, h + N, h + 2 * N, d);
-  int nb = 0;
   for (int t = T - 1; t >= 0; --t) {
     AWK_LAUNCH(k_ldmm_bwd, dim3(splat_tile_grid(nx, ny, nz)), dim3(kSplatTileThreads), 0, s, hs, hs + N, hs + 2 * N, h, h + N,
                h + 2 * N, v[3 * t], v[3 * t + 1], v[3 * t + 2], l->diff,
@@ -337,51 +616,11 @@ static awk_status ldmm_run(awk_ldmm *l, const float *I0, const float *I1, float 
       continue;
     }
     // u = L^-1 (grad(J_{t-1}) * alpha_t), grad(I0) at t = 0
-    XFwdArgs fa;
-    std::memset(&fa, 0, sizeof(fa));
-    fa.mode = XLOAD_GRADMUL_FX;
-    fa.idef = t > 0 ? l->J + (size_t)(t - 1) * N : I0;
-    fa.acc = l->acc;
-    fa.sp[0] = sx;
-    fa.sp[1] = sy;
-    fa.sp[2] = sz;
-    if ((st = fk_xfwd(l->k, fa, s, &nb))) return st;
-    if ((st = fk_ypass(l->k, false, s, nullptr))) return st;
-    if ((st = fk_zsolve(l->k, true, s))) return st;
-    if ((st = fk_ypass(l->k, true, s, nullptr))) return st;
-    XInvArgs ia;
-    std::memset(&ia, 0, sizeof(ia));
-    ia.mode = step_call ? XSTORE_MAXAXPY : XSTORE_AXPBY;
-    for (int c = 0; c < 3; ++c) ia.out[c] = v[3 * t + c];
-    ia.maxpart = l->maxpart + (size_t)t * l->nblk;
-    ia.step = l->step;
-    ia.sigma2 = l->sigma * l->sigma;
-    if ((st = fk_xinv(l->k, ia, s, &nb))) return st;
-    if (!step_call && vec_energy) {
-      // <L v_t, v_t> with L applied to a copy of v_t in the free hfield buffer
-      float *lv = hs;
-      for (int c = 0; c < 3; ++c)
-        AWK_HIP(hipMemcpyAsync(lv + (size_t)c * N, v[3 * t + c], (size_t)N * sizeof(float),
-                               hipMemcpyDeviceToDevice, s));
-      XFwdArgs pa;
-      std::memset(&pa, 0, sizeof(pa));
-      pa.mode = XLOAD_PLAIN;
-      for (int c = 0; c < 3; ++c) pa.in[c] = lv + (size_t)c * N;
-      if ((st = fk_xfwd(l->k, pa, s, &nb))) return st;
-      if ((st = fk_ypass(l->k, false, s, nullptr))) return st;
-      if ((st = fk_zsolve(l->k, false, s))) return st;
-      if ((st = fk_ypass(l->k, true, s, nullptr))) return st;
-      XInvArgs pi;
-      std::memset(&pi, 0, sizeof(pi));
-      pi.mode = XSTORE_PLAIN;
-      for (int c = 0; c < 3; ++c) pi.out[c] = lv + (size_t)c * N;
-      if ((st = fk_xinv(l->k, pi, s, &nb))) return st;
-      AWK_LAUNCH(k_ldmm_dot, dim3(egrid), dim3(kLdmmBlock), 0, s, lv, v[3 * t], v[3 * t + 1],
-                 v[3 * t + 2], N, l->partials);
-      AWK_LAUNCH(k_ldmm_vec_energy, dim3(1), dim3(256), 0, s, l->partials, (int)egrid, vol / T,
-                 energy);
-      AWK_LAUNCH_CHECK("vector energy");
-    }
+    if ((st = solve_update(l, v, t, t > 0 ? l->J + (size_t)(t - 1) * N : I0, nullptr, l->acc,
+                           step_call, s)))
+      return st;
+    // <L v_t, v_t> with L applied to a copy of v_t in the free hfield buffer
+    if (!step_call && vec_energy && (st = vec_energy_step(l, v, t, hs, energy, s))) return st;
   }
   if (step_call) {
     AWK_LAUNCH(k_ldmm_step, dim3(1), dim3(256), 0, s, l->maxpart, T * l->nblk, l->max_pert, l->step);
@@ -396,6 +635,7 @@ awk_status awk_ldmm_iterate(awk_ldmm_t l, const float *I0, const float *I1, floa
   if (!l || !I0 || !I1 || !v) return fail(AWK_ERR_INVALID, "awk_ldmm_iterate: null argument");
   for (int i = 0; i < 3 * l->nt; ++i)
     if (!v[i]) return fail(AWK_ERR_INVALID, "awk_ldmm_iterate: null velocity plane");
+  if (l->orig) return ldmm_run_orig(l, I0, I1, v, nullptr, energy_dev, vec_energy, as_stream(stream));
   return ldmm_run(l, I0, I1, v, nullptr, energy_dev, vec_energy, as_stream(stream));
 }
 
@@ -406,6 +646,7 @@ awk_status awk_ldmm_alphas(awk_ldmm_t l, const float *I0, const float *I1, float
     if (!v[i]) return fail(AWK_ERR_INVALID, "awk_ldmm_alphas: null velocity plane");
   for (int t = 0; t < l->nt; ++t)
     if (!alpha[t]) return fail(AWK_ERR_INVALID, "awk_ldmm_alphas: null alpha image");
+  if (l->orig) return ldmm_run_orig(l, I0, I1, v, alpha, nullptr, 0, as_stream(stream));
   return ldmm_run(l, I0, I1, v, alpha, nullptr, 0, as_stream(stream));
 }
 

@@ -1,11 +1,14 @@
-"""LDMM registration (adjoint-gradient ``IterateNew``) on the GPU.
+"""LDMM registration on the GPU.
 
-Mirrors ``LDMMIteratorGPU`` (Algorithms/LDMM/LDMMIteratorGPU.cxx:159-293), the
+Mirrors ``LDMMIteratorGPU`` (Algorithms/LDMM/LDMMIteratorGPU.cxx), the
 per-registration state of ``LDMMDeformationDataGPU`` (T velocity fields, I0, I1)
 and the single-scale loop of ``LDMMWarp::RunWarp`` (Algorithms/LDMM/LDMMWarp.txx:
-69-125).  Each ``Iterate`` is one ``awk_ldmm_iterate`` call: 2T fused gather
-kernels, T Navier solves with the force and the velocity update fused into the
-FFT x passes, no host synchronisation.
+69-125).  The gradient is ``IterateOrig`` (LDMMIteratorGPU.cxx:295-441, the
+Jacobian-determinant form, which the reference's constructor selects) unless
+``use_orig_gradient=False`` picks ``IterateNew`` (LDMMIteratorGPU.cxx:159-293,
+the adjoint splat).  Each ``Iterate`` is one ``awk_ldmm_iterate`` call: 2T fused
+gather kernels, T Navier solves with the force and the velocity update fused
+into the FFT x passes, no host synchronisation.
 """
 import ctypes
 
@@ -19,7 +22,8 @@ class LDMMIteratorParam:
     """LDMMIteratorParam (LDMMIteratorParam.h) + DiffOperParam."""
 
     def __init__(self, sigma=0.05, max_pert=0.1, alpha=0.01, beta=0.01, gamma=0.001,
-                 use_adaptive_step_size=True, step_size=0.005):
+                 use_adaptive_step_size=True, step_size=0.005, use_orig_gradient=True):
+        self.UseOrigGradientCalc = use_orig_gradient  # LDMMIteratorGPU.cxx:37
         self.Sigma = sigma
         self.MaxPert = max_pert
         self.alpha, self.beta, self.gamma = alpha, beta, gamma
@@ -79,6 +83,7 @@ class LDMMIteratorGPU:
         _lib.call("awk_ldmm_create", ctypes.byref(h), *self.size, *self.spacing, p.alpha, p.beta,
                   p.gamma, self.n_time_steps, p.Sigma, p.MaxPert)
         self._h = h
+        _lib.call("awk_ldmm_set_gradient", self._h, int(bool(p.UseOrigGradientCalc)))
         if not p.UseAdaptiveStepSize:
             _lib.call("awk_ldmm_set_step", self._h, float(p.StepSize))
 
@@ -105,7 +110,7 @@ class LDMMIteratorGPU:
             data.cur_iter += 1
 
     def ComputeAlphas(self, data, stream=None):
-        """IterateNew(computeAlphasOnly=true): [T, nz, ny, nx] alpha images."""
+        """Iterate(computeAlphasOnly=true): [T, nz, ny, nx] alpha images."""
         self._check(data)
         alpha = torch.empty((self.n_time_steps,) + tuple(data.I0.shape), dtype=torch.float32,
                             device=data.I0.device)

@@ -65,8 +65,9 @@ def _events_ms(fn, reps):
     return a.elapsed_time(b) / reps
 
 
-def bench_ldmm(dev, n=128, T=10, subjects=8, reps=5):
-    """IterateNew at n^3 with T velocity fields; subjects run back to back."""
+def bench_ldmm(dev, n=128, T=10, subjects=8, reps=5, orig=True):
+    """One LDMM iteration at n^3 with T velocity fields (IterateOrig, the
+    reference's default gradient, or IterateNew); subjects run back to back."""
     import torch
     from atlaswerks_amd import hfield, ldmm, synth
     shape = (n, n, n)
@@ -76,7 +77,8 @@ def bench_ldmm(dev, n=128, T=10, subjects=8, reps=5):
         u = torch.from_numpy(synth.identity(shape) + synth.sinusoidal_displacement(shape, 3.0, seed=300 + s)).to(dev)
         tgt = torch.empty_like(t)
         hfield.apply(tgt, t, u)
-        w = ldmm.LDMMWarp(t, tgt, n_time_steps=T, n_iter=0, param=ldmm.LDMMIteratorParam(sigma=0.05))
+        prm = ldmm.LDMMIteratorParam(sigma=0.05, use_orig_gradient=orig)
+        w = ldmm.LDMMWarp(t, tgt, n_time_steps=T, n_iter=0, param=prm)
         w.RunWarp()  # step-size call
         warps.append(w)
 
@@ -85,7 +87,8 @@ def bench_ldmm(dev, n=128, T=10, subjects=8, reps=5):
             w.iterator.Iterate(w.data, record_energy=False)
     ms = _events_ms(one, reps)
     N = n ** 3
-    return {"ms_per_iteration_all_subjects": round(ms, 3), "subjects": subjects, "T": T,
+    return {"gradient": "IterateOrig" if orig else "IterateNew",
+            "ms_per_iteration_all_subjects": round(ms, 3), "subjects": subjects, "T": T,
             "subject_iter_per_s": round(1000.0 * subjects / ms, 1),
             "ms_per_subject_iteration": round(ms / subjects, 3),
             "size": [n, n, n], "N": N}
@@ -254,6 +257,8 @@ def main():
     ap.add_argument("--no-microbench", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--only", choices=("ldmm", "ldmm_new", "atlas", "unet", "warm_start"),
+                    help="run one secondary workload alone (for profiling) and print its JSON")
     ap.add_argument("--copy-ref", action="store_true",
                     help="also run a plain 1 GiB device copy (counter scaling reference)")
     args = ap.parse_args()
@@ -261,6 +266,16 @@ def main():
         print("bench.py: this build runs on one GPU only (--gpus %d requested); the "
               "multi-GPU design is subject sharding, see DESIGN.md" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.only:
+        import torch
+        dev = torch.device("cuda:0")
+        fn = {"ldmm": lambda: bench_ldmm(dev, subjects=2, reps=3),
+              "ldmm_new": lambda: bench_ldmm(dev, subjects=2, reps=3, orig=False),
+              "atlas": lambda: bench_atlas(dev, reps=3), "unet": lambda: bench_unet(dev),
+              "warm_start": lambda: bench_warm_start(dev)}[args.only]
+        print(json.dumps({args.only: fn()}), flush=True)
+        return
 
     import numpy as np
     import torch
@@ -414,6 +429,7 @@ def main():
     #      greedy atlas over uneven grids (template 128^3); not part of the headline
     if not args.no_secondary:
         out["ldmm_128_T10"] = bench_ldmm(dev)
+        out["ldmm_new_128_T10"] = bench_ldmm(dev, orig=False)
         out["atlas_16_uneven"] = bench_atlas(dev)
         out["unet_128"] = bench_unet(dev)
         out["warm_start_128"] = bench_warm_start(dev)

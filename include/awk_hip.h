@@ -317,40 +317,50 @@ awk_status awk_atlas_mean(awk_atlas_t a, float *mean_out, awk_stream_t stream);
 awk_status awk_atlas_deformed_image(awk_atlas_t a, int s, float *out, awk_stream_t stream);
 
 /* ====================================================================== */
-/* LDMM (adjoint-gradient iteration, IterateNew)                            */
+/* LDMM (IterateOrig, the reference's default, and IterateNew)              */
 /* ====================================================================== */
 
 typedef struct awk_ldmm *awk_ldmm_t;
 
-/* LDMMIteratorGPU (Algorithms/LDMM/LDMMIteratorGPU.h, LDMMIteratorGPU.cxx:159-293)
+/* LDMMIteratorGPU (Algorithms/LDMM/LDMMIteratorGPU.h, LDMMIteratorGPU.cxx:26-60)
  * for one nx*ny*nz grid with n_time_steps velocity fields, the Navier operator
  * (alpha, beta, gamma), sigma and MaxPert.  The first iterate computes the step
  * size (UseAdaptiveStepSize, LDMMWarp.txx:97-103) unless awk_ldmm_set_step is
- * called first.  At most 2^28 voxels. */
+ * called first.  The gradient is IterateOrig's, as the reference's constructor
+ * sets mUseOrigGradientCalc (LDMMIteratorGPU.cxx:37); awk_ldmm_set_gradient
+ * switches.  At most 2^28 voxels. */
 awk_status awk_ldmm_create(awk_ldmm_t *out, int nx, int ny, int nz, float spx, float spy,
                            float spz, float alpha, float beta, float gamma, int n_time_steps,
                            float sigma, float max_pert);
 awk_status awk_ldmm_destroy(awk_ldmm_t l);
+/* mUseOrigGradientCalc (LDMMIteratorGPU.cxx:37, :151-155): 1 = IterateOrig
+ * (Jacobian-determinant gradient, the default), 0 = IterateNew (adjoint splat). */
+awk_status awk_ldmm_set_gradient(awk_ldmm_t l, int use_orig);
 /* LDMMIteratorGPU::UpdateStepSizeNextIteration: the next iterate only sets the step. */
 awk_status awk_ldmm_update_step_next(awk_ldmm_t l);
 awk_status awk_ldmm_set_step(awk_ldmm_t l, float step);
 awk_status awk_ldmm_get_step(awk_ldmm_t l, float *step); /* synchronises */
 
-/* One IterateNew(computeAlphasOnly = false) (LDMMIteratorCPU.cxx:529-631).
+/* One Iterate(computeAlphasOnly = false) (LDMMIteratorGPU.cxx:147-156).
  * v: 3 * n_time_steps device planes, v[3t + c] = component c of v_t, updated in
- * place (v_t = (1 - 2 eps) v_t + (2 eps / sigma^2) L^-1(grad(J_{t-1}) alpha_t)).
+ * place: v_t = (1 - 2 eps) v_t + (2 eps / sigma^2) L^-1(f_t), with
+ *   IterateOrig (LDMMIteratorGPU.cxx:295-441): f_t = grad(J_t) (J_t - JT_t) JacDet_t,
+ *     J_t = I0 o h_t through the already updated v_0..v_{t-1};
+ *   IterateNew (LDMMIteratorGPU.cxx:159-293): f_t = grad(J_{t-1}) alpha_t.
  * energy_dev (3 device doubles, may be NULL): image energy Prod(sp) sum(diff^2) /
- * sigma^2 before the update, vector energy mean_t Prod(sp) <L v_t, v_t> after it
- * (only when vec_energy != 0: one extra operator application per time step),
- * and their sum (LDMMEnergy.cxx:48-64).  FP64, fixed summation order.  No host
- * synchronisation. */
+ * sigma^2 (IterateOrig: after the update; IterateNew: before it), vector energy
+ * mean_t Prod(sp) <L v_t, v_t> after the update (only when vec_energy != 0: one
+ * extra operator application per time step), and their sum (LDMMEnergy.cxx:48-64).
+ * FP64, fixed summation order.  No host synchronisation. */
 awk_status awk_ldmm_iterate(awk_ldmm_t l, const float *I0, const float *I1, float *const v[],
                             double *energy_dev, int vec_energy, awk_stream_t stream);
-/* IterateNew(computeAlphasOnly = true): alpha[t] = splat(J_{T-1} - I1, h_t) / sigma^2
- * (LDMMIteratorGPU.cxx:197-204) for every t; v is not changed. */
+/* Iterate(computeAlphasOnly = true); v is not changed.  IterateOrig:
+ * alpha[t] = (J_t - JT_t) JacDet_t / sigma^2 (LDMMIteratorGPU.cxx:366-371);
+ * IterateNew: alpha[t] = splat(J_{T-1} - I1, h_t) / sigma^2 (LDMMIteratorGPU.cxx:197-204). */
 awk_status awk_ldmm_alphas(awk_ldmm_t l, const float *I0, const float *I1, float *const v[],
                            float *const alpha[], awk_stream_t stream);
-/* Copies J_t = I0 o h_t of the last iterate (the reference's mdJ0t[t]). */
+/* Copies J_t = I0 o h_t of the last iterate, h_t composed through v_0..v_t (the
+ * reference's mdJ0t[t]). */
 awk_status awk_ldmm_deformed_image(awk_ldmm_t l, int t, float *out, awk_stream_t stream);
 
 /* ====================================================================== */

@@ -288,6 +288,29 @@ def ldmm_run(I0, I1, n_t, n_iter, v=None, step=0.0, sp=(1, 1, 1), alpha=0.01, be
     return v, ei, ev, st.value
 
 
+def ldmm_run_orig(I0, I1, n_t, n_iter, v=None, step=0.0, sp=(1, 1, 1), alpha=0.01, beta=0.01,
+                  gamma=0.001, sigma=0.05, max_pert=0.1):
+    """LDMMWarp single scale with IterateOrig (FP64).  Returns (v, e_img, e_vec, step)."""
+    I0, I1 = _f32(I0), _f32(I1)
+    p = _ldmm_params(I0.shape, n_t, n_iter, sp, alpha, beta, gamma, sigma, max_pert)
+    v = (np.zeros((n_t,) + I0.shape + (3,)) if v is None
+         else np.ascontiguousarray(v, dtype=np.float64).copy())
+    ei = np.zeros(n_iter)
+    ev = np.zeros(n_iter)
+    st = ctypes.c_double(step)
+    lib().orc_ldmm_run_orig_d(ctypes.byref(p), _p(I0), _p(I1), _p(v), _p(ei), _p(ev),
+                              ctypes.byref(st))
+    return v, ei, ev, st.value
+
+
+def jacobian_det(hw, sp=(1, 1, 1)):
+    """det of the Jacobian of an AoS field [z, y, x, 3] in world units."""
+    hw = np.ascontiguousarray(hw, dtype=np.float64)
+    out = np.zeros(hw.shape[:3])
+    lib().orc_jacobian_det_d(_p(hw), _size(hw.shape[:3]), _sp(sp), _p(out))
+    return out
+
+
 def ldmm_alphas(I0, I1, v, sp=(1, 1, 1), sigma=0.05):
     """IterateNew(computeAlphasOnly): alpha_t images [T, z, y, x]."""
     I0, I1 = _f32(I0), _f32(I1)
@@ -297,6 +320,29 @@ def ldmm_alphas(I0, I1, v, sp=(1, 1, 1), sigma=0.05):
     a = np.zeros((n_t,) + I0.shape, np.float32)
     lib().orc_ldmm_alphas_d(ctypes.byref(p), _p(I0), _p(I1), _p(v), _p(a))
     return a
+
+
+def ldmm_alphas_orig(I0, I1, v, sp=(1, 1, 1), sigma=0.05):
+    """IterateOrig(computeAlphasOnly): alpha_t images [T, z, y, x]."""
+    I0, I1 = _f32(I0), _f32(I1)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    n_t = v.shape[0]
+    p = _ldmm_params(I0.shape, n_t, 1, sp, 0.01, 0.01, 0.001, sigma, 0.1)
+    a = np.zeros((n_t,) + I0.shape, np.float32)
+    lib().orc_ldmm_alphas_orig_d(ctypes.byref(p), _p(I0), _p(I1), _p(v), _p(a))
+    return a
+
+
+def ldmm_orig_terms(I1, v, sp=(1, 1, 1)):
+    """IterateOrig's backward sweep: (JT, JacDet), each [T, z, y, x]."""
+    I1 = _f32(I1)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    n_t = v.shape[0]
+    p = _ldmm_params(I1.shape, n_t, 1, sp, 0.01, 0.01, 0.001, 0.05, 0.1)
+    JT = np.zeros((n_t,) + I1.shape)
+    JD = np.zeros((n_t,) + I1.shape)
+    lib().orc_ldmm_orig_terms_d(ctypes.byref(p), _p(I1), _p(v), _p(JT), _p(JD))
+    return JT, JD
 
 
 def gaussian_filter(img, sigma, ksize):

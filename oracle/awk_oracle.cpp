@@ -906,6 +906,188 @@ void ldmm_iterate(const orc_ldmm_params *p, const float *I0, const float *I1, T 
   if (e_vec) *e_vec = update_step ? 0.0 : vec / nt;
 }
 
+/* HField3DUtils::jacobian (HField3DUtils.txx:2370-2420): det of the gradients
+ * (Array3DUtils::computeGradient, wrap = false) of the three components of an AoS
+ * field given in world units, with spacing sp. */
+template <class T>
+void jacobian_det(const T *hw, const int *size, const double *sp, T *det) {
+  const size_t N = (size_t)size[0] * size[1] * size[2];
+  std::vector<T> c(N), g1(3 * N), g2(3 * N), g3(3 * N);
+  for (int k = 0; k < 3; ++k) {
+    for (size_t i = 0; i < N; ++i) c[i] = hw[3 * i + k];
+    gradient_t<T>(c.data(), size, sp, 0, k == 0 ? g1.data() : k == 1 ? g2.data() : g3.data());
+  }
+  for (size_t i = 0; i < N; ++i) {
+    const T *a = &g1[3 * i], *b = &g2[3 * i], *e = &g3[3 * i];
+    const T t1 = a[0] * (b[1] * e[2] - b[2] * e[1]);
+    const T t2 = a[1] * (b[0] * e[2] - b[2] * e[0]);
+    const T t3 = a[2] * (b[0] * e[1] - b[1] * e[0]);
+    det[i] = t1 - t2 + t3;
+  }
+}
+
+/* Backward sweep of LDMMIteratorCPU::IterateOrig (LDMMIteratorCPU.cxx:391-428),
+ * t = T-1..0: h <- composeHV(h, v_t) (PARTIAL_ID), JT_t = I1(h) (VAL 0),
+ * |D(x sp + v_t)| and JacDet_t = applyU(JacDet_{t+1}, v_t) |D(x sp + v_t)|
+ * (HField3DUtils::applyU, HField3DUtils.txx:1029-1053: the image at x + v/sp,
+ * VAL 0), JacDet_{T-1} = the determinant. */
+template <class T>
+void ldmm_orig_backward(const orc_ldmm_params *p, const float *I1, const T *v, T *JT, T *JD) {
+  const int *size = p->size;
+  const int sx = size[0], sy = size[1], sz = size[2];
+  const size_t N = (size_t)sx * sy * sz;
+  const int nt = p->n_time_steps;
+  std::vector<T> h(3 * N), hs(3 * N), det(N), wv(3 * N);
+  size_t i = 0;
+  for (int z = 0; z < sz; ++z)
+    for (int y = 0; y < sy; ++y)
+      for (int x = 0; x < sx; ++x, ++i) {
+        h[3 * i] = (T)x; h[3 * i + 1] = (T)y; h[3 * i + 2] = (T)z;
+      }
+  for (int t = nt - 1; t >= 0; --t) {
+    const T *vt = v + (size_t)t * 3 * N;
+    i = 0;
+    for (int z = 0; z < sz; ++z)
+      for (int y = 0; y < sy; ++y)
+        for (int x = 0; x < sx; ++x, ++i) {
+          double o[3];
+          trilerp_vec<T>(h.data(), size, x + (double)vt[3 * i] / p->sp[0],
+                         y + (double)vt[3 * i + 1] / p->sp[1], z + (double)vt[3 * i + 2] / p->sp[2],
+                         o, ORC_BG_PARTIAL_ID);
+          hs[3 * i] = (T)o[0]; hs[3 * i + 1] = (T)o[1]; hs[3 * i + 2] = (T)o[2];
+          wv[3 * i] = (T)(x * p->sp[0]) + vt[3 * i];
+          wv[3 * i + 1] = (T)(y * p->sp[1]) + vt[3 * i + 1];
+          wv[3 * i + 2] = (T)(z * p->sp[2]) + vt[3 * i + 2];
+        }
+    std::swap(h, hs);
+    T *JTt = JT + (size_t)t * N;
+    for (size_t j = 0; j < N; ++j)
+      JTt[j] = (T)trilerp_scalar<float>(I1, size, (double)h[3 * j], (double)h[3 * j + 1],
+                                        (double)h[3 * j + 2], ORC_BG_VAL, 0.0);
+    jacobian_det<T>(wv.data(), size, p->sp, det.data());
+    T *JDt = JD + (size_t)t * N;
+    if (t == nt - 1) {
+      for (size_t j = 0; j < N; ++j) JDt[j] = det[j];
+    } else {
+      const T *JDn = JD + (size_t)(t + 1) * N;
+      size_t q = 0;
+      for (int z = 0; z < sz; ++z)
+        for (int y = 0; y < sy; ++y)
+          for (int x = 0; x < sx; ++x, ++q)
+            JDt[q] = (T)(trilerp_scalar<T>(JDn, size, x + (double)vt[3 * q] / p->sp[0],
+                                           y + (double)vt[3 * q + 1] / p->sp[1],
+                                           z + (double)vt[3 * q + 2] / p->sp[2], ORC_BG_VAL, 0.0) *
+                         det[q]);
+    }
+  }
+}
+
+/* LDMMIteratorCPU::IterateOrig (LDMMIteratorCPU.cxx:379-527), the
+ * Jacobian-determinant gradient (the reference's default: mUseOrigGradientCalc is
+ * set true in both iterators' constructors).  After the backward sweep, the
+ * forward sweep t = 0..T-1: J = I0(h) (I0 at t = 0), m = (J - JT_t) JacDet_t;
+ * alphas-only: alpha_t = m / sigma^2; else u = L^-1 (grad(J) m) and the update of
+ * v_t (or the step-size maximum); then h <- composeHVInv(h, v_t) with the
+ * updated v_t.  Image energy after the sweep: Prod(sp) sum(I0(h) - I1)^2 /
+ * sigma^2 (LDMMIteratorGPU.cxx:430-441, LDMMEnergy.h). */
+template <class T>
+void ldmm_iterate_orig(const orc_ldmm_params *p, const float *I0, const float *I1, T *v,
+                       bool update_step, double &step, double *e_img, double *e_vec,
+                       float *alpha) {
+  const int *size = p->size;
+  const int sx = size[0], sy = size[1], sz = size[2];
+  const size_t N = (size_t)sx * sy * sz;
+  const int nt = p->n_time_steps;
+  const double sig2 = p->sigma * p->sigma;
+  const double vol = p->sp[0] * p->sp[1] * p->sp[2];
+  std::vector<T> h(3 * N), hs(3 * N), JT((size_t)nt * N), JD((size_t)nt * N), f(3 * N), J(N), lv;
+  ldmm_orig_backward<T>(p, I1, v, JT.data(), JD.data());
+  size_t i = 0;
+  for (int z = 0; z < sz; ++z)
+    for (int y = 0; y < sy; ++y)
+      for (int x = 0; x < sx; ++x, ++i) {
+        h[3 * i] = (T)x; h[3 * i + 1] = (T)y; h[3 * i + 2] = (T)z;
+      }
+  double maxdisp = 0, vec = 0;
+  for (int t = 0; t < nt; ++t) {
+    T *vt = v + (size_t)t * 3 * N;
+    if (t == 0) {
+      for (size_t j = 0; j < N; ++j) J[j] = (T)I0[j];
+    } else {
+      for (size_t j = 0; j < N; ++j)
+        J[j] = (T)trilerp_scalar<float>(I0, size, (double)h[3 * j], (double)h[3 * j + 1],
+                                        (double)h[3 * j + 2], ORC_BG_VAL, 0.0);
+    }
+    const T *JTt = JT.data() + (size_t)t * N, *JDt = JD.data() + (size_t)t * N;
+    if (alpha) {
+      for (size_t j = 0; j < N; ++j)
+        alpha[(size_t)t * N + j] = (float)((J[j] - JTt[j]) * JDt[j] / sig2);
+    } else {
+      gradient_t<T>(J.data(), size, p->sp, 0, f.data());
+      for (size_t j = 0; j < N; ++j) {
+        const T a = (T)((J[j] - JTt[j]) * JDt[j]);
+        for (int c = 0; c < 3; ++c) f[3 * j + c] = (T)(f[3 * j + c] * a);
+      }
+      diffoper<T>(size, p->sp, p->alpha, p->beta, p->gamma, 1.0, 0, f.data(), true);
+      if (update_step) {
+        for (size_t j = 0; j < N; ++j) {
+          double l2 = 0;
+          for (int c = 0; c < 3; ++c) {
+            const double q = 2.0 * (double)vt[3 * j + c] - (2.0 / sig2) * (double)f[3 * j + c];
+            l2 += q * q;
+          }
+          maxdisp = std::max(maxdisp, std::sqrt(l2));
+        }
+      } else {
+        const T a = (T)(1.0 - 2.0 * step), b = (T)(2.0 * step / sig2);
+        for (size_t j = 0; j < 3 * N; ++j) vt[j] = (T)(a * vt[j] + b * f[j]);
+        if (e_vec) {
+          lv.assign(vt, vt + 3 * N);
+          diffoper<T>(size, p->sp, p->alpha, p->beta, p->gamma, 1.0, 0, lv.data(), false);
+          double d = 0;
+          for (size_t j = 0; j < 3 * N; ++j) d += (double)lv[j] * (double)vt[j];
+          vec += vol * d;
+        }
+      }
+    }
+    i = 0;
+    for (int z = 0; z < sz; ++z)
+      for (int y = 0; y < sy; ++y)
+        for (int x = 0; x < sx; ++x, ++i) {
+          double o[3];
+          trilerp_vec<T>(h.data(), size, x - (double)vt[3 * i] / p->sp[0],
+                         y - (double)vt[3 * i + 1] / p->sp[1], z - (double)vt[3 * i + 2] / p->sp[2],
+                         o, ORC_BG_PARTIAL_ID);
+          hs[3 * i] = (T)o[0]; hs[3 * i + 1] = (T)o[1]; hs[3 * i + 2] = (T)o[2];
+        }
+    std::swap(h, hs);
+  }
+  if (alpha) return;
+  if (update_step) step = p->max_pert / maxdisp;
+  if (e_img) {
+    double ssd = 0;
+    for (size_t j = 0; j < N; ++j) {
+      const double d = trilerp_scalar<float>(I0, size, (double)h[3 * j], (double)h[3 * j + 1],
+                                             (double)h[3 * j + 2], ORC_BG_VAL, 0.0) - (double)I1[j];
+      ssd += d * d;
+    }
+    *e_img = vol * ssd / sig2;
+  }
+  if (e_vec) *e_vec = update_step ? 0.0 : vec / nt;
+}
+
+template <class T>
+int ldmm_run_orig(const orc_ldmm_params *p, const float *I0, const float *I1, T *v, double *e_img,
+                  double *e_vec, double *step_out) {
+  double step = step_out ? *step_out : 0.0;
+  if (step <= 0.0) ldmm_iterate_orig<T>(p, I0, I1, v, true, step, nullptr, nullptr, nullptr);
+  for (int it = 0; it < p->n_iter; ++it)
+    ldmm_iterate_orig<T>(p, I0, I1, v, false, step, e_img ? e_img + it : nullptr,
+                         e_vec ? e_vec + it : nullptr, nullptr);
+  if (step_out) *step_out = step;
+  return 0;
+}
+
 template <class T>
 int ldmm_run(const orc_ldmm_params *p, const float *I0, const float *I1, T *v, double *e_img,
              double *e_vec, double *step_out) {
@@ -1232,6 +1414,28 @@ int orc_atlas_run_d(const orc_atlas_params *p, const float *const *I0, const int
 int orc_ldmm_run_d(const orc_ldmm_params *p, const float *I0, const float *I1, double *v,
                    double *e_img, double *e_vec, double *step) {
   return ldmm_run<double>(p, I0, I1, v, e_img, e_vec, step);
+}
+
+int orc_ldmm_run_orig_d(const orc_ldmm_params *p, const float *I0, const float *I1, double *v,
+                        double *e_img, double *e_vec, double *step) {
+  return ldmm_run_orig<double>(p, I0, I1, v, e_img, e_vec, step);
+}
+
+void orc_ldmm_orig_terms_d(const orc_ldmm_params *p, const float *I1, const double *v, double *JT,
+                           double *JD) {
+  ldmm_orig_backward<double>(p, I1, v, JT, JD);
+}
+
+void orc_ldmm_alphas_orig_d(const orc_ldmm_params *p, const float *I0, const float *I1,
+                            const double *v, float *alpha) {
+  const size_t n = (size_t)p->n_time_steps * 3 * p->size[0] * p->size[1] * p->size[2];
+  std::vector<double> vc(v, v + n);
+  double step = 0.0;
+  ldmm_iterate_orig<double>(p, I0, I1, vc.data(), false, step, nullptr, nullptr, alpha);
+}
+
+void orc_jacobian_det_d(const double *hw, const int *size, const double *sp, double *det) {
+  jacobian_det<double>(hw, size, sp, det);
 }
 
 int orc_ldmm_run_f(const orc_ldmm_params *p, const float *I0, const float *I1, float *v,

@@ -171,6 +171,20 @@ int orc_ldmm_run_d(const orc_ldmm_params *p, const float *I0, const float *I1, d
                    double *e_img, double *e_vec, double *step);
 int orc_ldmm_run_f(const orc_ldmm_params *p, const float *I0, const float *I1, float *v,
                    double *e_img, double *e_vec, double *step);
+/* The same loop with LDMMIteratorCPU::IterateOrig (LDMMIteratorCPU.cxx:379-527),
+ * the Jacobian-determinant gradient; the image energy is measured after the
+ * update (IterateOrig's order). */
+int orc_ldmm_run_orig_d(const orc_ldmm_params *p, const float *I0, const float *I1, double *v,
+                        double *e_img, double *e_vec, double *step);
+/* IterateOrig's backward sweep: JT_t = I1(h_t) and JacDet_t, [T][nz][ny][nx] each. */
+void orc_ldmm_orig_terms_d(const orc_ldmm_params *p, const float *I1, const double *v, double *JT,
+                           double *JD);
+/* IterateOrig(computeAlphasOnly = true): alpha_t = (I0(h_t) - JT_t) JacDet_t / sigma^2. */
+void orc_ldmm_alphas_orig_d(const orc_ldmm_params *p, const float *I0, const float *I1,
+                            const double *v, float *alpha);
+/* HField3DUtils::jacobian (HField3DUtils.txx:2370-2420) of an AoS field in world
+ * units with spacing sp. */
+void orc_jacobian_det_d(const double *hw, const int *size, const double *sp, double *det);
 /* IterateNew(computeAlphasOnly = true): alpha_t = splat(J0t_T - I1, h_t) / sigma^2
  * for every t (n_time_steps images, float). */
 void orc_ldmm_alphas_d(const orc_ldmm_params *p, const float *I0, const float *I1,

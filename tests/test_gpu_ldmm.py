@@ -1,9 +1,11 @@
-"""LDMM IterateNew on the GPU vs the FP64 oracle (LDMMIteratorCPU.cxx:529-631).
+"""LDMM on the GPU vs the FP64 oracle, both gradient forms: IterateOrig
+(LDMMIteratorCPU.cxx:379-527, the reference's default) and IterateNew
+(LDMMIteratorCPU.cxx:529-631).
 
 Bars: velocity fields relative L2 <= 1e-4 after several iterations (float GPU vs
-FP64 restatement; the splat is int64 fixed point, 2^-33 per contribution),
-energies and step relative 1e-5, alpha images relative L2 <= 1e-5; repeated runs
-bit-identical (deterministic splat and FP64 reductions)."""
+FP64 restatement; IterateNew's splat is int64 fixed point, 2^-33 per
+contribution), energies and step relative 1e-5, alpha images relative L2 <= 1e-5;
+repeated runs bit-identical (deterministic splat and FP64 reductions)."""
 import numpy as np
 import pytest
 
@@ -27,16 +29,24 @@ CASES = [((16, 20, 24), 5, 4, (1.0, 1.0, 1.0)),
          ((64, 64, 64), 10, 2, (1.0, 1.0, 1.0))]
 
 
+GRADIENTS = [True, False]  # IterateOrig, IterateNew
+
+
+def _oracle_run(orig):
+    return oracle.ldmm_run_orig if orig else oracle.ldmm_run
+
+
+@pytest.mark.parametrize("orig", GRADIENTS)
 @pytest.mark.parametrize("shape,nt,nit,sp", CASES)
-def test_ldmm_run_vs_oracle(shape, nt, nit, sp):
+def test_ldmm_run_vs_oracle(shape, nt, nit, sp, orig):
     import torch
     from atlaswerks_amd import ldmm
     from gpu_util import rel_l2, to_dev
-    print("ldmm", shape, nt, nit, sp, flush=True)
+    print("ldmm", "orig" if orig else "new", shape, nt, nit, sp, flush=True)
     I0, I1 = _pair(shape, seed=5)
     sigma, mp = 0.1, 0.2
-    v_ref, ei, ev, st = oracle.ldmm_run(I0, I1, nt, nit, sp=sp, sigma=sigma, max_pert=mp)
-    prm = ldmm.LDMMIteratorParam(sigma=sigma, max_pert=mp)
+    v_ref, ei, ev, st = _oracle_run(orig)(I0, I1, nt, nit, sp=sp, sigma=sigma, max_pert=mp)
+    prm = ldmm.LDMMIteratorParam(sigma=sigma, max_pert=mp, use_orig_gradient=orig)
     w = ldmm.LDMMWarp(to_dev(I0), to_dev(I1), n_time_steps=nt, n_iter=nit, spacing=sp, param=prm)
     data = w.RunWarp(vec_energy=True)
     torch.cuda.synchronize()
@@ -48,14 +58,16 @@ def test_ldmm_run_vs_oracle(shape, nt, nit, sp):
     np.testing.assert_allclose(e[:, 2], ei + ev, rtol=1e-5)
 
 
-def test_ldmm_step_call_leaves_v_and_deterministic():
+@pytest.mark.parametrize("orig", GRADIENTS)
+def test_ldmm_step_call_leaves_v_and_deterministic(orig):
     import torch
     from atlaswerks_amd import ldmm
     from gpu_util import to_dev
+    print("ldmm determinism", "orig" if orig else "new", flush=True)
     shape = (24, 28, 20)
     I0, I1 = _pair(shape, seed=6)
-    _, _, _, st = oracle.ldmm_run(I0, I1, 3, 0, sigma=0.1, max_pert=0.2)
-    prm = ldmm.LDMMIteratorParam(sigma=0.1, max_pert=0.2)
+    _, _, _, st = _oracle_run(orig)(I0, I1, 3, 0, sigma=0.1, max_pert=0.2)
+    prm = ldmm.LDMMIteratorParam(sigma=0.1, max_pert=0.2, use_orig_gradient=orig)
     outs = []
     for rep in range(2):
         w = ldmm.LDMMWarp(to_dev(I0), to_dev(I1), n_time_steps=3, n_iter=0, param=prm)
@@ -70,18 +82,20 @@ def test_ldmm_step_call_leaves_v_and_deterministic():
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
 
 
-def test_ldmm_alphas_vs_oracle():
+@pytest.mark.parametrize("orig", GRADIENTS)
+def test_ldmm_alphas_vs_oracle(orig):
     import torch
     from atlaswerks_amd import ldmm
     from gpu_util import rel_l2, to_dev
+    print("ldmm alphas", "orig" if orig else "new", flush=True)
     shape = (20, 24, 28)
     nt = 4
     I0, I1 = _pair(shape, seed=7)
     rng = np.random.default_rng(7)
     v = np.stack([np.moveaxis(synth.sinusoidal_displacement(shape, max_disp=1.5, seed=10 + t), 0, -1)
                   for t in range(nt)]).astype(np.float64)
-    want = oracle.ldmm_alphas(I0, I1, v, sigma=0.1)
-    prm = ldmm.LDMMIteratorParam(sigma=0.1)
+    want = (oracle.ldmm_alphas_orig if orig else oracle.ldmm_alphas)(I0, I1, v, sigma=0.1)
+    prm = ldmm.LDMMIteratorParam(sigma=0.1, use_orig_gradient=orig)
     data = ldmm.LDMMDeformationDataGPU(to_dev(I0), to_dev(I1), nt)
     data.v.copy_(torch.from_numpy(_soa_v(v.astype(np.float32))))
     it = ldmm.LDMMIteratorGPU(shape[::-1], (1.0, 1.0, 1.0), nt, prm)
