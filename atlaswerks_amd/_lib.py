@@ -72,12 +72,23 @@ SIGNATURES = {
     "awk_ldmm_create": [ctypes.POINTER(_P), _I, _I, _I, _F, _F, _F, _F, _F, _F, _I, _F, _F],
     "awk_ldmm_destroy": [_P],
     "awk_ldmm_set_gradient": [_P, _I],
+    "awk_ldmm_set_splat": [_P, _I],
     "awk_ldmm_update_step_next": [_P],
     "awk_ldmm_set_step": [_P, _F],
     "awk_ldmm_get_step": [_P, ctypes.POINTER(_F)],
     "awk_ldmm_iterate": [_P, _P, _P, _P, _P, _I, _P],
     "awk_ldmm_alphas": [_P, _P, _P, _P, _P, _P],
     "awk_ldmm_deformed_image": [_P, _I, _P, _P],
+    "awk_ldmm_atlas_create": [ctypes.POINTER(_P), _I, _I, _I, _F, _F, _F, _F, _F, _F, _I, _F, _F, _I, _I],
+    "awk_ldmm_atlas_destroy": [_P],
+    "awk_ldmm_atlas_set_subject": [_P, _I, _P, _F, _P, _P],
+    "awk_ldmm_atlas_set_gradient": [_P, _I],
+    "awk_ldmm_atlas_set_splat": [_P, _I],
+    "awk_ldmm_atlas_update_step_next": [_P],
+    "awk_ldmm_atlas_set_step": [_P, _F],
+    "awk_ldmm_atlas_get_step": [_P, _I, ctypes.POINTER(_F)],
+    "awk_ldmm_atlas_iterate": [_P, _I, _P, _I, _P],
+    "awk_ldmm_atlas_mean": [_P, _P, _P],
     "awk_greedy_update_step_next": [_P],
     "awk_greedy_set_step": [_P, _F],
     "awk_greedy_get_step": [_P, ctypes.POINTER(_F)],

@@ -24,6 +24,7 @@
 #include <utility>
 #include <vector>
 
+#include "awk_ldmm.h"
 #include "awk_fluid.h"
 #include "awk_ops.h"
 #include "awk_splat_tile.h"
@@ -103,6 +104,31 @@ k_ldmm_bwd(float *__restrict__ nx_, float *__restrict__ ny_, float *__restrict__
   SplatPosLdmm p{hx, hy, hz, make_buf(vx, N), make_buf(vy, N), make_buf(vz, N), d, sx, sy, sz,
                  nx_, ny_, nz_};
   splat_tile(p, diff, acc, d);
+}
+
+// composeVHInv(v_t, h, PARTIAL_ZERO) alone, for the sort/scan splat: the same
+// positions as the fused tiled kernel's (SplatPosLdmm).
+__global__ void __launch_bounds__(kLdmmBlock)
+k_ldmm_vhinv(float *__restrict__ nx_, float *__restrict__ ny_, float *__restrict__ nz_,
+             const float *__restrict__ hx, const float *__restrict__ hy,
+             const float *__restrict__ hz, const float *__restrict__ vx,
+             const float *__restrict__ vy, const float *__restrict__ vz, Dims d, float sx, float sy,
+             float sz) {
+  const int N = d.nx * d.ny * d.nz;
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= N) return;
+  const float a = hx[i], b = hy[i], c = hz[i];
+  const float3 o = trilerp3g<AWK_BG_PARTIAL_ZERO>(make_buf(vx, N), make_buf(vy, N), make_buf(vz, N), d,
+                                                  a, b, c);
+  nx_[i] = o.x / -sx + a;
+  ny_[i] = o.y / -sy + b;
+  nz_[i] = o.z / -sz + c;
+}
+
+__global__ void __launch_bounds__(kLdmmBlock)
+k_ldmm_scale(float *__restrict__ out, const float *__restrict__ in, int N, float k) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < N) out[i] = in[i] * k;
 }
 
 // alpha_t = splat / sigma^2 (LDMMIteratorGPU.cxx:197-204); clears the accumulator
@@ -330,55 +356,21 @@ __global__ void k_orig_image_energy(const double *__restrict__ partials, int n, 
 
 }  // namespace awk
 
-struct awk_ldmm {
-  awk::FluidKernel k;
-  int nx = 0, ny = 0, nz = 0, nt = 0;
-  float sp[3] = {1, 1, 1};
-  float sigma = 0.05f, max_pert = 0.1f;
-  bool update_step = true;
-  float *h = nullptr, *hs = nullptr;  // 3 planes each
-  float *J = nullptr;                  // nt images
-  float *diff = nullptr;               // IterateOrig: the force multiplier m
-  bool orig = true;                    // mUseOrigGradientCalc (LDMMIteratorGPU.cxx:37)
-  float *jt = nullptr, *jd = nullptr;  // IterateOrig: JT_t, JacDet_t, nt images each
-  long long *acc = nullptr;
-  double *partials = nullptr;  // max(forward grid, dot grid) doubles
-  float *maxpart = nullptr;    // nt * nblk
-  float *step = nullptr;
-  double *energy = nullptr;    // 3 doubles (internal when the caller passes none)
-  int nblk = 0, egrid = 0;
-};
-
 using namespace awk;
 
-static void ldmm_free(awk_ldmm *l) {
+void awk::ldmm_release(awk_ldmm *l) {
   fk_release(l->k);
-  void *bufs[] = {l->h,        l->hs,      l->J,    l->diff,   l->acc, l->partials,
-                  l->maxpart, l->step, l->energy, l->jt, l->jd};
+  void *bufs[] = {l->h,      l->hs,     l->J,  l->diff,      l->acc,   l->partials, l->maxpart,
+                  l->step,   l->energy, l->jt, l->jd, l->sort_work, l->msplat};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
 }
 
-extern "C" {
-
-awk_status awk_ldmm_create(awk_ldmm_t *out, int nx, int ny, int nz, float spx, float spy, float spz,
-                           float alpha, float beta, float gamma, int n_time_steps, float sigma,
-                           float max_pert) {
-  if (!out) return fail(AWK_ERR_INVALID, "awk_ldmm_create: null handle pointer");
-  *out = nullptr;
-  if (!(spx > 0 && spy > 0 && spz > 0)) return fail(AWK_ERR_INVALID, "awk_ldmm_create: spacing must be > 0");
-  if (n_time_steps < 1) return fail(AWK_ERR_INVALID, "awk_ldmm_create: need at least one time step");
-  if (!(sigma > 0) || !(max_pert > 0)) return fail(AWK_ERR_INVALID, "awk_ldmm_create: sigma and MaxPert must be > 0");
-  if ((long long)nx * ny * nz > (1LL << 28)) return fail(AWK_ERR_INVALID, "awk_ldmm_create: at most 2^28 voxels");
-  awk_ldmm *l = new (std::nothrow) awk_ldmm();
-  if (!l) return fail(AWK_ERR_ALLOC, "awk_ldmm_create: host allocation failed");
-  const float sp[3] = {spx, spy, spz};
+awk_status awk::ldmm_init(awk_ldmm *l, int nx, int ny, int nz, const float sp[3], float alpha,
+                          float beta, float gamma, int n_time_steps, float sigma, float max_pert,
+                          bool terms) {
   awk_status st = fk_init(l->k, nx, ny, nz, sp);
-  if (st) {
-    ldmm_free(l);
-    delete l;
-    return st;
-  }
+  if (st) return st;
   l->k.alpha = alpha;
   l->k.beta = beta;
   l->k.gamma = gamma;
@@ -404,51 +396,11 @@ awk_status awk_ldmm_create(awk_ldmm_t *out, int nx, int ny, int nz, float spx, f
   if (e == hipSuccess) e = hipMalloc(&l->maxpart, (size_t)l->nt * l->nblk * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&l->step, sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&l->energy, 3 * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc(&l->jt, (size_t)l->nt * N * sizeof(float));
-  if (e == hipSuccess) e = hipMalloc(&l->jd, (size_t)l->nt * N * sizeof(float));
+  if (terms && e == hipSuccess) e = hipMalloc(&l->jt, (size_t)l->nt * N * sizeof(float));
+  if (terms && e == hipSuccess) e = hipMalloc(&l->jd, (size_t)l->nt * N * sizeof(float));
   if (e == hipSuccess) e = hipMemset(l->acc, 0, N * sizeof(long long));
   if (e == hipSuccess) e = hipMemset(l->step, 0, sizeof(float));
-  if (e != hipSuccess) {
-    st = hip_check(e, "awk_ldmm_create: allocation");
-    ldmm_free(l);
-    delete l;
-    return st;
-  }
-  *out = l;
-  return AWK_OK;
-}
-
-awk_status awk_ldmm_destroy(awk_ldmm_t l) {
-  if (!l) return AWK_OK;
-  ldmm_free(l);
-  delete l;
-  return AWK_OK;
-}
-
-awk_status awk_ldmm_set_gradient(awk_ldmm_t l, int use_orig) {
-  if (!l) return fail(AWK_ERR_INVALID, "awk_ldmm_set_gradient: null handle");
-  l->orig = use_orig != 0;
-  return AWK_OK;
-}
-
-awk_status awk_ldmm_update_step_next(awk_ldmm_t l) {
-  if (!l) return fail(AWK_ERR_INVALID, "awk_ldmm_update_step_next: null handle");
-  l->update_step = true;
-  return AWK_OK;
-}
-
-awk_status awk_ldmm_set_step(awk_ldmm_t l, float step) {
-  if (!l) return fail(AWK_ERR_INVALID, "awk_ldmm_set_step: null handle");
-  AWK_HIP(hipMemcpy(l->step, &step, sizeof(float), hipMemcpyHostToDevice));
-  l->update_step = false;
-  return AWK_OK;
-}
-
-awk_status awk_ldmm_get_step(awk_ldmm_t l, float *step) {
-  if (!l || !step) return fail(AWK_ERR_INVALID, "awk_ldmm_get_step: null argument");
-  AWK_HIP(hipDeviceSynchronize());
-  AWK_HIP(hipMemcpy(step, l->step, sizeof(float), hipMemcpyDeviceToHost));
-  return AWK_OK;
+  return e == hipSuccess ? AWK_OK : hip_check(e, "LDMM iterator allocation");
 }
 
 // energy[1] += Prod(sp) <L v_t, v_t> / T, with L applied to a copy of v_t in lv (3N).
@@ -485,7 +437,7 @@ static awk_status vec_energy_step(awk_ldmm *l, float *const v[], int t, float *l
 // u_t = L^-1 (grad(src) * m) and the update of v_t (or the step-size maximum),
 // the force and the update fused into the FFT x passes.
 static awk_status solve_update(awk_ldmm *l, float *const v[], int t, const float *src,
-                               const float *m, const long long *acc, bool step_call,
+                               const float *m, const long long *acc, float *step, bool step_call,
                                hipStream_t s) {
   awk_status st;
   int nb = 0;
@@ -505,16 +457,34 @@ static awk_status solve_update(awk_ldmm *l, float *const v[], int t, const float
   ia.mode = step_call ? XSTORE_MAXAXPY : XSTORE_AXPBY;
   for (int c = 0; c < 3; ++c) ia.out[c] = v[3 * t + c];
   ia.maxpart = l->maxpart + (size_t)t * l->nblk;
-  ia.step = l->step;
+  ia.step = step;
   ia.sigma2 = l->sigma * l->sigma;
   return fk_xinv(l->k, ia, s, &nb);
 }
 
-// IterateOrig: backward sweep for JT_t and JacDet_t, then the forward sweep that
-// updates each v_t and composes h with the updated field.
-static awk_status ldmm_run_orig(awk_ldmm *l, const float *I0, const float *I1, float *const v[],
-                                float *const alpha[], double *energy_dev, int vec_energy,
-                                hipStream_t s) {
+awk_status awk::ldmm_orig_backward(awk_ldmm *l, const float *I1, float *const v[], float *jt,
+                                   float *jd, hipStream_t s) {
+  const int nx = l->nx, ny = l->ny, nz = l->nz, T = l->nt;
+  const int N = nx * ny * nz;
+  const Dims d{nx, ny, nz};
+  const unsigned lgrid = (unsigned)((N + kLdmmBlock - 1) / kLdmmBlock);
+  float *h = l->h, *hs = l->hs;
+  AWK_LAUNCH(k_ldmm_identity, dim3(lgrid), dim3(kLdmmBlock), 0, s, h, h + N, h + 2 * N, d);
+  for (int t = T - 1; t >= 0; --t) {
+    AWK_LAUNCH(k_orig_bwd, dim3(l->egrid), dim3(kLdmmBlock), 0, s, hs, hs + N, hs + 2 * N, h,
+               h + N, h + 2 * N, v[3 * t], v[3 * t + 1], v[3 * t + 2], I1,
+               t == T - 1 ? nullptr : jd + (size_t)(t + 1) * N, jt + (size_t)t * N,
+               jd + (size_t)t * N, d, l->sp[0], l->sp[1], l->sp[2]);
+    AWK_LAUNCH_CHECK("k_orig_bwd");
+    std::swap(h, hs);
+  }
+  return AWK_OK;
+}
+
+awk_status awk::ldmm_orig_forward(awk_ldmm *l, const float *I0, const float *I1, float *const v[],
+                                  const float *jt, const float *jd, float *step, bool step_call,
+                                  float *const alpha[], double *energy, int vec_energy,
+                                  hipStream_t s) {
   const int nx = l->nx, ny = l->ny, nz = l->nz, T = l->nt;
   const int N = nx * ny * nz;
   const Dims d{nx, ny, nz};
@@ -522,34 +492,24 @@ static awk_status ldmm_run_orig(awk_ldmm *l, const float *I0, const float *I1, f
   const unsigned egrid = (unsigned)l->egrid;
   const unsigned lgrid = (unsigned)((N + kLdmmBlock - 1) / kLdmmBlock);
   const double vol = (double)sx * sy * sz;
-  const bool step_call = !alpha && l->update_step;
   const float inv_sig2 = 1.f / (l->sigma * l->sigma);
-  double *energy = energy_dev ? energy_dev : l->energy;
+  if (alpha) step_call = false;
   awk_status st;
   float *h = l->h, *hs = l->hs;
-  AWK_LAUNCH(k_ldmm_identity, dim3(lgrid), dim3(kLdmmBlock), 0, s, h, h + N, h + 2 * N, d);
-  for (int t = T - 1; t >= 0; --t) {
-    AWK_LAUNCH(k_orig_bwd, dim3(egrid), dim3(kLdmmBlock), 0, s, hs, hs + N, hs + 2 * N, h, h + N,
-               h + 2 * N, v[3 * t], v[3 * t + 1], v[3 * t + 2], I1,
-               t == T - 1 ? nullptr : l->jd + (size_t)(t + 1) * N, l->jt + (size_t)t * N,
-               l->jd + (size_t)t * N, d, sx, sy, sz);
-    AWK_LAUNCH_CHECK("k_orig_bwd");
-    std::swap(h, hs);
-  }
   if (!alpha) AWK_LAUNCH(k_orig_energy_zero, dim3(1), dim3(64), 0, s, energy);
-  AWK_LAUNCH(k_orig_init, dim3(lgrid), dim3(kLdmmBlock), 0, s, h, h + N, h + 2 * N, I0, l->jt,
-             l->jd, alpha ? alpha[0] : l->diff, alpha ? inv_sig2 : 1.f, d);
+  AWK_LAUNCH(k_orig_init, dim3(lgrid), dim3(kLdmmBlock), 0, s, h, h + N, h + 2 * N, I0, jt, jd,
+             alpha ? alpha[0] : l->diff, alpha ? inv_sig2 : 1.f, d);
   AWK_LAUNCH_CHECK("k_orig_init");
   for (int t = 0; t < T; ++t) {
     const bool last = (t == T - 1);
     if (!alpha &&
         (st = solve_update(l, v, t, t > 0 ? l->J + (size_t)(t - 1) * N : I0, l->diff, nullptr,
-                           step_call, s)))
+                           step, step_call, s)))
       return st;
     AWK_LAUNCH(k_orig_fwd, dim3(egrid), dim3(kLdmmBlock), 0, s, hs, hs + N, hs + 2 * N, h, h + N,
                h + 2 * N, v[3 * t], v[3 * t + 1], v[3 * t + 2], I0,
-               alpha ? nullptr : l->J + (size_t)t * N, last ? nullptr : l->jt + (size_t)(t + 1) * N,
-               last ? nullptr : l->jd + (size_t)(t + 1) * N,
+               alpha ? nullptr : l->J + (size_t)t * N, last ? nullptr : jt + (size_t)(t + 1) * N,
+               last ? nullptr : jd + (size_t)(t + 1) * N,
                alpha ? (last ? nullptr : alpha[t + 1]) : l->diff, alpha ? inv_sig2 : 1.f,
                (last && !alpha) ? I1 : nullptr, (last && !alpha) ? l->partials : nullptr, d, sx, sy,
                sz);
@@ -564,17 +524,31 @@ static awk_status ldmm_run_orig(awk_ldmm *l, const float *I0, const float *I1, f
     if (!alpha && !step_call && vec_energy && (st = vec_energy_step(l, v, t, hs, energy, s)))
       return st;
   }
-  if (alpha) return AWK_OK;
   if (step_call) {
-    AWK_LAUNCH(k_ldmm_step, dim3(1), dim3(256), 0, s, l->maxpart, T * l->nblk, l->max_pert, l->step);
+    AWK_LAUNCH(k_ldmm_step, dim3(1), dim3(256), 0, s, l->maxpart, T * l->nblk, l->max_pert, step);
     AWK_LAUNCH_CHECK("k_ldmm_step");
-    l->update_step = false;
   }
   return AWK_OK;
 }
 
-static awk_status ldmm_run(awk_ldmm *l, const float *I0, const float *I1, float *const v[],
-                           float *const alpha[], double *energy_dev, int vec_energy, hipStream_t s) {
+// IterateOrig: backward sweep for JT_t and JacDet_t, then the forward sweep that
+// updates each v_t and composes h with the updated field.
+static awk_status ldmm_run_orig(awk_ldmm *l, const float *I0, const float *I1, float *const v[],
+                                float *const alpha[], double *energy_dev, int vec_energy,
+                                hipStream_t s) {
+  const bool step_call = !alpha && l->update_step;
+  awk_status st = ldmm_orig_backward(l, I1, v, l->jt, l->jd, s);
+  if (st) return st;
+  st = ldmm_orig_forward(l, I0, I1, v, l->jt, l->jd, l->step, step_call, alpha,
+                         energy_dev ? energy_dev : l->energy, vec_energy, s);
+  if (st) return st;
+  if (step_call) l->update_step = false;
+  return AWK_OK;
+}
+
+awk_status awk::ldmm_new_iterate(awk_ldmm *l, const float *I0, const float *I1, float *const v[],
+                                 float *step, bool step_call, float *const alpha[], double *energy,
+                                 int vec_energy, hipStream_t s) {
   const int nx = l->nx, ny = l->ny, nz = l->nz, T = l->nt;
   const int N = nx * ny * nz;
   const Dims d{nx, ny, nz};
@@ -582,8 +556,7 @@ static awk_status ldmm_run(awk_ldmm *l, const float *I0, const float *I1, float 
   const unsigned egrid = (unsigned)l->egrid;
   const unsigned lgrid = (unsigned)((N + kLdmmBlock - 1) / kLdmmBlock);
   const double vol = (double)sx * sy * sz;
-  const bool step_call = !alpha && l->update_step;
-  double *energy = energy_dev ? energy_dev : l->energy;
+  if (alpha) step_call = false;
   awk_status st;
   float *h = l->h, *hs = l->hs;
   // ---- forward: J_t = I0(h_t), h_t = h_{t-1}(x - v_t / sp)
@@ -603,30 +576,127 @@ static awk_status ldmm_run(awk_ldmm *l, const float *I0, const float *I1, float 
   // ---- backward: h_t = h_{t+1} - v_t(h_{t+1}) / sp; alpha_t = splat(diff, h_t)
   AWK_LAUNCH(k_ldmm_zero, dim3(lgrid), dim3(kLdmmBlock), 0, s, l->acc, N);
   AWK_LAUNCH(k_ldmm_identity, dim3(lgrid), dim3(kLdmmBlock), 0, s, h, h + N, h + 2 * N, d);
+  const bool sorted = l->splat_mode == 1;
   for (int t = T - 1; t >= 0; --t) {
-    AWK_LAUNCH(k_ldmm_bwd, dim3(splat_tile_grid(nx, ny, nz)), dim3(kSplatTileThreads), 0, s, hs, hs + N, hs + 2 * N, h, h + N,
-               h + 2 * N, v[3 * t], v[3 * t + 1], v[3 * t + 2], l->diff,
-               reinterpret_cast<unsigned long long *>(l->acc), d, sx, sy, sz);
-    AWK_LAUNCH_CHECK("k_ldmm_bwd");
-    std::swap(h, hs);
+    if (sorted) {
+      // composeVHInv, then the sort/scan splat of diff to the new h (floor, as the
+      // atomic splat the reference's iterator uses)
+      AWK_LAUNCH(k_ldmm_vhinv, dim3(lgrid), dim3(kLdmmBlock), 0, s, hs, hs + N, hs + 2 * N, h,
+                 h + N, h + 2 * N, v[3 * t], v[3 * t + 1], v[3 * t + 2], d, sx, sy, sz);
+      AWK_LAUNCH_CHECK("k_ldmm_vhinv");
+      std::swap(h, hs);
+      if ((st = launch_splat_sorted(l->msplat, l->diff, h, h + N, h + 2 * N, nx, ny, nz, 0, nullptr,
+                                    nullptr, s, 1, l->sort_work)))
+        return st;
+    } else {
+      AWK_LAUNCH(k_ldmm_bwd, dim3(splat_tile_grid(nx, ny, nz)), dim3(kSplatTileThreads), 0, s, hs,
+                 hs + N, hs + 2 * N, h, h + N, h + 2 * N, v[3 * t], v[3 * t + 1], v[3 * t + 2],
+                 l->diff, reinterpret_cast<unsigned long long *>(l->acc), d, sx, sy, sz);
+      AWK_LAUNCH_CHECK("k_ldmm_bwd");
+      std::swap(h, hs);
+    }
     if (alpha) {
-      AWK_LAUNCH(k_ldmm_alpha, dim3(lgrid), dim3(kLdmmBlock), 0, s, alpha[t], l->acc, N,
-                 1.f / (l->sigma * l->sigma));
-      AWK_LAUNCH_CHECK("k_ldmm_alpha");
+      if (sorted)
+        AWK_LAUNCH(k_ldmm_scale, dim3(lgrid), dim3(kLdmmBlock), 0, s, alpha[t], l->msplat, N,
+                   1.f / (l->sigma * l->sigma));
+      else
+        AWK_LAUNCH(k_ldmm_alpha, dim3(lgrid), dim3(kLdmmBlock), 0, s, alpha[t], l->acc, N,
+                   1.f / (l->sigma * l->sigma));
+      AWK_LAUNCH_CHECK("alpha");
       continue;
     }
     // u = L^-1 (grad(J_{t-1}) * alpha_t), grad(I0) at t = 0
-    if ((st = solve_update(l, v, t, t > 0 ? l->J + (size_t)(t - 1) * N : I0, nullptr, l->acc,
-                           step_call, s)))
+    if ((st = solve_update(l, v, t, t > 0 ? l->J + (size_t)(t - 1) * N : I0,
+                           sorted ? l->msplat : nullptr, sorted ? nullptr : l->acc, step, step_call,
+                           s)))
       return st;
     // <L v_t, v_t> with L applied to a copy of v_t in the free hfield buffer
     if (!step_call && vec_energy && (st = vec_energy_step(l, v, t, hs, energy, s))) return st;
   }
   if (step_call) {
-    AWK_LAUNCH(k_ldmm_step, dim3(1), dim3(256), 0, s, l->maxpart, T * l->nblk, l->max_pert, l->step);
+    AWK_LAUNCH(k_ldmm_step, dim3(1), dim3(256), 0, s, l->maxpart, T * l->nblk, l->max_pert, step);
     AWK_LAUNCH_CHECK("k_ldmm_step");
-    l->update_step = false;
   }
+  return AWK_OK;
+}
+
+static awk_status ldmm_run(awk_ldmm *l, const float *I0, const float *I1, float *const v[],
+                           float *const alpha[], double *energy_dev, int vec_energy, hipStream_t s) {
+  const bool step_call = !alpha && l->update_step;
+  awk_status st = ldmm_new_iterate(l, I0, I1, v, l->step, step_call, alpha,
+                                   energy_dev ? energy_dev : l->energy, vec_energy, s);
+  if (st) return st;
+  if (step_call) l->update_step = false;
+  return AWK_OK;
+}
+
+extern "C" {
+
+awk_status awk_ldmm_create(awk_ldmm_t *out, int nx, int ny, int nz, float spx, float spy, float spz,
+                           float alpha, float beta, float gamma, int n_time_steps, float sigma,
+                           float max_pert) {
+  if (!out) return fail(AWK_ERR_INVALID, "awk_ldmm_create: null handle pointer");
+  *out = nullptr;
+  if (!(spx > 0 && spy > 0 && spz > 0)) return fail(AWK_ERR_INVALID, "awk_ldmm_create: spacing must be > 0");
+  if (n_time_steps < 1) return fail(AWK_ERR_INVALID, "awk_ldmm_create: need at least one time step");
+  if (!(sigma > 0) || !(max_pert > 0)) return fail(AWK_ERR_INVALID, "awk_ldmm_create: sigma and MaxPert must be > 0");
+  if ((long long)nx * ny * nz > (1LL << 28)) return fail(AWK_ERR_INVALID, "awk_ldmm_create: at most 2^28 voxels");
+  awk_ldmm *l = new (std::nothrow) awk_ldmm();
+  if (!l) return fail(AWK_ERR_ALLOC, "awk_ldmm_create: host allocation failed");
+  const float sp[3] = {spx, spy, spz};
+  awk_status st = ldmm_init(l, nx, ny, nz, sp, alpha, beta, gamma, n_time_steps, sigma, max_pert, true);
+  if (st) {
+    ldmm_release(l);
+    delete l;
+    return st;
+  }
+  *out = l;
+  return AWK_OK;
+}
+
+awk_status awk_ldmm_destroy(awk_ldmm_t l) {
+  if (!l) return AWK_OK;
+  ldmm_release(l);
+  delete l;
+  return AWK_OK;
+}
+
+awk_status awk_ldmm_set_gradient(awk_ldmm_t l, int use_orig) {
+  if (!l) return fail(AWK_ERR_INVALID, "awk_ldmm_set_gradient: null handle");
+  l->orig = use_orig != 0;
+  return AWK_OK;
+}
+
+awk_status awk_ldmm_set_splat(awk_ldmm_t l, int mode) {
+  if (!l) return fail(AWK_ERR_INVALID, "awk_ldmm_set_splat: null handle");
+  if (mode != 0 && mode != 1) return fail(AWK_ERR_INVALID, "awk_ldmm_set_splat: mode must be 0 or 1");
+  if (mode == 1 && !l->sort_work) {
+    const size_t N = (size_t)l->nx * l->ny * l->nz;
+    if (8 * N > (size_t)((1LL << 31) - 1)) return fail(AWK_ERR_INVALID, "awk_ldmm_set_splat: sort/scan splat takes at most 2^28 voxels");
+    AWK_HIP(hipMalloc(&l->sort_work, splat_sorted_scratch_bytes(l->nx, l->ny, l->nz)));
+    AWK_HIP(hipMalloc(&l->msplat, N * sizeof(float)));
+  }
+  l->splat_mode = mode;
+  return AWK_OK;
+}
+
+awk_status awk_ldmm_update_step_next(awk_ldmm_t l) {
+  if (!l) return fail(AWK_ERR_INVALID, "awk_ldmm_update_step_next: null handle");
+  l->update_step = true;
+  return AWK_OK;
+}
+
+awk_status awk_ldmm_set_step(awk_ldmm_t l, float step) {
+  if (!l) return fail(AWK_ERR_INVALID, "awk_ldmm_set_step: null handle");
+  AWK_HIP(hipMemcpy(l->step, &step, sizeof(float), hipMemcpyHostToDevice));
+  l->update_step = false;
+  return AWK_OK;
+}
+
+awk_status awk_ldmm_get_step(awk_ldmm_t l, float *step) {
+  if (!l || !step) return fail(AWK_ERR_INVALID, "awk_ldmm_get_step: null argument");
+  AWK_HIP(hipDeviceSynchronize());
+  AWK_HIP(hipMemcpy(step, l->step, sizeof(float), hipMemcpyDeviceToHost));
   return AWK_OK;
 }
 

@@ -32,13 +32,16 @@ constexpr int kScanChunk = kSortBlock * kSortItems;
 __global__ void __launch_bounds__(256)
 k_create_index(uint4 *__restrict__ keys, float4 *__restrict__ vals, const float *__restrict__ src,
                const float *__restrict__ vx, const float *__restrict__ vy,
-               const float *__restrict__ vz, int w, int h, int l, int truncate) {
+               const float *__restrict__ vz, int w, int h, int l, int truncate, int absolute) {
   const int N = w * h * l;
   const int id = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (id >= N) return;
   const int i = id % w, rest = id / w, j = rest % h, k = rest / h;
   const float mass = src[id];
-  const float x = (float)i + vx[id], y = (float)j + vy[id], z = (float)k + vz[id];
+  // displacements (cplSplating) or, absolute, the target positions themselves
+  const float x = absolute ? vx[id] : (float)i + vx[id];
+  const float y = absolute ? vy[id] : (float)j + vy[id];
+  const float z = absolute ? vz[id] : (float)k + vz[id];
   int xi = (int)x, yi = (int)y, zi = (int)z;  // the reference truncates (cudaSplat.cu:50-52)
   if (!truncate) {
     if (x < 0.f && x != (float)xi) --xi;
@@ -210,6 +213,11 @@ k_radix_scatter(const unsigned *__restrict__ kin, const float *__restrict__ vin,
 }
 
 // ---------------------------------------------------------------- segment sums
+__global__ void __launch_bounds__(256) k_zero_image(float *__restrict__ dst, long long N) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) dst[i] = 0.f;
+}
+
 __global__ void __launch_bounds__(256)
 k_segment_sum(const unsigned *__restrict__ keys, const float *__restrict__ vals, long long M,
               unsigned N, float *__restrict__ dst) {
@@ -223,9 +231,19 @@ k_segment_sum(const unsigned *__restrict__ keys, const float *__restrict__ vals,
   dst[k] = acc;
 }
 
+size_t splat_sorted_scratch_bytes(int nx, int ny, int nz) {
+  const long long M = 8LL * nx * ny * nz;
+  const int ntiles = (int)((M + kTile - 1) / kTile);
+  const long long nh = (long long)kRadix * ntiles;
+  const int nchunks = (int)((nh + kScanChunk - 1) / kScanChunk);
+  // two key/value buffers, histogram, chunk sums
+  return 2 * (size_t)M * 4 + 2 * (size_t)M * 4 + (size_t)nh * 4 + (size_t)nchunks * 4 + 256;
+}
+
 awk_status launch_splat_sorted(float *dst, const float *src, const float *vx, const float *vy,
                                const float *vz, int nx, int ny, int nz, int truncate,
-                               unsigned *keys_out, float *vals_out, hipStream_t s) {
+                               unsigned *keys_out, float *vals_out, hipStream_t s, int absolute,
+                               void *work) {
   const long long N = (long long)nx * ny * nz;
   const long long M = 8 * N;
   int bits = 1;
@@ -234,10 +252,9 @@ awk_status launch_splat_sorted(float *dst, const float *src, const float *vx, co
   const int ntiles = (int)((M + kTile - 1) / kTile);
   const long long nh = (long long)kRadix * ntiles;
   const int nchunks = (int)((nh + kScanChunk - 1) / kScanChunk);
-  // scratch: two key/value buffers, histogram, chunk sums (stream-ordered)
-  const size_t bytes = 2 * (size_t)M * 4 + 2 * (size_t)M * 4 + (size_t)nh * 4 + (size_t)nchunks * 4 + 256;
-  char *scratch = nullptr;
-  AWK_HIP(hipMallocAsync((void **)&scratch, bytes, s));
+  // scratch: the caller's work buffer (splat_sorted_scratch_bytes) or stream-ordered
+  char *scratch = static_cast<char *>(work);
+  if (!scratch) AWK_HIP(hipMallocAsync((void **)&scratch, splat_sorted_scratch_bytes(nx, ny, nz), s));
   unsigned *ka = reinterpret_cast<unsigned *>(scratch);
   unsigned *kb = ka + M;
   float *va = reinterpret_cast<float *>(kb + M);
@@ -246,7 +263,7 @@ awk_status launch_splat_sorted(float *dst, const float *src, const float *vx, co
   unsigned *sums = hist + nh;
   AWK_LAUNCH(k_create_index, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s,
              reinterpret_cast<uint4 *>(ka), reinterpret_cast<float4 *>(va), src, vx, vy, vz, nx, ny,
-             nz, truncate);
+             nz, truncate, absolute);
   AWK_LAUNCH_CHECK("k_create_index");
   for (int p = 0; p < passes; ++p) {
     const int shift = p * kRadixBits;
@@ -260,13 +277,13 @@ awk_status launch_splat_sorted(float *dst, const float *src, const float *vx, co
     std::swap(ka, kb);
     std::swap(va, vb);
   }
-  AWK_HIP(hipMemsetAsync(dst, 0, (size_t)N * 4, s));
+  AWK_LAUNCH(k_zero_image, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, dst, N);
   AWK_LAUNCH(k_segment_sum, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, ka, va, M,
              (unsigned)N, dst);
   AWK_LAUNCH_CHECK("k_segment_sum");
   if (keys_out) AWK_HIP(hipMemcpyAsync(keys_out, ka, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
   if (vals_out) AWK_HIP(hipMemcpyAsync(vals_out, va, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
-  AWK_HIP(hipFreeAsync(scratch, s));
+  if (!work) AWK_HIP(hipFreeAsync(scratch, s));
   return AWK_OK;
 }
 
@@ -282,5 +299,5 @@ extern "C" awk_status awk_splat_sorted(float *dst, const float *src, const float
     return fail(AWK_ERR_INVALID, "awk_splat_sorted: bad sizes (at most 2^28 voxels)");
   if (!dst || !src || !vx || !vy || !vz) return fail(AWK_ERR_INVALID, "awk_splat_sorted: null pointer");
   return launch_splat_sorted(dst, src, vx, vy, vz, nx, ny, nz, reference_truncation ? 1 : 0, keys_out,
-                             vals_out, as_stream(stream));
+                             vals_out, as_stream(stream), 0, nullptr);
 }

@@ -18,12 +18,19 @@ from . import _lib
 from .device import ptr, stream_handle
 
 
+SPLAT_MODES = {"fixed": 0, "sorted": 1}
+
+
 class LDMMIteratorParam:
     """LDMMIteratorParam (LDMMIteratorParam.h) + DiffOperParam."""
 
     def __init__(self, sigma=0.05, max_pert=0.1, alpha=0.01, beta=0.01, gamma=0.001,
-                 use_adaptive_step_size=True, step_size=0.005, use_orig_gradient=True):
+                 use_adaptive_step_size=True, step_size=0.005, use_orig_gradient=True,
+                 splat="fixed"):
         self.UseOrigGradientCalc = use_orig_gradient  # LDMMIteratorGPU.cxx:37
+        if splat not in SPLAT_MODES:
+            raise ValueError("splat must be one of %s" % sorted(SPLAT_MODES))
+        self.Splat = splat  # IterateNew's splat: int64 fixed point or sort/scan
         self.Sigma = sigma
         self.MaxPert = max_pert
         self.alpha, self.beta, self.gamma = alpha, beta, gamma
@@ -84,6 +91,7 @@ class LDMMIteratorGPU:
                   p.gamma, self.n_time_steps, p.Sigma, p.MaxPert)
         self._h = h
         _lib.call("awk_ldmm_set_gradient", self._h, int(bool(p.UseOrigGradientCalc)))
+        _lib.call("awk_ldmm_set_splat", self._h, SPLAT_MODES[p.Splat])
         if not p.UseAdaptiveStepSize:
             _lib.call("awk_ldmm_set_step", self._h, float(p.StepSize))
 
@@ -156,3 +164,80 @@ class LDMMWarp:
         for _ in range(self.n_iter):
             self.iterator.Iterate(self.data, vec_energy=vec_energy)
         return self.data
+
+
+class LDMMAtlasBuilderGPU:
+    """LDMMAtlasBuilder (Algorithms/LDMM/LDMMAtlasBuilder.txx) with LDMMAtlasThreadGPU
+    (LDMMAtlasThreadGPU.cxx), single scale, one GPU: every subject is an LDMM
+    registration of the mean to the subject image (IterateOrig unless the param
+    says otherwise); the mean is re-estimated after
+    every iteration, scaled by the Jacobian determinants when ``jacobian_scale``
+    (LDMMAtlasParam JacobianScale, on by default).  ``awk_ldmm_atlas_*``."""
+
+    def __init__(self, images, weights=None, n_time_steps=5, spacing=(1.0, 1.0, 1.0), param=None,
+                 jacobian_scale=True):
+        if not images:
+            raise ValueError("need at least one image")
+        shape = tuple(images[0].shape)
+        if any(tuple(im.shape) != shape for im in images):
+            raise ValueError("all images must be on the template grid")
+        S = len(images)
+        self.param = param or LDMMIteratorParam()
+        self.weights = [1.0 / S] * S if weights is None else [float(w) for w in weights]
+        if len(self.weights) != S:
+            raise ValueError("one weight per image")
+        nz, ny, nx = shape
+        self.size = (nx, ny, nz)
+        self.n_time_steps = int(n_time_steps)
+        dev = images[0].device
+        self.images = [im.contiguous() for im in images]
+        self.v = torch.zeros((S, self.n_time_steps, 3, nz, ny, nx), dtype=torch.float32, device=dev)
+        p = self.param
+        h = ctypes.c_void_p()
+        _lib.call("awk_ldmm_atlas_create", ctypes.byref(h), nx, ny, nz, *[float(x) for x in spacing],
+                  p.alpha, p.beta, p.gamma, self.n_time_steps, p.Sigma, p.MaxPert, S,
+                  int(bool(jacobian_scale)))
+        self._h = h
+        _lib.call("awk_ldmm_atlas_set_gradient", self._h, int(bool(p.UseOrigGradientCalc)))
+        _lib.call("awk_ldmm_atlas_set_splat", self._h, SPLAT_MODES[p.Splat])
+        for s in range(S):
+            arr = (ctypes.c_void_p * (3 * self.n_time_steps))()
+            for t in range(self.n_time_steps):
+                for c in range(3):
+                    arr[3 * t + c] = self.v[s, t, c].data_ptr()
+            _lib.call("awk_ldmm_atlas_set_subject", self._h, s, ptr(self.images[s]),
+                      self.weights[s], arr, None)
+        if not p.UseAdaptiveStepSize:
+            _lib.call("awk_ldmm_atlas_set_step", self._h, float(p.StepSize))
+        self._energy = []
+
+    def Iterate(self, n_iter=1, vec_energy=True, stream=None):
+        e = torch.zeros((n_iter, 3), dtype=torch.float64, device=self.v.device)
+        _lib.call("awk_ldmm_atlas_iterate", self._h, int(n_iter), ctypes.c_void_p(e.data_ptr()),
+                  int(bool(vec_energy)), stream_handle(stream))
+        self._energy.append(e)
+
+    def Mean(self, stream=None):
+        nx, ny, nz = self.size
+        out = torch.empty((nz, ny, nx), dtype=torch.float32, device=self.v.device)
+        _lib.call("awk_ldmm_atlas_mean", self._h, ptr(out), stream_handle(stream))
+        return out
+
+    def StepSize(self, s):
+        out = ctypes.c_float()
+        _lib.call("awk_ldmm_atlas_get_step", self._h, int(s), ctypes.byref(out))
+        return out.value
+
+    def energies(self):
+        """[iterations, 3]: weighted image, vector and total energy (synchronises)."""
+        if not self._energy:
+            return torch.zeros((0, 3), dtype=torch.float64).numpy()
+        return torch.cat(self._energy).cpu().numpy()
+
+    def __del__(self):
+        try:
+            if self._h is not None and self._h.value:
+                _lib.call("awk_ldmm_atlas_destroy", self._h)
+            self._h = None
+        except Exception:
+            pass

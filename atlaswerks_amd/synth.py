@@ -48,6 +48,14 @@ def sinusoidal_displacement(shape, max_disp=3.0, n_modes=4, seed=0):
     return u.astype(np.float32)
 
 
+def rician(img, sigma, seed=0):
+    """Rician noise: |img + n1 + i n2| with n1, n2 ~ N(0, sigma^2)."""
+    rng = np.random.default_rng(seed)
+    n1 = rng.normal(0.0, sigma, img.shape)
+    n2 = rng.normal(0.0, sigma, img.shape)
+    return np.sqrt((img.astype(np.float64) + n1) ** 2 + n2 ** 2).astype(np.float32)
+
+
 def identity(shape):
     nz, ny, nx = shape
     z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")

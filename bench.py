@@ -94,6 +94,48 @@ def bench_ldmm(dev, n=128, T=10, subjects=8, reps=5, orig=True):
             "size": [n, n, n], "N": N}
 
 
+def bench_ldmm_atlas(dev, n=128, S=8, T=10, n_iter=40):
+    """LDMM atlas: S subjects at n^3 (a common 8-blob template warped by its own
+    sinusoidal displacement, max 5 voxels, plus Rician noise 0.02), T = 10 velocity
+    fields, sigma 0.05, alpha 0.01, beta 0.01, gamma 0.001, n_iter iterations; each
+    gradient / splat variant timed as one whole build (step-size calls, n_iter
+    iterations and the final mean)."""
+    import torch
+    from atlaswerks_amd import hfield, ldmm, synth
+    shape = (n, n, n)
+    base = synth.blobs(shape, seed=2)
+    ims = []
+    for s in range(S):
+        h = synth.identity(shape) + synth.sinusoidal_displacement(shape, 5.0, seed=3000 + s)
+        ht = torch.from_numpy(h).to(dev)
+        im = torch.empty(shape, dtype=torch.float32, device=dev)
+        hfield.apply(im, torch.from_numpy(base).to(dev), ht)
+        ims.append(torch.from_numpy(synth.rician(im.cpu().numpy(), 0.02, seed=4000 + s)).to(dev))
+        del ht
+    out = {"subjects": S, "size": [n, n, n], "T": T, "iterations": n_iter}
+    for name, orig, splat in (("IterateOrig", True, "fixed"), ("IterateNew_fixed64", False, "fixed"),
+                              ("IterateNew_sortscan", False, "sorted")):
+        prm = ldmm.LDMMIteratorParam(sigma=0.05, max_pert=0.1, alpha=0.01, beta=0.01, gamma=0.001,
+                                     use_orig_gradient=orig, splat=splat)
+        b = ldmm.LDMMAtlasBuilderGPU(ims, n_time_steps=T, param=prm)
+        b.Mean()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.Iterate(n_iter, vec_energy=False)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        en = b.energies()
+        out[name] = {"build_s": round(ms / 1000.0, 3), "ms_per_iteration": round(ms / n_iter, 3),
+                     "subject_iter_per_s": round(1000.0 * S * n_iter / ms, 1),
+                     "image_energy_first_last": [round(float(en[0, 0]), 3), round(float(en[-1, 0]), 3)]}
+        del b
+        torch.cuda.empty_cache()
+    return out
+
+
 def bench_atlas(dev, reps=10):
     """16 subjects (6 x 96^3, 6 x 128^3, 4 x 160x192x144) of one world-space blob
     template, each on its own grid, atlas on a 128^3 template grid."""
@@ -257,7 +299,7 @@ def main():
     ap.add_argument("--no-microbench", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
-    ap.add_argument("--only", choices=("ldmm", "ldmm_new", "atlas", "unet", "warm_start"),
+    ap.add_argument("--only", choices=("ldmm", "ldmm_new", "ldmm_atlas", "atlas", "unet", "warm_start"),
                     help="run one secondary workload alone (for profiling) and print its JSON")
     ap.add_argument("--copy-ref", action="store_true",
                     help="also run a plain 1 GiB device copy (counter scaling reference)")
@@ -272,6 +314,7 @@ def main():
         dev = torch.device("cuda:0")
         fn = {"ldmm": lambda: bench_ldmm(dev, subjects=2, reps=3),
               "ldmm_new": lambda: bench_ldmm(dev, subjects=2, reps=3, orig=False),
+              "ldmm_atlas": lambda: bench_ldmm_atlas(dev, n_iter=4),
               "atlas": lambda: bench_atlas(dev, reps=3), "unet": lambda: bench_unet(dev),
               "warm_start": lambda: bench_warm_start(dev)}[args.only]
         print(json.dumps({args.only: fn()}), flush=True)
@@ -430,6 +473,7 @@ def main():
     if not args.no_secondary:
         out["ldmm_128_T10"] = bench_ldmm(dev)
         out["ldmm_new_128_T10"] = bench_ldmm(dev, orig=False)
+        out["ldmm_atlas_8x128_T10"] = bench_ldmm_atlas(dev)
         out["atlas_16_uneven"] = bench_atlas(dev)
         out["unet_128"] = bench_unet(dev)
         out["warm_start_128"] = bench_warm_start(dev)

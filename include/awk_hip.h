@@ -336,6 +336,13 @@ awk_status awk_ldmm_destroy(awk_ldmm_t l);
 /* mUseOrigGradientCalc (LDMMIteratorGPU.cxx:37, :151-155): 1 = IterateOrig
  * (Jacobian-determinant gradient, the default), 0 = IterateNew (adjoint splat). */
 awk_status awk_ldmm_set_gradient(awk_ldmm_t l, int use_orig);
+/* IterateNew's splat of the image difference to the backward-composed positions:
+ * 0 = int64 fixed point through LDS tiles (cplSplat3DH's role, cudaSplatAtomic.cu:
+ * 185-247, made order-independent; the default), 1 = the sort/scan splat
+ * (cplSplating, cudaSplat.cu:175-238: createIndex keys, stable radix sort,
+ * sequential segment sums; positions floored as the atomic splat does).  Both are
+ * deterministic.  Mode 1 allocates its scratch here (about 130 bytes per voxel). */
+awk_status awk_ldmm_set_splat(awk_ldmm_t l, int mode);
 /* LDMMIteratorGPU::UpdateStepSizeNextIteration: the next iterate only sets the step. */
 awk_status awk_ldmm_update_step_next(awk_ldmm_t l);
 awk_status awk_ldmm_set_step(awk_ldmm_t l, float step);
@@ -362,6 +369,48 @@ awk_status awk_ldmm_alphas(awk_ldmm_t l, const float *I0, const float *I1, float
 /* Copies J_t = I0 o h_t of the last iterate, h_t composed through v_0..v_t (the
  * reference's mdJ0t[t]). */
 awk_status awk_ldmm_deformed_image(awk_ldmm_t l, int t, float *out, awk_stream_t stream);
+
+/* ====================================================================== */
+/* LDMM atlas                                                               */
+/* ====================================================================== */
+
+typedef struct awk_ldmm_atlas *awk_ldmm_atlas_t;
+
+/* LDMMAtlasBuilder (Algorithms/LDMM/LDMMAtlasBuilder.txx) with LDMMAtlasThreadGPU
+ * (LDMMAtlasThreadGPU.cxx) on one nx*ny*nz grid, single scale: n_subjects LDMM
+ * registrations of the mean (I0) to the subjects (I1), each with n_time_steps
+ * velocity fields; jacobian_scale = LDMMAtlasParam JacobianScale (default true in
+ * the reference, LDMMAtlasParam.h:43-46). */
+awk_status awk_ldmm_atlas_create(awk_ldmm_atlas_t *out, int nx, int ny, int nz, float spx,
+                                 float spy, float spz, float alpha, float beta, float gamma,
+                                 int n_time_steps, float sigma, float max_pert, int n_subjects,
+                                 int jacobian_scale);
+awk_status awk_ldmm_atlas_destroy(awk_ldmm_atlas_t a);
+/* Subject s: its image (device, copied), weight (WeightedImageSet::GetWeight) and
+ * 3 * n_time_steps device velocity planes v[3t + c], updated in place by
+ * awk_ldmm_atlas_iterate; the planes stay the caller's and must outlive the atlas
+ * (or the next set_subject of s).  Synchronises the stream. */
+awk_status awk_ldmm_atlas_set_subject(awk_ldmm_atlas_t a, int s, const float *image, float weight,
+                                      float *const v[], awk_stream_t stream);
+/* The subjects' gradient (awk_ldmm_set_gradient; IterateOrig by default) and
+ * IterateNew's splat (awk_ldmm_set_splat). */
+awk_status awk_ldmm_atlas_set_gradient(awk_ldmm_atlas_t a, int use_orig);
+awk_status awk_ldmm_atlas_set_splat(awk_ldmm_atlas_t a, int mode);
+/* Every subject computes its step size at its next iteration
+ * (LDMMAtlasThreadGPU::BeginImageIteration with UseAdaptiveStepSize). */
+awk_status awk_ldmm_atlas_update_step_next(awk_ldmm_atlas_t a);
+awk_status awk_ldmm_atlas_set_step(awk_ldmm_atlas_t a, float step);
+awk_status awk_ldmm_atlas_get_step(awk_ldmm_atlas_t a, int s, float *step); /* synchronises */
+/* n_iter iterations of AtlasThread::AtlasThreadMain's loop (AtlasThread.txx): each
+ * subject's LDMM iteration against the current mean, then the new mean
+ * sum_s w_s I_s(h_0) JacDet_0 / sum_s w_s JacDet_0 (ComputeThreadMean and
+ * LDMMAtlasBuilder::ComputeMean, LDMMAtlasBuilder.txx:270-340).  energy_dev
+ * (3 * n_iter device doubles, may be NULL): sum_s w_s (image, vector, total)
+ * energy per iteration, FP64 in subject order.  No host synchronisation. */
+awk_status awk_ldmm_atlas_iterate(awk_ldmm_atlas_t a, int n_iter, double *energy_dev,
+                                  int vec_energy, awk_stream_t stream);
+/* The mean of the current fields (the initial mean before any iteration). */
+awk_status awk_ldmm_atlas_mean(awk_ldmm_atlas_t a, float *mean_out, awk_stream_t stream);
 
 /* ====================================================================== */
 /* U-Net warm start (no reference counterpart)                              */

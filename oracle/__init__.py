@@ -333,6 +333,26 @@ def ldmm_alphas_orig(I0, I1, v, sp=(1, 1, 1), sigma=0.05):
     return a
 
 
+def ldmm_atlas_run(images, n_t, n_iter, weights=None, v=None, steps=None, jacobian_scale=True,
+                   sp=(1, 1, 1), alpha=0.01, beta=0.01, gamma=0.001, sigma=0.05, max_pert=0.1,
+                   use_orig=True):
+    """LDMM atlas, single scale (FP64).  images [S, z, y, x]; v [S, T, z, y, x, 3].
+    Returns (mean, v, steps, energy [n_iter, 3])."""
+    images = _f32(images)
+    S = images.shape[0]
+    shape = images.shape[1:]
+    p = _ldmm_params(shape, n_t, n_iter, sp, alpha, beta, gamma, sigma, max_pert)
+    w = np.full(S, 1.0 / S) if weights is None else np.ascontiguousarray(weights, np.float64)
+    v = (np.zeros((S, n_t) + shape + (3,)) if v is None
+         else np.ascontiguousarray(v, dtype=np.float64).copy())
+    st = np.zeros(S) if steps is None else np.ascontiguousarray(steps, np.float64).copy()
+    mean = np.zeros(shape)
+    e = np.zeros((n_iter, 3))
+    lib().orc_ldmm_atlas_run_d(ctypes.byref(p), S, _p(images), _p(w), int(bool(jacobian_scale)),
+                               int(bool(use_orig)), _p(v), _p(st), _p(mean), _p(e))
+    return mean, v, st, e
+
+
 def ldmm_orig_terms(I1, v, sp=(1, 1, 1)):
     """IterateOrig's backward sweep: (JT, JacDet), each [T, z, y, x]."""
     I1 = _f32(I1)

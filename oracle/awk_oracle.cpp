@@ -1088,6 +1088,65 @@ int ldmm_run_orig(const orc_ldmm_params *p, const float *I0, const float *I1, T 
   return 0;
 }
 
+/* LDMM atlas (LDMMAtlasBuilder.txx:270-340, LDMMAtlasThreadGPU.cxx ComputeThreadMean,
+ * AtlasThread.txx AtlasThreadMain), single scale, one thread: the mean of the
+ * subjects' I_s(h_0) (GetDefTTo1(0): composeHV of v_{T-1}..v_0) weighted by w_s
+ * and, with JacobianScale, by JacDet_0 and divided by sum_s w_s JacDet_0; then per
+ * iteration each subject's IterateOrig against the mean (I0 = mean, I1 = I_s; a
+ * step-size call first at iteration 0 when adaptive), the new mean, and the
+ * energy sum_s w_s E_s. */
+void ldmm_atlas_mean(const orc_ldmm_params *p, int S, const float *I, const double *w, int jscale,
+                     const double *v, double *mean) {
+  const size_t N = (size_t)p->size[0] * p->size[1] * p->size[2];
+  const int nt = p->n_time_steps;
+  std::vector<double> JT((size_t)nt * N), JD((size_t)nt * N), den(N, 0.0);
+  std::fill(mean, mean + N, 0.0);
+  for (int s = 0; s < S; ++s) {
+    ldmm_orig_backward<double>(p, I + (size_t)s * N, v + (size_t)s * nt * 3 * N, JT.data(), JD.data());
+    for (size_t i = 0; i < N; ++i) {
+      if (jscale) {
+        mean[i] += JT[i] * JD[i] * w[s];
+        den[i] += JD[i] * w[s];
+      } else {
+        mean[i] += JT[i] * w[s];
+      }
+    }
+  }
+  if (jscale)
+    for (size_t i = 0; i < N; ++i) mean[i] /= den[i];
+}
+
+int ldmm_atlas_run(const orc_ldmm_params *p, int S, const float *I, const double *w, int jscale,
+                   int use_orig, double *v, double *steps, double *mean_out, double *energy) {
+  auto iterate = use_orig ? ldmm_iterate_orig<double> : ldmm_iterate<double>;
+  const size_t N = (size_t)p->size[0] * p->size[1] * p->size[2];
+  const int nt = p->n_time_steps;
+  std::vector<double> mean(N);
+  std::vector<float> meanf(N);
+  ldmm_atlas_mean(p, S, I, w, jscale, v, mean.data());
+  for (int it = 0; it < p->n_iter; ++it) {
+    for (size_t i = 0; i < N; ++i) meanf[i] = (float)mean[i];
+    double e[3] = {0, 0, 0};
+    for (int s = 0; s < S; ++s) {
+      double *vs = v + (size_t)s * nt * 3 * N;
+      if (it == 0 && steps[s] <= 0.0)
+        iterate(p, meanf.data(), I + (size_t)s * N, vs, true, steps[s], nullptr, nullptr, nullptr);
+      double ei = 0, ev = 0;
+      iterate(p, meanf.data(), I + (size_t)s * N, vs, false, steps[s], &ei, &ev, nullptr);
+      e[0] += w[s] * ei;
+      e[1] += w[s] * ev;
+    }
+    if (energy) {
+      energy[3 * it] = e[0];
+      energy[3 * it + 1] = e[1];
+      energy[3 * it + 2] = e[0] + e[1];
+    }
+    ldmm_atlas_mean(p, S, I, w, jscale, v, mean.data());
+  }
+  std::copy(mean.begin(), mean.end(), mean_out);
+  return 0;
+}
+
 template <class T>
 int ldmm_run(const orc_ldmm_params *p, const float *I0, const float *I1, T *v, double *e_img,
              double *e_vec, double *step_out) {
@@ -1432,6 +1491,13 @@ void orc_ldmm_alphas_orig_d(const orc_ldmm_params *p, const float *I0, const flo
   std::vector<double> vc(v, v + n);
   double step = 0.0;
   ldmm_iterate_orig<double>(p, I0, I1, vc.data(), false, step, nullptr, nullptr, alpha);
+}
+
+int orc_ldmm_atlas_run_d(const orc_ldmm_params *p, int n_subjects, const float *images,
+                         const double *weights, int jacobian_scale, int use_orig, double *v,
+                         double *steps, double *mean, double *energy) {
+  return ldmm_atlas_run(p, n_subjects, images, weights, jacobian_scale, use_orig, v, steps, mean,
+                        energy);
 }
 
 void orc_jacobian_det_d(const double *hw, const int *size, const double *sp, double *det) {

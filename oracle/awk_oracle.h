@@ -182,6 +182,15 @@ void orc_ldmm_orig_terms_d(const orc_ldmm_params *p, const float *I1, const doub
 /* IterateOrig(computeAlphasOnly = true): alpha_t = (I0(h_t) - JT_t) JacDet_t / sigma^2. */
 void orc_ldmm_alphas_orig_d(const orc_ldmm_params *p, const float *I0, const float *I1,
                             const double *v, float *alpha);
+/* LDMM atlas (LDMMAtlasBuilder.txx:270-340, LDMMAtlasThreadGPU.cxx, AtlasThread.txx),
+ * single scale: images [S][N] (the subjects, I1 of each registration), weights
+ * [S], v [S][T][N][3] in/out, steps [S] in/out (<= 0: computed by a step-size call
+ * at the first iteration), mean [N] out (after the last iteration), energy
+ * [n_iter][3] (image, vector, total; sum_s w_s E_s).  p->n_iter iterations; each
+ * subject's iteration is IterateOrig (use_orig) or IterateNew. */
+int orc_ldmm_atlas_run_d(const orc_ldmm_params *p, int n_subjects, const float *images,
+                         const double *weights, int jacobian_scale, int use_orig, double *v,
+                         double *steps, double *mean, double *energy);
 /* HField3DUtils::jacobian (HField3DUtils.txx:2370-2420) of an AoS field in world
  * units with spacing sp. */
 void orc_jacobian_det_d(const double *hw, const int *size, const double *sp, double *det);

@@ -29,24 +29,26 @@ CASES = [((16, 20, 24), 5, 4, (1.0, 1.0, 1.0)),
          ((64, 64, 64), 10, 2, (1.0, 1.0, 1.0))]
 
 
-GRADIENTS = [True, False]  # IterateOrig, IterateNew
+# (IterateOrig?, IterateNew's splat)
+MODES = [(True, "fixed"), (False, "fixed"), (False, "sorted")]
+GRADIENTS = [True, False]
 
 
 def _oracle_run(orig):
     return oracle.ldmm_run_orig if orig else oracle.ldmm_run
 
 
-@pytest.mark.parametrize("orig", GRADIENTS)
+@pytest.mark.parametrize("orig,splat", MODES)
 @pytest.mark.parametrize("shape,nt,nit,sp", CASES)
-def test_ldmm_run_vs_oracle(shape, nt, nit, sp, orig):
+def test_ldmm_run_vs_oracle(shape, nt, nit, sp, orig, splat):
     import torch
     from atlaswerks_amd import ldmm
     from gpu_util import rel_l2, to_dev
-    print("ldmm", "orig" if orig else "new", shape, nt, nit, sp, flush=True)
+    print("ldmm", "orig" if orig else "new", splat, shape, nt, nit, sp, flush=True)
     I0, I1 = _pair(shape, seed=5)
     sigma, mp = 0.1, 0.2
     v_ref, ei, ev, st = _oracle_run(orig)(I0, I1, nt, nit, sp=sp, sigma=sigma, max_pert=mp)
-    prm = ldmm.LDMMIteratorParam(sigma=sigma, max_pert=mp, use_orig_gradient=orig)
+    prm = ldmm.LDMMIteratorParam(sigma=sigma, max_pert=mp, use_orig_gradient=orig, splat=splat)
     w = ldmm.LDMMWarp(to_dev(I0), to_dev(I1), n_time_steps=nt, n_iter=nit, spacing=sp, param=prm)
     data = w.RunWarp(vec_energy=True)
     torch.cuda.synchronize()
@@ -58,16 +60,16 @@ def test_ldmm_run_vs_oracle(shape, nt, nit, sp, orig):
     np.testing.assert_allclose(e[:, 2], ei + ev, rtol=1e-5)
 
 
-@pytest.mark.parametrize("orig", GRADIENTS)
-def test_ldmm_step_call_leaves_v_and_deterministic(orig):
+@pytest.mark.parametrize("orig,splat", MODES)
+def test_ldmm_step_call_leaves_v_and_deterministic(orig, splat):
     import torch
     from atlaswerks_amd import ldmm
     from gpu_util import to_dev
-    print("ldmm determinism", "orig" if orig else "new", flush=True)
+    print("ldmm determinism", "orig" if orig else "new", splat, flush=True)
     shape = (24, 28, 20)
     I0, I1 = _pair(shape, seed=6)
     _, _, _, st = _oracle_run(orig)(I0, I1, 3, 0, sigma=0.1, max_pert=0.2)
-    prm = ldmm.LDMMIteratorParam(sigma=0.1, max_pert=0.2, use_orig_gradient=orig)
+    prm = ldmm.LDMMIteratorParam(sigma=0.1, max_pert=0.2, use_orig_gradient=orig, splat=splat)
     outs = []
     for rep in range(2):
         w = ldmm.LDMMWarp(to_dev(I0), to_dev(I1), n_time_steps=3, n_iter=0, param=prm)
@@ -82,12 +84,12 @@ def test_ldmm_step_call_leaves_v_and_deterministic(orig):
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize("orig", GRADIENTS)
-def test_ldmm_alphas_vs_oracle(orig):
+@pytest.mark.parametrize("orig,splat", MODES)
+def test_ldmm_alphas_vs_oracle(orig, splat):
     import torch
     from atlaswerks_amd import ldmm
     from gpu_util import rel_l2, to_dev
-    print("ldmm alphas", "orig" if orig else "new", flush=True)
+    print("ldmm alphas", "orig" if orig else "new", splat, flush=True)
     shape = (20, 24, 28)
     nt = 4
     I0, I1 = _pair(shape, seed=7)
@@ -95,7 +97,7 @@ def test_ldmm_alphas_vs_oracle(orig):
     v = np.stack([np.moveaxis(synth.sinusoidal_displacement(shape, max_disp=1.5, seed=10 + t), 0, -1)
                   for t in range(nt)]).astype(np.float64)
     want = (oracle.ldmm_alphas_orig if orig else oracle.ldmm_alphas)(I0, I1, v, sigma=0.1)
-    prm = ldmm.LDMMIteratorParam(sigma=0.1, use_orig_gradient=orig)
+    prm = ldmm.LDMMIteratorParam(sigma=0.1, use_orig_gradient=orig, splat=splat)
     data = ldmm.LDMMDeformationDataGPU(to_dev(I0), to_dev(I1), nt)
     data.v.copy_(torch.from_numpy(_soa_v(v.astype(np.float32))))
     it = ldmm.LDMMIteratorGPU(shape[::-1], (1.0, 1.0, 1.0), nt, prm)
