@@ -138,7 +138,10 @@ def bench_ldmm_atlas(dev, n=128, S=8, T=10, n_iter=40):
 
 def bench_atlas(dev, reps=10):
     """16 subjects (6 x 96^3, 6 x 128^3, 4 x 160x192x144) of one world-space blob
-    template, each on its own grid, atlas on a 128^3 template grid."""
+    template, each on its own grid warped by its own sinusoidal displacement (max
+    5 voxels) plus Rician noise 0.02; atlas on a 128^3 template grid.  Per-iteration
+    time at full resolution (eager and hipGraph replay), then one whole 3-level
+    build (1/4, 1/2, 1 with 100 / 50 / 25 iterations)."""
     import torch
     from atlaswerks_amd import atlas, hfield, synth
     world = 128.0
@@ -150,10 +153,11 @@ def bench_atlas(dev, reps=10):
         sp = (world / nx, world / ny, world / nz)
         h = torch.empty((3, nz, ny, nx), dtype=torch.float32, device=dev)
         hfield.initializeFromAffine(h, hfield.grid_affine((0, 0, 0), sp, (0, 0, 0), (1, 1, 1)))
-        u = torch.from_numpy(synth.sinusoidal_displacement(g, 4.0, seed=2000 + k)).to(dev)
+        u = torch.from_numpy(synth.sinusoidal_displacement(g, 5.0, seed=2000 + k)).to(dev)
         h += u
         im = torch.empty(g, dtype=torch.float32, device=dev)
         hfield.apply(im, base, h)
+        im = torch.from_numpy(synth.rician(im.cpu().numpy(), 0.02, seed=5000 + k)).to(dev)
         subs.append(atlas.AtlasSubject(im, spacing=sp))
     out = {}
     for use_graph in (False, True):
@@ -165,6 +169,17 @@ def bench_atlas(dev, reps=10):
     out["subjects"] = len(grids)
     out["template"] = [128, 128, 128]
     out["subject_iter_per_s_graph"] = round(1000.0 * len(grids) / out["graph"]["ms_per_atlas_iteration"], 1)
+    levels = [(4, 100), (2, 50), (1, 25)]
+    b = atlas.GreedyAtlasBuilderGPU(subs, (128, 128, 128), max_pert=0.1, use_graph=True,
+                                    scale_levels=levels)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    b.Run()
+    torch.cuda.synchronize()
+    e = b.energies()
+    out["multiscale_build"] = {"levels": [list(x) for x in levels], "build_s": round(time.time() - t0, 3),
+                               "energy_first_last": [round(float(e[0]), 4), round(float(e[-1]), 4)]}
+    del b
     return out
 
 
