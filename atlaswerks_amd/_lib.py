@@ -53,6 +53,8 @@ SIGNATURES = {
     "awk_atlas_create": [ctypes.POINTER(_P), _I, _I, _I, _F, _F, _F, _F, _F, _F, _F, _I],
     "awk_atlas_destroy": [_P],
     "awk_atlas_set_subject": [_P, _I, _P, _I, _I, _I, _F, ctypes.POINTER(_P3), _P],
+    "awk_atlas_set_subject_inverse": [_P, _I, _P, _P],
+    "awk_atlas_get_hinv": [_P, _I, _P, _P],
     "awk_atlas_get_hfield": [_P, _I, ctypes.POINTER(_P3), _P],
     "awk_atlas_update_step_next": [_P],
     "awk_atlas_set_graph": [_P, _I],

@@ -38,7 +38,7 @@ class _AtlasLevel:
     """One awk_atlas handle: template grid, subjects' (level) images, hfields."""
 
     def __init__(self, images, weights, size, spacing, max_pert, alpha, beta, gamma, use_graph,
-                 h0s):
+                 h0s, hinv0s=None):
         self.size = tuple(int(v) for v in size)
         self.images = images  # kept alive: the library holds the pointers
         h = ctypes.c_void_p()
@@ -50,12 +50,22 @@ class _AtlasLevel:
             iz, iy, ix = im.shape
             _lib.call("awk_atlas_set_subject", self._h, i, ptr(im), ix, iy, iz, float(w),
                       planes_arr(h0), stream_handle(None))
+            if hinv0s is not None:
+                _lib.call("awk_atlas_set_subject_inverse", self._h, i, planes_arr(hinv0s[i]),
+                          stream_handle(None))
         torch.cuda.current_stream().synchronize()
+        self.has_inverse = hinv0s is not None
 
     def field(self, i, stream=None):
         nx, ny, nz = self.size
         out = torch.empty((3, nz, ny, nx), dtype=torch.float32, device="cuda")
         _lib.call("awk_atlas_get_hfield", self._h, int(i), planes_arr(out), stream_handle(stream))
+        return out
+
+    def inverse(self, i, stream=None):
+        nx, ny, nz = self.size
+        out = torch.empty((3, nz, ny, nx), dtype=torch.float32, device="cuda")
+        _lib.call("awk_atlas_get_hinv", self._h, int(i), planes_arr(out), stream_handle(stream))
         return out
 
     def close(self):
@@ -67,7 +77,7 @@ class _AtlasLevel:
 class GreedyAtlasBuilderGPU:
     def __init__(self, subjects, template_size, template_origin=(0.0, 0.0, 0.0),
                  template_spacing=(1.0, 1.0, 1.0), max_pert=0.1, alpha=0.01, beta=0.01,
-                 gamma=0.001, use_graph=True, scale_levels=None):
+                 gamma=0.001, use_graph=True, scale_levels=None, compute_inverse=False):
         if not subjects:
             raise ValueError("an atlas needs at least one subject")
         self.subjects = list(subjects)
@@ -78,6 +88,15 @@ class GreedyAtlasBuilderGPU:
         ns = len(self.subjects)
         self.weights = [s.weight if s.weight is not None else 1.0 / ns for s in self.subjects]
         self.scale_levels = list(scale_levels) if scale_levels else None
+        # Def0To1 per subject (GreedyDeformationData::ComputeInverseHField); the
+        # subjects must share the template grid, as the reference's atlas requires
+        self.compute_inverse = bool(compute_inverse)
+        if self.compute_inverse:
+            for s in self.subjects:
+                iz, iy, ix = s.image.shape
+                if ((ix, iy, iz) != self.size or s.spacing != self.spacing or
+                        s.origin != self.origin or s.affine is not None):
+                    raise ValueError("inverse fields need every subject on the template grid")
         self._energy = []
         self._level = None
         self._factor = None
@@ -105,7 +124,7 @@ class GreedyAtlasBuilderGPU:
     def _enter_level(self, f):
         size, sp = self._template_grid(f)
         nx, ny, nz = size
-        images, h0s = [], []
+        images, h0s, hinv0s = [], [], []
         for s in self.subjects:
             if f == 1:
                 im = s.image
@@ -118,16 +137,23 @@ class GreedyAtlasBuilderGPU:
             if self._level is None:
                 h0 = torch.empty((3, nz, ny, nx), dtype=torch.float32, device="cuda")
                 self.initial_hfield(s, h0, f)
+                if self.compute_inverse:
+                    hi = torch.empty_like(h0)
+                    hfield.setToIdentity(hi)
+                    hinv0s.append(hi)
             else:
                 h0 = hfield.upsampleHField(self._level.field(i), size, rescale=True,
                                            base_in=self._base_map(s, self._factor),
                                            base_out=self._base_map(s, f))
+                if self.compute_inverse:
+                    # GreedyDeformationData::SetScaleLevel: Def0To1 upsampled as Def1To0
+                    hinv0s.append(hfield.upsampleHField(self._level.inverse(i), size, rescale=True))
             h0s.append(h0)
         if self._level is not None:
             self._level.close()
         max_pert, alpha, beta, gamma, use_graph = self.prm
         self._level = _AtlasLevel(images, self.weights, size, sp, max_pert, alpha, beta, gamma,
-                                  use_graph, h0s)
+                                  use_graph, h0s, hinv0s if self.compute_inverse else None)
         self._factor = f
 
     @property
@@ -174,6 +200,12 @@ class GreedyAtlasBuilderGPU:
 
     def HField(self, i, stream=None):
         return self._level.field(i, stream)
+
+    def HFieldInverse(self, i, stream=None):
+        """Def0To1 of subject i (compute_inverse=True)."""
+        if not self.compute_inverse:
+            raise ValueError("the builder was created without compute_inverse")
+        return self._level.inverse(i, stream)
 
     def DeformedImage(self, i, stream=None):
         nx, ny, nz = self._level.size

@@ -71,6 +71,7 @@ struct Subject {
   int dims[3] = {0, 0, 0};
   float w = 0.f;
   float *h[2] = {nullptr, nullptr};  // 3 planes each, ping-pong
+  float *hinv[2] = {nullptr, nullptr};  // Def0To1, 3 planes each, when tracked
   float *idef = nullptr;
   float *step = nullptr;
   double *prev = nullptr;
@@ -121,7 +122,7 @@ static void atlas_free(awk_atlas *a) {
   atlas_drop_graphs(a);
   fk_release(a->k);
   for (Subject &s : a->subj) {
-    void *b[] = {s.h[0], s.h[1], s.idef, s.step, s.prev, s.have_prev};
+    void *b[] = {s.h[0], s.h[1], s.hinv[0], s.hinv[1], s.idef, s.step, s.prev, s.have_prev};
     for (void *p : b)
       if (p) (void)hipFree(p);
   }
@@ -185,18 +186,32 @@ static awk_status atlas_iteration(awk_atlas *a, hipStream_t s) {
     XInvArgs ia;
     std::memset(&ia, 0, sizeof(ia));
     for (int c = 0; c < 3; ++c) ia.sp[c] = a->sp[c];
-    if (sb.update_step) {
-      // iteration 0 of a scale level: v, step = MaxPert / max|v|, then composeHV
-      ia.mode = XSTORE_MAXNORM;
+    const bool inv = sb.hinv[0] != nullptr;
+    if (sb.update_step || inv) {
+      // v into vbuf (iteration 0 of a scale level: step = MaxPert / max|v|), then
+      // composeHV and, when the inverse is tracked, composeVHInv
+      // (GreedyIteratorCPU::updateDeformation, GreedyIteratorCPU.cxx:82-95)
+      ia.mode = sb.update_step ? XSTORE_MAXNORM : XSTORE_PLAIN;
       for (int c = 0; c < 3; ++c) ia.out[c] = a->vbuf + (size_t)c * N;
       ia.maxpart = a->maxpart;
       if ((st = fk_xinv(a->k, ia, s, &nb))) return st;
-      AWK_LAUNCH(k_atlas_step, dim3(1), dim3(256), 0, s, a->maxpart, nb, a->max_pert, sb.step);
-      AWK_LAUNCH_CHECK("k_atlas_step");
-      if ((st = launch_compose_hv(hn, hn + N, hn + 2 * (size_t)N, h, h + N, h + 2 * (size_t)N,
-                                  a->vbuf, a->vbuf + N, a->vbuf + 2 * (size_t)N, a->nx, a->ny,
-                                  a->nz, a->sp, 1.f, sb.step, 0, AWK_BG_PARTIAL_ID, s)))
+      if (sb.update_step) {
+        AWK_LAUNCH(k_atlas_step, dim3(1), dim3(256), 0, s, a->maxpart, nb, a->max_pert, sb.step);
+        AWK_LAUNCH_CHECK("k_atlas_step");
+      }
+      const float *vx = a->vbuf, *vy = a->vbuf + N, *vz = a->vbuf + 2 * (size_t)N;
+      if ((st = launch_compose_hv(hn, hn + N, hn + 2 * (size_t)N, h, h + N, h + 2 * (size_t)N, vx,
+                                  vy, vz, a->nx, a->ny, a->nz, a->sp, 1.f, sb.step, 0,
+                                  AWK_BG_PARTIAL_ID, s)))
         return st;
+      if (inv) {
+        const float *hi = sb.hinv[a->cur];
+        float *hin = sb.hinv[nxt];
+        if ((st = launch_compose_vh(hin, hin + N, hin + 2 * (size_t)N, vx, vy, vz, hi, hi + N,
+                                    hi + 2 * (size_t)N, a->nx, a->ny, a->nz, a->sp, sb.step, 1,
+                                    AWK_BG_PARTIAL_ZERO, s)))
+          return st;
+      }
     } else {
       ia.mode = XSTORE_COMPOSE;
       for (int c = 0; c < 3; ++c) {
@@ -312,6 +327,12 @@ awk_status awk_atlas_set_subject(awk_atlas_t a, int s, const float *I0, int nx_i
   if (nx_img < 1 || ny_img < 1 || nz_img < 1 || (long long)nx_img * ny_img * nz_img > kBufMaxElems)
     return fail(AWK_ERR_INVALID, "awk_atlas_set_subject: bad image size");
   Subject &sb = a->subj[s];
+  for (int b = 0; b < 2; ++b)  // a new subject starts without a tracked inverse
+    if (sb.hinv[b]) {
+      AWK_HIP(hipStreamSynchronize(a->as));
+      AWK_HIP(hipFree(sb.hinv[b]));
+      sb.hinv[b] = nullptr;
+    }
   sb.I0 = I0;
   sb.dims[0] = nx_img;
   sb.dims[1] = ny_img;
@@ -337,6 +358,38 @@ awk_status awk_atlas_get_hfield(awk_atlas_t a, int s, float *const h_out[3], awk
   AWK_HIP(hipStreamWaitEvent(cs, a->ev_out, 0));
   for (int c = 0; c < 3; ++c)
     AWK_HIP(hipMemcpyAsync(h_out[c], a->subj[s].h[a->cur] + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, cs));
+  return AWK_OK;
+}
+
+awk_status awk_atlas_set_subject_inverse(awk_atlas_t a, int s, const float *const hinv0[3],
+                                         awk_stream_t stream) {
+  if (!a || !hinv0 || !hinv0[0] || !hinv0[1] || !hinv0[2])
+    return fail(AWK_ERR_INVALID, "awk_atlas_set_subject_inverse: null argument");
+  if (s < 0 || s >= a->ns) return fail(AWK_ERR_INVALID, "awk_atlas_set_subject_inverse: subject index out of range");
+  Subject &sb = a->subj[s];
+  if (!sb.set) return fail(AWK_ERR_INVALID, "awk_atlas_set_subject_inverse: subject not set");
+  if (sb.dims[0] != a->nx || sb.dims[1] != a->ny || sb.dims[2] != a->nz)
+    return fail(AWK_ERR_INVALID, "awk_atlas_set_subject_inverse: the subject's image must be on the template grid");
+  const size_t N = (size_t)a->nx * a->ny * a->nz;
+  for (int b = 0; b < 2; ++b)
+    if (!sb.hinv[b]) AWK_HIP(hipMalloc(&sb.hinv[b], 3 * N * sizeof(float)));
+  hipStream_t cs = as_stream(stream);
+  for (int c = 0; c < 3; ++c)
+    AWK_HIP(hipMemcpyAsync(sb.hinv[a->cur] + c * N, hinv0[c], N * sizeof(float), hipMemcpyDeviceToDevice, cs));
+  atlas_drop_graphs(a);  // the update path of s changes
+  return AWK_OK;
+}
+
+awk_status awk_atlas_get_hinv(awk_atlas_t a, int s, float *const out[3], awk_stream_t stream) {
+  if (!a || !out || !out[0] || !out[1] || !out[2]) return fail(AWK_ERR_INVALID, "awk_atlas_get_hinv: null argument");
+  if (s < 0 || s >= a->ns) return fail(AWK_ERR_INVALID, "awk_atlas_get_hinv: subject index out of range");
+  if (!a->subj[s].hinv[0]) return fail(AWK_ERR_INVALID, "awk_atlas_get_hinv: the subject's inverse is not tracked");
+  const size_t N = (size_t)a->nx * a->ny * a->nz;
+  hipStream_t cs = as_stream(stream);
+  AWK_HIP(hipEventRecord(a->ev_out, a->as));
+  AWK_HIP(hipStreamWaitEvent(cs, a->ev_out, 0));
+  for (int c = 0; c < 3; ++c)
+    AWK_HIP(hipMemcpyAsync(out[c], a->subj[s].hinv[a->cur] + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, cs));
   return AWK_OK;
 }
 

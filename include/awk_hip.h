@@ -301,6 +301,17 @@ awk_status awk_atlas_set_subject(awk_atlas_t a, int s, const float *I0, int nx_i
                                  int nz_img, float weight, const float *const h0[3],
                                  awk_stream_t stream);
 awk_status awk_atlas_get_hfield(awk_atlas_t a, int s, float *const h_out[3], awk_stream_t stream);
+/* Track subject s's inverse field Def0To1 from hinv0 (device, copied; identity, or
+ * the inverse of the initial map), updated every iteration by composeVHInv of the
+ * step, PARTIAL_ZERO, as GreedyIteratorCPU::updateDeformation does for
+ * ComputeInverseHField (GreedyIteratorCPU.cxx:82-95; the reference's GPU atlas
+ * rejects inverse fields, GreedyAtlasBuilderGPU.cxx:451-458).  The subject's image
+ * must be on the template grid; call after awk_atlas_set_subject (which drops a
+ * tracked inverse).  Such subjects take the unfused update (v, then composeHV and
+ * composeVHInv). */
+awk_status awk_atlas_set_subject_inverse(awk_atlas_t a, int s, const float *const hinv0[3],
+                                         awk_stream_t stream);
+awk_status awk_atlas_get_hinv(awk_atlas_t a, int s, float *const out[3], awk_stream_t stream);
 /* Every subject's next iteration sets its step size (iteration 0 of a scale
  * level, GreedyAtlasThreadGPU::BeginImageIteration). */
 awk_status awk_atlas_update_step_next(awk_atlas_t a);

@@ -413,10 +413,11 @@ def apply_grid(img, h, background=0.0):
 
 
 def atlas_run(images, h0, weights, n_iter, template_shape, sp=(1, 1, 1), alpha=0.01, beta=0.01,
-              gamma=0.001, max_pert=0.1):
+              gamma=0.001, max_pert=0.1, hinv0=None):
     """GreedyAtlasBuilder, one scale level, FP64.  images: list of [z,y,x] arrays
     (any sizes); h0: list of AoS hfields on the template grid in each image's
-    voxels.  Returns (h list, mean, energies)."""
+    voxels; hinv0: None, or initial inverse fields (Def0To1) for every subject.
+    Returns (h list, mean, energies), plus the hinv list when hinv0 is given."""
     p = AtlasParams()
     nz, ny, nx = template_shape
     p.size[:] = [nx, ny, nz]
@@ -431,7 +432,14 @@ def atlas_run(images, h0, weights, n_iter, template_shape, sp=(1, 1, 1), alpha=0
     w = np.ascontiguousarray(weights, dtype=np.float64)
     mean = np.zeros(template_shape)
     energy = np.zeros(n_iter)
-    lib().orc_atlas_run_d(ctypes.byref(p), I0, _p(isz), _p(w), H, _p(mean), _p(energy))
+    his = None
+    HI = None
+    if hinv0 is not None:
+        his = [np.ascontiguousarray(h, dtype=np.float64).copy() for h in hinv0]
+        HI = (ctypes.c_void_p * len(his))(*[h.ctypes.data for h in his])
+    lib().orc_atlas_run_d(ctypes.byref(p), I0, _p(isz), _p(w), H, HI, _p(mean), _p(energy))
+    if his is not None:
+        return hs, mean, energy, his
     return hs, mean, energy
 
 

@@ -752,14 +752,15 @@ void gauss_axis(const float *in, float *out, const int *size, int axis, float si
  * iteration runs one greedy Iterate per subject against the current mean (the
  * step size set at iteration 0); the iteration energy is sum_i w_i E_i. */
 int atlas_run_d(const orc_atlas_params *p, const float *const *I0, const int *isizes,
-                const double *w, double *const *h, double *mean, double *energy) {
+                const double *w, double *const *h, double *const *hinv, double *mean,
+                double *energy) {
   const int *size = p->size;
   const size_t N = (size_t)size[0] * size[1] * size[2];
   const int ns = p->n_subjects;
   orc_greedy_params gp;
   for (int i = 0; i < 3; ++i) { gp.size[i] = size[i]; gp.sp[i] = p->sp[i]; }
   gp.alpha = p->alpha; gp.beta = p->beta; gp.gamma = p->gamma; gp.max_pert = p->max_pert;
-  gp.n_iter = 1; gp.compute_inverse = 0; gp.auto_step_reduce = 0;
+  gp.n_iter = 1; gp.compute_inverse = hinv ? 1 : 0; gp.auto_step_reduce = 0;
   GreedyState<double> st(size, p->sp);
   std::vector<double> steps(ns, 0.0), m(N);
   auto compute_mean = [&]() {
@@ -773,8 +774,9 @@ int atlas_run_d(const orc_atlas_params *p, const float *const *I0, const int *is
   for (int it = 0; it < p->n_iter; ++it) {
     double e = 0;
     for (int s = 0; s < ns; ++s)
-      e += w[s] * greedy_iterate<double, double>(st, &gp, I0[s], m.data(), h[s], nullptr, it == 0,
-                                                 steps[s], nullptr, isizes + 3 * s);
+      e += w[s] * greedy_iterate<double, double>(st, &gp, I0[s], m.data(), h[s],
+                                                 hinv ? hinv[s] : nullptr, it == 0, steps[s],
+                                                 nullptr, isizes + 3 * s);
     if (energy) energy[it] = e;
     compute_mean();
   }
@@ -1466,8 +1468,9 @@ void orc_resample_hfield_d(const float *in, const int *in_size, const int *out_s
 }
 
 int orc_atlas_run_d(const orc_atlas_params *p, const float *const *I0, const int *isizes,
-                    const double *weights, double *const *h, double *mean, double *energy) {
-  return atlas_run_d(p, I0, isizes, weights, h, mean, energy);
+                    const double *weights, double *const *h, double *const *hinv, double *mean,
+                    double *energy) {
+  return atlas_run_d(p, I0, isizes, weights, h, hinv, mean, energy);
 }
 
 int orc_ldmm_run_d(const orc_ldmm_params *p, const float *I0, const float *I1, double *v,

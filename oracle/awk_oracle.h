@@ -149,7 +149,8 @@ typedef struct {
  * coordinates of subject s (in/out); weights w_s.  mean (N, may be NULL): the
  * final weighted mean; energy (n_iter, may be NULL): sum_s w_s E_s per iteration. */
 int orc_atlas_run_d(const orc_atlas_params *p, const float *const *I0, const int *isizes,
-                    const double *weights, double *const *h, double *mean, double *energy);
+                    const double *weights, double *const *h, double *const *hinv, double *mean,
+                    double *energy);
 
 /* ---- LDMM (LDMMIteratorCPU.cxx:175-212, 529-631; LDMMEnergy.cxx:48-64) ---- */
 typedef struct {

@@ -96,3 +96,51 @@ def test_atlas_graph_replays_match_eager():
         for a, c in zip(runs[0][0][it], runs[1][0][it]):
             np.testing.assert_array_equal(a, c)
     np.testing.assert_array_equal(runs[0][1], runs[1][1])
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_atlas_inverse_fields_vs_oracle(use_graph):
+    """Def0To1 per subject (composeVHInv of each step, GreedyIteratorCPU.cxx:82-95)."""
+    import torch
+    from atlaswerks_amd import atlas
+    from gpu_util import aos, rel_l2, to_dev
+    print("atlas inverse graph=%d" % use_graph, flush=True)
+    shape = (16, 20, 24)
+    sp = (1.0, 1.2, 0.9)
+    ims = _subjects(shape, 3, seed=70)
+    nz, ny, nx = shape
+    subs = [atlas.AtlasSubject(to_dev(im), spacing=sp) for im in ims]
+    b = atlas.GreedyAtlasBuilderGPU(subs, (nx, ny, nz), template_spacing=sp, max_pert=0.2,
+                                    use_graph=use_graph, compute_inverse=True)
+    b.RunScaleLevel(2)
+    b.Iterate(4)  # steady-state iterations (captured and replayed with use_graph)
+    ident = oracle.identity(shape, np.float64)
+    hs_ref, mean_ref, en_ref, his_ref = oracle.atlas_run(ims, [ident] * 3, [1 / 3] * 3, 6, shape,
+                                                         sp=sp, max_pert=0.2, hinv0=[ident] * 3)
+    for i in range(3):
+        assert rel_l2(aos(b.HField(i).cpu().numpy()), hs_ref[i]) <= 1e-4, i
+        assert rel_l2(aos(b.HFieldInverse(i).cpu().numpy()), his_ref[i]) <= 1e-4, i
+    assert rel_l2(b.Mean().cpu().numpy(), mean_ref) <= 1e-5
+    np.testing.assert_allclose(b.energies(), en_ref, rtol=1e-4)
+
+
+def test_atlas_inverse_multiscale_consistent():
+    """Two scale levels with inverse fields: h(hinv(x)) stays near x in the interior."""
+    import torch
+    from atlaswerks_amd import atlas
+    from gpu_util import aos, to_dev
+    print("atlas inverse multiscale", flush=True)
+    shape = (32, 32, 32)
+    ims = _subjects(shape, 3, seed=80)
+    subs = [atlas.AtlasSubject(to_dev(im)) for im in ims]
+    b = atlas.GreedyAtlasBuilderGPU(subs, (32, 32, 32), max_pert=0.2, scale_levels=[(2, 10), (1, 10)],
+                                    compute_inverse=True)
+    b.Run()
+    ident = oracle.identity(shape, np.float64)
+    for i in range(3):
+        h = aos(b.HField(i).cpu().numpy()).astype(np.float64)
+        hi = aos(b.HFieldInverse(i).cpu().numpy()).astype(np.float64)
+        err = np.sqrt(((oracle.compose(h, hi) - ident) ** 2).sum(-1))[4:-4, 4:-4, 4:-4]
+        disp = np.sqrt(((h - ident) ** 2).sum(-1))[4:-4, 4:-4, 4:-4]
+        assert disp.max() > 0.1
+        assert err.max() < 0.15 * disp.max(), (err.max(), disp.max())

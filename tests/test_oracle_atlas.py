@@ -61,3 +61,29 @@ def test_atlas_energy_decreases():
     h0 = [oracle.identity(shape, np.float64) for _ in ims]
     _, _, en = oracle.atlas_run(ims, h0, np.full(3, 1 / 3), 6, shape, max_pert=0.2)
     assert np.all(np.diff(en) < 0)
+
+
+def test_atlas_inverse_fields():
+    # Def0To1 of each subject after one atlas iteration equals the single greedy
+    # iteration's inverse against the initial mean (GreedyIteratorCPU.cxx:82-95);
+    # after several iterations h(hinv(x)) stays within a small fraction of a voxel
+    # of x away from the border (the inverse is first order in the step)
+    shape = (16, 16, 16)
+    ims = _subjects(shape, 3, seed=40)
+    w = np.full(3, 1.0 / 3)
+    h0 = [oracle.identity(shape, np.float64) for _ in ims]
+    hs, mean, en, his = oracle.atlas_run(ims, h0, w, 1, shape, max_pert=0.2, hinv0=h0)
+    mean0 = sum(wi * im.astype(np.float64) for wi, im in zip(w, ims))
+    for i, im in enumerate(ims):
+        h, hinv, _, _, _ = oracle.greedy_run(im, mean0.astype(np.float32), 1, max_pert=0.2,
+                                             compute_inverse=True, dtype=np.float64)
+        assert np.abs(hs[i] - h).max() < 1e-5
+        assert np.abs(his[i] - hinv).max() < 1e-5
+    hs, _, _, his = oracle.atlas_run(ims, h0, w, 10, shape, max_pert=0.2, hinv0=h0)
+    ident = oracle.identity(shape, np.float64)
+    for h, hi in zip(hs, his):
+        hh = oracle.compose(h, hi)
+        err = np.sqrt(((hh - ident) ** 2).sum(-1))[3:-3, 3:-3, 3:-3]
+        disp = np.sqrt(((h - ident) ** 2).sum(-1))[3:-3, 3:-3, 3:-3]
+        assert disp.max() > 0.1           # the fields moved
+        assert err.max() < 0.1 * disp.max()
