@@ -98,6 +98,8 @@ class GreedyAtlasBuilderGPU:
                         s.origin != self.origin or s.affine is not None):
                     raise ValueError("inverse fields need every subject on the template grid")
         self._energy = []
+        self._energy_level = []
+        self._level_index = -1
         self._level = None
         self._factor = None
         if self.scale_levels is None:
@@ -152,6 +154,7 @@ class GreedyAtlasBuilderGPU:
         if self._level is not None:
             self._level.close()
         max_pert, alpha, beta, gamma, use_graph = self.prm
+        self._level_index += 1
         self._level = _AtlasLevel(images, self.weights, size, sp, max_pert, alpha, beta, gamma,
                                   use_graph, h0s, hinv0s if self.compute_inverse else None)
         self._factor = f
@@ -169,6 +172,7 @@ class GreedyAtlasBuilderGPU:
         _lib.call("awk_atlas_iterate", self._h, int(n_iter), ctypes.c_void_p(e.data_ptr()),
                   stream_handle(stream))
         self._energy.append(e[:n_iter])
+        self._energy_level.append(self._level_index)
 
     def RunScaleLevel(self, n_iter):
         """One scale level: step sizes reset, then n_iter iterations."""
@@ -188,6 +192,19 @@ class GreedyAtlasBuilderGPU:
         if not self._energy:
             return torch.empty(0, dtype=torch.float64).numpy()
         return torch.cat(self._energy).cpu().numpy()
+
+    def EnergyHistory(self):
+        """The per-iteration atlas energies as the reference's EnergyHistory
+        (GreedyAtlasBuilder::FinishIteration and WriteEnergy,
+        GreedyAtlasBuilder.txx:340-348, 494-503)."""
+        from .energy_history import EnergyHistory
+        h = EnergyHistory()
+        count = {}
+        for lvl, e in zip(self._energy_level, self._energy):
+            for v in e.cpu().numpy():
+                h.add_iteration(lvl, count.get(lvl, 0), float(v))
+                count[lvl] = count.get(lvl, 0) + 1
+        return h
 
     def level_size(self):
         return self._level.size

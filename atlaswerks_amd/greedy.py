@@ -190,3 +190,9 @@ class GreedyWarp:
             self.iterator = it
         self.data = prev
         return prev
+
+    def EnergyHistory(self):
+        """The run's energies as the reference's EnergyHistory (``save_xml`` writes
+        GreedyWarp::WriteEnergy's file, GreedyWarp.txx:220-230)."""
+        from .energy_history import EnergyHistory
+        return EnergyHistory.from_levels(list(enumerate(self.energy_history)))

@@ -228,6 +228,16 @@ class LDMMAtlasBuilderGPU:
         _lib.call("awk_ldmm_atlas_get_step", self._h, int(s), ctypes.byref(out))
         return out.value
 
+    def EnergyHistory(self):
+        """The per-iteration weighted energies as the reference's EnergyHistory
+        (LDMMAtlasBuilder::WriteEnergy, LDMMAtlasBuilder.txx:684-693); the
+        VecEnergy element carries the mean over the time steps."""
+        from .energy_history import EnergyHistory, LDMMEnergy
+        h = EnergyHistory()
+        for i, (ei, ev, _) in enumerate(self.energies()):
+            h.add_iteration(0, i, LDMMEnergy(ei, ev, self.n_time_steps))
+        return h
+
     def energies(self):
         """[iterations, 3]: weighted image, vector and total energy (synchronises)."""
         if not self._energy:

@@ -144,3 +144,8 @@ def test_atlas_inverse_multiscale_consistent():
         disp = np.sqrt(((h - ident) ** 2).sum(-1))[4:-4, 4:-4, 4:-4]
         assert disp.max() > 0.1
         assert err.max() < 0.15 * disp.max(), (err.max(), disp.max())
+    # the energies as the reference's EnergyHistory file: one Iteration per level step
+    import xml.etree.ElementTree as ET
+    it = ET.fromstring(b.EnergyHistory().to_xml()).findall("Iteration")
+    assert [(i.get("Scale"), i.get("Iter")) for i in it] == (
+        [("0", str(k)) for k in range(10)] + [("1", str(k)) for k in range(10)])
