@@ -14,7 +14,7 @@ import ctypes
 
 import torch
 
-from . import _lib
+from . import _lib, hfield
 from .device import ptr, stream_handle
 
 
@@ -146,36 +146,98 @@ class LDMMIteratorGPU:
             pass
 
 
-class LDMMWarp:
-    """LDMMWarp::RunWarp for one scale level (LDMMWarp.txx:94-125): an extra
-    step-size call when the step is adaptive, then n_iter iterations."""
+def _downsample(img, f):
+    """MultiscaleManager::DownsampleToLevel (ImageUtils::gaussianDownsample); the
+    level grid is the original size divided by f (MultiscaleManager.cxx:105-125)."""
+    if f == 1:
+        return img
+    nz, ny, nx = img.shape
+    out = torch.empty((nz // f, ny // f, nx // f), dtype=torch.float32, device=img.device)
+    hfield.gaussianDownsample(out, img)
+    return out
 
-    def __init__(self, I0, I1, n_time_steps=10, n_iter=40, spacing=(1.0, 1.0, 1.0), param=None):
-        self.data = LDMMDeformationDataGPU(I0, I1, n_time_steps, spacing)
-        nz, ny, nx = I0.shape
+
+def _upsample_v(v, shape):
+    """MultiscaleManager::UpsampleToLevel of every velocity field: resampleNew with
+    CLAMP and, as the LDMM builders set SetScaleVectorFields(false)
+    (LDMMAtlasBuilder.txx:39), no rescaling of the vectors (world units)."""
+    nz, ny, nx = shape
+    out = torch.empty(v.shape[:-3] + (nz, ny, nx), dtype=torch.float32, device=v.device)
+    flat_in = v.reshape((-1, 3) + tuple(v.shape[-3:]))
+    flat_out = out.reshape((-1, 3, nz, ny, nx))
+    for k in range(flat_in.shape[0]):
+        hfield.resampleNew(flat_out[k], flat_in[k], hfield.BACKGROUND_STRATEGY_CLAMP, False)
+    return out
+
+
+def _levels(scale_levels, n_iter, param):
+    """[(factor, iterations, param)] in run order.  Entries (factor, iterations) take
+    the common param."""
+    if not scale_levels:
+        return [(1, n_iter, param)]
+    out = []
+    for lv in scale_levels:
+        f, n = int(lv[0]), int(lv[1])
+        out.append((f, n, lv[2] if len(lv) > 2 and lv[2] is not None else param))
+    return out
+
+
+class LDMMWarp:
+    """LDMMWarp::RunWarp (LDMMWarp.txx:69-125): per scale level (coarse to fine) an
+    extra step-size call when the step is adaptive, then the level's iterations;
+    between levels the images are downsampled from the originals and the velocity
+    fields upsampled (LDMMDeformationData::SetScaleLevel, LDMMDeformationData.cxx:
+    119-161).  scale_levels: [(factor, iterations[, LDMMIteratorParam])]."""
+
+    def __init__(self, I0, I1, n_time_steps=10, n_iter=40, spacing=(1.0, 1.0, 1.0), param=None,
+                 scale_levels=None):
+        self.I0orig, self.I1orig = I0.contiguous(), I1.contiguous()
+        self.n_time_steps = int(n_time_steps)
+        self.spacing = tuple(float(s) for s in spacing)
         self.param = param or LDMMIteratorParam()
-        self.iterator = LDMMIteratorGPU((nx, ny, nz), spacing, n_time_steps, self.param)
+        self.levels = _levels(scale_levels, n_iter, self.param)
         self.n_iter = n_iter
+        self.energy_history = []
+        f0, _, p0 = self.levels[0]
+        self._enter(f0, p0, None)
+
+    def _enter(self, f, prm, v_prev):
+        I0, I1 = _downsample(self.I0orig, f), _downsample(self.I1orig, f)
+        nz, ny, nx = I0.shape
+        sp = tuple(s * f for s in self.spacing)
+        self.data = LDMMDeformationDataGPU(I0, I1, self.n_time_steps, sp)
+        if v_prev is not None:
+            self.data.v.copy_(_upsample_v(v_prev, (nz, ny, nx)))
+        self.iterator = LDMMIteratorGPU((nx, ny, nz), sp, self.n_time_steps, prm)
 
     def RunWarp(self, vec_energy=True):
-        if self.param.UseAdaptiveStepSize:
-            self.iterator.UpdateStepSizeNextIteration()
-            self.iterator.Iterate(self.data, record_energy=False)
-        for _ in range(self.n_iter):
-            self.iterator.Iterate(self.data, vec_energy=vec_energy)
+        for k, (f, n, prm) in enumerate(self.levels):
+            if k > 0:
+                self._enter(f, prm, self.data.v)
+            if prm.UseAdaptiveStepSize:
+                self.iterator.UpdateStepSizeNextIteration()
+                self.iterator.Iterate(self.data, record_energy=False)
+            for _ in range(n):
+                self.iterator.Iterate(self.data, vec_energy=vec_energy)
+            self.energy_history.append(self.data.energies())
         return self.data
 
 
 class LDMMAtlasBuilderGPU:
     """LDMMAtlasBuilder (Algorithms/LDMM/LDMMAtlasBuilder.txx) with LDMMAtlasThreadGPU
-    (LDMMAtlasThreadGPU.cxx), single scale, one GPU: every subject is an LDMM
-    registration of the mean to the subject image (IterateOrig unless the param
-    says otherwise); the mean is re-estimated after
-    every iteration, scaled by the Jacobian determinants when ``jacobian_scale``
-    (LDMMAtlasParam JacobianScale, on by default).  ``awk_ldmm_atlas_*``."""
+    (LDMMAtlasThreadGPU.cxx) on one GPU: every subject is an LDMM registration of
+    the mean to the subject image (IterateOrig unless the param says otherwise);
+    the mean is re-estimated after every iteration, scaled by the Jacobian
+    determinants when ``jacobian_scale`` (LDMMAtlasParam JacobianScale, on by
+    default).  ``awk_ldmm_atlas_*``, one handle per scale level.
+
+    scale_levels: [(factor, iterations[, LDMMIteratorParam])] in run order (coarse
+    to fine); ``Run()`` runs them, each starting with the subjects' step-size calls
+    when adaptive, the images downsampled from the originals and the velocity
+    fields upsampled in between (LDMMDeformationData::SetScaleLevel)."""
 
     def __init__(self, images, weights=None, n_time_steps=5, spacing=(1.0, 1.0, 1.0), param=None,
-                 jacobian_scale=True):
+                 jacobian_scale=True, scale_levels=None):
         if not images:
             raise ValueError("need at least one image")
         shape = tuple(images[0].shape)
@@ -186,17 +248,34 @@ class LDMMAtlasBuilderGPU:
         self.weights = [1.0 / S] * S if weights is None else [float(w) for w in weights]
         if len(self.weights) != S:
             raise ValueError("one weight per image")
-        nz, ny, nx = shape
-        self.size = (nx, ny, nz)
+        self.spacing = tuple(float(x) for x in spacing)
         self.n_time_steps = int(n_time_steps)
-        dev = images[0].device
-        self.images = [im.contiguous() for im in images]
-        self.v = torch.zeros((S, self.n_time_steps, 3, nz, ny, nx), dtype=torch.float32, device=dev)
-        p = self.param
+        self.jacobian_scale = bool(jacobian_scale)
+        self.images_orig = [im.contiguous() for im in images]
+        self.levels = _levels(scale_levels, 0, self.param)
+        self._h = None
+        self._energy = []
+        self._energy_level = []
+        self._level = -1
+        self._enter(0)
+
+    def _enter(self, k):
+        f, _, p = self.levels[k]
+        ims = [_downsample(im, f) for im in self.images_orig]
+        nz, ny, nx = ims[0].shape
+        S = len(ims)
+        v_prev = getattr(self, "v", None)
+        self.v = torch.zeros((S, self.n_time_steps, 3, nz, ny, nx), dtype=torch.float32,
+                             device=ims[0].device)
+        if v_prev is not None:
+            self.v.copy_(_upsample_v(v_prev, (nz, ny, nx)))
+        self._destroy()
+        self.images = ims
+        self.size = (nx, ny, nz)
+        sp = tuple(x * f for x in self.spacing)
         h = ctypes.c_void_p()
-        _lib.call("awk_ldmm_atlas_create", ctypes.byref(h), nx, ny, nz, *[float(x) for x in spacing],
-                  p.alpha, p.beta, p.gamma, self.n_time_steps, p.Sigma, p.MaxPert, S,
-                  int(bool(jacobian_scale)))
+        _lib.call("awk_ldmm_atlas_create", ctypes.byref(h), nx, ny, nz, *sp, p.alpha, p.beta,
+                  p.gamma, self.n_time_steps, p.Sigma, p.MaxPert, S, int(self.jacobian_scale))
         self._h = h
         _lib.call("awk_ldmm_atlas_set_gradient", self._h, int(bool(p.UseOrigGradientCalc)))
         _lib.call("awk_ldmm_atlas_set_splat", self._h, SPLAT_MODES[p.Splat])
@@ -205,17 +284,25 @@ class LDMMAtlasBuilderGPU:
             for t in range(self.n_time_steps):
                 for c in range(3):
                     arr[3 * t + c] = self.v[s, t, c].data_ptr()
-            _lib.call("awk_ldmm_atlas_set_subject", self._h, s, ptr(self.images[s]),
-                      self.weights[s], arr, None)
+            _lib.call("awk_ldmm_atlas_set_subject", self._h, s, ptr(ims[s]), self.weights[s], arr,
+                      None)
         if not p.UseAdaptiveStepSize:
             _lib.call("awk_ldmm_atlas_set_step", self._h, float(p.StepSize))
-        self._energy = []
+        self._level = k
+
+    def Run(self, vec_energy=True):
+        """Every scale level (AtlasThread::AtlasThreadMain)."""
+        for k, (_, n, _) in enumerate(self.levels):
+            if k != self._level:
+                self._enter(k)
+            self.Iterate(n, vec_energy=vec_energy)
 
     def Iterate(self, n_iter=1, vec_energy=True, stream=None):
         e = torch.zeros((n_iter, 3), dtype=torch.float64, device=self.v.device)
         _lib.call("awk_ldmm_atlas_iterate", self._h, int(n_iter), ctypes.c_void_p(e.data_ptr()),
                   int(bool(vec_energy)), stream_handle(stream))
         self._energy.append(e)
+        self._energy_level.append(self._level)
 
     def Mean(self, stream=None):
         nx, ny, nz = self.size
@@ -234,8 +321,11 @@ class LDMMAtlasBuilderGPU:
         VecEnergy element carries the mean over the time steps."""
         from .energy_history import EnergyHistory, LDMMEnergy
         h = EnergyHistory()
-        for i, (ei, ev, _) in enumerate(self.energies()):
-            h.add_iteration(0, i, LDMMEnergy(ei, ev, self.n_time_steps))
+        count = {}
+        for lvl, e in zip(self._energy_level, self._energy):
+            for ei, ev, _ in e.cpu().numpy():
+                h.add_iteration(lvl, count.get(lvl, 0), LDMMEnergy(ei, ev, self.n_time_steps))
+                count[lvl] = count.get(lvl, 0) + 1
         return h
 
     def energies(self):
@@ -244,10 +334,13 @@ class LDMMAtlasBuilderGPU:
             return torch.zeros((0, 3), dtype=torch.float64).numpy()
         return torch.cat(self._energy).cpu().numpy()
 
+    def _destroy(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.call("awk_ldmm_atlas_destroy", self._h)
+        self._h = None
+
     def __del__(self):
         try:
-            if self._h is not None and self._h.value:
-                _lib.call("awk_ldmm_atlas_destroy", self._h)
-            self._h = None
+            self._destroy()
         except Exception:
             pass

@@ -106,3 +106,33 @@ def test_ldmm_alphas_vs_oracle(orig, splat):
     for t in range(nt):
         assert rel_l2(a[t].cpu().numpy(), want[t]) <= 1e-5, t
     del rng
+
+
+def test_ldmm_warp_scale_levels_vs_oracle():
+    """Two scale levels (LDMMWarp.txx:69-125, LDMMDeformationData.cxx:119-161): the
+    oracle chain is a coarse run on the Gaussian-downsampled images, the velocity
+    fields resampled (CLAMP, no rescaling) to the fine grid, then a fine run that
+    starts with a new step-size call."""
+    import torch
+    from atlaswerks_amd import ldmm
+    from gpu_util import rel_l2, to_dev
+    print("ldmm scale levels", flush=True)
+    shape = (24, 28, 32)
+    sp = (1.0, 1.1, 0.9)
+    I0, I1 = _pair(shape, seed=9)
+    T, kw = 4, dict(sigma=0.1, max_pert=0.2)
+    cshape = tuple(n // 2 for n in shape)
+    c0, c1 = oracle.gaussian_downsample(I0, cshape), oracle.gaussian_downsample(I1, cshape)
+    vc, eic, evc, _ = oracle.ldmm_run_orig(c0, c1, T, 3, sp=tuple(2 * s for s in sp), **kw)
+    v0 = np.stack([oracle.resample_hfield(vc[t], shape) for t in range(T)])
+    vf, eif, evf, stf = oracle.ldmm_run_orig(I0, I1, T, 2, v=v0, sp=sp, **kw)
+    prm = ldmm.LDMMIteratorParam(**kw)
+    w = ldmm.LDMMWarp(to_dev(I0), to_dev(I1), n_time_steps=T, spacing=sp, param=prm,
+                      scale_levels=[(2, 3), (1, 2)])
+    data = w.RunWarp(vec_energy=True)
+    torch.cuda.synchronize()
+    assert abs(w.iterator.StepSize() / stf - 1) < 1e-4
+    assert rel_l2(data.v.cpu().numpy(), _soa_v(vf)) <= 2e-4
+    np.testing.assert_allclose(w.energy_history[0][:, 0], eic, rtol=1e-4)
+    np.testing.assert_allclose(w.energy_history[1][:, 0], eif, rtol=1e-4)
+    np.testing.assert_allclose(w.energy_history[1][:, 1], evf, rtol=1e-4)
