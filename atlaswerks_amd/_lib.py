@@ -91,6 +91,7 @@ SIGNATURES = {
     "awk_ldmm_atlas_get_step": [_P, _I, ctypes.POINTER(_F)],
     "awk_ldmm_atlas_iterate": [_P, _I, _P, _I, _P],
     "awk_ldmm_atlas_mean": [_P, _P, _P],
+    "awk_ldmm_atlas_deformed_image": [_P, _I, _I, _P, _P],
     "awk_greedy_update_step_next": [_P],
     "awk_greedy_set_step": [_P, _F],
     "awk_greedy_get_step": [_P, ctypes.POINTER(_F)],

@@ -289,4 +289,19 @@ awk_status awk_ldmm_atlas_mean(awk_ldmm_atlas_t a, float *mean_out, awk_stream_t
   return AWK_OK;
 }
 
+awk_status awk_ldmm_atlas_deformed_image(awk_ldmm_atlas_t a, int s, int t, float *out,
+                                         awk_stream_t stream) {
+  awk_status st = la_check(a, "awk_ldmm_atlas_deformed_image");
+  if (st) return st;
+  if (!out) return fail(AWK_ERR_INVALID, "awk_ldmm_atlas_deformed_image: null output");
+  if (s < 0 || s >= a->S || t < 0 || t >= a->eng.nt)
+    return fail(AWK_ERR_INVALID, "awk_ldmm_atlas_deformed_image: subject or time step out of range");
+  hipStream_t hs = as_stream(stream);
+  if (!a->current && (st = la_refresh(a, hs))) return st;
+  const size_t N = (size_t)a->eng.nx * a->eng.ny * a->eng.nz;
+  AWK_HIP(hipMemcpyAsync(out, a->subs[s].jt + (size_t)t * N, N * sizeof(float),
+                         hipMemcpyDeviceToDevice, hs));
+  return AWK_OK;
+}
+
 }  // extern "C"

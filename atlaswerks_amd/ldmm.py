@@ -310,6 +310,14 @@ class LDMMAtlasBuilderGPU:
         _lib.call("awk_ldmm_atlas_mean", self._h, ptr(out), stream_handle(stream))
         return out
 
+    def DefToMean(self, s, t, stream=None):
+        """Subject s at time t, I_s(h_t) (GetI1AtT; the DefToMeanTime images)."""
+        nx, ny, nz = self.size
+        out = torch.empty((nz, ny, nx), dtype=torch.float32, device=self.v.device)
+        _lib.call("awk_ldmm_atlas_deformed_image", self._h, int(s), int(t), ptr(out),
+                  stream_handle(stream))
+        return out
+
     def StepSize(self, s):
         out = ctypes.c_float()
         _lib.call("awk_ldmm_atlas_get_step", self._h, int(s), ctypes.byref(out))

@@ -422,6 +422,11 @@ awk_status awk_ldmm_atlas_iterate(awk_ldmm_atlas_t a, int n_iter, double *energy
                                   int vec_energy, awk_stream_t stream);
 /* The mean of the current fields (the initial mean before any iteration). */
 awk_status awk_ldmm_atlas_mean(awk_ldmm_atlas_t a, float *mean_out, awk_stream_t stream);
+/* Subject s's image at time t, I_s(h_t) with h_t = GetDefTTo1(t) of the current
+ * fields (LDMMDeformationData::GetI1AtT, the DefToMeanTime images of
+ * LDMMAtlasBuilder::WriteIntermediateImages, LDMMAtlasBuilder.txx:730-750). */
+awk_status awk_ldmm_atlas_deformed_image(awk_ldmm_atlas_t a, int s, int t, float *out,
+                                         awk_stream_t stream);
 
 /* ====================================================================== */
 /* U-Net warm start (no reference counterpart)                              */

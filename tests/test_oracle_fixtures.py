@@ -45,3 +45,16 @@ def test_fixture_catches_planted_mistakes(fw, mutation):
     finally:
         oracle.set_mutation(0)
     assert not (max(d) <= 10.0)  # NaN also counts as caught
+
+
+def test_ldmm_atlas_fixtures_load():
+    """The LDMM atlas regression fixtures (the reference's Testing data) parse and
+    share the inputs' grid."""
+    import os
+    from atlaswerks_amd import metaimage
+    g = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    ins = [metaimage.read_nrrd(os.path.join(g, "BullseyeTestBlur%02d.nhdr" % i)) for i in range(4)]
+    m = metaimage.read_metaimage(os.path.join(g, "ldmm_atlas", "LDMMAtlasMeanImage.mha"))
+    assert m.data.shape == (36, 31, 32) and all(im.data.shape == (36, 31, 32) for im in ins)
+    assert np.allclose(m.spacing, ins[0].spacing, atol=1e-5)
+    assert 0.0 <= float(m.data.min()) and float(m.data.max()) <= 1.0
