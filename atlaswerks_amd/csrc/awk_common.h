@@ -132,8 +132,10 @@ __device__ __forceinline__ int wrap_mod(int r, int b) {
 // Vector trilerp of a SoA field at (x, y, z) in voxel coordinates, following
 // HField3DUtils::trilerp (UtilitiesDataTypes/HField3DUtils.txx:1969-2300):
 // corner order v0..v7, nested lerp, CPU semantics for PARTIAL_ID (out-of-volume
-// corners take their own integer coordinates).  FP32 weights.
-template <int BG, class L>
+// corners take their own integer coordinates).  FP32 weights.  FAST: a
+// wave-uniform path for waves whose points are all inside the volume (fewer
+// instructions, a few more registers; off where registers bound occupancy).
+template <int BG, class L, bool FAST = true>
 __device__ __forceinline__ float3 trilerp3g(const L &fx, const L &fy, const L &fz, Dims d, float x,
                                             float y, float z) {
   const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
@@ -171,6 +173,27 @@ __device__ __forceinline__ float3 trilerp3g(const L &fx, const L &fy, const L &f
     const bool xin0 = (x0 >= 0) && (x0 < d.nx), xin1 = (x0 >= -1) && (x0 < d.nx - 1);
     const bool yin0 = (y0 >= 0) && (y0 < d.ny), yin1 = (y0 >= -1) && (y0 < d.ny - 1);
     const bool zin0 = (z0 >= 0) && (z0 < d.nz), zin1 = (z0 >= -1) && (z0 < d.nz - 1);
+    const bool inside = xin0 && xin1 && yin0 && yin1 && zin0 && zin1;
+    if (FAST && __all(inside)) {
+      // every lane of the wave interpolates inside the volume (the common case):
+      // corner indices are the base plus fixed offsets, nothing to substitute;
+      // the same loads and values as the general path below
+      const int b0 = (z0 * d.ny + y0) * d.nx + x0;
+      const int b1 = b0 + d.nx, b2 = b0 + d.nx * d.ny, b3 = b2 + d.nx;
+      const int idx[8] = {b0, b0 + 1, b1 + 1, b1, b3, b3 + 1, b2 + 1, b2};
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        c[i][0] = fx(idx[i]);
+        c[i][1] = fy(idx[i]);
+        c[i][2] = fz(idx[i]);
+      }
+      float r[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        r[q] = omt * (omu * (c[0][q] * omv + c[7][q] * v) + u * (c[3][q] * omv + c[4][q] * v)) +
+               t * (omu * (c[1][q] * omv + c[6][q] * v) + u * (c[2][q] * omv + c[5][q] * v));
+      return make_float3(r[0], r[1], r[2]);
+    }
     const int xc0 = min(max(x0, 0), d.nx - 1), xc1 = xin1 ? x0 + 1 : xc0;
     const int yc0 = min(max(y0, 0), d.ny - 1), yc1 = yin1 ? y0 + 1 : yc0;
     const int zc0 = min(max(z0, 0), d.nz - 1), zc1 = zin1 ? z0 + 1 : zc0;
@@ -184,7 +207,6 @@ __device__ __forceinline__ float3 trilerp3g(const L &fx, const L &fy, const L &f
       c[i][1] = fy(idx[i]);
       c[i][2] = fz(idx[i]);
     }
-    const bool inside = xin0 && xin1 && yin0 && yin1 && zin0 && zin1;
     if (BG == AWK_BG_ID || BG == AWK_BG_ZERO) {
       if (!inside) return BG == AWK_BG_ID ? make_float3(x, y, z) : make_float3(0.f, 0.f, 0.f);
     } else {

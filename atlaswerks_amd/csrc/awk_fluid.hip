@@ -330,7 +330,7 @@ k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
       float3 hn[XU];
 #pragma unroll
       for (int u = 0; u < XU; ++u)
-        if (row_[u] < NR) hn[u] = trilerp3g<AWK_BG_PARTIAL_ID>(hx, hy, hz, d, px[u], py[u], pz[u]);
+        if (row_[u] < NR) hn[u] = trilerp3g<AWK_BG_PARTIAL_ID, decltype(hx), false>(hx, hy, hz, d, px[u], py[u], pz[u]);
 #pragma unroll
       for (int u = 0; u < XU; ++u)
         if (row_[u] < NR) {
