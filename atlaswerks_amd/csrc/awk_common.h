@@ -94,14 +94,25 @@ __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y
 // Gathers go through a loader: plain pointers, or raw buffer resources (32-bit
 // byte offsets, hardware range check: an offset past num_records reads 0).  The
 // buffer form saves the 64-bit address arithmetic of every gather.
+// pair(i) loads elements i and i + 1 with one 8-byte access (4-byte aligned, which
+// multi-dword loads allow): the two x-neighbour corners of a trilinear gather.
+typedef float f2a4 __attribute__((ext_vector_type(2), aligned(4)));
 struct PtrLoad {
   const float *__restrict__ p;
   __device__ __forceinline__ float operator()(int i) const { return p[i]; }
+  __device__ __forceinline__ float2 pair(int i) const {
+    const f2a4 v = *reinterpret_cast<const f2a4 *>(p + i);
+    return make_float2(v.x, v.y);
+  }
 };
 struct BufLoad {
   __amdgpu_buffer_rsrc_t r;
   __device__ __forceinline__ float operator()(int i) const {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, i * 4, 0, 0));
+  }
+  __device__ __forceinline__ float2 pair(int i) const {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, i * 4, 0, 0);
+    return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
   }
 };
 // Buffer descriptor word 3 for gfx9-family raw buffers (32-bit data format).
@@ -180,12 +191,16 @@ __device__ __forceinline__ float3 trilerp3g(const L &fx, const L &fy, const L &f
       // the same loads and values as the general path below
       const int b0 = (z0 * d.ny + y0) * d.nx + x0;
       const int b1 = b0 + d.nx, b2 = b0 + d.nx * d.ny, b3 = b2 + d.nx;
-      const int idx[8] = {b0, b0 + 1, b1 + 1, b1, b3, b3 + 1, b2 + 1, b2};
+      // x-neighbour corners in pairs: (c0, c1) at b0, (c3, c2) at b1, (c4, c5) at b3,
+      // (c7, c6) at b2
+      const int pb[4] = {b0, b1, b3, b2};
+      const int lo[4] = {0, 3, 4, 7}, hi[4] = {1, 2, 5, 6};
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        c[i][0] = fx(idx[i]);
-        c[i][1] = fy(idx[i]);
-        c[i][2] = fz(idx[i]);
+      for (int k = 0; k < 4; ++k) {
+        const float2 qx = fx.pair(pb[k]), qy = fy.pair(pb[k]), qz = fz.pair(pb[k]);
+        c[lo[k]][0] = qx.x; c[hi[k]][0] = qx.y;
+        c[lo[k]][1] = qy.x; c[hi[k]][1] = qy.y;
+        c[lo[k]][2] = qz.x; c[hi[k]][2] = qz.y;
       }
       float r[3];
 #pragma unroll
