@@ -109,7 +109,7 @@ def reference_grads(blob, t, s, target):
     for w, b in zip(Ws, Bs):
         parts += [w.grad.numpy().ravel(), b.grad.numpy().ravel()]
     parts += [HW.grad.numpy().ravel(), HB.grad.numpy().ravel()]
-    return np.concatenate(parts), float(loss)
+    return np.concatenate(parts), float(loss.detach())
 
 
 def test_unet_gradients_vs_torch():
