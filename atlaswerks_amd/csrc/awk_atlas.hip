@@ -82,6 +82,19 @@ struct Subject {
 
 constexpr int kHistCap = 1 << 16;
 
+// Subjects' updates after the mean are independent: they run on up to kLanes
+// streams (lane 0 is the iteration's own stream), each lane with its own operator
+// spectrum and reduction scratch; the operator tables are shared.
+constexpr int kLanes = 3;
+struct Lane {
+  awk::FluidKernel k;  // copy of the atlas operator with its own spectrum
+  hipStream_t st = nullptr;
+  hipEvent_t join = nullptr;
+  double *partials = nullptr;
+  float *maxpart = nullptr;
+  float *vbuf = nullptr;
+};
+
 }  // namespace
 
 struct awk_atlas {
@@ -102,7 +115,8 @@ struct awk_atlas {
   long long host_cursor = 0;
   float *vbuf = nullptr;  // 3 planes: velocity of a step-size iteration
   hipStream_t as = nullptr;
-  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_fork = nullptr;
+  std::vector<Lane> lanes;  // lanes 1..; lane 0 is the atlas operator itself
   bool use_graph = true;
   bool tables_dirty = true;
   hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -130,6 +144,14 @@ static void atlas_free(awk_atlas *a) {
                a->hist, a->cursor, a->vbuf};
   for (void *p : b)
     if (p) (void)hipFree(p);
+  for (Lane &L : a->lanes) {
+    void *lb[] = {L.k.spec, L.partials, L.maxpart, L.vbuf};
+    for (void *p : lb)
+      if (p) (void)hipFree(p);
+    if (L.join) (void)hipEventDestroy(L.join);
+    if (L.st) (void)hipStreamDestroy(L.st);
+  }
+  if (a->ev_fork) (void)hipEventDestroy(a->ev_fork);
   if (a->ev_in) (void)hipEventDestroy(a->ev_in);
   if (a->ev_out) (void)hipEventDestroy(a->ev_out);
   if (a->as) (void)hipStreamDestroy(a->as);
@@ -158,31 +180,43 @@ static awk_status atlas_iteration(awk_atlas *a, hipStream_t s) {
   const int N = a->nx * a->ny * a->nz;
   const int nxt = 1 - a->cur;
   if ((st = atlas_mean_stage(a, s))) return st;
+  const int nl = 1 + (int)a->lanes.size();
+  if (nl > 1) {
+    AWK_HIP(hipEventRecord(a->ev_fork, s));
+    for (Lane &L : a->lanes) AWK_HIP(hipStreamWaitEvent(L.st, a->ev_fork, 0));
+  }
   for (int si = 0; si < a->ns; ++si) {
     Subject &sb = a->subj[si];
     const float *h = sb.h[a->cur];
     float *hn = sb.h[nxt];
+    // this subject's lane: stream, operator spectrum, reduction scratch
+    const int li = si % nl;
+    FluidKernel &k = li ? a->lanes[li - 1].k : a->k;
+    const hipStream_t ls = li ? a->lanes[li - 1].st : s;
+    double *partials = li ? a->lanes[li - 1].partials : a->partials;
+    float *maxpart = li ? a->lanes[li - 1].maxpart : a->maxpart;
+    float *vbuf = li ? a->lanes[li - 1].vbuf : a->vbuf;
     XFwdArgs fa;
     std::memset(&fa, 0, sizeof(fa));
     fa.mode = XLOAD_FORCE;
     fa.idef = sb.idef;
     fa.i1 = a->mean;
     for (int c = 0; c < 3; ++c) fa.sp[c] = a->sp[c];
-    fa.partials = a->partials;
+    fa.partials = partials;
     int nb = 0;
-    if ((st = fk_xfwd(a->k, fa, s, &nb))) return st;
+    if ((st = fk_xfwd(k, fa, ls, &nb))) return st;
     ReduceArgs ra;
-    ra.partials = a->partials;
-    ra.n = (a->ny * a->nz + a->k.rt - 1) / a->k.rt;
+    ra.partials = partials;
+    ra.n = (a->ny * a->nz + k.rt - 1) / k.rt;
     ra.scale = (double)a->sp[0] * a->sp[1] * a->sp[2];
     ra.energy_out = a->esub + si;
     ra.prev = sb.prev;
     ra.have_prev = sb.have_prev;
     ra.step = sb.step;
     ra.auto_reduce = 0;
-    if ((st = fk_ypass(a->k, false, s, &ra))) return st;
-    if ((st = fk_zsolve(a->k, true, s))) return st;
-    if ((st = fk_ypass(a->k, true, s, nullptr))) return st;
+    if ((st = fk_ypass(k, false, ls, &ra))) return st;
+    if ((st = fk_zsolve(k, true, ls))) return st;
+    if ((st = fk_ypass(k, true, ls, nullptr))) return st;
     XInvArgs ia;
     std::memset(&ia, 0, sizeof(ia));
     for (int c = 0; c < 3; ++c) ia.sp[c] = a->sp[c];
@@ -192,24 +226,24 @@ static awk_status atlas_iteration(awk_atlas *a, hipStream_t s) {
       // composeHV and, when the inverse is tracked, composeVHInv
       // (GreedyIteratorCPU::updateDeformation, GreedyIteratorCPU.cxx:82-95)
       ia.mode = sb.update_step ? XSTORE_MAXNORM : XSTORE_PLAIN;
-      for (int c = 0; c < 3; ++c) ia.out[c] = a->vbuf + (size_t)c * N;
-      ia.maxpart = a->maxpart;
-      if ((st = fk_xinv(a->k, ia, s, &nb))) return st;
+      for (int c = 0; c < 3; ++c) ia.out[c] = vbuf + (size_t)c * N;
+      ia.maxpart = maxpart;
+      if ((st = fk_xinv(k, ia, ls, &nb))) return st;
       if (sb.update_step) {
-        AWK_LAUNCH(k_atlas_step, dim3(1), dim3(256), 0, s, a->maxpart, nb, a->max_pert, sb.step);
+        AWK_LAUNCH(k_atlas_step, dim3(1), dim3(256), 0, ls, maxpart, nb, a->max_pert, sb.step);
         AWK_LAUNCH_CHECK("k_atlas_step");
       }
-      const float *vx = a->vbuf, *vy = a->vbuf + N, *vz = a->vbuf + 2 * (size_t)N;
+      const float *vx = vbuf, *vy = vbuf + N, *vz = vbuf + 2 * (size_t)N;
       if ((st = launch_compose_hv(hn, hn + N, hn + 2 * (size_t)N, h, h + N, h + 2 * (size_t)N, vx,
                                   vy, vz, a->nx, a->ny, a->nz, a->sp, 1.f, sb.step, 0,
-                                  AWK_BG_PARTIAL_ID, s)))
+                                  AWK_BG_PARTIAL_ID, ls)))
         return st;
       if (inv) {
         const float *hi = sb.hinv[a->cur];
         float *hin = sb.hinv[nxt];
         if ((st = launch_compose_vh(hin, hin + N, hin + 2 * (size_t)N, vx, vy, vz, hi, hi + N,
                                     hi + 2 * (size_t)N, a->nx, a->ny, a->nz, a->sp, sb.step, 1,
-                                    AWK_BG_PARTIAL_ZERO, s)))
+                                    AWK_BG_PARTIAL_ZERO, ls)))
           return st;
       }
     } else {
@@ -219,9 +253,14 @@ static awk_status atlas_iteration(awk_atlas *a, hipStream_t s) {
         ia.hnew[c] = hn + (size_t)c * N;
       }
       ia.step = sb.step;
-      if ((st = fk_xinv(a->k, ia, s, &nb))) return st;
+      if ((st = fk_xinv(k, ia, ls, &nb))) return st;
     }
   }
+  if (nl > 1)
+    for (Lane &L : a->lanes) {
+      AWK_HIP(hipEventRecord(L.join, L.st));
+      AWK_HIP(hipStreamWaitEvent(s, L.join, 0));
+    }
   AWK_LAUNCH(k_atlas_energy, dim3(1), dim3(64), 0, s, a->esub, a->w_dev, a->ns, a->hist, a->cursor,
              kHistCap);
   AWK_LAUNCH_CHECK("k_atlas_energy");
@@ -301,6 +340,21 @@ awk_status awk_atlas_create(awk_atlas_t *out, int nx, int ny, int nz, float spx,
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&a->as, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&a->ev_in, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&a->ev_out, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&a->ev_fork, hipEventDisableTiming);
+  const int extra = std::min(n_subjects, kLanes) - 1;
+  for (int l = 0; l < extra && e == hipSuccess; ++l) {
+    a->lanes.emplace_back();
+    Lane &L = a->lanes.back();
+    L.k = a->k;  // shares the operator tables
+    L.k.spec = nullptr;
+    L.k.eig = nullptr;
+    e = hipMalloc(&L.k.spec, 3 * a->k.spec_plane() * sizeof(float2));
+    if (e == hipSuccess) e = hipMalloc(&L.partials, (size_t)nblk * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&L.maxpart, (size_t)nblk * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&L.vbuf, 3 * N * sizeof(float));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&L.join, hipEventDisableTiming);
+  }
   if (e != hipSuccess) {
     st = hip_check(e, "awk_atlas_create: allocation");
     atlas_free(a);
