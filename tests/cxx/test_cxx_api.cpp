@@ -4,12 +4,15 @@
 //  * DiffOperRoutineTests.cxx:161-196 -- GPU L(L^-1 g) = g, Sum |d|^2 < 0.1
 //  * cudaHField3DUtilsRoutineTests.cxx:61-72 -- <apply(I1,h), I2> = <I1, splat(I2,h)>
 //  * GreedyIteratorGPU drives a few iterations with decreasing energy.
+//  * LDMMIteratorGPU (IterateOrig) drives a few iterations with decreasing total
+//    energy; computeAlphasOnly fills GetAlpha; LDMMAtlasGPU builds a finite mean.
 #include <cmath>
 #include <cstdio>
 #include <vector>
 
 #include "awk/DiffOperGPU.h"
 #include "awk/GreedyIteratorGPU.h"
+#include "awk/LDMMIteratorGPU.h"
 #include "awk/cudaHField3DUtils.h"
 #include "awk/cudaSplat.h"
 
@@ -123,6 +126,50 @@ int main() {
     std::vector<double> e = data.Energies();
     EXPECT(e.size() == 10 && e.back() < e.front(), "energy did not decrease");
     std::printf("greedy energy %g -> %g, step %g\n", e.front(), e.back(), it.StepSize());
+    // ---- LDMM (IterateOrig): total energy decreases; alphas; a two-subject atlas
+    {
+      const unsigned T = 4;
+      std::vector<cplVector3DArray> v(T), v2(T);
+      for (unsigned t = 0; t < T; ++t) {
+        allocateDeviceVector3DArray(v[t], N);
+        allocateDeviceVector3DArray(v2[t], N);
+        for (int c = 0; c < 3; ++c) {
+          (void)hipMemset(v[t].elementArray(c), 0, N * 4);
+          (void)hipMemset(v2[t].elementArray(c), 0, N * 4);
+        }
+      }
+      LDMMIteratorParam lp;
+      lp.Sigma() = 0.5f;
+      lp.MaxPert() = 0.2f;
+      LDMMIteratorGPU li(SizeType(nx, ny, nz), SpacingType(1, 1, 1), T, lp);
+      LDMMDeformationDataGPU ld(d1, d2, v);
+      li.UpdateStepSizeNextIteration();
+      li.Iterate(ld);  // step-size call (energies recorded from here on)
+      for (int k = 0; k < 5; ++k) li.Iterate(ld);
+      std::vector<double> le = ld.Energies();  // 6 x (image, vector, total)
+      EXPECT(le.size() == 18 && le[17] < le[5], "LDMM total energy did not decrease");
+      std::printf("LDMM total energy %g -> %g, step %g\n", le[5], le[17], li.StepSize());
+      li.Iterate(ld, true);
+      std::vector<float> al(N);
+      (void)hipMemcpy(al.data(), li.GetAlpha(0), N * 4, hipMemcpyDeviceToHost);
+      bool finite = true;
+      for (float q : al) finite = finite && std::isfinite(q);
+      EXPECT(finite, "LDMM alphas not finite");
+      LDMMAtlasGPU atlas(SizeType(nx, ny, nz), SpacingType(1, 1, 1), T, 2, lp);
+      atlas.SetSubject(0, d1, 0.5f, v);
+      atlas.SetSubject(1, d2, 0.5f, v2);
+      atlas.Iterate(3);
+      atlas.GetMean(o1);
+      (void)hipMemcpy(al.data(), o1, N * 4, hipMemcpyDeviceToHost);
+      double msum = 0.0;
+      for (float q : al) msum += q;
+      EXPECT(std::isfinite(msum), "LDMM atlas mean not finite");
+      std::printf("LDMM atlas mean sum %g\n", msum);
+      for (unsigned t = 0; t < T; ++t) {
+        freeDeviceVector3DArray(v[t]);
+        freeDeviceVector3DArray(v2[t]);
+      }
+    }
     freeDeviceVector3DArray(hw);
     freeDeviceVector3DArray(h);
     (void)hipFree(d1); (void)hipFree(d2); (void)hipFree(o1); (void)hipFree(o2);

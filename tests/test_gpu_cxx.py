@@ -1,6 +1,6 @@
 """The C++ drop-in API (include/awk/*.h over the C-ABI) on the GPU: the reference's
 DiffOperRoutineTests GPU round trip, the apply/splat adjoint identity of
-cudaHField3DUtilsRoutineTests, and GreedyIteratorGPU iterations."""
+cudaHField3DUtilsRoutineTests, GreedyIteratorGPU iterations, and LDMMIteratorGPU / LDMMAtlasGPU."""
 import os
 import subprocess
 
