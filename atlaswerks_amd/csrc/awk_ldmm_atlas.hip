@@ -14,6 +14,7 @@
 // iteration's update (the reference runs it twice).  Subjects run in stream order on one shared iterator
 // state; each keeps its own JT_t / JacDet_t, step and energy.  No host
 // synchronisation; the mean and the energies are summed in subject order.
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <string>
@@ -73,10 +74,22 @@ struct LaSubject {
   bool set = false;
 };
 
+// Subjects are independent within a sweep: they run on up to kLaLanes streams,
+// lane 0 on the caller's stream with the atlas engine, the others each with an
+// engine (operator, hfields, scratch) of their own.
+constexpr int kLaLanes = 3;
+struct LaLane {
+  awk_ldmm eng;
+  hipStream_t st = nullptr;
+  hipEvent_t join = nullptr;
+};
+
 }  // namespace
 
 struct awk_ldmm_atlas {
-  awk_ldmm eng;  // shared iterator state (operator, hfields, scratch), no JT/JacDet
+  awk_ldmm eng;  // lane 0's iterator state (operator, hfields, scratch), no JT/JacDet
+  std::vector<LaLane> lanes;  // lanes 1..
+  hipEvent_t fork = nullptr;
   int S = 0;
   bool jscale = true;
   std::vector<LaSubject> subs;
@@ -92,6 +105,12 @@ struct awk_ldmm_atlas {
 
 static void la_free(awk_ldmm_atlas *a) {
   ldmm_release(&a->eng);
+  for (LaLane &L : a->lanes) {
+    ldmm_release(&L.eng);
+    if (L.join) (void)hipEventDestroy(L.join);
+    if (L.st) (void)hipStreamDestroy(L.st);
+  }
+  if (a->fork) (void)hipEventDestroy(a->fork);
   for (LaSubject &s : a->subs) {
     void *b[] = {s.I, s.jt, s.jd, s.step};
     for (void *p : b)
@@ -102,14 +121,40 @@ static void la_free(awk_ldmm_atlas *a) {
     if (b) (void)hipFree(b);
 }
 
+// lane of subject k: engine and stream
+static awk_ldmm *lane_eng(awk_ldmm_atlas *a, int k) {
+  const int li = k % (1 + (int)a->lanes.size());
+  return li ? &a->lanes[li - 1].eng : &a->eng;
+}
+static hipStream_t lane_stream(awk_ldmm_atlas *a, int k, hipStream_t s) {
+  const int li = k % (1 + (int)a->lanes.size());
+  return li ? a->lanes[li - 1].st : s;
+}
+static awk_status lanes_fork(awk_ldmm_atlas *a, hipStream_t s) {
+  if (a->lanes.empty()) return AWK_OK;
+  AWK_HIP(hipEventRecord(a->fork, s));
+  for (LaLane &L : a->lanes) AWK_HIP(hipStreamWaitEvent(L.st, a->fork, 0));
+  return AWK_OK;
+}
+static awk_status lanes_join(awk_ldmm_atlas *a, hipStream_t s) {
+  for (LaLane &L : a->lanes) {
+    AWK_HIP(hipEventRecord(L.join, L.st));
+    AWK_HIP(hipStreamWaitEvent(s, L.join, 0));
+  }
+  return AWK_OK;
+}
+
 // backward sweep of every subject, then the mean
 static awk_status la_refresh(awk_ldmm_atlas *a, hipStream_t s) {
   const int N = a->eng.nx * a->eng.ny * a->eng.nz;
+  awk_status st = lanes_fork(a, s);
+  if (st) return st;
   for (int k = 0; k < a->S; ++k) {
     LaSubject &sb = a->subs[k];
-    awk_status st = ldmm_orig_backward(&a->eng, sb.I, sb.v.data(), sb.jt, sb.jd, s);
+    st = ldmm_orig_backward(lane_eng(a, k), sb.I, sb.v.data(), sb.jt, sb.jd, lane_stream(a, k, s));
     if (st) return st;
   }
+  if ((st = lanes_join(a, s))) return st;
   AWK_LAUNCH(k_la_mean, dim3((N + kLaBlock - 1) / kLaBlock), dim3(kLaBlock), 0, s, a->mean, a->jt0,
              a->jd0, a->w, a->S, N, a->jscale ? 1 : 0);
   AWK_LAUNCH_CHECK("k_la_mean");
@@ -143,6 +188,16 @@ awk_status awk_ldmm_atlas_create(awk_ldmm_atlas_t *out, int nx, int ny, int nz, 
   const float sp[3] = {spx, spy, spz};
   awk_status st = ldmm_init(&a->eng, nx, ny, nz, sp, alpha, beta, gamma, n_time_steps, sigma,
                             max_pert, false);
+  hipError_t le = hipSuccess;
+  if (!st) le = hipEventCreateWithFlags(&a->fork, hipEventDisableTiming);
+  for (int l = 0; !st && le == hipSuccess && l < std::min(n_subjects, kLaLanes) - 1; ++l) {
+    a->lanes.emplace_back();
+    LaLane &L = a->lanes.back();
+    st = ldmm_init(&L.eng, nx, ny, nz, sp, alpha, beta, gamma, n_time_steps, sigma, max_pert, false);
+    if (!st) le = hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking);
+    if (!st && le == hipSuccess) le = hipEventCreateWithFlags(&L.join, hipEventDisableTiming);
+  }
+  if (!st && le != hipSuccess) st = hip_check(le, "awk_ldmm_atlas_create: lanes");
   const size_t N = (size_t)nx * ny * nz;
   const int S = n_subjects;
   a->S = S;
@@ -219,7 +274,10 @@ awk_status awk_ldmm_atlas_set_gradient(awk_ldmm_atlas_t a, int use_orig) {
 
 awk_status awk_ldmm_atlas_set_splat(awk_ldmm_atlas_t a, int mode) {
   if (!a) return fail(AWK_ERR_INVALID, "awk_ldmm_atlas_set_splat: null handle");
-  return awk_ldmm_set_splat(&a->eng, mode);
+  awk_status st = awk_ldmm_set_splat(&a->eng, mode);
+  for (LaLane &L : a->lanes)
+    if (!st) st = awk_ldmm_set_splat(&L.eng, mode);
+  return st;
 }
 
 awk_status awk_ldmm_atlas_update_step_next(awk_ldmm_atlas_t a) {
@@ -253,22 +311,27 @@ awk_status awk_ldmm_atlas_iterate(awk_ldmm_atlas_t a, int n_iter, double *energy
   hipStream_t s = as_stream(stream);
   for (int it = 0; it < n_iter; ++it) {
     if (!a->current && (st = la_refresh(a, s))) return st;
+    if ((st = lanes_fork(a, s))) return st;
     for (int k = 0; k < a->S; ++k) {
       LaSubject &sb = a->subs[k];
+      awk_ldmm *eng = lane_eng(a, k);
+      const hipStream_t ls = lane_stream(a, k, s);
       for (int pass = sb.update_step ? 0 : 1; pass < 2; ++pass) {
-        // pass 0: LDMMAtlasThreadGPU::BeginImageIteration's step-size call
+        // pass 0: LDMMAtlasThreadGPU::BeginImageIteration's step-size call (its
+        // energies go to the lane's own unused slot)
         const bool step_call = pass == 0;
-        double *e = step_call ? a->energy : a->esub + 3 * (size_t)k;
+        double *e = step_call ? eng->energy : a->esub + 3 * (size_t)k;
         if (a->orig)
-          st = ldmm_orig_forward(&a->eng, a->mean, sb.I, sb.v.data(), sb.jt, sb.jd, sb.step,
-                                 step_call, nullptr, e, step_call ? 0 : vec_energy, s);
+          st = ldmm_orig_forward(eng, a->mean, sb.I, sb.v.data(), sb.jt, sb.jd, sb.step, step_call,
+                                 nullptr, e, step_call ? 0 : vec_energy, ls);
         else
-          st = ldmm_new_iterate(&a->eng, a->mean, sb.I, sb.v.data(), sb.step, step_call, nullptr, e,
-                                step_call ? 0 : vec_energy, s);
+          st = ldmm_new_iterate(eng, a->mean, sb.I, sb.v.data(), sb.step, step_call, nullptr, e,
+                                step_call ? 0 : vec_energy, ls);
         if (st) return st;
       }
       sb.update_step = false;
     }
+    if ((st = lanes_join(a, s))) return st;
     a->current = false;
     AWK_LAUNCH(k_la_energy, dim3(1), dim3(64), 0, s, a->esub, a->w, a->S,
                energy_dev ? energy_dev + 3 * (size_t)it : a->energy);
