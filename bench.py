@@ -329,7 +329,7 @@ def main():
         dev = torch.device("cuda:0")
         fn = {"ldmm": lambda: bench_ldmm(dev, subjects=2, reps=3),
               "ldmm_new": lambda: bench_ldmm(dev, subjects=2, reps=3, orig=False),
-              "ldmm_atlas": lambda: bench_ldmm_atlas(dev, n_iter=4),
+              "ldmm_atlas": lambda: bench_ldmm_atlas(dev),
               "atlas": lambda: bench_atlas(dev, reps=3), "unet": lambda: bench_unet(dev),
               "warm_start": lambda: bench_warm_start(dev)}[args.only]
         print(json.dumps({args.only: fn()}), flush=True)
