@@ -140,6 +140,19 @@ __device__ __forceinline__ int wrap_mod(int r, int b) {
   return m < 0 ? m + b : m;
 }
 
+// The nested trilinear lerp of corners c0..c7 (HField3DUtils::trilerp's corner
+// order: c0 (x0,y0,z0), c1 (x1,y0,z0), c2 (x1,y1,z0), c3 (x0,y1,z0), c4 (x0,y1,z1),
+// c5 (x1,y1,z1), c6 (x1,y0,z1), c7 (x0,y0,z1)): z first, then y, then x, each as
+// one multiply and one explicit fma, so every caller rounds identically whatever
+// contraction the compiler would choose at its site.
+__device__ __forceinline__ float lerp8(float c0, float c1, float c2, float c3, float c4, float c5,
+                                       float c6, float c7, float t, float u, float v) {
+  const float omt = 1.f - t, omu = 1.f - u, omv = 1.f - v;
+  const float a00 = fmaf(c7, v, c0 * omv), a01 = fmaf(c4, v, c3 * omv);
+  const float a10 = fmaf(c6, v, c1 * omv), a11 = fmaf(c5, v, c2 * omv);
+  return fmaf(t, fmaf(u, a11, omu * a10), omt * fmaf(u, a01, omu * a00));
+}
+
 // Vector trilerp of a SoA field at (x, y, z) in voxel coordinates, following
 // HField3DUtils::trilerp (UtilitiesDataTypes/HField3DUtils.txx:1969-2300):
 // corner order v0..v7, nested lerp, CPU semantics for PARTIAL_ID (out-of-volume
@@ -205,8 +218,7 @@ __device__ __forceinline__ float3 trilerp3g(const L &fx, const L &fy, const L &f
       float r[3];
 #pragma unroll
       for (int q = 0; q < 3; ++q)
-        r[q] = omt * (omu * (c[0][q] * omv + c[7][q] * v) + u * (c[3][q] * omv + c[4][q] * v)) +
-               t * (omu * (c[1][q] * omv + c[6][q] * v) + u * (c[2][q] * omv + c[5][q] * v));
+        r[q] = lerp8(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], c[5][q], c[6][q], c[7][q], t, u, v);
       return make_float3(r[0], r[1], r[2]);
     }
     const int xc0 = min(max(x0, 0), d.nx - 1), xc1 = xin1 ? x0 + 1 : xc0;
@@ -249,8 +261,7 @@ __device__ __forceinline__ float3 trilerp3g(const L &fx, const L &fy, const L &f
   float r[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q)
-    r[q] = omt * (omu * (c[0][q] * omv + c[7][q] * v) + u * (c[3][q] * omv + c[4][q] * v)) +
-           t * (omu * (c[1][q] * omv + c[6][q] * v) + u * (c[2][q] * omv + c[5][q] * v));
+    r[q] = lerp8(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], c[5][q], c[6][q], c[7][q], t, u, v);
   return make_float3(r[0], r[1], r[2]);
 }
 

@@ -322,9 +322,10 @@ k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
             v0 = t.x; v1 = t.y; v2 = sm[x * LS + 3 * p + 1].y;
           }
           const int y = row_[u] % ny, z = row_[u] / ny;
-          px[u] = (float)x + (v0 * st) * isx;
-          py[u] = (float)y + (v1 * st) * isy;
-          pz[u] = (float)z + (v2 * st) * isz;
+          // explicit fma: the same rounding wherever the position is formed
+          px[u] = fmaf(v0 * st, isx, (float)x);
+          py[u] = fmaf(v1 * st, isy, (float)y);
+          pz[u] = fmaf(v2 * st, isz, (float)z);
         }
       }
       float3 hn[XU];

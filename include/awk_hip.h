@@ -50,7 +50,8 @@ enum {
 
 /* Launch-shape knobs chosen by the autotuner (atlaswerks_amd/autotune.py):
  * "apply_vpt", "compose_vpt" (voxels per thread: 1, 2, 4), "xinv_unroll"
- * (voxels per step of the fused inverse-x epilogue: 1, 2, 4).  Process-wide;
+ * (voxels per step of the fused inverse-x epilogue: 1, 2, 4), "splat_tile"
+ * (0 or 1: the fixed-point splat through per-block LDS windows).  Process-wide;
  * results do not depend on them.  No counterpart in the reference. */
 awk_status awk_tune_set(const char *name, int value);
 awk_status awk_tune_get(const char *name, int *value);
