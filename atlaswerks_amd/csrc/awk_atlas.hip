@@ -52,7 +52,7 @@ __global__ void k_atlas_step(const float *__restrict__ maxpart, int n, float max
                              float *__restrict__ step) {
   __shared__ float sm[256];
   float m = 0.f;
-  for (int i = threadIdx.x; i < n; i += 256) m = fmaxf(m, maxpart[i]);
+  m = strided_fold(maxpart, n, (int)threadIdx.x, 256, m, FoldMax());
   sm[threadIdx.x] = m;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {

@@ -52,6 +52,31 @@ LaunchEvents *&launch_events();
       hipLaunchKernelGGL(K, G, B, L, S, __VA_ARGS__);                                      \
   } while (0)
 
+// One thread's part of a block-level fold: op over p[start], p[start + stride], ...
+// in that order (so a sum rounds exactly as the plain loop would), with the loads
+// issued 16 at a time.  A block that folds thousands of partials while the rest of
+// the grid saturates HBM then waits a few memory latencies, not one per element.
+template <class T, class Op>
+__device__ __forceinline__ T strided_fold(const T *__restrict__ p, int n, int start, int stride,
+                                          T acc, Op op) {
+  int i = start;
+  for (; i + 15 * stride < n; i += 16 * stride) {
+    T v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = p[i + k * stride];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc = op(acc, v[k]);
+  }
+  for (; i < n; i += stride) acc = op(acc, p[i]);
+  return acc;
+}
+struct FoldAdd {
+  __device__ double operator()(double a, double b) const { return a + b; }
+};
+struct FoldMax {
+  __device__ float operator()(float a, float b) const { return fmaxf(a, b); }
+};
+
 // Launch-shape knobs (set by the autotuner through awk_tune_set).
 struct Tuning {
   int apply_vpt = 1;    // voxels per thread, apply

@@ -68,7 +68,7 @@ __device__ __forceinline__ void line_targets(int line, int r0, int &rowA, int &c
 // GreedyDeformationData::AddEnergy (GreedyDeformationData.cxx:162-184).
 __device__ __forceinline__ void energy_block(const ReduceArgs &ra, double *tree, int bdim) {
   double acc = 0.0;
-  for (int i = threadIdx.x; i < ra.n; i += bdim) acc += ra.partials[i];
+  acc = strided_fold(ra.partials, ra.n, (int)threadIdx.x, bdim, acc, FoldAdd());
   tree[threadIdx.x] = acc;
   __syncthreads();
   for (int w = 512; w > 0; w >>= 1) {

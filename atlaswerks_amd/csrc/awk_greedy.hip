@@ -29,7 +29,7 @@ __global__ void __launch_bounds__(256)
 k_step_from_max(const float *__restrict__ maxpart, int n, float max_pert, float *__restrict__ step) {
   __shared__ float s[256];
   float m = 0.f;
-  for (int i = threadIdx.x; i < n; i += 256) m = fmaxf(m, maxpart[i]);
+  m = strided_fold(maxpart, n, (int)threadIdx.x, 256, m, FoldMax());
   s[threadIdx.x] = m;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {

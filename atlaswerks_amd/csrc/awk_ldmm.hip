@@ -167,7 +167,7 @@ __global__ void k_ldmm_image_energy(const double *__restrict__ partials, int n, 
                                     double *__restrict__ energy) {
   __shared__ double s[256];
   double a = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) a += partials[i];
+  a = strided_fold(partials, n, (int)threadIdx.x, 256, a, FoldAdd());
   s[threadIdx.x] = a;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
@@ -187,7 +187,7 @@ __global__ void k_ldmm_vec_energy(const double *__restrict__ partials, int n, do
                                   double *__restrict__ energy) {
   __shared__ double s[256];
   double a = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) a += partials[i];
+  a = strided_fold(partials, n, (int)threadIdx.x, 256, a, FoldAdd());
   s[threadIdx.x] = a;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
@@ -205,7 +205,7 @@ __global__ void k_ldmm_step(const float *__restrict__ maxpart, int n, float max_
                             float *__restrict__ step) {
   __shared__ float s[256];
   float m = 0.f;
-  for (int i = threadIdx.x; i < n; i += 256) m = fmaxf(m, maxpart[i]);
+  m = strided_fold(maxpart, n, (int)threadIdx.x, 256, m, FoldMax());
   s[threadIdx.x] = m;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
@@ -341,7 +341,7 @@ __global__ void k_orig_image_energy(const double *__restrict__ partials, int n, 
                                     double *__restrict__ energy) {
   __shared__ double s[256];
   double a = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) a += partials[i];
+  a = strided_fold(partials, n, (int)threadIdx.x, 256, a, FoldAdd());
   s[threadIdx.x] = a;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
