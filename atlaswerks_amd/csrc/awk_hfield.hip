@@ -116,7 +116,13 @@ k_apply(float *__restrict__ out, const float *__restrict__ img, const float *__r
 #pragma unroll
   for (int v = 0; v < VPT; ++v) {
     const int i = i0 + v * kBlock;
-    if (i < N) { px[v] = hx[i]; py[v] = hy[i]; pz[v] = hz[i]; }
+    if (i < N) {
+      // h is read once: streamed past the memory-side cache, which keeps the
+      // output for its reader
+      px[v] = __builtin_nontemporal_load(hx + i);
+      py[v] = __builtin_nontemporal_load(hy + i);
+      pz[v] = __builtin_nontemporal_load(hz + i);
+    }
   }
   float r[VPT];
   const BufLoad ib = make_buf(img, d.nx * d.ny * d.nz);
