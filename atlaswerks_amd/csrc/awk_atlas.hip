@@ -201,7 +201,7 @@ static awk_status atlas_iteration(awk_atlas *a, hipStream_t s) {
     fa.mode = XLOAD_FORCE;
     fa.idef = sb.idef;
     fa.i1 = a->mean;
-    for (int c = 0; c < 3; ++c) fa.sp[c] = a->sp[c];
+    xfwd_spacing(fa, a->sp);
     fa.partials = partials;
     int nb = 0;
     if ((st = fk_xfwd(k, fa, ls, &nb))) return st;

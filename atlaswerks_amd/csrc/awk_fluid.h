@@ -39,9 +39,19 @@ struct XFwdArgs {
   const float *idef;    // FORCE: deformed image
   const float *i1;      // FORCE: target image
   float sp[3];          // FORCE: spacing for the gradient
+  float rsp[3], r2sp[3];  // 1 / sp and 1 / (2 sp), rounded (set by xfwd_spacing)
   double *partials;     // FORCE: one double per block (may be null)
   long long *acc;       // GRADMUL_FX: fixed-point multiplier, cleared after reading
 };
+
+// The gradient's spacing and the reciprocals of its two divisors.
+inline void xfwd_spacing(XFwdArgs &a, const float sp[3]) {
+  for (int c = 0; c < 3; ++c) {
+    a.sp[c] = sp[c];
+    a.rsp[c] = 1.f / sp[c];
+    a.r2sp[c] = 1.f / (2.f * sp[c]);
+  }
+}
 
 // Energy reduction of the greedy step (done by block 0 of the forward y pass).
 struct ReduceArgs {

@@ -209,7 +209,7 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
   fa.mode = XLOAD_FORCE;
   fa.idef = g->idef;
   fa.i1 = I1;
-  for (int c = 0; c < 3; ++c) fa.sp[c] = g->sp[c];
+  xfwd_spacing(fa, g->sp);
   fa.partials = g->partials;
   int nb = 0;
   if ((st = fk_xfwd(g->k, fa, s, &nb))) return st;

@@ -447,7 +447,7 @@ static awk_status solve_update(awk_ldmm *l, float *const v[], int t, const float
   fa.idef = src;
   fa.i1 = m;
   fa.acc = const_cast<long long *>(acc);
-  for (int c = 0; c < 3; ++c) fa.sp[c] = l->sp[c];
+  xfwd_spacing(fa, l->sp);
   if ((st = fk_xfwd(l->k, fa, s, &nb))) return st;
   if ((st = fk_ypass(l->k, false, s, nullptr))) return st;
   if ((st = fk_zsolve(l->k, true, s))) return st;
