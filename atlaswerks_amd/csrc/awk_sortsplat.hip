@@ -10,8 +10,9 @@
 //        k_scan_*      exclusive scan of the counts (3 kernels, integer: exact)
 //        k_radix_scatter  stable ranks from wave ballots + per-wave digit counts
 //   3. k_segment_sum   one thread per segment head, sequential sum in sorted
-//      order (float): the result is a pure function of the inputs, identical
-//      run to run and to a CPU sequential sum in stable-sorted order.
+//      order (float) over LDS tiles of the sorted pairs: the result is a pure
+//      function of the inputs, identical run to run and to a CPU sequential sum
+//      in stable-sorted order.
 // The reference sums with a parallel CUDPP segmented scan; the keys are the
 // bit-exact target, the sums match to rounding.
 #include <algorithm>
@@ -81,10 +82,29 @@ k_radix_hist(const unsigned *__restrict__ keys, long long M, int shift, int ntil
   cnt[threadIdx.x] = 0;
   __syncthreads();
   const long long base = (long long)t * kTile;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  unsigned kr[kSortItems];
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     const long long e = base + r * kSortBlock + threadIdx.x;
-    if (e < M) atomicAdd(&cnt[(keys[e] >> shift) & (kRadix - 1)], 1u);
+    kr[r] = e < M ? keys[e] : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const long long e = base + r * kSortBlock + threadIdx.x;
+    const bool valid = e < M;
+    const unsigned d = (kr[r] >> shift) & (kRadix - 1);
+    // lanes with equal digits (8 ballots) add once, from their lowest lane: no
+    // serialised atomics when a wave's keys share a digit
+    unsigned long long same = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < kRadixBits; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const unsigned long long m = __ballot(bit);
+      same &= bit ? m : ~m;
+    }
+    if (valid && (same & lt_mask) == 0ull) atomicAdd(&cnt[d], (unsigned)__popcll(same));
   }
   __syncthreads();
   hist[(size_t)threadIdx.x * ntiles + t] = cnt[threadIdx.x];
@@ -163,26 +183,63 @@ k_scan_down(unsigned *__restrict__ a, long long n, const unsigned *__restrict__ 
 
 // Stable scatter of one pass.  Elements are visited in their original order
 // (round r, thread t <-> element r*256 + t of the tile); within a wave the rank
-// among equal digits comes from 8 ballots, across waves from LDS counts.
+// among equal digits comes from 8 ballots, across waves from LDS counts.  Each
+// element first goes to its tile-local sorted position in LDS (digit-major, stable
+// within a digit); the tile is then written out in that order, so consecutive
+// threads store consecutive addresses of one digit's run instead of scattering
+// every element.  The tile's digit counts are differences of consecutive entries
+// of the exclusive scan (digit-major: entry d * ntiles + t), the last against M.
 __global__ void __launch_bounds__(kSortBlock)
 k_radix_scatter(const unsigned *__restrict__ kin, const float *__restrict__ vin,
                 unsigned *__restrict__ kout, float *__restrict__ vout, long long M, int shift,
                 int ntiles, const unsigned *__restrict__ offs) {
   constexpr int kWaves = kSortBlock / 64;
-  __shared__ unsigned base_off[kRadix];
+  static_assert(kSortBlock == kRadix, "one thread per digit");
+  __shared__ unsigned gbase[kRadix];   // global start of this tile's run of each digit
+  __shared__ unsigned lstart[kRadix];  // tile-local start of each digit
+  __shared__ unsigned lrun[kRadix];    // next tile-local slot of each digit
   __shared__ unsigned wcnt[kWaves][kRadix];
+  __shared__ unsigned lk[kTile];
+  __shared__ float lv[kTile];
   const int t = blockIdx.x;
-  base_off[threadIdx.x] = offs[(size_t)threadIdx.x * ntiles + t];
+  const long long nh = (long long)kRadix * ntiles;
+  const long long hi = (long long)threadIdx.x * ntiles + t;
+  const unsigned g0 = offs[hi];
+  const unsigned cnt = (hi + 1 < nh ? offs[hi + 1] : (unsigned)M) - g0;
+  gbase[threadIdx.x] = g0;
+  // exclusive scan of the tile's 256 digit counts
+  lstart[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int off = 1; off < kRadix; off <<= 1) {
+    const unsigned v = (int)threadIdx.x >= off ? lstart[threadIdx.x - off] : 0u;
+    __syncthreads();
+    lstart[threadIdx.x] += v;
+    __syncthreads();
+  }
+  const unsigned excl = lstart[threadIdx.x] - cnt;
+  __syncthreads();
+  lstart[threadIdx.x] = excl;
+  lrun[threadIdx.x] = excl;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const long long tile0 = (long long)t * kTile;
+  // the whole tile's loads first: one memory latency per tile, not per round
+  unsigned kr[kSortItems];
+  float vr[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const long long e = tile0 + r * kSortBlock + threadIdx.x;
+    kr[r] = e < M ? kin[e] : 0u;
+    vr[r] = e < M ? vin[e] : 0.f;
+  }
+#pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     for (int q = threadIdx.x; q < kWaves * kRadix; q += kSortBlock) (&wcnt[0][0])[q] = 0u;
     __syncthreads();
     const long long e = tile0 + r * kSortBlock + threadIdx.x;
     const bool valid = e < M;
-    const unsigned key = valid ? kin[e] : 0u;
-    const float val = valid ? vin[e] : 0.f;
+    const unsigned key = kr[r];
+    const float val = vr[r];
     const unsigned d = (key >> shift) & (kRadix - 1);
     unsigned long long same = __ballot(valid);
 #pragma unroll
@@ -196,19 +253,29 @@ k_radix_scatter(const unsigned *__restrict__ kin, const float *__restrict__ vin,
     if (valid && (same & lt_mask) == 0ull) wcnt[wave][d] = (unsigned)__popcll(same);
     __syncthreads();
     if (valid) {
-      unsigned before = base_off[d];
+      unsigned before = lrun[d];
       for (int w2 = 0; w2 < wave; ++w2) before += wcnt[w2][d];
       const unsigned pos = before + rank;
-      kout[pos] = key;
-      vout[pos] = val;
+      lk[pos] = key;
+      lv[pos] = val;
     }
     __syncthreads();
-    // advance the per-digit running offsets by this round's counts
+    // advance the per-digit running slots by this round's counts
     unsigned add = 0;
 #pragma unroll
     for (int w2 = 0; w2 < kWaves; ++w2) add += wcnt[w2][threadIdx.x];
-    base_off[threadIdx.x] += add;
+    lrun[threadIdx.x] += add;
     __syncthreads();
+  }
+  // the tile in digit-major order: slot p of digit d goes to gbase[d] + p - lstart[d]
+  const long long left = M - tile0;
+  const int n = left < kTile ? (int)left : kTile;
+  for (int p = threadIdx.x; p < n; p += kSortBlock) {
+    const unsigned key = lk[p];
+    const unsigned d = (key >> shift) & (kRadix - 1);
+    const unsigned g = gbase[d] + ((unsigned)p - lstart[d]);
+    kout[g] = key;
+    vout[g] = lv[p];
   }
 }
 
@@ -218,17 +285,42 @@ __global__ void __launch_bounds__(256) k_zero_image(float *__restrict__ dst, lon
   if (i < N) dst[i] = 0.f;
 }
 
+// Segment sums over the sorted pairs: the thread at a segment's head adds the
+// segment's values in sorted order.  A block stages a tile of kSegTile pairs in
+// LDS with coalesced loads, so the heads walk their segments in LDS; a segment
+// that runs past the tile's end continues in global memory.  The same additions
+// in the same order as one walk over global memory.
+constexpr int kSegItems = 8;
+constexpr int kSegTile = 256 * kSegItems;
 __global__ void __launch_bounds__(256)
 k_segment_sum(const unsigned *__restrict__ keys, const float *__restrict__ vals, long long M,
               unsigned N, float *__restrict__ dst) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M) return;
-  const unsigned k = keys[i];
-  if (k >= N) return;                       // sentinel segment: corners outside the volume
-  if (i > 0 && keys[i - 1] == k) return;    // not a segment head
-  float acc = 0.f;
-  for (long long j = i; j < M && keys[j] == k; ++j) acc += vals[j];
-  dst[k] = acc;
+  __shared__ unsigned sk[kSegTile + 1];  // sk[0]: the key before the tile
+  __shared__ float sv[kSegTile];
+  const long long t0 = (long long)blockIdx.x * kSegTile;
+  // past the pairs: a key no cell has (cells are < N <= 2^31), which ends every walk
+  constexpr unsigned kNone = 0xffffffffu;
+#pragma unroll
+  for (int q = 0; q < kSegItems; ++q) {
+    const int l = q * 256 + (int)threadIdx.x;
+    const long long e = t0 + l;
+    sk[l + 1] = e < M ? keys[e] : kNone;
+    sv[l] = e < M ? vals[e] : 0.f;
+  }
+  if (threadIdx.x == 0) sk[0] = t0 > 0 ? keys[t0 - 1] : kNone;
+  __syncthreads();
+#pragma unroll 1
+  for (int q = 0; q < kSegItems; ++q) {
+    const int l = q * 256 + (int)threadIdx.x;
+    const unsigned k = sk[l + 1];
+    if (k >= N || sk[l] == k) continue;  // past the pairs, the outside-volume cell, or not a head
+    float acc = 0.f;
+    int j = l;
+    while (j < kSegTile && sk[j + 1] == k) acc += sv[j++];
+    if (j == kSegTile)
+      for (long long g = t0 + kSegTile; g < M && keys[g] == k; ++g) acc += vals[g];
+    dst[k] = acc;
+  }
 }
 
 size_t splat_sorted_scratch_bytes(int nx, int ny, int nz) {
@@ -278,7 +370,7 @@ awk_status launch_splat_sorted(float *dst, const float *src, const float *vx, co
     std::swap(va, vb);
   }
   AWK_LAUNCH(k_zero_image, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, dst, N);
-  AWK_LAUNCH(k_segment_sum, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, ka, va, M,
+  AWK_LAUNCH(k_segment_sum, dim3((unsigned)((M + kSegTile - 1) / kSegTile)), dim3(256), 0, s, ka, va, M,
              (unsigned)N, dst);
   AWK_LAUNCH_CHECK("k_segment_sum");
   if (keys_out) AWK_HIP(hipMemcpyAsync(keys_out, ka, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
