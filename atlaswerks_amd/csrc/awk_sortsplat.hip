@@ -5,7 +5,8 @@
 // sentinel cell = N for corners outside the volume); the pairs are stably
 // sorted by cell; each cell's contributions are summed.  Here:
 //   1. k_create_index  keys/values, two uint4/float4 stores per voxel
-//   2. stable LSD radix sort, 8-bit digits, ceil(log2(N+1)/8) passes:
+//   2. stable LSD radix sort, 8-bit digits (a last 9-bit one where that saves a
+//      pass), about log2(N+1)/8 passes:
 //        k_radix_hist  per-tile digit counts (LDS), digit-major layout
 //        k_scan_*      exclusive scan of the counts (3 kernels, integer: exact)
 //        k_radix_scatter  stable ranks from wave ballots + per-wave digit counts
@@ -17,6 +18,7 @@
 // bit-exact target, the sums match to rounding.
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "awk_ops.h"
 
@@ -74,12 +76,16 @@ k_create_index(uint4 *__restrict__ keys, float4 *__restrict__ vals, const float 
 }
 
 // ---------------------------------------------------------------- radix sort
+// Per-tile digit counts of one pass, RB-bit digits, digit-major layout
+// (hist[d * ntiles + tile]).
+template <int RB>
 __global__ void __launch_bounds__(kSortBlock)
 k_radix_hist(const unsigned *__restrict__ keys, long long M, int shift, int ntiles,
              unsigned *__restrict__ hist) {
-  __shared__ unsigned cnt[kRadix];
+  constexpr int R = 1 << RB;
+  __shared__ unsigned cnt[R];
   const int t = blockIdx.x;
-  cnt[threadIdx.x] = 0;
+  for (int q = threadIdx.x; q < R; q += kSortBlock) cnt[q] = 0;
   __syncthreads();
   const long long base = (long long)t * kTile;
   const int lane = threadIdx.x & 63;
@@ -94,12 +100,12 @@ k_radix_hist(const unsigned *__restrict__ keys, long long M, int shift, int ntil
   for (int r = 0; r < kSortItems; ++r) {
     const long long e = base + r * kSortBlock + threadIdx.x;
     const bool valid = e < M;
-    const unsigned d = (kr[r] >> shift) & (kRadix - 1);
-    // lanes with equal digits (8 ballots) add once, from their lowest lane: no
+    const unsigned d = (kr[r] >> shift) & (R - 1);
+    // lanes with equal digits (RB ballots) add once, from their lowest lane: no
     // serialised atomics when a wave's keys share a digit
     unsigned long long same = __ballot(valid);
 #pragma unroll
-    for (int b = 0; b < kRadixBits; ++b) {
+    for (int b = 0; b < RB; ++b) {
       const bool bit = (d >> b) & 1u;
       const unsigned long long m = __ballot(bit);
       same &= bit ? m : ~m;
@@ -107,7 +113,7 @@ k_radix_hist(const unsigned *__restrict__ keys, long long M, int shift, int ntil
     if (valid && (same & lt_mask) == 0ull) atomicAdd(&cnt[d], (unsigned)__popcll(same));
   }
   __syncthreads();
-  hist[(size_t)threadIdx.x * ntiles + t] = cnt[threadIdx.x];
+  for (int q = threadIdx.x; q < R; q += kSortBlock) hist[(size_t)q * ntiles + t] = cnt[q];
 }
 
 // exclusive scan of n unsigned values, in three steps (exact integer arithmetic)
@@ -189,37 +195,44 @@ k_scan_down(unsigned *__restrict__ a, long long n, const unsigned *__restrict__ 
 // threads store consecutive addresses of one digit's run instead of scattering
 // every element.  The tile's digit counts are differences of consecutive entries
 // of the exclusive scan (digit-major: entry d * ntiles + t), the last against M.
+template <int RB>
 __global__ void __launch_bounds__(kSortBlock)
 k_radix_scatter(const unsigned *__restrict__ kin, const float *__restrict__ vin,
                 unsigned *__restrict__ kout, float *__restrict__ vout, long long M, int shift,
                 int ntiles, const unsigned *__restrict__ offs) {
   constexpr int kWaves = kSortBlock / 64;
-  static_assert(kSortBlock == kRadix, "one thread per digit");
-  __shared__ unsigned gbase[kRadix];   // global start of this tile's run of each digit
-  __shared__ unsigned lstart[kRadix];  // tile-local start of each digit
-  __shared__ unsigned lrun[kRadix];    // next tile-local slot of each digit
-  __shared__ unsigned wcnt[kWaves][kRadix];
+  constexpr int R = 1 << RB, DPT = R / kSortBlock;  // digits per thread
+  static_assert(DPT * kSortBlock == R, "whole digits per thread");
+  __shared__ unsigned gbase[R];   // global start of this tile's run of each digit
+  __shared__ unsigned lstart[R];  // tile-local start of each digit
+  __shared__ unsigned lrun[R];    // next tile-local slot of each digit
+  __shared__ unsigned wcnt[kWaves][R];
   __shared__ unsigned lk[kTile];
   __shared__ float lv[kTile];
+  __shared__ unsigned scan_s[kSortBlock];
   const int t = blockIdx.x;
-  const long long nh = (long long)kRadix * ntiles;
-  const long long hi = (long long)threadIdx.x * ntiles + t;
-  const unsigned g0 = offs[hi];
-  const unsigned cnt = (hi + 1 < nh ? offs[hi + 1] : (unsigned)M) - g0;
-  gbase[threadIdx.x] = g0;
-  // exclusive scan of the tile's 256 digit counts
-  lstart[threadIdx.x] = cnt;
-  __syncthreads();
-  for (int off = 1; off < kRadix; off <<= 1) {
-    const unsigned v = (int)threadIdx.x >= off ? lstart[threadIdx.x - off] : 0u;
-    __syncthreads();
-    lstart[threadIdx.x] += v;
-    __syncthreads();
+  const long long nh = (long long)R * ntiles;
+  // thread t owns digits t * DPT ... t * DPT + DPT - 1
+  unsigned c[DPT], tsum = 0;
+#pragma unroll
+  for (int k = 0; k < DPT; ++k) {
+    const int d = (int)threadIdx.x * DPT + k;
+    const long long hi = (long long)d * ntiles + t;
+    const unsigned g0 = offs[hi];
+    c[k] = (hi + 1 < nh ? offs[hi + 1] : (unsigned)M) - g0;
+    gbase[d] = g0;
+    tsum += c[k];
   }
-  const unsigned excl = lstart[threadIdx.x] - cnt;
-  __syncthreads();
-  lstart[threadIdx.x] = excl;
-  lrun[threadIdx.x] = excl;
+  // exclusive scan of the tile's digit counts
+  unsigned total;
+  unsigned ex = block_exclusive_scan(tsum, scan_s, total);
+#pragma unroll
+  for (int k = 0; k < DPT; ++k) {
+    const int d = (int)threadIdx.x * DPT + k;
+    lstart[d] = ex;
+    lrun[d] = ex;
+    ex += c[k];
+  }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const long long tile0 = (long long)t * kTile;
@@ -234,16 +247,16 @@ k_radix_scatter(const unsigned *__restrict__ kin, const float *__restrict__ vin,
   }
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    for (int q = threadIdx.x; q < kWaves * kRadix; q += kSortBlock) (&wcnt[0][0])[q] = 0u;
+    for (int q = threadIdx.x; q < kWaves * R; q += kSortBlock) (&wcnt[0][0])[q] = 0u;
     __syncthreads();
     const long long e = tile0 + r * kSortBlock + threadIdx.x;
     const bool valid = e < M;
     const unsigned key = kr[r];
     const float val = vr[r];
-    const unsigned d = (key >> shift) & (kRadix - 1);
+    const unsigned d = (key >> shift) & (R - 1);
     unsigned long long same = __ballot(valid);
 #pragma unroll
-    for (int b = 0; b < kRadixBits; ++b) {
+    for (int b = 0; b < RB; ++b) {
       const bool bit = (d >> b) & 1u;
       const unsigned long long m = __ballot(bit);
       same &= bit ? m : ~m;
@@ -261,10 +274,14 @@ k_radix_scatter(const unsigned *__restrict__ kin, const float *__restrict__ vin,
     }
     __syncthreads();
     // advance the per-digit running slots by this round's counts
-    unsigned add = 0;
 #pragma unroll
-    for (int w2 = 0; w2 < kWaves; ++w2) add += wcnt[w2][threadIdx.x];
-    lrun[threadIdx.x] += add;
+    for (int k = 0; k < DPT; ++k) {
+      const int dd = (int)threadIdx.x * DPT + k;
+      unsigned add = 0;
+#pragma unroll
+      for (int w2 = 0; w2 < kWaves; ++w2) add += wcnt[w2][dd];
+      lrun[dd] += add;
+    }
     __syncthreads();
   }
   // the tile in digit-major order: slot p of digit d goes to gbase[d] + p - lstart[d]
@@ -272,7 +289,7 @@ k_radix_scatter(const unsigned *__restrict__ kin, const float *__restrict__ vin,
   const int n = left < kTile ? (int)left : kTile;
   for (int p = threadIdx.x; p < n; p += kSortBlock) {
     const unsigned key = lk[p];
-    const unsigned d = (key >> shift) & (kRadix - 1);
+    const unsigned d = (key >> shift) & (R - 1);
     const unsigned g = gbase[d] + ((unsigned)p - lstart[d]);
     kout[g] = key;
     vout[g] = lv[p];
@@ -326,7 +343,7 @@ k_segment_sum(const unsigned *__restrict__ keys, const float *__restrict__ vals,
 size_t splat_sorted_scratch_bytes(int nx, int ny, int nz) {
   const long long M = 8LL * nx * ny * nz;
   const int ntiles = (int)((M + kTile - 1) / kTile);
-  const long long nh = (long long)kRadix * ntiles;
+  const long long nh = 2LL * kRadix * ntiles;  // room for a 9-bit pass
   const int nchunks = (int)((nh + kScanChunk - 1) / kScanChunk);
   // two key/value buffers, histogram, chunk sums
   return 2 * (size_t)M * 4 + 2 * (size_t)M * 4 + (size_t)nh * 4 + (size_t)nchunks * 4 + 256;
@@ -340,10 +357,16 @@ awk_status launch_splat_sorted(float *dst, const float *src, const float *vx, co
   const long long M = 8 * N;
   int bits = 1;
   while ((1LL << bits) < N + 1) ++bits;
-  const int passes = (bits + kRadixBits - 1) / kRadixBits;
+  // 8-bit passes; 9 remaining bits go in one pass (a power-of-two volume's keys,
+  // cells 0 .. N - 1 and the outside-volume cell N, need one bit past a multiple of 8)
+  std::vector<int> plan;
+  for (int rem = bits; rem > 0;) {
+    const int b = rem == 9 ? 9 : std::min(rem, kRadixBits);
+    plan.push_back(b);
+    rem -= b;
+  }
   const int ntiles = (int)((M + kTile - 1) / kTile);
-  const long long nh = (long long)kRadix * ntiles;
-  const int nchunks = (int)((nh + kScanChunk - 1) / kScanChunk);
+  const long long nh_max = 2LL * kRadix * ntiles;
   // scratch: the caller's work buffer (splat_sorted_scratch_bytes) or stream-ordered
   char *scratch = static_cast<char *>(work);
   if (!scratch) AWK_HIP(hipMallocAsync((void **)&scratch, splat_sorted_scratch_bytes(nx, ny, nz), s));
@@ -352,20 +375,28 @@ awk_status launch_splat_sorted(float *dst, const float *src, const float *vx, co
   float *va = reinterpret_cast<float *>(kb + M);
   float *vb = va + M;
   unsigned *hist = reinterpret_cast<unsigned *>(vb + M);
-  unsigned *sums = hist + nh;
+  unsigned *sums = hist + nh_max;
   AWK_LAUNCH(k_create_index, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s,
              reinterpret_cast<uint4 *>(ka), reinterpret_cast<float4 *>(va), src, vx, vy, vz, nx, ny,
              nz, truncate, absolute);
   AWK_LAUNCH_CHECK("k_create_index");
-  for (int p = 0; p < passes; ++p) {
-    const int shift = p * kRadixBits;
-    AWK_LAUNCH(k_radix_hist, dim3(ntiles), dim3(kSortBlock), 0, s, ka, M, shift, ntiles, hist);
+  int shift = 0;
+  for (const int b : plan) {
+    const long long nh = (long long)(b == 9 ? 2 * kRadix : kRadix) * ntiles;
+    const int nchunks = (int)((nh + kScanChunk - 1) / kScanChunk);
+    if (b == 9) AWK_LAUNCH(k_radix_hist<9>, dim3(ntiles), dim3(kSortBlock), 0, s, ka, M, shift, ntiles, hist);
+    else AWK_LAUNCH(k_radix_hist<8>, dim3(ntiles), dim3(kSortBlock), 0, s, ka, M, shift, ntiles, hist);
     AWK_LAUNCH(k_scan_reduce, dim3(nchunks), dim3(kSortBlock), 0, s, hist, nh, sums);
     AWK_LAUNCH(k_scan_top, dim3(1), dim3(kSortBlock), 0, s, sums, nchunks);
     AWK_LAUNCH(k_scan_down, dim3(nchunks), dim3(kSortBlock), 0, s, hist, nh, sums);
-    AWK_LAUNCH(k_radix_scatter, dim3(ntiles), dim3(kSortBlock), 0, s, ka, va, kb, vb, M, shift,
-               ntiles, hist);
+    if (b == 9)
+      AWK_LAUNCH(k_radix_scatter<9>, dim3(ntiles), dim3(kSortBlock), 0, s, ka, va, kb, vb, M, shift,
+                 ntiles, hist);
+    else
+      AWK_LAUNCH(k_radix_scatter<8>, dim3(ntiles), dim3(kSortBlock), 0, s, ka, va, kb, vb, M, shift,
+                 ntiles, hist);
     AWK_LAUNCH_CHECK("radix pass");
+    shift += b;
     std::swap(ka, kb);
     std::swap(va, vb);
   }

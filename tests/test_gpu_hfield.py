@@ -134,8 +134,11 @@ def test_splat_adjoint_identity():
     assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
 
 
-@pytest.mark.parametrize("shape", [(1, 1, 1), (1, 15, 17), (4, 8, 8), (17, 9, 5),
-                                   (24, 40, 36), (64, 64, 64)])
+# (6, 7, 8): keys of 9 bits, one 9-bit radix pass; (32, 32, 64): N = 2^16, keys of
+# 17 bits, an 8-bit then a 9-bit pass (the plan of every power-of-two volume
+# whose exponent is a multiple of 8, 256^3 among them)
+@pytest.mark.parametrize("shape", [(1, 1, 1), (1, 15, 17), (4, 8, 8), (17, 9, 5), (6, 7, 8),
+                                   (24, 40, 36), (32, 32, 64), (64, 64, 64)])
 @pytest.mark.parametrize("trunc", [True, False])
 def test_splat_sorted_bit_exact(shape, trunc):
     """Sort/scan splat: sorted keys, sorted contributions and cell sums equal the
