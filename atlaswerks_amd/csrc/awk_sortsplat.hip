@@ -187,26 +187,30 @@ k_scan_down(unsigned *__restrict__ a, long long n, const unsigned *__restrict__ 
   }
 }
 
-// Stable scatter of one pass.  Elements are visited in their original order
-// (round r, thread t <-> element r*256 + t of the tile); within a wave the rank
-// among equal digits comes from 8 ballots, across waves from LDS counts.  Each
-// element first goes to its tile-local sorted position in LDS (digit-major, stable
-// within a digit); the tile is then written out in that order, so consecutive
-// threads store consecutive addresses of one digit's run instead of scattering
-// every element.  The tile's digit counts are differences of consecutive entries
-// of the exclusive scan (digit-major: entry d * ntiles + t), the last against M.
+// Stable scatter of one pass.  Wave w owns the tile's elements
+// [w * 1024, (w + 1) * 1024), loaded in 16 coalesced rounds of 64; a stable order
+// is index order, i.e. (wave, round, lane).  Within a wave the rank among equal
+// digits comes from RB ballots plus the wave's running count of each digit, kept
+// in the wave's own LDS row (a wave's LDS operations complete in program order,
+// so the rounds need no barrier).  One barrier later the per-wave totals give each
+// wave's start within each digit's run of the tile, every element goes to its
+// tile-local sorted position in LDS (digit-major), and the tile is written out in
+// that order: consecutive threads store consecutive addresses of one digit's run.
+// The tile's digit counts are differences of consecutive entries of the exclusive
+// scan (digit-major: entry d * ntiles + t), the last against M.
 template <int RB>
 __global__ void __launch_bounds__(kSortBlock)
 k_radix_scatter(const unsigned *__restrict__ kin, const float *__restrict__ vin,
                 unsigned *__restrict__ kout, float *__restrict__ vout, long long M, int shift,
                 int ntiles, const unsigned *__restrict__ offs) {
   constexpr int kWaves = kSortBlock / 64;
+  constexpr int kSpan = kTile / kWaves;  // elements per wave
+  constexpr int kRounds = kSpan / 64;
   constexpr int R = 1 << RB, DPT = R / kSortBlock;  // digits per thread
-  static_assert(DPT * kSortBlock == R, "whole digits per thread");
+  static_assert(DPT * kSortBlock == R && kRounds * 64 * kWaves == kTile, "tile shape");
   __shared__ unsigned gbase[R];   // global start of this tile's run of each digit
   __shared__ unsigned lstart[R];  // tile-local start of each digit
-  __shared__ unsigned lrun[R];    // next tile-local slot of each digit
-  __shared__ unsigned wcnt[kWaves][R];
+  __shared__ unsigned wrun[kWaves][R];
   __shared__ unsigned lk[kTile];
   __shared__ float lv[kTile];
   __shared__ unsigned scan_s[kSortBlock];
@@ -223,37 +227,32 @@ k_radix_scatter(const unsigned *__restrict__ kin, const float *__restrict__ vin,
     gbase[d] = g0;
     tsum += c[k];
   }
-  // exclusive scan of the tile's digit counts
+  for (int q = threadIdx.x; q < kWaves * R; q += kSortBlock) (&wrun[0][0])[q] = 0u;
+  // exclusive scan of the tile's digit counts (its barriers also order the zeroing)
   unsigned total;
   unsigned ex = block_exclusive_scan(tsum, scan_s, total);
 #pragma unroll
   for (int k = 0; k < DPT; ++k) {
-    const int d = (int)threadIdx.x * DPT + k;
-    lstart[d] = ex;
-    lrun[d] = ex;
+    lstart[(int)threadIdx.x * DPT + k] = ex;
     ex += c[k];
   }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const long long tile0 = (long long)t * kTile;
-  // the whole tile's loads first: one memory latency per tile, not per round
-  unsigned kr[kSortItems];
-  float vr[kSortItems];
+  const long long w0 = (long long)t * kTile + (long long)wave * kSpan;
+  // the wave's loads first: one memory latency per tile, not per round
+  unsigned kr[kRounds], dr[kRounds], pr[kRounds];
+  float vr[kRounds];
 #pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
-    const long long e = tile0 + r * kSortBlock + threadIdx.x;
+  for (int r = 0; r < kRounds; ++r) {
+    const long long e = w0 + r * 64 + lane;
     kr[r] = e < M ? kin[e] : 0u;
     vr[r] = e < M ? vin[e] : 0.f;
   }
+  unsigned *run = wrun[wave];
 #pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
-    for (int q = threadIdx.x; q < kWaves * R; q += kSortBlock) (&wcnt[0][0])[q] = 0u;
-    __syncthreads();
-    const long long e = tile0 + r * kSortBlock + threadIdx.x;
-    const bool valid = e < M;
-    const unsigned key = kr[r];
-    const float val = vr[r];
-    const unsigned d = (key >> shift) & (R - 1);
+  for (int r = 0; r < kRounds; ++r) {
+    const bool valid = w0 + r * 64 + lane < M;
+    const unsigned d = (kr[r] >> shift) & (R - 1);
     unsigned long long same = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < RB; ++b) {
@@ -261,31 +260,37 @@ k_radix_scatter(const unsigned *__restrict__ kin, const float *__restrict__ vin,
       const unsigned long long m = __ballot(bit);
       same &= bit ? m : ~m;
     }
-    const unsigned rank = (unsigned)__popcll(same & lt_mask);
-    // the lowest lane of each digit group publishes the group size
-    if (valid && (same & lt_mask) == 0ull) wcnt[wave][d] = (unsigned)__popcll(same);
-    __syncthreads();
-    if (valid) {
-      unsigned before = lrun[d];
-      for (int w2 = 0; w2 < wave; ++w2) before += wcnt[w2][d];
-      const unsigned pos = before + rank;
-      lk[pos] = key;
-      lv[pos] = val;
-    }
-    __syncthreads();
-    // advance the per-digit running slots by this round's counts
-#pragma unroll
-    for (int k = 0; k < DPT; ++k) {
-      const int dd = (int)threadIdx.x * DPT + k;
-      unsigned add = 0;
-#pragma unroll
-      for (int w2 = 0; w2 < kWaves; ++w2) add += wcnt[w2][dd];
-      lrun[dd] += add;
-    }
-    __syncthreads();
+    const unsigned before = valid ? run[d] : 0u;
+    // the lowest lane of each digit group advances the wave's count
+    if (valid && (same & lt_mask) == 0ull) run[d] = before + (unsigned)__popcll(same);
+    dr[r] = d;
+    pr[r] = before + (unsigned)__popcll(same & lt_mask);
   }
+  __syncthreads();
+  // each wave's start within each digit's run: exclusive prefix over the waves
+#pragma unroll
+  for (int k = 0; k < DPT; ++k) {
+    const int d = (int)threadIdx.x * DPT + k;
+    unsigned acc = lstart[d];
+#pragma unroll
+    for (int w2 = 0; w2 < kWaves; ++w2) {
+      const unsigned n = wrun[w2][d];
+      wrun[w2][d] = acc;
+      acc += n;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    if (w0 + r * 64 + lane < M) {
+      const unsigned pos = run[dr[r]] + pr[r];
+      lk[pos] = kr[r];
+      lv[pos] = vr[r];
+    }
+  }
+  __syncthreads();
   // the tile in digit-major order: slot p of digit d goes to gbase[d] + p - lstart[d]
-  const long long left = M - tile0;
+  const long long left = M - (long long)t * kTile;
   const int n = left < kTile ? (int)left : kTile;
   for (int p = threadIdx.x; p < n; p += kSortBlock) {
     const unsigned key = lk[p];
