@@ -90,15 +90,23 @@ k_radix_hist(const unsigned *__restrict__ keys, long long M, int shift, int ntil
   const long long base = (long long)t * kTile;
   const int lane = threadIdx.x & 63;
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // counts do not depend on the order: each thread takes 4 groups of 4 consecutive
+  // keys, 16-byte loads where the group lies inside the pairs
   unsigned kr[kSortItems];
 #pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
-    const long long e = base + r * kSortBlock + threadIdx.x;
-    kr[r] = e < M ? keys[e] : 0u;
+  for (int g = 0; g < kSortItems / 4; ++g) {
+    const long long e = base + 4 * (g * kSortBlock + (long long)threadIdx.x);
+    if (e + 3 < M) {
+      const uint4 q = *reinterpret_cast<const uint4 *>(keys + e);
+      kr[4 * g] = q.x; kr[4 * g + 1] = q.y; kr[4 * g + 2] = q.z; kr[4 * g + 3] = q.w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) kr[4 * g + j] = e + j < M ? keys[e + j] : 0u;
+    }
   }
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    const long long e = base + r * kSortBlock + threadIdx.x;
+    const long long e = base + 4 * ((r / 4) * kSortBlock + (long long)threadIdx.x) + (r % 4);
     const bool valid = e < M;
     const unsigned d = (kr[r] >> shift) & (R - 1);
     // lanes with equal digits (RB ballots) add once, from their lowest lane: no
