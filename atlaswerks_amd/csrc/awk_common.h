@@ -191,7 +191,6 @@ __device__ __forceinline__ float3 trilerp3g(const L &fx, const L &fy, const L &f
   int x0 = (int)flx, y0 = (int)fly, z0 = (int)flz;
   int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
   const float t = x - flx, u = y - fly, v = z - flz;
-  const float omt = 1.f - t, omu = 1.f - u, omv = 1.f - v;
   float c[8][3];
   if (BG == AWK_BG_WRAP || BG == AWK_BG_CLAMP) {
     if (BG == AWK_BG_WRAP) {

@@ -89,7 +89,6 @@ k_radix_hist(const unsigned *__restrict__ keys, long long M, int shift, int ntil
   __syncthreads();
   const long long base = (long long)t * kTile;
   const int lane = threadIdx.x & 63;
-  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   // counts do not depend on the order: each thread takes 4 groups of 4 consecutive
   // keys, 16-byte loads where the group lies inside the pairs
   unsigned kr[kSortItems];
@@ -109,16 +108,17 @@ k_radix_hist(const unsigned *__restrict__ keys, long long M, int shift, int ntil
     const long long e = base + 4 * ((r / 4) * kSortBlock + (long long)threadIdx.x) + (r % 4);
     const bool valid = e < M;
     const unsigned d = (kr[r] >> shift) & (R - 1);
-    // lanes with equal digits (RB ballots) add once, from their lowest lane: no
-    // serialised atomics when a wave's keys share a digit
-    unsigned long long same = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < RB; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const unsigned long long m = __ballot(bit);
-      same &= bit ? m : ~m;
+    // a wave whose keys share one digit (the common case past the first pass:
+    // cells of neighbouring voxels) adds once; otherwise each lane adds, with
+    // little contention when the digits differ.  Valid lanes are a prefix, so
+    // lane 0 holds a valid digit whenever any lane does.
+    const unsigned d0 = __builtin_amdgcn_readfirstlane(d);
+    if (__all(!valid || d == d0)) {
+      const unsigned long long vm = __ballot(valid);
+      if (lane == 0 && vm) atomicAdd(&cnt[d0], (unsigned)__popcll(vm));
+    } else if (valid) {
+      atomicAdd(&cnt[d], 1u);
     }
-    if (valid && (same & lt_mask) == 0ull) atomicAdd(&cnt[d], (unsigned)__popcll(same));
   }
   __syncthreads();
   for (int q = threadIdx.x; q < R; q += kSortBlock) hist[(size_t)q * ntiles + t] = cnt[q];
@@ -262,11 +262,16 @@ k_radix_scatter(const unsigned *__restrict__ kin, const float *__restrict__ vin,
     const bool valid = w0 + r * 64 + lane < M;
     const unsigned d = (kr[r] >> shift) & (R - 1);
     unsigned long long same = __ballot(valid);
+    // lanes with this lane's digit: all valid lanes when the wave shares one digit
+    // (valid lanes are a prefix, so lane 0's digit is a valid one), else RB ballots
+    const unsigned d0 = __builtin_amdgcn_readfirstlane(d);
+    if (!__all(!valid || d == d0)) {
 #pragma unroll
-    for (int b = 0; b < RB; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const unsigned long long m = __ballot(bit);
-      same &= bit ? m : ~m;
+      for (int b = 0; b < RB; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const unsigned long long m = __ballot(bit);
+        same &= bit ? m : ~m;
+      }
     }
     const unsigned before = valid ? run[d] : 0u;
     // the lowest lane of each digit group advances the wave's count
