@@ -88,25 +88,27 @@ k_radix_hist(const unsigned *__restrict__ keys, long long M, int shift, int ntil
   for (int q = threadIdx.x; q < R; q += kSortBlock) cnt[q] = 0;
   __syncthreads();
   const long long base = (long long)t * kTile;
+  const unsigned *__restrict__ kt = keys + base;
+  const int n = M - base < kTile ? (int)(M - base) : kTile;
   const int lane = threadIdx.x & 63;
   // counts do not depend on the order: each thread takes 4 groups of 4 consecutive
   // keys, 16-byte loads where the group lies inside the pairs
   unsigned kr[kSortItems];
 #pragma unroll
   for (int g = 0; g < kSortItems / 4; ++g) {
-    const long long e = base + 4 * (g * kSortBlock + (long long)threadIdx.x);
-    if (e + 3 < M) {
-      const uint4 q = *reinterpret_cast<const uint4 *>(keys + e);
+    const int e = 4 * (g * kSortBlock + (int)threadIdx.x);
+    if (e + 3 < n) {
+      const uint4 q = *reinterpret_cast<const uint4 *>(kt + e);
       kr[4 * g] = q.x; kr[4 * g + 1] = q.y; kr[4 * g + 2] = q.z; kr[4 * g + 3] = q.w;
     } else {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) kr[4 * g + j] = e + j < M ? keys[e + j] : 0u;
+      for (int j = 0; j < 4; ++j) kr[4 * g + j] = e + j < n ? kt[e + j] : 0u;
     }
   }
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    const long long e = base + 4 * ((r / 4) * kSortBlock + (long long)threadIdx.x) + (r % 4);
-    const bool valid = e < M;
+    const int e = 4 * ((r / 4) * kSortBlock + (int)threadIdx.x) + (r % 4);
+    const bool valid = e < n;
     const unsigned d = (kr[r] >> shift) & (R - 1);
     // a wave whose keys share one digit (the common case past the first pass:
     // cells of neighbouring voxels) adds once; otherwise each lane adds, with
@@ -246,20 +248,25 @@ k_radix_scatter(const unsigned *__restrict__ kin, const float *__restrict__ vin,
   }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const long long w0 = (long long)t * kTile + (long long)wave * kSpan;
+  // tile-local 32-bit indices: element w0 + r * 64 + lane of the tile
+  const long long tile0 = (long long)t * kTile;
+  const unsigned *__restrict__ kt = kin + tile0;
+  const float *__restrict__ vt = vin + tile0;
+  const int n = M - tile0 < kTile ? (int)(M - tile0) : kTile;
+  const int w0 = wave * kSpan + lane;
   // the wave's loads first: one memory latency per tile, not per round
   unsigned kr[kRounds], dr[kRounds], pr[kRounds];
   float vr[kRounds];
 #pragma unroll
   for (int r = 0; r < kRounds; ++r) {
-    const long long e = w0 + r * 64 + lane;
-    kr[r] = e < M ? kin[e] : 0u;
-    vr[r] = e < M ? vin[e] : 0.f;
+    const int e = w0 + r * 64;
+    kr[r] = e < n ? kt[e] : 0u;
+    vr[r] = e < n ? vt[e] : 0.f;
   }
   unsigned *run = wrun[wave];
 #pragma unroll
   for (int r = 0; r < kRounds; ++r) {
-    const bool valid = w0 + r * 64 + lane < M;
+    const bool valid = w0 + r * 64 < n;
     const unsigned d = (kr[r] >> shift) & (R - 1);
     unsigned long long same = __ballot(valid);
     // lanes with this lane's digit: all valid lanes when the wave shares one digit
@@ -295,7 +302,7 @@ k_radix_scatter(const unsigned *__restrict__ kin, const float *__restrict__ vin,
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kRounds; ++r) {
-    if (w0 + r * 64 + lane < M) {
+    if (w0 + r * 64 < n) {
       const unsigned pos = run[dr[r]] + pr[r];
       lk[pos] = kr[r];
       lv[pos] = vr[r];
@@ -303,8 +310,6 @@ k_radix_scatter(const unsigned *__restrict__ kin, const float *__restrict__ vin,
   }
   __syncthreads();
   // the tile in digit-major order: slot p of digit d goes to gbase[d] + p - lstart[d]
-  const long long left = M - (long long)t * kTile;
-  const int n = left < kTile ? (int)left : kTile;
   for (int p = threadIdx.x; p < n; p += kSortBlock) {
     const unsigned key = lk[p];
     const unsigned d = (key >> shift) & (R - 1);
