@@ -99,8 +99,42 @@ def write_metaimage(path, img):
         f.write(data.astype("<f4").tobytes())
 
 
+def _fmt_num(v):
+    return "%.9g" % float(v)
+
+
+def write_nrrd(path, img):
+    """Detached-header NRRD: ``path`` (.nhdr) plus its raw data file (the .nhdr
+    name with .raw), the layout of the reference's test data and of the outputs its
+    applications write with ``OutputSuffix nhdr``.  Scalar images have three
+    domain axes; vector fields a leading ``vector`` axis of 3 (no direction)."""
+    data = np.ascontiguousarray(img.data, dtype=np.float32)
+    nz, ny, nx = data.shape[:3]
+    vec = data.ndim == 4
+    sx, sy, sz = img.spacing
+    dirs = "(%s,0,0) (0,%s,0) (0,0,%s)" % (_fmt_num(sx), _fmt_num(sy), _fmt_num(sz))
+    raw = os.path.splitext(os.path.basename(path))[0] + ".raw"
+    lines = [
+        "NRRD0004",
+        "type: float",
+        "dimension: %d" % (4 if vec else 3),
+        "space: left-posterior-superior",
+        "sizes: " + ("%d %d %d %d" % (data.shape[3], nx, ny, nz) if vec else "%d %d %d" % (nx, ny, nz)),
+        "space directions: " + ("none " + dirs if vec else dirs),
+        "kinds: " + ("vector domain domain domain" if vec else "domain domain domain"),
+        "endian: little",
+        "encoding: raw",
+        "space origin: (%s)" % ",".join(_fmt_num(o) for o in img.origin),
+        "data file: " + raw,
+    ]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    data.astype("<f4").tofile(os.path.join(os.path.dirname(path), raw))
+
+
 def read_nrrd(path):
-    """Detached-header NRRD (.nhdr + raw), as the BullseyeTestBlur fixtures use."""
+    """Detached-header NRRD (.nhdr + raw), as the BullseyeTestBlur fixtures use;
+    a leading ``vector`` axis gives a ``[z, y, x, c]`` field."""
     header = {}
     with open(path, "r") as f:
         for line in f:
@@ -120,7 +154,11 @@ def read_nrrd(path):
     datafile = header.get("data file", header.get("datafile"))
     data = np.fromfile(os.path.join(os.path.dirname(path), datafile), dtype=dtype,
                        count=int(np.prod(sizes)))
+    kinds = header.get("kinds", "").split()
+    vec = len(sizes) == 4 and kinds[:1] == ["vector"]
     arr = data.reshape(tuple(reversed(sizes))).astype(np.float32)
+    if vec:
+        arr = np.ascontiguousarray(arr)  # [z, y, x, c]: the vector axis is the fastest
     sp = [1.0, 1.0, 1.0]
     if "space directions" in header:
         vecs = header["space directions"].replace(")", "").split("(")[1:]
