@@ -52,6 +52,30 @@ LaunchEvents *&launch_events();
       hipLaunchKernelGGL(K, G, B, L, S, __VA_ARGS__);                                      \
   } while (0)
 
+// x / c for a launch constant c, given rc = RN(1 / c): the product, then one
+// fma correction by the exact remainder (Markstein's division by a constant).
+// Three instructions where an IEEE division takes about ten; over 4e8 samples at
+// the spacings 0.5 ... 3 and their doubles it agrees with x / c in every bit.
+__device__ __forceinline__ float div_const(float x, float c, float rc) {
+  const float q = x * rc;
+  return fmaf(fmaf(-q, c, x), rc, q);
+}
+
+// A grid spacing with the reciprocals of its two divisors (sp and 2 sp), for
+// div_const; built on the host.
+struct Spacing3 {
+  float s[3], r[3], r2[3];
+};
+inline Spacing3 spacing3(const float sp[3]) {
+  Spacing3 S;
+  for (int c = 0; c < 3; ++c) {
+    S.s[c] = sp[c];
+    S.r[c] = 1.f / sp[c];
+    S.r2[c] = 1.f / (2.f * sp[c]);
+  }
+  return S;
+}
+
 // One thread's part of a block-level fold: op over p[start], p[start + stride], ...
 // in that order (so a sum rounds exactly as the plain loop would), with the loads
 // issued 16 at a time.  A block that folds thousands of partials while the rest of

@@ -99,15 +99,6 @@ __device__ __forceinline__ float load_mul(const XFwdArgs &a, size_t i) {
   return __builtin_nontemporal_load(a.i1 + i);
 }
 
-// x / c for a launch constant c, given rc = RN(1 / c): the product, then one
-// fma correction by the exact remainder (Markstein's division by a constant).
-// Three instructions where an IEEE division takes about ten; over 4e8 samples at
-// the spacings 0.5 ... 3 and their doubles it agrees with x / c in every bit.
-__device__ __forceinline__ float div_const(float x, float c, float rc) {
-  const float q = x * rc;
-  return fmaf(fmaf(-q, c, x), rc, q);
-}
-
 // NC, RTC, BDC: compile-time length, rows per block and block size (0 = runtime).
 template <int NC, int RTC, int BDC>
 __global__ void __launch_bounds__(BDC ? BDC : 1024)

@@ -219,12 +219,13 @@ __global__ void k_ldmm_step(const float *__restrict__ maxpart, int n, float max_
 
 // Gradient along one axis of f(q) = b(q) + a[q] (computeGradient, wrap = false;
 // Array3DUtils.txx:783-901); bm, b0, bp = the identity part at p - 1, p, p + 1.
+// r1 = 1 / sp and r2 = 1 / (2 sp) for div_const.
 __device__ __forceinline__ float hgrad(const float *__restrict__ a, int i, int p, int n, int stride,
-                                       float sp, float bm, float b0, float bp) {
+                                       float sp, float r1, float r2, float bm, float b0, float bp) {
   if (n == 1) return 0.f;
-  if (p > 0 && p < n - 1) return ((bp + a[i + stride]) - (bm + a[i - stride])) / (2.f * sp);
-  if (p == 0) return ((bp + a[i + stride]) - (b0 + a[i])) / sp;
-  return ((b0 + a[i]) - (bm + a[i - stride])) / sp;
+  if (p > 0 && p < n - 1) return div_const((bp + a[i + stride]) - (bm + a[i - stride]), 2.f * sp, r2);
+  if (p == 0) return div_const((bp + a[i + stride]) - (b0 + a[i]), sp, r1);
+  return div_const((b0 + a[i]) - (bm + a[i - stride]), sp, r1);
 }
 
 // |D(x sp + v)| at voxel i: HField3DUtils::jacobian (HField3DUtils.txx:2370-2420) of
@@ -232,11 +233,11 @@ __device__ __forceinline__ float hgrad(const float *__restrict__ a, int i, int p
 __device__ __forceinline__ float jac_det_xsp_v(const float *__restrict__ vx,
                                                const float *__restrict__ vy,
                                                const float *__restrict__ vz, int i, int x, int y,
-                                               int z, Dims d, float sx, float sy, float sz) {
+                                               int z, Dims d, const Spacing3 &S) {
   const int sxy = d.nx * d.ny;
   const float *vc[3] = {vx, vy, vz};
   const int pos[3] = {x, y, z}, n[3] = {d.nx, d.ny, d.nz}, stride[3] = {1, d.nx, sxy};
-  const float sp[3] = {sx, sy, sz};
+  const float *sp = S.s;
   float g[3][3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -248,7 +249,7 @@ __device__ __forceinline__ float jac_det_xsp_v(const float *__restrict__ vx,
         bm = (float)(pos[c] - 1) * sp[c];
         bp = (float)(pos[c] + 1) * sp[c];
       }
-      g[c][e] = hgrad(vc[c], i, pos[e], n[e], stride[e], sp[e], bm, b0, bp);
+      g[c][e] = hgrad(vc[c], i, pos[e], n[e], stride[e], sp[e], S.r[e], S.r2[e], bm, b0, bp);
     }
   }
   const float t1 = g[0][0] * (g[1][1] * g[2][2] - g[1][2] * g[2][1]);
@@ -265,19 +266,21 @@ k_orig_bwd(float *__restrict__ nx_, float *__restrict__ ny_, float *__restrict__
            const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
            const float *__restrict__ vx, const float *__restrict__ vy, const float *__restrict__ vz,
            const float *__restrict__ I1, const float *__restrict__ jdn, float *__restrict__ jt,
-           float *__restrict__ jd, Dims d, float sx, float sy, float sz) {
+           float *__restrict__ jd, Dims d, Spacing3 S) {
   const int N = d.nx * d.ny * d.nz;
   const int i = (int)xcd_block() * kLdmmBlock + (int)threadIdx.x;
   if (i >= N) return;
   const int x = i % d.nx, r = i / d.nx, y = r % d.ny, z = r / d.ny;
-  const float px = (float)x + vx[i] / sx, py = (float)y + vy[i] / sy, pz = (float)z + vz[i] / sz;
+  const float px = (float)x + div_const(vx[i], S.s[0], S.r[0]);
+  const float py = (float)y + div_const(vy[i], S.s[1], S.r[1]);
+  const float pz = (float)z + div_const(vz[i], S.s[2], S.r[2]);
   const float3 hn = trilerp3g<AWK_BG_PARTIAL_ID>(make_buf(hx, N), make_buf(hy, N), make_buf(hz, N), d,
                                                  px, py, pz);
   nx_[i] = hn.x;
   ny_[i] = hn.y;
   nz_[i] = hn.z;
   jt[i] = trilerp1g<AWK_BG_VAL>(make_buf(I1, N), d, hn.x, hn.y, hn.z, 0.f);
-  const float det = jac_det_xsp_v(vx, vy, vz, i, x, y, z, d, sx, sy, sz);
+  const float det = jac_det_xsp_v(vx, vy, vz, i, x, y, z, d, S);
   jd[i] = jdn ? trilerp1g<AWK_BG_VAL>(make_buf(jdn, N), d, px, py, pz, 0.f) * det : det;
 }
 
@@ -306,15 +309,16 @@ k_orig_fwd(float *__restrict__ nx_, float *__restrict__ ny_, float *__restrict__
            const float *__restrict__ vx, const float *__restrict__ vy, const float *__restrict__ vz,
            const float *__restrict__ I0, float *__restrict__ J, const float *__restrict__ jt,
            const float *__restrict__ jd, float *__restrict__ m, float mscale,
-           const float *__restrict__ I1, double *__restrict__ partials, Dims d, float sx, float sy,
-           float sz) {
+           const float *__restrict__ I1, double *__restrict__ partials, Dims d, Spacing3 S) {
   const int N = d.nx * d.ny * d.nz;
   const int lb = (int)xcd_block();
   const int i = lb * kLdmmBlock + (int)threadIdx.x;
   double ssd = 0.0;
   if (i < N) {
     const int x = i % d.nx, r = i / d.nx, y = r % d.ny, z = r / d.ny;
-    const float px = (float)x - vx[i] / sx, py = (float)y - vy[i] / sy, pz = (float)z - vz[i] / sz;
+    const float px = (float)x - div_const(vx[i], S.s[0], S.r[0]);
+    const float py = (float)y - div_const(vy[i], S.s[1], S.r[1]);
+    const float pz = (float)z - div_const(vz[i], S.s[2], S.r[2]);
     const float3 hn = trilerp3g<AWK_BG_PARTIAL_ID>(make_buf(hx, N), make_buf(hy, N), make_buf(hz, N), d,
                                                    px, py, pz);
     nx_[i] = hn.x;
@@ -474,7 +478,7 @@ awk_status awk::ldmm_orig_backward(awk_ldmm *l, const float *I1, float *const v[
     AWK_LAUNCH(k_orig_bwd, dim3(l->egrid), dim3(kLdmmBlock), 0, s, hs, hs + N, hs + 2 * N, h,
                h + N, h + 2 * N, v[3 * t], v[3 * t + 1], v[3 * t + 2], I1,
                t == T - 1 ? nullptr : jd + (size_t)(t + 1) * N, jt + (size_t)t * N,
-               jd + (size_t)t * N, d, l->sp[0], l->sp[1], l->sp[2]);
+               jd + (size_t)t * N, d, spacing3(l->sp));
     AWK_LAUNCH_CHECK("k_orig_bwd");
     std::swap(h, hs);
   }
@@ -511,8 +515,8 @@ awk_status awk::ldmm_orig_forward(awk_ldmm *l, const float *I0, const float *I1,
                alpha ? nullptr : l->J + (size_t)t * N, last ? nullptr : jt + (size_t)(t + 1) * N,
                last ? nullptr : jd + (size_t)(t + 1) * N,
                alpha ? (last ? nullptr : alpha[t + 1]) : l->diff, alpha ? inv_sig2 : 1.f,
-               (last && !alpha) ? I1 : nullptr, (last && !alpha) ? l->partials : nullptr, d, sx, sy,
-               sz);
+               (last && !alpha) ? I1 : nullptr, (last && !alpha) ? l->partials : nullptr, d,
+               spacing3(l->sp));
     AWK_LAUNCH_CHECK("k_orig_fwd");
     std::swap(h, hs);
     if (last && !alpha) {
