@@ -320,8 +320,9 @@ def main():
                     help="also run a plain 1 GiB device copy (counter scaling reference)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (--gpus %d requested); the "
-              "multi-GPU design is subject sharding, see DESIGN.md" % args.gpus, file=sys.stderr)
+        print("bench.py: this path runs on one GPU only (--gpus %d requested); across GPUs it "
+              "would shard an atlas by subject, one process per GPU, with one all-reduce of the "
+              "mean image and energy per atlas iteration" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     if args.only:
