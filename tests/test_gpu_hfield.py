@@ -165,6 +165,32 @@ def test_splat_sorted_bit_exact(shape, trunc):
     np.testing.assert_array_equal(host(out), d)  # deterministic, also without key export
 
 
+@pytest.mark.parametrize("shape", [(16, 16, 16), (20, 24, 28)])
+def test_splat_sorted_collapse(shape):
+    """Every voxel displaced onto one point: 8 cells receive N contributions each,
+    segments far longer than a segment-sum tile (2048 pairs), so each head's walk
+    continues past several tile ends.  Bit-exact against the oracle."""
+    import torch
+    from atlaswerks_amd import hfield
+    from gpu_util import host, soa, to_dev
+    print("splat_sorted collapse", shape, flush=True)
+    nz, ny, nx = shape
+    rng = np.random.default_rng(7)
+    src = (rng.random(shape) + 0.05).astype(np.float32)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    target = np.array([nx * 0.43, ny * 0.57, nz * 0.38], dtype=np.float32)
+    v = np.stack([target[0] - x, target[1] - y, target[2] - z], axis=-1).astype(np.float32)
+    ref_d, ref_k, ref_v = oracle.splat_sorted(src, v, floor_fix=False)
+    out = torch.empty(shape, dtype=torch.float32, device="cuda")
+    keys = torch.empty(8 * src.size, dtype=torch.int32, device="cuda")
+    vals = torch.empty(8 * src.size, dtype=torch.float32, device="cuda")
+    hfield.splat_sorted(out, to_dev(src), to_dev(soa(v)), keys=keys, vals=vals)
+    np.testing.assert_array_equal(host(keys).view(np.uint32), ref_k)
+    np.testing.assert_array_equal(host(vals), ref_v)
+    np.testing.assert_array_equal(host(out), ref_d)
+    assert np.count_nonzero(ref_d) <= 8
+
+
 @pytest.mark.parametrize("shape,amp", [((17, 9, 5), 3.0), ((64, 48, 40), 3.0), ((40, 36, 100), 1.0),
                                        ((32, 32, 32), 40.0)])
 def test_splat_tiled_equals_direct(shape, amp):
