@@ -356,7 +356,23 @@ k_segment_sum(const unsigned *__restrict__ keys, const float *__restrict__ vals,
     if (k >= N || sk[l] == k) continue;  // past the pairs, the outside-volume cell, or not a head
     float acc = 0.f;
     int j = l;
-    while (j < kSegTile && sk[j + 1] == k) acc += sv[j++];
+    // four pairs per LDS round trip, added one by one in order while the key holds
+    bool open = true;
+    while (open && j + 4 <= kSegTile) {
+      const unsigned k0 = sk[j + 1], k1 = sk[j + 2], k2 = sk[j + 3], k3 = sk[j + 4];
+      const float v0 = sv[j], v1 = sv[j + 1], v2 = sv[j + 2], v3 = sv[j + 3];
+      if (k0 != k) { open = false; break; }
+      acc += v0;
+      if (k1 != k) { j += 1; open = false; break; }
+      acc += v1;
+      if (k2 != k) { j += 2; open = false; break; }
+      acc += v2;
+      if (k3 != k) { j += 3; open = false; break; }
+      acc += v3;
+      j += 4;
+    }
+    if (open)
+      while (j < kSegTile && sk[j + 1] == k) acc += sv[j++];
     if (j == kSegTile)
       for (long long g = t0 + kSegTile; g < M && keys[g] == k; ++g) acc += vals[g];
     dst[k] = acc;
