@@ -546,8 +546,12 @@ __device__ __forceinline__ void solve_one(const SolveArgs &s, int kx, int ky, in
   b2 = cscale(v2, s.scale);
 }
 
+// The three components' z-lines stay in registers through the solve; the
+// occupancy floor (3 blocks of 256 at N = 256, 4 below) makes the compiler fit
+// them in 168 / 128 registers without spilling, one more resident wave per SIMD
+// than it picks unasked (256^3 greedy: 111 -> 96 us).
 template <int N, int RA, int RB, int TX>
-__global__ void __launch_bounds__(TX *(N / RA > N / RB ? N / RA : N / RB))
+__global__ void __launch_bounds__(TX *(N / RA > N / RB ? N / RA : N / RB), N >= 256 ? 3 : 4)
 k_zsolve_reg(const float2 *__restrict__ tw, SolveArgs sa, int ny, int nxh, int nxh_pad,
              float2 *__restrict__ spec, size_t plane) {
   constexpr int PA = N / RA, PB = N / RB;  // PB = RA: threads holding the spectrum
