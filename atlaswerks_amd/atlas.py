@@ -15,7 +15,9 @@ carried to the next level as displacements from the grid-to-grid map, resampled
 with CLAMP and rescaled (ScaleVectorFields, GreedyAtlasBuilder.txx:34).
 """
 import ctypes
+import os
 
+import numpy as np
 import torch
 
 from . import _lib, hfield
@@ -229,6 +231,58 @@ class GreedyAtlasBuilderGPU:
         out = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda")
         _lib.call("awk_atlas_deformed_image", self._h, int(i), ptr(out), stream_handle(stream))
         return out
+
+    def WriteOutputs(self, out_dir=".", prefix="Greedy", suffix="nhdr", names=None, mean=True,
+                     def_images=True, def_fields_mean_to_im=False, def_fields_im_to_mean=False,
+                     energy=False):
+        """The reference's output files (GreedyAtlasBuilder::GenerateOutput and its
+        writers, GreedyAtlasBuilder.txx:395-590), without ITK:
+        ``<prefix>MeanImage.<suffix>``, ``<prefix><name>DefToMean.<suffix>``,
+        ``<prefix><name>DefFieldMeanToIm.<suffix>`` (Def1To0) and
+        ``<prefix><name>DefFieldImToMean.<suffix>`` (Def0To1, compute_inverse=True) as
+        voxel-coordinate hfields on the template grid (SaveHFieldITK writes h as
+        stored), and ``<prefix>Energy.xml``.  suffix: nhdr / nrrd (detached NRRD) or
+        mha / mhd (MetaImage).  names: per-subject names, the reference's input file
+        base names (default Image00, Image01, ...).  Returns the paths written."""
+        from .metaimage import Image, write_metaimage, write_nrrd
+        if suffix in ("nhdr", "nrrd"):
+            write = write_nrrd
+            suffix = "nhdr"
+        elif suffix in ("mha", "mhd"):
+            write = write_metaimage
+            suffix = "mha"
+        else:
+            raise ValueError("suffix must be nhdr, nrrd, mha or mhd")
+        ns = len(self.subjects)
+        names = list(names) if names is not None else ["Image%02d" % i for i in range(ns)]
+        if len(names) != ns:
+            raise ValueError("one name per subject")
+        f = self._factor or 1
+        sp = tuple(s * f for s in self.spacing)
+        os.makedirs(out_dir, exist_ok=True)
+        paths = []
+
+        def put(fname, arr):
+            path = os.path.join(out_dir, fname)
+            write(path, Image(arr, sp, self.origin))
+            paths.append(path)
+
+        if mean:
+            put("%sMeanImage.%s" % (prefix, suffix), self.Mean().cpu().numpy())
+        for i, name in enumerate(names):
+            if def_images:
+                put("%s%sDefToMean.%s" % (prefix, name, suffix), self.DeformedImage(i).cpu().numpy())
+            if def_fields_mean_to_im:
+                h = self.HField(i).cpu().numpy()
+                put("%s%sDefFieldMeanToIm.%s" % (prefix, name, suffix), np.moveaxis(h, 0, -1))
+            if def_fields_im_to_mean:
+                h = self.HFieldInverse(i).cpu().numpy()
+                put("%s%sDefFieldImToMean.%s" % (prefix, name, suffix), np.moveaxis(h, 0, -1))
+        if energy:
+            path = os.path.join(out_dir, "%sEnergy.xml" % prefix)
+            self.EnergyHistory().save_xml(path)
+            paths.append(path)
+        return paths
 
     def __del__(self):
         try:

@@ -5,6 +5,8 @@ Bars: hfields relative L2 <= 1e-4 after several iterations, the mean relative L2
 <= 1e-5, energies relative 1e-4 (float mean against the FP64 mean); hipGraph
 replays bit-identical to eager launches, replay after replay with host
 synchronisation between them."""
+import os
+
 import numpy as np
 import pytest
 
@@ -149,3 +151,55 @@ def test_atlas_inverse_multiscale_consistent():
     it = ET.fromstring(b.EnergyHistory().to_xml()).findall("Iteration")
     assert [(i.get("Scale"), i.get("Iter")) for i in it] == (
         [("0", str(k)) for k in range(10)] + [("1", str(k)) for k in range(10)])
+
+
+def test_atlas_write_outputs(tmp_path, golden_dir):
+    """The reference's GreedyAtlas output file set from the builder (NRRD, no ITK):
+    names as GreedyAtlasBuilder writes them, contents equal to the builder's
+    tensors, headers on the input images' grid, the energy history as XML."""
+    import xml.etree.ElementTree as ET
+
+    import torch
+    from atlaswerks_amd import atlas, metaimage
+    names = ["BullseyeTestBlur%02d" % i for i in range(4)]
+    subs = []
+    for n in names:
+        im = metaimage.read_nrrd(os.path.join(golden_dir, n + ".nhdr"))
+        subs.append(atlas.AtlasSubject(torch.from_numpy(im.data).cuda(), origin=im.origin,
+                                       spacing=im.spacing))
+    nz, ny, nx = subs[0].image.shape
+    b = atlas.GreedyAtlasBuilderGPU(subs, (nx, ny, nz), template_origin=subs[0].origin,
+                                    template_spacing=subs[0].spacing, max_pert=0.2, alpha=0.02,
+                                    beta=0.02, gamma=0.001, scale_levels=[(2, 3), (1, 3)],
+                                    compute_inverse=True)
+    b.Run()
+    paths = b.WriteOutputs(str(tmp_path), names=names, def_fields_mean_to_im=True,
+                           def_fields_im_to_mean=True, energy=True)
+    want = ["GreedyMeanImage.nhdr"]
+    for n in names:
+        want += ["Greedy%sDefToMean.nhdr" % n, "Greedy%sDefFieldMeanToIm.nhdr" % n,
+                 "Greedy%sDefFieldImToMean.nhdr" % n]
+    want.append("GreedyEnergy.xml")
+    assert sorted(os.path.basename(p) for p in paths) == sorted(want)
+    mean = metaimage.read_nrrd(str(tmp_path / "GreedyMeanImage.nhdr"))
+    assert np.array_equal(mean.data, b.Mean().cpu().numpy())
+    d1 = metaimage.read_nrrd(str(tmp_path / ("Greedy%sDefToMean.nhdr" % names[1])))
+    assert np.array_equal(d1.data, b.DeformedImage(1).cpu().numpy())
+    h2 = metaimage.read_nrrd(str(tmp_path / ("Greedy%sDefFieldMeanToIm.nhdr" % names[2])))
+    assert np.array_equal(h2.data, np.moveaxis(b.HField(2).cpu().numpy(), 0, -1))
+    g3 = metaimage.read_nrrd(str(tmp_path / ("Greedy%sDefFieldImToMean.nhdr" % names[3])))
+    assert np.array_equal(g3.data, np.moveaxis(b.HFieldInverse(3).cpu().numpy(), 0, -1))
+
+    def fields(path):
+        out = {}
+        for line in open(path):
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                out[k.strip()] = v.strip()
+        return out
+    ref = fields(os.path.join(golden_dir, names[0] + ".nhdr"))
+    got = fields(str(tmp_path / "GreedyMeanImage.nhdr"))
+    for k in ("sizes", "space directions", "space origin", "type", "endian", "encoding"):
+        assert got[k] == ref[k], (k, got[k], ref[k])
+    root = ET.parse(str(tmp_path / "GreedyEnergy.xml")).getroot()
+    assert root.tag == "EnergyHistory" and len(root.findall("Iteration")) == 6
