@@ -318,6 +318,56 @@ class LDMMAtlasBuilderGPU:
                   stream_handle(stream))
         return out
 
+    def WriteOutputs(self, out_dir=".", prefix="LDMMAtlas", suffix="mha", names=None,
+                     origin=(0.0, 0.0, 0.0), mean=True, def_to_mean_time=True, vel_fields=False,
+                     energy=False):
+        """LDMMAtlasBuilder's output files (its writers, LDMMAtlasBuilder.txx:566-775),
+        without ITK: ``<prefix>MeanImage.<suffix>``, every subject's image at every
+        time step ``<prefix><name>DefToMeanTime<tt>.<suffix>`` (WriteIntermediateImages,
+        GetI1AtT), the velocity fields ``<prefix><name>VelFieldTime<tt>.<suffix>``
+        (WriteVelFields: v as a 3-vector image) and ``<prefix>Energy.xml``, on the
+        current level's grid.  suffix: mha / mhd (MetaImage) or nhdr / nrrd.  names:
+        the reference's input file base names (default Image00, ...); origin: the
+        images' origin (the iteration does not use it).  Returns the paths written."""
+        import os
+
+        from .metaimage import Image, write_metaimage, write_nrrd
+        if suffix in ("nhdr", "nrrd"):
+            write, suffix = write_nrrd, "nhdr"
+        elif suffix in ("mha", "mhd"):
+            write, suffix = write_metaimage, "mha"
+        else:
+            raise ValueError("suffix must be nhdr, nrrd, mha or mhd")
+        S = len(self.images_orig)
+        names = list(names) if names is not None else ["Image%02d" % i for i in range(S)]
+        if len(names) != S:
+            raise ValueError("one name per subject")
+        f = self.levels[self._level][0]
+        sp = tuple(x * f for x in self.spacing)
+        os.makedirs(out_dir, exist_ok=True)
+        paths = []
+
+        def put(fname, arr):
+            path = os.path.join(out_dir, fname)
+            write(path, Image(arr, sp, origin))
+            paths.append(path)
+
+        if mean:
+            put("%sMeanImage.%s" % (prefix, suffix), self.Mean().cpu().numpy())
+        for s, name in enumerate(names):
+            for t in range(self.n_time_steps):
+                if def_to_mean_time:
+                    put("%s%sDefToMeanTime%02d.%s" % (prefix, name, t, suffix),
+                        self.DefToMean(s, t).cpu().numpy())
+                if vel_fields:
+                    put("%s%sVelFieldTime%02d.%s" % (prefix, name, t, suffix),
+                        self.v[s, t].permute(1, 2, 3, 0).contiguous().cpu().numpy())
+        if energy:
+            path = os.path.join(out_dir, "%sEnergy.xml" % prefix)
+            self.EnergyHistory().save_xml(path)
+            paths.append(path)
+        return paths
+
     def StepSize(self, s):
         out = ctypes.c_float()
         _lib.call("awk_ldmm_atlas_get_step", self._h, int(s), ctypes.byref(out))

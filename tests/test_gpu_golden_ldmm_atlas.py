@@ -23,7 +23,7 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def test_ldmm_atlas_reproduces_reference_outputs():
+def test_ldmm_atlas_reproduces_reference_outputs(tmp_path):
     import torch
     from atlaswerks_amd import ldmm, metaimage
     print("golden LDMM atlas: 4 bullseyes, 2 levels x 100 iterations, T = 10", flush=True)
@@ -50,3 +50,18 @@ def test_ldmm_atlas_reproduces_reference_outputs():
             assert np.abs(got - want).max() <= 5e-5, (s, t)
     e = b.energies()
     assert e.shape == (200, 3) and np.all(np.isfinite(e))
+    # the output files as the reference application writes them: every fixture is
+    # one of them, under the same name, with the same contents to the same bars
+    written = b.WriteOutputs(str(tmp_path), prefix="LDMMAtlas", suffix="mha",
+                             names=["BullseyeTestBlur%02d" % i for i in range(4)],
+                             origin=raw[0].origin, energy=True)
+    names = {os.path.basename(p) for p in written}
+    assert len(names) == 1 + 4 * 10 + 1
+    for fx in sorted(os.listdir(os.path.join(GOLD, "ldmm_atlas"))):
+        assert fx in names, fx
+        got = metaimage.read_metaimage(str(tmp_path / fx))
+        want = metaimage.read_metaimage(os.path.join(GOLD, "ldmm_atlas", fx))
+        assert got.data.shape == want.data.shape
+        np.testing.assert_allclose(got.spacing, want.spacing, rtol=1e-5)
+        bar = 2e-5 if fx.endswith("MeanImage.mha") else 5e-5
+        assert np.abs(got.data.astype(np.float64) - want.data).max() <= bar, fx
