@@ -196,3 +196,61 @@ class GreedyWarp:
         GreedyWarp::WriteEnergy's file, GreedyWarp.txx:220-230)."""
         from .energy_history import EnergyHistory
         return EnergyHistory.from_levels(list(enumerate(self.energy_history)))
+
+    def WriteOutputs(self, name, out_dir=".", prefix="Greedy", suffix="mha", origin=(0.0, 0.0, 0.0),
+                     def_image=True, inv_def_image=False, def_field=False, inv_def_field=False,
+                     energy=False):
+        """GreedyWarp's output files (GreedyWarp.txx:150-230) after RunWarp, without
+        ITK: ``<prefix><name>DefImage.<suffix>`` (I0 at Def1To0),
+        ``<prefix><name>InvDefImage.<suffix>`` (GetI1At0: I1 at Def0To1 with
+        compute_inverse; without it the reference applies I0 through the zeroth-order
+        inverse of Def1To0, GreedyDeformationData.cxx:66-79, and so does this),
+        ``<prefix><name>DefField1To0`` / ``DefField0To1`` (voxel-coordinate hfields)
+        and ``<prefix><name>Energy.xml``.  name: the reference's
+        ``<moving base>_to_<static base>``.  suffix: mha / mhd or nhdr / nrrd."""
+        import os
+
+        import numpy as np
+
+        from .metaimage import Image, write_metaimage, write_nrrd
+        if self.data is None:
+            raise ValueError("RunWarp first")
+        if suffix in ("nhdr", "nrrd"):
+            write, suffix = write_nrrd, "nhdr"
+        elif suffix in ("mha", "mhd"):
+            write, suffix = write_metaimage, "mha"
+        else:
+            raise ValueError("suffix must be nhdr, nrrd, mha or mhd")
+        d = self.data
+        os.makedirs(out_dir, exist_ok=True)
+        paths = []
+
+        def put(kind, arr):
+            path = os.path.join(out_dir, "%s%s%s.%s" % (prefix, name, kind, suffix))
+            write(path, Image(arr, self.spacing, origin))
+            paths.append(path)
+
+        if def_image:
+            out = torch.empty_like(self.I0orig)
+            hfield.apply(out, self.I0orig, d.Def1To0)
+            put("DefImage", out.cpu().numpy())
+        if inv_def_image:
+            out = torch.empty_like(self.I0orig)
+            if self.compute_inverse:
+                hfield.apply(out, self.I1orig, d.Def0To1)
+            else:
+                hinv = torch.empty_like(d.Def1To0)
+                hfield.computeInverseZerothOrder(d.Def1To0, hinv)
+                hfield.apply(out, self.I0orig, hinv)
+            put("InvDefImage", out.cpu().numpy())
+        if def_field:
+            put("DefField1To0", np.moveaxis(d.Def1To0.cpu().numpy(), 0, -1))
+        if inv_def_field:
+            if not self.compute_inverse:
+                raise ValueError("DefField0To1 needs compute_inverse=True")
+            put("DefField0To1", np.moveaxis(d.Def0To1.cpu().numpy(), 0, -1))
+        if energy:
+            path = os.path.join(out_dir, "%s%sEnergy.xml" % (prefix, name))
+            self.EnergyHistory().save_xml(path)
+            paths.append(path)
+        return paths

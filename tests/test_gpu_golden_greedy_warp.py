@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def test_greedy_warp_reproduces_reference_output():
+def test_greedy_warp_reproduces_reference_output(tmp_path):
     import torch
     from atlaswerks_amd import greedy, hfield, metaimage
     print("golden GreedyWarp: bullseye 00 -> 01, levels 2 -> 1, 50 + 50 iterations", flush=True)
@@ -40,3 +40,11 @@ def test_greedy_warp_reproduces_reference_output():
     got = out.cpu().numpy().astype(np.float64)
     assert np.linalg.norm(got - want) / np.linalg.norm(want) <= 1e-4
     assert np.abs(got - want).max() <= 2e-4
+    # the reference application's output file, under its name, to the same bar
+    paths = w.WriteOutputs("BullseyeTestBlur00_to_BullseyeTestBlur01", str(tmp_path), origin=a.origin,
+                           inv_def_image=True, def_field=True, inv_def_field=True, energy=True)
+    fx = "GreedyBullseyeTestBlur00_to_BullseyeTestBlur01DefImage.mha"
+    assert fx in {os.path.basename(p) for p in paths} and len(paths) == 5
+    written = metaimage.read_metaimage(str(tmp_path / fx))
+    assert np.array_equal(written.data, out.cpu().numpy())
+    np.testing.assert_allclose(written.spacing, a.spacing, rtol=1e-6)
