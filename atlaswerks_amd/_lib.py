@@ -9,7 +9,8 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libawk_hip.so")
 
-AWK_OK, AWK_ERR_INVALID, AWK_ERR_HIP, AWK_ERR_ALLOC, AWK_ERR_UNSUPPORTED = range(5)
+AWK_OK, AWK_ERR_INVALID, AWK_ERR_HIP, AWK_ERR_ALLOC, AWK_ERR_UNSUPPORTED, AWK_ERR_NUMERIC = range(6)
+FLAG_NAN_ENERGY, FLAG_ENERGY_INCREASE, FLAG_BAD_STEP = 1, 2, 4
 BG_ID, BG_ZERO, BG_PARTIAL_ID, BG_PARTIAL_ZERO, BG_CLAMP, BG_WRAP, BG_VAL = range(7)
 FK_EIGEN, FK_DIVFREE = 1, 2
 SPLAT_FIXED64, SPLAT_FIXED20, SPLAT_FLOAT = range(3)
@@ -76,8 +77,8 @@ SIGNATURES = {
     "awk_ldmm_set_gradient": [_P, _I],
     "awk_ldmm_set_splat": [_P, _I],
     "awk_ldmm_update_step_next": [_P],
-    "awk_ldmm_set_step": [_P, _F],
-    "awk_ldmm_get_step": [_P, ctypes.POINTER(_F)],
+    "awk_ldmm_set_step": [_P, _F, _P],
+    "awk_ldmm_get_step": [_P, ctypes.POINTER(_F), _P],
     "awk_ldmm_iterate": [_P, _P, _P, _P, _P, _I, _P],
     "awk_ldmm_alphas": [_P, _P, _P, _P, _P, _P],
     "awk_ldmm_deformed_image": [_P, _I, _P, _P],
@@ -87,17 +88,21 @@ SIGNATURES = {
     "awk_ldmm_atlas_set_gradient": [_P, _I],
     "awk_ldmm_atlas_set_splat": [_P, _I],
     "awk_ldmm_atlas_update_step_next": [_P],
-    "awk_ldmm_atlas_set_step": [_P, _F],
-    "awk_ldmm_atlas_get_step": [_P, _I, ctypes.POINTER(_F)],
+    "awk_ldmm_atlas_set_step": [_P, _F, _P],
+    "awk_ldmm_atlas_get_step": [_P, _I, ctypes.POINTER(_F), _P],
     "awk_ldmm_atlas_iterate": [_P, _I, _P, _I, _P],
     "awk_ldmm_atlas_mean": [_P, _P, _P],
     "awk_ldmm_atlas_deformed_image": [_P, _I, _I, _P, _P],
     "awk_greedy_update_step_next": [_P],
-    "awk_greedy_set_step": [_P, _F],
-    "awk_greedy_get_step": [_P, ctypes.POINTER(_F)],
+    "awk_greedy_set_step": [_P, _F, _P],
+    "awk_greedy_get_step": [_P, ctypes.POINTER(_F), _P],
     "awk_greedy_set_auto_step_reduce": [_P, _I],
     "awk_greedy_set_profiling": [_P, _I],
     "awk_greedy_get_profile": [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)],
+    "awk_greedy_get_flags": [_P, ctypes.POINTER(ctypes.c_uint), _I, _P],
+    "awk_atlas_get_flags": [_P, ctypes.POINTER(ctypes.c_uint), _I, _P],
+    "awk_ldmm_get_flags": [_P, ctypes.POINTER(ctypes.c_uint), _I, _P],
+    "awk_ldmm_atlas_get_flags": [_P, ctypes.POINTER(ctypes.c_uint), _I, _P],
     "awk_greedy_iterate": [_P, _P, _P, ctypes.POINTER(_P3), ctypes.POINTER(_P3),
                            ctypes.POINTER(_P3), ctypes.POINTER(_P3), _P, _P],
 }
@@ -111,6 +116,11 @@ class AwkError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("awk status %d: %s" % (code, msg))
         self.code = code
+
+
+class NumericalError(AwkError, FloatingPointError):
+    """AWK_ERR_NUMERIC: a NaN energy or an invalid step size (the reference's
+    "NaN encountered" / "max displacement is NaN" exceptions)."""
 
 
 def load():
@@ -130,8 +140,20 @@ def load():
 
 def check(code):
     if code != AWK_OK:
-        raise AwkError(code, load().awk_last_error().decode())
+        cls = NumericalError if code == AWK_ERR_NUMERIC else AwkError
+        raise cls(code, load().awk_last_error().decode())
     return code
+
+
+def get_flags(name, handle, clear=True, stream=None):
+    """(flags, status) of an awk_*_get_flags call, without raising on
+    AWK_ERR_NUMERIC; waits for the stream."""
+    from .device import stream_handle
+    f = ctypes.c_uint(0)
+    code = getattr(load(), name)(handle, ctypes.byref(f), int(bool(clear)), stream_handle(stream))
+    if code not in (AWK_OK, AWK_ERR_NUMERIC):
+        check(code)
+    return f.value, code
 
 
 def call(name, *args):

@@ -16,6 +16,7 @@ with CLAMP and rescaled (ScaleVectorFields, GreedyAtlasBuilder.txx:34).
 """
 import ctypes
 import os
+import warnings
 
 import numpy as np
 import torch
@@ -101,6 +102,7 @@ class GreedyAtlasBuilderGPU:
                     raise ValueError("inverse fields need every subject on the template grid")
         self._energy = []
         self._energy_level = []
+        self.flags = 0  # numerical flags of the run (FLAG_* of _lib)
         self._level_index = -1
         self._level = None
         self._factor = None
@@ -177,9 +179,20 @@ class GreedyAtlasBuilderGPU:
         self._energy_level.append(self._level_index)
 
     def RunScaleLevel(self, n_iter):
-        """One scale level: step sizes reset, then n_iter iterations."""
+        """One scale level: step sizes reset, then n_iter iterations.  A NaN energy
+        is reported as GreedyDeformationData::AddEnergy logs it
+        (GreedyDeformationData.cxx:170-174): a warning, and the run goes on."""
         self.UpdateStepSizeNextIteration()
         self.Iterate(n_iter)
+        flags = self.GetFlags()
+        self.flags |= flags
+        if flags & _lib.FLAG_NAN_ENERGY:
+            warnings.warn("NaN detected in an atlas deformation (scale level %d)" % self._level_index,
+                          RuntimeWarning)
+
+    def GetFlags(self, clear=True, stream=None):
+        """Numerical flags of all subjects since the last clear (FLAG_* of _lib)."""
+        return _lib.get_flags("awk_atlas_get_flags", self._h, clear, stream)[0]
 
     def Run(self):
         """All scale levels (AtlasThread::AtlasThreadMain)."""

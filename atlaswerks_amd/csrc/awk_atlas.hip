@@ -48,20 +48,6 @@ __global__ void k_atlas_energy(const double *__restrict__ esub, const float *__r
   *cursor = c + 1;
 }
 
-__global__ void k_atlas_step(const float *__restrict__ maxpart, int n, float max_pert,
-                             float *__restrict__ step) {
-  __shared__ float sm[256];
-  float m = 0.f;
-  m = strided_fold(maxpart, n, (int)threadIdx.x, 256, m, FoldMax());
-  sm[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sm[threadIdx.x] = fmaxf(sm[threadIdx.x], sm[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *step = max_pert / sm[0];
-}
-
 }  // namespace awk
 
 namespace {
@@ -112,6 +98,7 @@ struct awk_atlas {
   float *maxpart = nullptr;
   double *hist = nullptr;
   int *cursor = nullptr;
+  unsigned *flags = nullptr;  // sticky AWK_FLAG_* word of all subjects
   long long host_cursor = 0;
   float *vbuf = nullptr;  // 3 planes: velocity of a step-size iteration
   hipStream_t as = nullptr;
@@ -141,7 +128,7 @@ static void atlas_free(awk_atlas *a) {
       if (p) (void)hipFree(p);
   }
   void *b[] = {a->mean, (void *)a->idef_tab, a->w_dev, a->esub, a->partials, a->maxpart,
-               a->hist, a->cursor, a->vbuf};
+               a->hist, a->cursor, a->vbuf, a->flags};
   for (void *p : b)
     if (p) (void)hipFree(p);
   for (Lane &L : a->lanes) {
@@ -214,6 +201,7 @@ static awk_status atlas_iteration(awk_atlas *a, hipStream_t s) {
     ra.have_prev = sb.have_prev;
     ra.step = sb.step;
     ra.auto_reduce = 0;
+    ra.flags = a->flags;
     if ((st = fk_ypass(k, false, ls, &ra))) return st;
     if ((st = fk_zsolve(k, true, ls))) return st;
     if ((st = fk_ypass(k, true, ls, nullptr))) return st;
@@ -230,8 +218,7 @@ static awk_status atlas_iteration(awk_atlas *a, hipStream_t s) {
       ia.maxpart = maxpart;
       if ((st = fk_xinv(k, ia, ls, &nb))) return st;
       if (sb.update_step) {
-        AWK_LAUNCH(k_atlas_step, dim3(1), dim3(256), 0, ls, maxpart, nb, a->max_pert, sb.step);
-        AWK_LAUNCH_CHECK("k_atlas_step");
+        if ((st = launch_step_from_max(maxpart, nb, a->max_pert, sb.step, a->flags, ls))) return st;
       }
       const float *vx = vbuf, *vy = vbuf + N, *vz = vbuf + 2 * (size_t)N;
       if ((st = launch_compose_hv(hn, hn + N, hn + 2 * (size_t)N, h, h + N, h + 2 * (size_t)N, vx,
@@ -336,6 +323,8 @@ awk_status awk_atlas_create(awk_atlas_t *out, int nx, int ny, int nz, float spx,
   if (e == hipSuccess) e = hipMalloc(&a->hist, kHistCap * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&a->cursor, sizeof(int));
   if (e == hipSuccess) e = hipMemset(a->cursor, 0, sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&a->flags, sizeof(unsigned));
+  if (e == hipSuccess) e = hipMemset(a->flags, 0, sizeof(unsigned));
   if (e == hipSuccess) e = hipMalloc(&a->vbuf, 3 * N * sizeof(float));
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&a->as, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&a->ev_in, hipEventDisableTiming);
@@ -451,6 +440,14 @@ awk_status awk_atlas_update_step_next(awk_atlas_t a) {
   if (!a) return fail(AWK_ERR_INVALID, "awk_atlas_update_step_next: null handle");
   for (Subject &s : a->subj) s.update_step = true;
   return AWK_OK;
+}
+
+awk_status awk_atlas_get_flags(awk_atlas_t a, unsigned *flags, int clear, awk_stream_t stream) {
+  if (!a) return fail(AWK_ERR_INVALID, "awk_atlas_get_flags: null handle");
+  hipStream_t cs = as_stream(stream);
+  AWK_HIP(hipEventRecord(a->ev_out, a->as));
+  AWK_HIP(hipStreamWaitEvent(cs, a->ev_out, 0));
+  return read_flags(&a->flags, 1, flags, clear, cs, "awk_atlas_get_flags");
 }
 
 awk_status awk_atlas_set_graph(awk_atlas_t a, int on) {

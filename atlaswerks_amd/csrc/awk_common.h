@@ -35,6 +35,58 @@ awk_status hip_check(hipError_t e, const char *what);
 
 inline hipStream_t as_stream(awk_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
+// One device scalar written in stream order (a kernel argument carries the value,
+// so nothing on the host has to outlive the call and a graph may capture it).
+template <class T>
+__global__ void k_store_scalar(T *p, T v) {
+  *p = v;
+}
+template <class T>
+inline hipError_t store_scalar_async(T *dev, T v, hipStream_t s) {
+  hipLaunchKernelGGL(k_store_scalar<T>, dim3(1), dim3(1), 0, s, dev, v);
+  return hipGetLastError();
+}
+// Splat accumulators: int64 fixed point with 32 fractional bits, formed in FP32: scaling by 2^32 is
+// exact, and the nearest-even rounding of that exact value to an integer is the
+// same in any precision (|w| < 2^31).
+__device__ __forceinline__ unsigned long long splat_q(float w) {
+  return (unsigned long long)(long long)rintf(w * 4294967296.0f);
+}
+// Back to float: one correctly rounded int64 -> float conversion and an exact
+// scaling by 2^-32.
+__device__ __forceinline__ float splat_unq(long long q) {
+  return (float)q * (1.0f / 4294967296.0f);
+}
+
+// Sticky numerical flags (AWK_FLAG_*): a vector atomic OR into the handle's word.
+__device__ __forceinline__ void raise_flags(unsigned *flags, unsigned f) {
+  if (flags && f) atomicOr(flags, f);
+}
+// Reads (and optionally clears) n device flag words after the work queued on s;
+// AWK_ERR_NUMERIC when a NaN energy or an invalid step is among them.
+awk_status read_flags(unsigned *const *dev, int n, unsigned *out, int clear, hipStream_t s,
+                      const char *who);
+
+// True when any of the three output planes (n floats each) overlaps any of the
+// three input planes: the update must then go through a staging buffer.
+inline bool planes_alias(const float *const out[3], const float *const in[3], size_t n) {
+  const uintptr_t bytes = n * sizeof(float);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      const uintptr_t o = reinterpret_cast<uintptr_t>(out[a]);
+      const uintptr_t i = reinterpret_cast<uintptr_t>(in[b]);
+      if (o < i + bytes && i < o + bytes) return true;
+    }
+  return false;
+}
+// One device scalar read back after the work queued on s.
+template <class T>
+inline hipError_t load_scalar_sync(T *host, const T *dev, hipStream_t s) {
+  hipError_t e = hipMemcpyAsync(host, dev, sizeof(T), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e;
+}
+
 // Kernel-attached timing events: while set, launches go through
 // hipExtLaunchKernelGGL with these start/stop events, which the dispatch itself
 // timestamps (kernel duration only, the figure rocprof reports).

@@ -63,6 +63,7 @@ struct ReduceArgs {
   int *have_prev;
   float *step;             // halved on an energy increase when auto_reduce
   int auto_reduce;
+  unsigned *flags;         // sticky AWK_FLAG_* word (may be null)
 };
 
 struct XInvArgs {
@@ -103,6 +104,11 @@ awk_status fk_prepare_eigen(FluidKernel &k, int dir, hipStream_t s);
 // Pass launchers.
 awk_status fk_xfwd(FluidKernel &k, const XFwdArgs &a, hipStream_t s, int *nblocks);
 awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s, const ReduceArgs *red);
+// step = MaxPert / max over n per-block maxima (GreedyIteratorCPU.cxx:65-79,
+// 129-134; LDMMIteratorGPU.cxx:286-289); a NaN, infinite or zero maximum raises
+// AWK_FLAG_BAD_STEP in flags (may be null).  The step is stored as computed.
+awk_status launch_step_from_max(const float *maxpart, int n, float max_pert, float *step,
+                                unsigned *flags, hipStream_t s);
 awk_status fk_zsolve(FluidKernel &k, bool inverse_op, hipStream_t s);
 awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblocks);
 int fk_xblocks(const FluidKernel &k);

@@ -78,7 +78,13 @@ __device__ __forceinline__ void energy_block(const ReduceArgs &ra, double *tree,
   if (threadIdx.x == 0) {
     const double e = ra.scale * tree[0];
     if (ra.energy_out) *ra.energy_out = e;
-    if (ra.auto_reduce && *ra.have_prev && e - *ra.prev > 0.0) *ra.step = *ra.step * 0.5f;
+    const bool have = *ra.have_prev != 0;
+    const double change = have ? e - *ra.prev : 0.0;
+    unsigned f = 0;
+    if (e != e || change != change) f |= AWK_FLAG_NAN_ENERGY;
+    if (change > 0.0) f |= AWK_FLAG_ENERGY_INCREASE;
+    raise_flags(ra.flags, f);
+    if (ra.auto_reduce && change > 0.0) *ra.step = *ra.step * 0.5f;
     *ra.prev = e;
     *ra.have_prev = 1;
   }
@@ -94,7 +100,7 @@ __device__ __forceinline__ float load_mul(const XFwdArgs &a, size_t i) {
   if (a.mode == XLOAD_GRADMUL_FX) {
     const long long q = a.acc[i];
     a.acc[i] = 0;
-    return (float)((double)q * (1.0 / 4294967296.0));
+    return splat_unq(q);
   }
   return __builtin_nontemporal_load(a.i1 + i);
 }

@@ -26,7 +26,8 @@ namespace awk {
 
 // step = MaxPert / max |v| (GreedyIteratorCPU.cxx:65-79, 129-134)
 __global__ void __launch_bounds__(256)
-k_step_from_max(const float *__restrict__ maxpart, int n, float max_pert, float *__restrict__ step) {
+k_step_from_max(const float *__restrict__ maxpart, int n, float max_pert, float *__restrict__ step,
+                unsigned *__restrict__ flags) {
   __shared__ float s[256];
   float m = 0.f;
   m = strided_fold(maxpart, n, (int)threadIdx.x, 256, m, FoldMax());
@@ -36,7 +37,18 @@ k_step_from_max(const float *__restrict__ maxpart, int n, float max_pert, float 
     if ((int)threadIdx.x < w) s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) *step = max_pert / s[0];
+  if (threadIdx.x == 0) {
+    const float mx = s[0];
+    *step = max_pert / mx;
+    if (!(mx > 0.f) || !(mx <= 3.402823466e38f)) raise_flags(flags, AWK_FLAG_BAD_STEP);
+  }
+}
+
+awk_status launch_step_from_max(const float *maxpart, int n, float max_pert, float *step,
+                                unsigned *flags, hipStream_t s) {
+  AWK_LAUNCH(k_step_from_max, dim3(1), dim3(256), 0, s, maxpart, n, max_pert, step, flags);
+  AWK_LAUNCH_CHECK("k_step_from_max");
+  return AWK_OK;
 }
 
 }  // namespace awk
@@ -56,6 +68,7 @@ struct awk_greedy {
   float *step = nullptr;  // device scalar
   double *prev = nullptr;
   int *have_prev = nullptr;
+  unsigned *flags = nullptr;  // sticky AWK_FLAG_* word
   int nblk = 0;
   // optional per-kernel timing (awk_greedy_set_profiling): start/stop events
   // attached to each stage's kernel dispatch
@@ -85,6 +98,7 @@ static void greedy_free(awk_greedy *g) {
   if (g->step) (void)hipFree(g->step);
   if (g->prev) (void)hipFree(g->prev);
   if (g->have_prev) (void)hipFree(g->have_prev);
+  if (g->flags) (void)hipFree(g->flags);
 }
 
 extern "C" {
@@ -125,6 +139,8 @@ awk_status awk_greedy_create(awk_greedy_t *out, int nx, int ny, int nz, float sp
   if (e == hipSuccess) e = hipMalloc(&g->step, sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&g->prev, sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&g->have_prev, sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&g->flags, sizeof(unsigned));
+  if (e == hipSuccess) e = hipMemset(g->flags, 0, sizeof(unsigned));
   if (e == hipSuccess) e = hipMemset(g->step, 0, sizeof(float));
   if (e == hipSuccess) e = hipMemset(g->prev, 0, sizeof(double));
   if (e == hipSuccess) e = hipMemset(g->have_prev, 0, sizeof(int));
@@ -151,18 +167,22 @@ awk_status awk_greedy_update_step_next(awk_greedy_t g) {
   return AWK_OK;
 }
 
-awk_status awk_greedy_set_step(awk_greedy_t g, float step) {
+awk_status awk_greedy_set_step(awk_greedy_t g, float step, awk_stream_t stream) {
   if (!g) return fail(AWK_ERR_INVALID, "awk_greedy_set_step: null handle");
-  AWK_HIP(hipMemcpy(g->step, &step, sizeof(float), hipMemcpyHostToDevice));
+  AWK_HIP(store_scalar_async(g->step, step, as_stream(stream)));
   g->update_step = false;
   return AWK_OK;
 }
 
-awk_status awk_greedy_get_step(awk_greedy_t g, float *step) {
+awk_status awk_greedy_get_step(awk_greedy_t g, float *step, awk_stream_t stream) {
   if (!g || !step) return fail(AWK_ERR_INVALID, "awk_greedy_get_step: null argument");
-  AWK_HIP(hipDeviceSynchronize());
-  AWK_HIP(hipMemcpy(step, g->step, sizeof(float), hipMemcpyDeviceToHost));
+  AWK_HIP(load_scalar_sync(step, g->step, as_stream(stream)));
   return AWK_OK;
+}
+
+awk_status awk_greedy_get_flags(awk_greedy_t g, unsigned *flags, int clear, awk_stream_t stream) {
+  if (!g) return fail(AWK_ERR_INVALID, "awk_greedy_get_flags: null handle");
+  return read_flags(&g->flags, 1, flags, clear, as_stream(stream), "awk_greedy_get_flags");
 }
 
 awk_status awk_greedy_set_auto_step_reduce(awk_greedy_t g, int on) {
@@ -224,6 +244,7 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
   ra.have_prev = g->have_prev;
   ra.step = g->step;
   ra.auto_reduce = g->auto_reduce;
+  ra.flags = g->flags;
   if ((st = fk_ypass(g->k, false, s, &ra))) return st;
   g->stage(3);
   if ((st = fk_zsolve(g->k, true, s))) return st;
@@ -231,7 +252,7 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
   if ((st = fk_ypass(g->k, true, s, nullptr))) return st;
   g->stage(5);
   // 5. velocity -> hfield update
-  const bool in_place = (h_out[0] == h_in[0]);
+  const bool in_place = planes_alias(h_out, h_in, N);
   float *dst[3];
   for (int c = 0; c < 3; ++c) dst[c] = in_place ? g->hbuf + c * N : h_out[c];
   XInvArgs ia;
@@ -243,8 +264,7 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
     ia.maxpart = g->maxpart;
     if ((st = fk_xinv(g->k, ia, s, &nb))) return st;
     if (g->update_step) {
-      AWK_LAUNCH(k_step_from_max, dim3(1), dim3(256), 0, s, g->maxpart, nb, g->max_pert, g->step);
-      AWK_LAUNCH_CHECK("k_step_from_max");
+      if ((st = launch_step_from_max(g->maxpart, nb, g->max_pert, g->step, g->flags, s))) return st;
       g->update_step = false;
     }
     float *vx = g->v, *vy = g->v + N, *vz = g->v + 2 * N;
@@ -256,7 +276,7 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
         AWK_HIP(hipMemcpyAsync(h_out[c], g->hbuf + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, s));
     if (inv) {
       // composeVHInv(v * step, hinv, PARTIAL_ZERO) (GreedyIteratorCPU.cxx:89-93)
-      const bool inv_in_place = (hinv_out[0] == hinv_in[0]);
+      const bool inv_in_place = planes_alias(hinv_out, hinv_in, N);
       float *idst[3];
       for (int c = 0; c < 3; ++c) idst[c] = inv_in_place ? g->hbuf + c * N : hinv_out[c];
       if ((st = launch_compose_vh(idst[0], idst[1], idst[2], vx, vy, vz, hinv_in[0], hinv_in[1],

@@ -26,6 +26,24 @@ awk_status hip_check(hipError_t e, const char *what) {
   return AWK_ERR_HIP;
 }
 
+awk_status read_flags(unsigned *const *dev, int n, unsigned *out, int clear, hipStream_t s,
+                      const char *who) {
+  if (!out) return fail(AWK_ERR_INVALID, std::string(who) + ": null flags pointer");
+  unsigned acc = 0;
+  for (int i = 0; i < n; ++i) {
+    unsigned f = 0;
+    AWK_HIP(load_scalar_sync(&f, dev[i], s));
+    acc |= f;
+    if (clear) AWK_HIP(store_scalar_async(dev[i], 0u, s));
+  }
+  *out = acc;
+  if (acc & (AWK_FLAG_NAN_ENERGY | AWK_FLAG_BAD_STEP))
+    return fail(AWK_ERR_NUMERIC, std::string(who) + ((acc & AWK_FLAG_NAN_ENERGY)
+                                                         ? ": NaN encountered in the energy"
+                                                         : ": max displacement is NaN, infinite or zero"));
+  return AWK_OK;
+}
+
 constexpr int kBlock = 256;
 
 LaunchEvents *&launch_events() {

@@ -20,6 +20,7 @@ struct awk_ldmm {
   float *maxpart = nullptr;    // nt * nblk
   float *step = nullptr;
   double *energy = nullptr;    // 3 doubles (internal when the caller passes none)
+  unsigned *flags = nullptr;   // sticky AWK_FLAG_* word
   int nblk = 0, egrid = 0;
   int splat_mode = 0;          // IterateNew's splat: 0 int64 fixed point (LDS tiles), 1 sort/scan
   void *sort_work = nullptr;   // sort/scan scratch

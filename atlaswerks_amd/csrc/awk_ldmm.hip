@@ -103,7 +103,7 @@ k_ldmm_bwd(float *__restrict__ nx_, float *__restrict__ ny_, float *__restrict__
   const int N = d.nx * d.ny * d.nz;
   SplatPosLdmm p{hx, hy, hz, make_buf(vx, N), make_buf(vy, N), make_buf(vz, N), d, sx, sy, sz,
                  nx_, ny_, nz_};
-  splat_tile(p, diff, acc, d);
+  splat_tile<SplatShapeLdmm>(p, diff, acc, d);
 }
 
 // composeVHInv(v_t, h, PARTIAL_ZERO) alone, for the sort/scan splat: the same
@@ -138,7 +138,7 @@ k_ldmm_alpha(float *__restrict__ alpha, long long *__restrict__ acc, int N, floa
   if (i >= N) return;
   const long long q = acc[i];
   acc[i] = 0;
-  alpha[i] = (float)((double)q * (1.0 / 4294967296.0)) * inv_sig2;
+  alpha[i] = splat_unq(q) * inv_sig2;
 }
 
 __global__ void __launch_bounds__(kLdmmBlock) k_ldmm_zero(long long *__restrict__ acc, int N) {
@@ -164,7 +164,7 @@ k_ldmm_dot(const float *__restrict__ a, const float *__restrict__ b0, const floa
 // energy[0] = Prod(sp) * sum(diff^2) / sigma^2 (LDMMIteratorGPU.cxx:276-283,
 // LDMMEnergy.h:42); energy[1] = 0; energy[2] = energy[0].  Fixed order.
 __global__ void k_ldmm_image_energy(const double *__restrict__ partials, int n, double scale,
-                                    double *__restrict__ energy) {
+                                    double *__restrict__ energy, unsigned *__restrict__ flags) {
   __shared__ double s[256];
   double a = 0.0;
   a = strided_fold(partials, n, (int)threadIdx.x, 256, a, FoldAdd());
@@ -178,13 +178,14 @@ __global__ void k_ldmm_image_energy(const double *__restrict__ partials, int n, 
     energy[0] = scale * s[0];
     energy[1] = 0.0;
     energy[2] = energy[0];
+    if (energy[0] != energy[0]) raise_flags(flags, AWK_FLAG_NAN_ENERGY);
   }
 }
 
 // energy[1] += Prod(sp) <L v_t, v_t> / T (LDMMIteratorGPU.cxx:250-256,
 // LDMMEnergy.cxx:57-64); energy[2] = energy[0] + energy[1].  Fixed order.
 __global__ void k_ldmm_vec_energy(const double *__restrict__ partials, int n, double scale,
-                                  double *__restrict__ energy) {
+                                  double *__restrict__ energy, unsigned *__restrict__ flags) {
   __shared__ double s[256];
   double a = 0.0;
   a = strided_fold(partials, n, (int)threadIdx.x, 256, a, FoldAdd());
@@ -197,22 +198,8 @@ __global__ void k_ldmm_vec_energy(const double *__restrict__ partials, int n, do
   if (threadIdx.x == 0) {
     energy[1] += scale * s[0];
     energy[2] = energy[0] + energy[1];
+    if (energy[1] != energy[1]) raise_flags(flags, AWK_FLAG_NAN_ENERGY);
   }
-}
-
-// step = MaxPert / max over all time steps and blocks (LDMMIteratorGPU.cxx:286-289)
-__global__ void k_ldmm_step(const float *__restrict__ maxpart, int n, float max_pert,
-                            float *__restrict__ step) {
-  __shared__ float s[256];
-  float m = 0.f;
-  m = strided_fold(maxpart, n, (int)threadIdx.x, 256, m, FoldMax());
-  s[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *step = max_pert / s[0];
 }
 
 // ---- IterateOrig (LDMMIteratorGPU.cxx:295-441, LDMMIteratorCPU.cxx:379-527) ----
@@ -342,7 +329,7 @@ __global__ void k_orig_energy_zero(double *__restrict__ energy) {
 // energy[0] = Prod(sp) sum(diff^2) / sigma^2 measured after the sweep
 // (LDMMIteratorGPU.cxx:430-441); energy[2] = energy[0] + energy[1].  Fixed order.
 __global__ void k_orig_image_energy(const double *__restrict__ partials, int n, double scale,
-                                    double *__restrict__ energy) {
+                                    double *__restrict__ energy, unsigned *__restrict__ flags) {
   __shared__ double s[256];
   double a = 0.0;
   a = strided_fold(partials, n, (int)threadIdx.x, 256, a, FoldAdd());
@@ -355,6 +342,7 @@ __global__ void k_orig_image_energy(const double *__restrict__ partials, int n, 
   if (threadIdx.x == 0) {
     energy[0] = scale * s[0];
     energy[2] = energy[0] + energy[1];
+    if (energy[0] != energy[0]) raise_flags(flags, AWK_FLAG_NAN_ENERGY);
   }
 }
 
@@ -365,7 +353,7 @@ using namespace awk;
 void awk::ldmm_release(awk_ldmm *l) {
   fk_release(l->k);
   void *bufs[] = {l->h,      l->hs,     l->J,  l->diff,      l->acc,   l->partials, l->maxpart,
-                  l->step,   l->energy, l->jt, l->jd, l->sort_work, l->msplat};
+                  l->step,   l->energy, l->jt, l->jd, l->sort_work, l->msplat, l->flags};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
 }
@@ -404,6 +392,8 @@ awk_status awk::ldmm_init(awk_ldmm *l, int nx, int ny, int nz, const float sp[3]
   if (terms && e == hipSuccess) e = hipMalloc(&l->jd, (size_t)l->nt * N * sizeof(float));
   if (e == hipSuccess) e = hipMemset(l->acc, 0, N * sizeof(long long));
   if (e == hipSuccess) e = hipMemset(l->step, 0, sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&l->flags, sizeof(unsigned));
+  if (e == hipSuccess) e = hipMemset(l->flags, 0, sizeof(unsigned));
   return e == hipSuccess ? AWK_OK : hip_check(e, "LDMM iterator allocation");
 }
 
@@ -433,7 +423,7 @@ static awk_status vec_energy_step(awk_ldmm *l, float *const v[], int t, float *l
   AWK_LAUNCH(k_ldmm_dot, dim3(l->egrid), dim3(kLdmmBlock), 0, s, lv, v[3 * t], v[3 * t + 1],
              v[3 * t + 2], N, l->partials);
   AWK_LAUNCH(k_ldmm_vec_energy, dim3(1), dim3(256), 0, s, l->partials, l->egrid, vol / l->nt,
-             energy);
+             energy, l->flags);
   AWK_LAUNCH_CHECK("vector energy");
   return AWK_OK;
 }
@@ -522,15 +512,15 @@ awk_status awk::ldmm_orig_forward(awk_ldmm *l, const float *I0, const float *I1,
     if (last && !alpha) {
       // reduce the SSD partials before the vector energy reuses the buffer
       AWK_LAUNCH(k_orig_image_energy, dim3(1), dim3(256), 0, s, l->partials, (int)egrid,
-                 vol * inv_sig2, energy);
+                 vol * inv_sig2, energy, l->flags);
       AWK_LAUNCH_CHECK("k_orig_image_energy");
     }
     if (!alpha && !step_call && vec_energy && (st = vec_energy_step(l, v, t, hs, energy, s)))
       return st;
   }
   if (step_call) {
-    AWK_LAUNCH(k_ldmm_step, dim3(1), dim3(256), 0, s, l->maxpart, T * l->nblk, l->max_pert, step);
-    AWK_LAUNCH_CHECK("k_ldmm_step");
+    if ((st = launch_step_from_max(l->maxpart, T * l->nblk, l->max_pert, step, l->flags, s)))
+      return st;
   }
   return AWK_OK;
 }
@@ -575,7 +565,7 @@ awk_status awk::ldmm_new_iterate(awk_ldmm *l, const float *I0, const float *I1, 
     std::swap(h, hs);
   }
   AWK_LAUNCH(k_ldmm_image_energy, dim3(1), dim3(256), 0, s, l->partials, (int)egrid,
-             vol / ((double)l->sigma * l->sigma), energy);
+             vol / ((double)l->sigma * l->sigma), energy, l->flags);
   AWK_LAUNCH_CHECK("k_ldmm_image_energy");
   // ---- backward: h_t = h_{t+1} - v_t(h_{t+1}) / sp; alpha_t = splat(diff, h_t)
   AWK_LAUNCH(k_ldmm_zero, dim3(lgrid), dim3(kLdmmBlock), 0, s, l->acc, N);
@@ -593,7 +583,7 @@ awk_status awk::ldmm_new_iterate(awk_ldmm *l, const float *I0, const float *I1, 
                                     nullptr, s, 1, l->sort_work)))
         return st;
     } else {
-      AWK_LAUNCH(k_ldmm_bwd, dim3(splat_tile_grid(nx, ny, nz)), dim3(kSplatTileThreads), 0, s, hs,
+      AWK_LAUNCH(k_ldmm_bwd, dim3(splat_tile_grid<SplatShapeLdmm>(nx, ny, nz)), dim3(kSplatTileThreads), 0, s, hs,
                  hs + N, hs + 2 * N, h, h + N, h + 2 * N, v[3 * t], v[3 * t + 1], v[3 * t + 2],
                  l->diff, reinterpret_cast<unsigned long long *>(l->acc), d, sx, sy, sz);
       AWK_LAUNCH_CHECK("k_ldmm_bwd");
@@ -618,8 +608,8 @@ awk_status awk::ldmm_new_iterate(awk_ldmm *l, const float *I0, const float *I1, 
     if (!step_call && vec_energy && (st = vec_energy_step(l, v, t, hs, energy, s))) return st;
   }
   if (step_call) {
-    AWK_LAUNCH(k_ldmm_step, dim3(1), dim3(256), 0, s, l->maxpart, T * l->nblk, l->max_pert, step);
-    AWK_LAUNCH_CHECK("k_ldmm_step");
+    if ((st = launch_step_from_max(l->maxpart, T * l->nblk, l->max_pert, step, l->flags, s)))
+      return st;
   }
   return AWK_OK;
 }
@@ -690,17 +680,21 @@ awk_status awk_ldmm_update_step_next(awk_ldmm_t l) {
   return AWK_OK;
 }
 
-awk_status awk_ldmm_set_step(awk_ldmm_t l, float step) {
+awk_status awk_ldmm_get_flags(awk_ldmm_t l, unsigned *flags, int clear, awk_stream_t stream) {
+  if (!l) return fail(AWK_ERR_INVALID, "awk_ldmm_get_flags: null handle");
+  return read_flags(&l->flags, 1, flags, clear, as_stream(stream), "awk_ldmm_get_flags");
+}
+
+awk_status awk_ldmm_set_step(awk_ldmm_t l, float step, awk_stream_t stream) {
   if (!l) return fail(AWK_ERR_INVALID, "awk_ldmm_set_step: null handle");
-  AWK_HIP(hipMemcpy(l->step, &step, sizeof(float), hipMemcpyHostToDevice));
+  AWK_HIP(store_scalar_async(l->step, step, as_stream(stream)));
   l->update_step = false;
   return AWK_OK;
 }
 
-awk_status awk_ldmm_get_step(awk_ldmm_t l, float *step) {
+awk_status awk_ldmm_get_step(awk_ldmm_t l, float *step, awk_stream_t stream) {
   if (!l || !step) return fail(AWK_ERR_INVALID, "awk_ldmm_get_step: null argument");
-  AWK_HIP(hipDeviceSynchronize());
-  AWK_HIP(hipMemcpy(step, l->step, sizeof(float), hipMemcpyDeviceToHost));
+  AWK_HIP(load_scalar_sync(step, l->step, as_stream(stream)));
   return AWK_OK;
 }
 

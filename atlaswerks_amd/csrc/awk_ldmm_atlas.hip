@@ -286,20 +286,29 @@ awk_status awk_ldmm_atlas_update_step_next(awk_ldmm_atlas_t a) {
   return AWK_OK;
 }
 
-awk_status awk_ldmm_atlas_set_step(awk_ldmm_atlas_t a, float step) {
+awk_status awk_ldmm_atlas_get_flags(awk_ldmm_atlas_t a, unsigned *flags, int clear,
+                                    awk_stream_t stream) {
+  if (!a) return fail(AWK_ERR_INVALID, "awk_ldmm_atlas_get_flags: null handle");
+  // the iterate calls join their lanes back into the caller's stream
+  std::vector<unsigned *> words{a->eng.flags};
+  for (LaLane &L : a->lanes) words.push_back(L.eng.flags);
+  return read_flags(words.data(), (int)words.size(), flags, clear, as_stream(stream),
+                    "awk_ldmm_atlas_get_flags");
+}
+
+awk_status awk_ldmm_atlas_set_step(awk_ldmm_atlas_t a, float step, awk_stream_t stream) {
   if (!a) return fail(AWK_ERR_INVALID, "awk_ldmm_atlas_set_step: null handle");
   for (LaSubject &sb : a->subs) {
-    AWK_HIP(hipMemcpy(sb.step, &step, sizeof(float), hipMemcpyHostToDevice));
+    AWK_HIP(store_scalar_async(sb.step, step, as_stream(stream)));
     sb.update_step = false;
   }
   return AWK_OK;
 }
 
-awk_status awk_ldmm_atlas_get_step(awk_ldmm_atlas_t a, int s, float *step) {
+awk_status awk_ldmm_atlas_get_step(awk_ldmm_atlas_t a, int s, float *step, awk_stream_t stream) {
   if (!a || !step) return fail(AWK_ERR_INVALID, "awk_ldmm_atlas_get_step: null argument");
   if (s < 0 || s >= a->S) return fail(AWK_ERR_INVALID, "awk_ldmm_atlas_get_step: subject out of range");
-  AWK_HIP(hipDeviceSynchronize());
-  AWK_HIP(hipMemcpy(step, a->subs[s].step, sizeof(float), hipMemcpyDeviceToHost));
+  AWK_HIP(load_scalar_sync(step, a->subs[s].step, as_stream(stream)));
   return AWK_OK;
 }
 

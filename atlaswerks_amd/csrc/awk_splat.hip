@@ -6,11 +6,14 @@
 //   FIXED64  int64 fixed point with 32 fractional bits: integer adds commute, so
 //            the result is bit-identical for any scheduling (deterministic);
 //            per-contribution rounding error <= 2^-33, |sum| < 2^31.  By default
-//            accumulated per block in LDS windows first (awk_splat_tile.h).
+//            accumulated per block in LDS windows first (awk_splat_tile.h), then
+//            converted to float by a pass that also clears the accumulator.
 //   FIXED20  the reference's int32 fixed point S2p20 = int(a * 2^20 + 0.5f)
 //            (CUDACommon/inc/cplMacro.h:17, 67-91), accumulated in the dst buffer
 //            itself and converted in place (FixedToFloating); |sum| < 2048.
 //   FLOAT    float atomics (fast; summation order, hence rounding, varies).
+// Buffers are cleared by fill kernels rather than memset nodes, so a captured
+// graph replays them correctly.
 #include "awk_ops.h"
 #include "awk_splat_tile.h"
 
@@ -40,8 +43,7 @@ k_splat(void *__restrict__ acc, const float *__restrict__ src, const float *__re
     // same factor order as the reference: mass * wx * wy * wz
     const float w = mass * wx[ox] * wy[oy] * wz[oz];
     if (MODE == AWK_SPLAT_FIXED64) {
-      const long long q = llrint((double)w * 4294967296.0);
-      atomicAdd(reinterpret_cast<unsigned long long *>(acc) + id, (unsigned long long)q);
+      atomicAdd(reinterpret_cast<unsigned long long *>(acc) + id, splat_q(w));
     } else if (MODE == AWK_SPLAT_FIXED20) {
       atomicAdd(reinterpret_cast<int *>(acc) + id, (int)(w * 1048576.f + 0.5f));
     } else {
@@ -54,17 +56,56 @@ __global__ void __launch_bounds__(kSplatTileThreads)
 k_splat_tiled(unsigned long long *__restrict__ acc, const float *__restrict__ src,
               const float *__restrict__ hx, const float *__restrict__ hy,
               const float *__restrict__ hz, Dims d) {
-  splat_tile(SplatPosH{hx, hy, hz}, src, acc, d);
+  splat_tile<SplatShapeEntry>(SplatPosH{hx, hy, hz}, src, acc, d);
 }
 
-__global__ void k_fixed64_to_float(float *__restrict__ dst, const long long *__restrict__ acc, int N) {
+template <class T>
+__global__ void k_fill(T *__restrict__ p, long long n, T v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// dst = acc * 2^-32 in float; the accumulator is left cleared for its next use.
+__global__ void k_fixed64_to_float(float *__restrict__ dst, long long *__restrict__ acc, int N) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (i < N) dst[i] = (float)((double)acc[i] * (1.0 / 4294967296.0));
+  if (i >= N) return;
+  const long long q = acc[i];
+  acc[i] = 0;
+  dst[i] = splat_unq(q);
 }
 
 __global__ void k_fixed20_to_float(float *__restrict__ dst, int N) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i < N) dst[i] = (float)reinterpret_cast<const int *>(dst)[i] / 1048576.f;
+}
+
+template <class T>
+awk_status fill_async(T *p, long long n, T v, hipStream_t s) {
+  if (n <= 0) return AWK_OK;
+  AWK_LAUNCH(k_fill<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  AWK_LAUNCH_CHECK("k_fill");
+  return AWK_OK;
+}
+
+awk_status clear_splat_acc(unsigned long long *acc, long long n, hipStream_t s) {
+  return fill_async(acc, n, 0ull, s);
+}
+
+awk_status launch_splat_acc(float *dst, const float *src, const float *hx, const float *hy,
+                            const float *hz, int nx, int ny, int nz, unsigned long long *acc,
+                            hipStream_t s) {
+  const long long N = (long long)nx * ny * nz;
+  if (tuning().splat_tile)
+    AWK_LAUNCH(k_splat_tiled, dim3(splat_tile_grid<SplatShapeEntry>(nx, ny, nz)),
+               dim3(kSplatTileThreads), 0, s, acc, src, hx, hy, hz, Dims{nx, ny, nz});
+  else
+    AWK_LAUNCH(k_splat<AWK_SPLAT_FIXED64>, dim3(grid_for(N, kSplatBlock)), dim3(kSplatBlock), 0,
+               s, (void *)acc, src, hx, hy, hz, nx, ny, nz);
+  AWK_LAUNCH_CHECK("k_splat<FIXED64>");
+  AWK_LAUNCH(k_fixed64_to_float, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, dst,
+             reinterpret_cast<long long *>(acc), (int)N);
+  AWK_LAUNCH_CHECK("k_fixed64_to_float");
+  return AWK_OK;
 }
 
 awk_status launch_splat(float *dst, const float *src, const float *hx, const float *hy,
@@ -73,23 +114,21 @@ awk_status launch_splat(float *dst, const float *src, const float *hx, const flo
   const unsigned grid = grid_for(N, kSplatBlock);
   const unsigned cgrid = (unsigned)((N + 255) / 256);
   if (mode == AWK_SPLAT_FIXED64) {
+    // a stream-ordered allocation: after the first call the pool hands the same
+    // block back without a device allocation
     void *acc = nullptr;
     AWK_HIP(hipMallocAsync(&acc, (size_t)N * 8, s));
-    AWK_HIP(hipMemsetAsync(acc, 0, (size_t)N * 8, s));
-    if (tuning().splat_tile)
-      AWK_LAUNCH(k_splat_tiled, dim3(splat_tile_grid(nx, ny, nz)), dim3(kSplatTileThreads), 0, s,
-                 reinterpret_cast<unsigned long long *>(acc), src, hx, hy, hz, Dims{nx, ny, nz});
-    else
-      AWK_LAUNCH(k_splat<AWK_SPLAT_FIXED64>, dim3(grid), dim3(kSplatBlock), 0, s, acc, src, hx,
-                 hy, hz, nx, ny, nz);
-    AWK_LAUNCH_CHECK("k_splat<FIXED64>");
-    AWK_LAUNCH(k_fixed64_to_float, dim3(cgrid), dim3(256), 0, s, dst,
-                       reinterpret_cast<const long long *>(acc), (int)N);
-    AWK_LAUNCH_CHECK("k_fixed64_to_float");
-    AWK_HIP(hipFreeAsync(acc, s));
+    awk_status st = clear_splat_acc(reinterpret_cast<unsigned long long *>(acc), N, s);
+    if (st == AWK_OK)
+      st = launch_splat_acc(dst, src, hx, hy, hz, nx, ny, nz,
+                            reinterpret_cast<unsigned long long *>(acc), s);
+    hipError_t e = hipFreeAsync(acc, s);
+    if (st != AWK_OK) return st;
+    AWK_HIP(e);
   } else if (mode == AWK_SPLAT_FIXED20) {
     if ((const void *)dst == (const void *)src) return fail(AWK_ERR_INVALID, "awk_splat: dst aliases src");
-    AWK_HIP(hipMemsetAsync(dst, 0, (size_t)N * 4, s));
+    awk_status st = fill_async(reinterpret_cast<int *>(dst), N, 0, s);
+    if (st != AWK_OK) return st;
     AWK_LAUNCH(k_splat<AWK_SPLAT_FIXED20>, dim3(grid), dim3(kSplatBlock), 0, s, (void *)dst,
                        src, hx, hy, hz, nx, ny, nz);
     AWK_LAUNCH_CHECK("k_splat<FIXED20>");
@@ -97,7 +136,8 @@ awk_status launch_splat(float *dst, const float *src, const float *hx, const flo
     AWK_LAUNCH_CHECK("k_fixed20_to_float");
   } else if (mode == AWK_SPLAT_FLOAT) {
     if ((const void *)dst == (const void *)src) return fail(AWK_ERR_INVALID, "awk_splat: dst aliases src");
-    AWK_HIP(hipMemsetAsync(dst, 0, (size_t)N * 4, s));
+    awk_status st = fill_async(dst, N, 0.f, s);
+    if (st != AWK_OK) return st;
     AWK_LAUNCH(k_splat<AWK_SPLAT_FLOAT>, dim3(grid), dim3(kSplatBlock), 0, s, (void *)dst,
                        src, hx, hy, hz, nx, ny, nz);
     AWK_LAUNCH_CHECK("k_splat<FLOAT>");

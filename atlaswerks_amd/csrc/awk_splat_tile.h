@@ -1,15 +1,22 @@
 // awk_splat_tile.h -- tiled fixed-point splatting shared by the splat entry point
 // and the LDMM backward sweep.
 //
-// A block takes a 64 x 8 x 8 tile of source voxels (16 per thread).  Pass A forms
-// every voxel's destination and the bounding box of the 8-corner footprints; when
-// the box (clipped to the volume) fits the LDS window, pass B accumulates the
+// A block takes a TX x TY x TZ tile of source voxels.  Pass A forms every voxel's
+// destination and the bounding box of the 8-corner footprints; when the box
+// (clipped to the volume) fits the LDS window of CAP cells, pass B accumulates the
 // int64 fixed-point contributions (2^-32 units, the same quantisation as the
-// direct kernel) with LDS atomics and one flush adds the non-zero window cells
-// to the global accumulator; otherwise pass B adds straight to global memory.
+// direct kernel) with LDS atomics and one flush adds the non-zero window cells to
+// the global accumulator; otherwise pass B adds straight to global memory.
 // Integer sums commute, so the result is bit-identical to the direct splat for
 // any tiling or scheduling.  Each thread handles the same voxels in both passes,
 // so a position it stored in pass A is read back by the same thread.
+//
+// Tile shapes: a tile's footprint is the tile itself widened by the variation of
+// the displacement across it.  For smooth fields of up to 8 voxels at 256^3 a
+// 16 x 8 x 8 tile's box stays under 4,100 cells (4,032 at most, 2,173 on average,
+// over all tiles of the sinusoid workload), where a 64 x 8 x 8 tile's exceeds
+// 7,168 in 99 % of tiles; the flush then issues about 1.3 global atomics per
+// source voxel instead of 8.
 #pragma once
 
 #include <climits>
@@ -18,13 +25,24 @@
 
 namespace awk {
 
-constexpr int kSTX = 64, kSTY = 8, kSTZ = 8;
 constexpr int kSplatTileThreads = 256;
-constexpr int kSplatTileVPT = kSTX * kSTY * kSTZ / kSplatTileThreads;  // 16
-constexpr int kSplatWinCap = 7168;                                      // 56 KiB of int64 cells
 
+template <int TX, int TY, int TZ, int CAP>
+struct SplatShape {
+  static constexpr int tx = TX, ty = TY, tz = TZ, cap = CAP;
+  static constexpr int vpt = TX * TY * TZ / kSplatTileThreads;
+  static_assert(TX * TY * TZ % kSplatTileThreads == 0, "tile must be a multiple of the block");
+  static_assert((TX & (TX - 1)) == 0 && (TY & (TY - 1)) == 0, "power-of-two tile rows");
+};
+// the splat entry points (1,024 voxels, 32 KiB window: five blocks per CU)
+using SplatShapeEntry = SplatShape<16, 8, 8, 4096>;
+// the LDMM backward sweep (4,096 voxels, 56 KiB window), as measured there
+using SplatShapeLdmm = SplatShape<64, 8, 8, 7168>;
+
+template <class S>
 inline unsigned splat_tile_grid(int nx, int ny, int nz) {
-  return (unsigned)(((nx + kSTX - 1) / kSTX) * ((ny + kSTY - 1) / kSTY) * ((nz + kSTZ - 1) / kSTZ));
+  return (unsigned)(((nx + S::tx - 1) / S::tx) * ((ny + S::ty - 1) / S::ty) *
+                    ((nz + S::tz - 1) / S::tz));
 }
 
 // Positions given by an hfield.
@@ -65,16 +83,13 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-// Fixed-point contribution of one corner: the same factor order and rounding as
-// the direct splat kernel.
-__device__ __forceinline__ unsigned long long splat_q(float w) {
-  return (unsigned long long)llrint((double)w * 4294967296.0);
-}
-
-template <class P>
+// One block's tile of shape S: contributions in the direct kernel's factor order
+// (mass * wx * wy * wz) and rounding (splat_q).
+template <class S, class P>
 __device__ void splat_tile(const P &pos, const float *__restrict__ mass,
                            unsigned long long *__restrict__ acc, Dims d) {
-  __shared__ unsigned long long win[kSplatWinCap];
+  constexpr int kSTX = S::tx, kSTY = S::ty, kSTZ = S::tz, kSplatTileVPT = S::vpt;
+  __shared__ unsigned long long win[S::cap];
   __shared__ int red[6][kSplatTileThreads / 64];
   const int ntx = (d.nx + kSTX - 1) / kSTX, nty = (d.ny + kSTY - 1) / kSTY;
   const int b = (int)blockIdx.x;
@@ -122,7 +137,7 @@ __device__ void splat_tile(const P &pos, const float *__restrict__ mass,
   }
   if (empty) return;  // uniform over the block: nothing lands inside the volume
   const long long cells = (long long)W[0] * W[1] * W[2];
-  const bool use_lds = cells <= kSplatWinCap;
+  const bool use_lds = cells <= S::cap;
   if (use_lds) {
     for (int c = t; c < (int)cells; c += kSplatTileThreads) win[c] = 0ull;
     __syncthreads();

@@ -8,6 +8,7 @@ Each ``Iterate`` is one call of ``awk_greedy_iterate`` (six HIP kernels, no host
 sync); energies stay on the device until ``energies()`` is read.
 """
 import ctypes
+import warnings
 
 import torch
 
@@ -84,16 +85,20 @@ class GreedyIteratorGPU:
     def UpdateStepSizeNextIteration(self):
         _lib.call("awk_greedy_update_step_next", self._h)
 
+    def GetFlags(self, clear=True, stream=None):
+        """Numerical flags since the last clear (FLAG_* of _lib); waits for the stream."""
+        return _lib.get_flags("awk_greedy_get_flags", self._h, clear, stream)[0]
+
     def SetAutoStepReduce(self, on):
         _lib.call("awk_greedy_set_auto_step_reduce", self._h, int(bool(on)))
 
-    def StepSize(self):
+    def StepSize(self, stream=None):
         s = ctypes.c_float()
-        _lib.call("awk_greedy_get_step", self._h, ctypes.byref(s))
+        _lib.call("awk_greedy_get_step", self._h, ctypes.byref(s), stream_handle(stream))
         return s.value
 
-    def SetStepSize(self, step):
-        _lib.call("awk_greedy_set_step", self._h, float(step))
+    def SetStepSize(self, step, stream=None):
+        _lib.call("awk_greedy_set_step", self._h, float(step), stream_handle(stream))
 
     STAGES = ("apply", "xfwd_force", "y_fwd", "z_solve", "y_inv", "xinv_compose")
 
@@ -155,6 +160,8 @@ class GreedyWarp:
         self.levels = list(scale_levels) if scale_levels else [(1, n_iterations)]
         self.n_iterations = n_iterations
         self.energy_history = []
+        self.step_history = []   # per level: the step size set at its first iteration
+        self.flags = 0           # numerical flags of the run (FLAG_* of _lib)
         self.data = None
 
     def _level_images(self, f):
@@ -185,6 +192,13 @@ class GreedyWarp:
                 if k == 0:
                     it.UpdateStepSizeNextIteration()
                 it.Iterate(data, stream)
+                if k == 0:
+                    self.step_history.append(it.StepSize(stream))
+            flags = it.GetFlags(stream=stream)
+            self.flags |= flags
+            if flags & _lib.FLAG_NAN_ENERGY:
+                # GreedyDeformationData::AddEnergy logs and goes on (GreedyDeformationData.cxx:170-174)
+                warnings.warn("NaN detected in deformation (scale level %d)" % level, RuntimeWarning)
             self.energy_history.append(data.energies())
             prev = data
             self.iterator = it
@@ -192,10 +206,30 @@ class GreedyWarp:
         return prev
 
     def EnergyHistory(self):
-        """The run's energies as the reference's EnergyHistory (``save_xml`` writes
-        GreedyWarp::WriteEnergy's file, GreedyWarp.txx:220-230)."""
+        """The run's EnergyHistory as the reference's deformation data collects it
+        (``save_xml`` writes GreedyWarp::WriteEnergy's file, GreedyWarp.txx:220-230):
+        one IterationEvent per iteration (DeformationDataInterface::AddEnergy), an
+        EnergyIncreaseEvent after each iteration whose energy exceeds the previous
+        one, across scale levels (GreedyDeformationData.cxx:162-184), with the
+        AutoStepReduce halving as a StepSizeChange, and the StepSizeChange that each
+        level's first iteration sets (GreedyIteratorCPU.cxx:129-134)."""
         from .energy_history import EnergyHistory
-        return EnergyHistory.from_levels(list(enumerate(self.energy_history)))
+        h = EnergyHistory()
+        prev = None
+        step = None
+        for level, energies in enumerate(self.energy_history):
+            for k, e in enumerate(energies):
+                h.add_iteration(level, k, e)
+                if prev is not None and e - prev > 0:
+                    h.add_energy_increase()
+                    if self.auto_step_reduce and step is not None:
+                        step = step / 2.0
+                        h.add_step_size(step)
+                prev = e
+                if k == 0 and level < len(self.step_history):
+                    step = self.step_history[level]
+                    h.add_step_size(step)
+        return h
 
     def WriteOutputs(self, name, out_dir=".", prefix="Greedy", suffix="mha", origin=(0.0, 0.0, 0.0),
                      def_image=True, inv_def_image=False, def_field=False, inv_def_field=False,

@@ -93,15 +93,26 @@ class LDMMIteratorGPU:
         _lib.call("awk_ldmm_set_gradient", self._h, int(bool(p.UseOrigGradientCalc)))
         _lib.call("awk_ldmm_set_splat", self._h, SPLAT_MODES[p.Splat])
         if not p.UseAdaptiveStepSize:
-            _lib.call("awk_ldmm_set_step", self._h, float(p.StepSize))
+            _lib.call("awk_ldmm_set_step", self._h, float(p.StepSize), stream_handle(None))
 
     def UpdateStepSizeNextIteration(self):
         _lib.call("awk_ldmm_update_step_next", self._h)
 
-    def StepSize(self):
+    def StepSize(self, stream=None):
         out = ctypes.c_float()
-        _lib.call("awk_ldmm_get_step", self._h, ctypes.byref(out))
+        _lib.call("awk_ldmm_get_step", self._h, ctypes.byref(out), stream_handle(stream))
         return out.value
+
+    def GetFlags(self, clear=True, stream=None):
+        """Numerical flags since the last clear (FLAG_* of _lib); waits for the stream."""
+        return _lib.get_flags("awk_ldmm_get_flags", self._h, clear, stream)[0]
+
+    def CheckFinite(self, stream=None):
+        """Raises _lib.NumericalError when an energy was NaN or a step size invalid
+        since the last check, as the reference's iterator throws
+        (LDMMIteratorGPU.cxx:238-241, 283-285, 439-441)."""
+        _lib.call("awk_ldmm_get_flags", self._h, ctypes.byref(ctypes.c_uint()), 1,
+                  stream_handle(stream))
 
     def _check(self, data):
         nz, ny, nx = data.I0.shape
@@ -219,6 +230,7 @@ class LDMMWarp:
                 self.iterator.Iterate(self.data, record_energy=False)
             for _ in range(n):
                 self.iterator.Iterate(self.data, vec_energy=vec_energy)
+            self.iterator.CheckFinite()
             self.energy_history.append(self.data.energies())
         return self.data
 
@@ -287,7 +299,7 @@ class LDMMAtlasBuilderGPU:
             _lib.call("awk_ldmm_atlas_set_subject", self._h, s, ptr(ims[s]), self.weights[s], arr,
                       None)
         if not p.UseAdaptiveStepSize:
-            _lib.call("awk_ldmm_atlas_set_step", self._h, float(p.StepSize))
+            _lib.call("awk_ldmm_atlas_set_step", self._h, float(p.StepSize), stream_handle(None))
         self._level = k
 
     def Run(self, vec_energy=True):
@@ -296,6 +308,16 @@ class LDMMAtlasBuilderGPU:
             if k != self._level:
                 self._enter(k)
             self.Iterate(n, vec_energy=vec_energy)
+            self.CheckFinite()
+
+    def GetFlags(self, clear=True, stream=None):
+        """Numerical flags of all subjects since the last clear (FLAG_* of _lib)."""
+        return _lib.get_flags("awk_ldmm_atlas_get_flags", self._h, clear, stream)[0]
+
+    def CheckFinite(self, stream=None):
+        """Raises _lib.NumericalError on a NaN energy or an invalid step size."""
+        _lib.call("awk_ldmm_atlas_get_flags", self._h, ctypes.byref(ctypes.c_uint()), 1,
+                  stream_handle(stream))
 
     def Iterate(self, n_iter=1, vec_energy=True, stream=None):
         e = torch.zeros((n_iter, 3), dtype=torch.float64, device=self.v.device)
@@ -368,9 +390,10 @@ class LDMMAtlasBuilderGPU:
             paths.append(path)
         return paths
 
-    def StepSize(self, s):
+    def StepSize(self, s, stream=None):
         out = ctypes.c_float()
-        _lib.call("awk_ldmm_atlas_get_step", self._h, int(s), ctypes.byref(out))
+        _lib.call("awk_ldmm_atlas_get_step", self._h, int(s), ctypes.byref(out),
+                  stream_handle(stream))
         return out.value
 
     def EnergyHistory(self):

@@ -85,10 +85,10 @@ def write_metaimage(path, img):
         "BinaryDataByteOrderMSB = False",
         "CompressedData = False",
         "TransformMatrix = 1 0 0 0 1 0 0 0 1",
-        "Offset = %g %g %g" % img.origin,
+        "Offset = %.9g %.9g %.9g" % img.origin,
         "CenterOfRotation = 0 0 0",
         "AnatomicalOrientation = RAI",
-        "ElementSpacing = %g %g %g" % img.spacing,
+        "ElementSpacing = %.9g %.9g %.9g" % img.spacing,
         "DimSize = %d %d %d" % (nx, ny, nz),
     ]
     if nch > 1:

@@ -84,7 +84,7 @@ class GreedyIteratorGPU {
   void UpdateStepSizeNextIteration() { AWK_CHECK(awk_greedy_update_step_next(mG)); }
   float StepSize() {
     float s = 0.f;
-    AWK_CHECK(awk_greedy_get_step(mG, &s));
+    AWK_CHECK(awk_greedy_get_step(mG, &s, nullptr));
     return s;
   }
   void SetAutoStepReduce(bool on) { AWK_CHECK(awk_greedy_set_auto_step_reduce(mG, on ? 1 : 0)); }

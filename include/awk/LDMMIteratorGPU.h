@@ -96,7 +96,7 @@ class LDMMIteratorGPU {
     AWK_CHECK(awk_ldmm_create(&mL, (int)size.x, (int)size.y, (int)size.z, spacing.x, spacing.y,
                               spacing.z, k.alpha, k.beta, k.gamma, (int)mNT, p.sigma, p.maxPert));
     AWK_CHECK(awk_ldmm_set_gradient(mL, mOrig ? 1 : 0));
-    if (!p.useAdaptiveStepSize) AWK_CHECK(awk_ldmm_set_step(mL, p.stepSize));
+    if (!p.useAdaptiveStepSize) AWK_CHECK(awk_ldmm_set_step(mL, p.stepSize, nullptr));
     const size_t N = (size_t)size.x * size.y * size.z;
     freeAlphas();
     for (unsigned t = 0; t < mNT; ++t) {
@@ -115,7 +115,7 @@ class LDMMIteratorGPU {
   void UpdateStepSizeNextIteration() { AWK_CHECK(awk_ldmm_update_step_next(mL)); }
   float StepSize() {
     float s = 0.f;
-    AWK_CHECK(awk_ldmm_get_step(mL, &s));
+    AWK_CHECK(awk_ldmm_get_step(mL, &s, nullptr));
     return s;
   }
 
@@ -163,7 +163,7 @@ class LDMMAtlasGPU {
     AWK_CHECK(awk_ldmm_atlas_create(&mA, (int)size.x, (int)size.y, (int)size.z, spacing.x,
                                     spacing.y, spacing.z, k.alpha, k.beta, k.gamma, (int)nTimeSteps,
                                     p.sigma, p.maxPert, (int)nSubjects, jacobianScale ? 1 : 0));
-    if (!p.useAdaptiveStepSize) AWK_CHECK(awk_ldmm_atlas_set_step(mA, p.stepSize));
+    if (!p.useAdaptiveStepSize) AWK_CHECK(awk_ldmm_atlas_set_step(mA, p.stepSize, nullptr));
   }
   ~LDMMAtlasGPU() {
     if (mA) awk_ldmm_atlas_destroy(mA);

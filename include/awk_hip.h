@@ -30,7 +30,23 @@ enum {
   AWK_ERR_INVALID = 1,      /* bad argument / size / null pointer */
   AWK_ERR_HIP = 2,          /* HIP runtime error */
   AWK_ERR_ALLOC = 3,        /* device allocation failed */
-  AWK_ERR_UNSUPPORTED = 4   /* parameter combination not implemented */
+  AWK_ERR_UNSUPPORTED = 4,  /* parameter combination not implemented */
+  AWK_ERR_NUMERIC = 5       /* a NaN energy or an invalid step size was detected */
+};
+
+/* Numerical events of an iterator or atlas: a sticky device word that the
+ * energy and step-size kernels set, read (and cleared) by awk_*_get_flags.
+ * Those calls wait for `stream` and return AWK_ERR_NUMERIC when a NaN energy or
+ * an invalid step is among the flags (AWK_OK otherwise, the flags still given);
+ * an energy increase alone is informational. */
+enum {
+  AWK_FLAG_NAN_ENERGY = 1,      /* an energy (or its change) was NaN
+                                   (GreedyDeformationData.cxx:169-174; the LDMM
+                                   iterator throws, LDMMIteratorGPU.cxx:283-285, 439-441) */
+  AWK_FLAG_ENERGY_INCREASE = 2, /* an energy rose over the previous iteration's
+                                   (EnergyIncreaseEvent, GreedyDeformationData.cxx:175-183) */
+  AWK_FLAG_BAD_STEP = 4         /* max|v| was NaN, infinite or zero where a step size was
+                                   computed (LDMMIteratorGPU.cxx:238-241) */
 };
 
 typedef void *awk_stream_t;
@@ -216,8 +232,11 @@ awk_status awk_greedy_destroy(awk_greedy_t g);
 /* Request a new step size (MaxPert / max|v|) at the next iteration
  * (DeformationIteratorInterface::UpdateStepSizeNextIteration; GreedyWarp.txx:99). */
 awk_status awk_greedy_update_step_next(awk_greedy_t g);
-awk_status awk_greedy_set_step(awk_greedy_t g, float step);
-awk_status awk_greedy_get_step(awk_greedy_t g, float *step);
+/* The step is a device scalar: set_step writes it in the order of `stream`, after
+ * the work already queued there; get_step reads it after that work (it waits for
+ * `stream`). */
+awk_status awk_greedy_set_step(awk_greedy_t g, float step, awk_stream_t stream);
+awk_status awk_greedy_get_step(awk_greedy_t g, float *step, awk_stream_t stream);
 
 /* One GreedyIteratorCPU::Iterate / GreedyIteratorGPU::Iterate
  * (GreedyIteratorCPU.cxx:97-138; GreedyIteratorGPU.cxx:92-135):
@@ -235,6 +254,12 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
                               const float *const hinv_in[3], float *const hinv_out[3],
                               double *energy_dev, awk_stream_t stream);
 
+/* Sticky numerical flags since the last clear (AWK_FLAG_*); waits for stream;
+ * clear != 0 resets them in stream order.  Returns AWK_ERR_NUMERIC on a NaN
+ * energy or an invalid step.  Replaces the NaN and energy-increase checks of
+ * GreedyDeformationData::AddEnergy (GreedyDeformationData.cxx:162-184), which
+ * the reference makes on the host after every iteration. */
+awk_status awk_greedy_get_flags(awk_greedy_t g, unsigned *flags, int clear, awk_stream_t stream);
 /* AutoStepReduce (GreedyParam.h:61-65, GreedyDeformationData.cxx:176-183): when
  * on, an energy increase over the previous iteration halves the step before
  * this iteration's update (done on the device). */
@@ -316,6 +341,8 @@ awk_status awk_atlas_get_hinv(awk_atlas_t a, int s, float *const out[3], awk_str
 /* Every subject's next iteration sets its step size (iteration 0 of a scale
  * level, GreedyAtlasThreadGPU::BeginImageIteration). */
 awk_status awk_atlas_update_step_next(awk_atlas_t a);
+/* Numerical flags of all subjects' iterations (AWK_FLAG_*), as awk_greedy_get_flags. */
+awk_status awk_atlas_get_flags(awk_atlas_t a, unsigned *flags, int clear, awk_stream_t stream);
 /* Steady-state iterations are captured into hipGraphs and replayed (default on). */
 awk_status awk_atlas_set_graph(awk_atlas_t a, int on);
 /* n_iter atlas iterations: each subject's greedy step against the current
@@ -357,8 +384,14 @@ awk_status awk_ldmm_set_gradient(awk_ldmm_t l, int use_orig);
 awk_status awk_ldmm_set_splat(awk_ldmm_t l, int mode);
 /* LDMMIteratorGPU::UpdateStepSizeNextIteration: the next iterate only sets the step. */
 awk_status awk_ldmm_update_step_next(awk_ldmm_t l);
-awk_status awk_ldmm_set_step(awk_ldmm_t l, float step);
-awk_status awk_ldmm_get_step(awk_ldmm_t l, float *step); /* synchronises */
+/* Numerical flags (AWK_FLAG_*), as awk_greedy_get_flags: the reference's LDMM
+ * iterator throws on a NaN image energy or max displacement
+ * (LDMMIteratorGPU.cxx:238-241, 283-285, 439-441); here the iterate calls stay
+ * asynchronous and AWK_ERR_NUMERIC comes from this call. */
+awk_status awk_ldmm_get_flags(awk_ldmm_t l, unsigned *flags, int clear, awk_stream_t stream);
+/* Stream-ordered like awk_greedy_set_step / awk_greedy_get_step. */
+awk_status awk_ldmm_set_step(awk_ldmm_t l, float step, awk_stream_t stream);
+awk_status awk_ldmm_get_step(awk_ldmm_t l, float *step, awk_stream_t stream); /* waits for stream */
 
 /* One Iterate(computeAlphasOnly = false) (LDMMIteratorGPU.cxx:147-156).
  * v: 3 * n_time_steps device planes, v[3t + c] = component c of v_t, updated in
@@ -411,8 +444,12 @@ awk_status awk_ldmm_atlas_set_splat(awk_ldmm_atlas_t a, int mode);
 /* Every subject computes its step size at its next iteration
  * (LDMMAtlasThreadGPU::BeginImageIteration with UseAdaptiveStepSize). */
 awk_status awk_ldmm_atlas_update_step_next(awk_ldmm_atlas_t a);
-awk_status awk_ldmm_atlas_set_step(awk_ldmm_atlas_t a, float step);
-awk_status awk_ldmm_atlas_get_step(awk_ldmm_atlas_t a, int s, float *step); /* synchronises */
+/* Numerical flags of all subjects' iterations (AWK_FLAG_*), as awk_ldmm_get_flags. */
+awk_status awk_ldmm_atlas_get_flags(awk_ldmm_atlas_t a, unsigned *flags, int clear,
+                                    awk_stream_t stream);
+awk_status awk_ldmm_atlas_set_step(awk_ldmm_atlas_t a, float step, awk_stream_t stream);
+awk_status awk_ldmm_atlas_get_step(awk_ldmm_atlas_t a, int s, float *step,
+                                   awk_stream_t stream); /* waits for stream */
 /* n_iter iterations of AtlasThread::AtlasThreadMain's loop (AtlasThread.txx): each
  * subject's LDMM iteration against the current mean, then the new mean
  * sum_s w_s I_s(h_0) JacDet_0 / sum_s w_s JacDet_0 (ComputeThreadMean and

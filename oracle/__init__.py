@@ -443,6 +443,51 @@ def atlas_run(images, h0, weights, n_iter, template_shape, sp=(1, 1, 1), alpha=0
     return hs, mean, energy
 
 
+def rescale_intensities(img, out_min=0.0, out_max=1.0):
+    """IntensityWindow::Rescale with the image's own min/max as the input window
+    (ImagePreprocessor.cxx:39-69, Array3DUtils::rescaleElements)."""
+    img = np.asarray(img, np.float32)
+    lo, hi = float(img.min()), float(img.max())
+    if hi == lo:
+        return np.full(img.shape, out_min, np.float32)
+    return ((img - lo) * ((out_max - out_min) / (hi - lo)) + out_min).astype(np.float32)
+
+
+def atlas_multiscale_run(images, weights, levels, sp=(1, 1, 1), alpha=0.01, beta=0.01,
+                         gamma=0.001, max_pert=0.1):
+    """GreedyAtlasBuilder over scale levels, FP64, every subject on the template
+    grid (AtlasThread::AtlasThreadMain, AtlasThread.txx:95-181).  levels: list of
+    (downsample factor, iterations), coarsest first.  At factor f the grid is
+    size // f with spacing * f (MultiscaleManager::AddScaleLevel,
+    MultiscaleManager.cxx:104-111) and the images are Gaussian-downsampled from the
+    originals (GreedyDeformationData::SetScaleLevel, GreedyDeformationData.cxx:
+    223-240); each hfield is carried to the next level as a voxel displacement
+    resampled with CLAMP and rescaled (GreedyDeformationData.cxx:210-214,
+    MultiscaleManager.cxx:327-341); the step sizes are recomputed at each level's
+    first iteration (GreedyAtlasThreadCPU.cxx:84-91).  sp is the full-resolution
+    spacing (x, y, z).  Returns (hfields, mean, deformed images, energies) at the
+    last level."""
+    shape = np.asarray(images[0]).shape
+    hs = None
+    energies = []
+    for f, n in levels:
+        ls = tuple(s // f for s in shape)
+        ims = [np.asarray(im, np.float32) if f == 1 else gaussian_downsample(im, ls)
+               for im in images]
+        ident = identity(ls, np.float64)
+        if hs is None:
+            hs = [ident.copy() for _ in images]
+        else:
+            hs = [resample_hfield(h - identity(h.shape[:3], np.float64), ls, bg=BG_CLAMP,
+                                  rescale=True) + ident for h in hs]
+        spl = tuple(float(s) * f for s in sp)
+        hs, mean, en = atlas_run(ims, hs, weights, int(n), ls, sp=spl, alpha=alpha, beta=beta,
+                                 gamma=gamma, max_pert=max_pert)
+        energies.append(en)
+    defs = [apply(im, h) for im, h in zip(images, hs)]
+    return hs, mean, defs, energies
+
+
 def greedy_time(shape3, n_iter=1, seed=0):
     """Seconds per FP32 greedy iteration of the oracle on one host thread."""
     nz, ny, nx = shape3

@@ -192,7 +192,7 @@ def test_splat_sorted_collapse(shape):
 
 
 @pytest.mark.parametrize("shape,amp", [((17, 9, 5), 3.0), ((64, 48, 40), 3.0), ((40, 36, 100), 1.0),
-                                       ((32, 32, 32), 40.0)])
+                                       ((32, 32, 32), 40.0), ((256, 256, 256), 8.0)])
 def test_splat_tiled_equals_direct(shape, amp):
     """The LDS-window splat and the direct atomic splat add the same int64
     contributions, so their FIXED64 results are bit-identical (40-voxel

@@ -87,3 +87,77 @@ def test_atlas_inverse_fields():
         disp = np.sqrt(((h - ident) ** 2).sum(-1))[3:-3, 3:-3, 3:-3]
         assert disp.max() > 0.1           # the fields moved
         assert err.max() < 0.1 * disp.max()
+
+
+def _atlas_from_primitives(images, w, levels, sp, alpha, beta, gamma, max_pert):
+    """The atlas scale loop written out from the separately pinned primitives
+    (apply, gradient, L^-1, composeHV, Gaussian downsample, hfield resample):
+    per level the step of each subject is MaxPert / max|v| at its first iteration
+    and held afterwards; the mean is recomputed after every iteration from all
+    subjects' updated fields; the fields are carried between levels as rescaled
+    voxel displacements."""
+    shape = images[0].shape
+    hs = None
+    for f, n in levels:
+        ls = tuple(s // f for s in shape)
+        ims = images if f == 1 else [oracle.gaussian_downsample(im, ls) for im in images]
+        ident = oracle.identity(ls, np.float64)
+        if hs is None:
+            hs = [ident.copy() for _ in images]
+        else:
+            hs = [oracle.resample_hfield(h - oracle.identity(h.shape[:3], np.float64), ls,
+                                         bg=oracle.BG_CLAMP, rescale=True) + ident for h in hs]
+        spl = tuple(s * f for s in sp)
+        mean = sum(wi * oracle.apply(im, h) for wi, im, h in zip(w, ims, hs))
+        steps = [0.0] * len(images)
+        for it in range(n):
+            for s, im in enumerate(ims):
+                idef = oracle.apply(im, hs[s])
+                force = oracle.gradient(idef, spl) * (mean - idef)[..., None]
+                v = oracle.diffoper(force, spl, alpha, beta, gamma, inverse=True)
+                if it == 0:
+                    steps[s] = max_pert / np.sqrt((v * v).sum(-1)).max()
+                hs[s] = oracle.compose_hv(hs[s], steps[s] * v, spl)
+            mean = sum(wi * oracle.apply(im, h) for wi, im, h in zip(w, ims, hs))
+    return hs, mean
+
+
+def test_atlas_two_levels_many_iterations_equal_the_primitive_loop():
+    # pins the multi-iteration, multi-level semantics of the atlas restatement
+    # (AtlasThread.txx:95-181, GreedyDeformationData.cxx:186-247) past one iteration
+    shape = (14, 13, 16)
+    ims = _subjects(shape, 3, seed=50)
+    w = np.array([0.5, 0.3, 0.2])
+    sp = (1.5, 1.2, 0.9)
+    levels = [(2, 4), (1, 5)]
+    hs, mean, defs, en = oracle.atlas_multiscale_run(ims, w, levels, sp=sp, alpha=0.02,
+                                                     beta=0.02, gamma=0.001, max_pert=0.2)
+    hs2, mean2 = _atlas_from_primitives(ims, w, levels, sp, 0.02, 0.02, 0.001, 0.2)
+    for a, b in zip(hs, hs2):
+        assert np.abs(a - b).max() < 1e-4
+    assert np.abs(mean - mean2).max() < 1e-5
+    assert [len(e) for e in en] == [4, 5]
+    for d, im, h in zip(defs, ims, hs):
+        np.testing.assert_allclose(d, oracle.apply(im, h), atol=1e-6)
+    # a mistake planted in the loop (the mean held at its level-start value) is caught
+    hs3 = _atlas_stale_mean(ims, w, levels, sp)
+    assert max(np.abs(a - b).max() for a, b in zip(hs, hs3)) > 1e-3
+
+
+def _atlas_stale_mean(images, w, levels, sp):
+    shape = images[0].shape
+    hs = None
+    for f, n in levels:
+        ls = tuple(s // f for s in shape)
+        ims = images if f == 1 else [oracle.gaussian_downsample(im, ls) for im in images]
+        ident = oracle.identity(ls, np.float64)
+        hs = [ident.copy() for _ in images] if hs is None else [
+            oracle.resample_hfield(h - oracle.identity(h.shape[:3], np.float64), ls,
+                                   bg=oracle.BG_CLAMP, rescale=True) + ident for h in hs]
+        spl = tuple(s * f for s in sp)
+        mean = sum(wi * oracle.apply(im, h) for wi, im, h in zip(w, ims, hs))
+        for s, im in enumerate(ims):
+            hs[s], _, _, _, _ = oracle.greedy_run(im, mean, n, sp=spl, alpha=0.02, beta=0.02,
+                                                  gamma=0.001, max_pert=0.2, h0=hs[s],
+                                                  dtype=np.float64)
+    return hs
