@@ -100,6 +100,8 @@ SIGNATURES = {
     "awk_greedy_set_profiling": [_P, _I],
     "awk_greedy_get_profile": [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)],
     "awk_greedy_get_flags": [_P, ctypes.POINTER(ctypes.c_uint), _I, _P],
+    "awk_vf_aos_to_soa": [_P, _P, _P, _P, ctypes.c_longlong, _P],
+    "awk_vf_soa_to_aos": [_P, _P, _P, _P, ctypes.c_longlong, _P],
     "awk_atlas_get_flags": [_P, ctypes.POINTER(ctypes.c_uint), _I, _P],
     "awk_ldmm_get_flags": [_P, ctypes.POINTER(ctypes.c_uint), _I, _P],
     "awk_ldmm_atlas_get_flags": [_P, ctypes.POINTER(ctypes.c_uint), _I, _P],

@@ -333,6 +333,26 @@ awk_status launch_compose_vh(float *ox, float *oy, float *oz, const float *vx, c
 
 using namespace awk;
 
+// AoS (x0 y0 z0 x1 ...) <-> SoA planes, one voxel per thread: the three planes
+// are read or written coalesced, the AoS side as 12-byte strides that the vector
+// L1 combines into whole lines.
+__global__ void k_aos_to_soa(const float *__restrict__ aos, float *__restrict__ x,
+                             float *__restrict__ y, float *__restrict__ z, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] = aos[3 * i];
+  y[i] = aos[3 * i + 1];
+  z[i] = aos[3 * i + 2];
+}
+__global__ void k_soa_to_aos(const float *__restrict__ x, const float *__restrict__ y,
+                             const float *__restrict__ z, float *__restrict__ aos, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  aos[3 * i] = x[i];
+  aos[3 * i + 1] = y[i];
+  aos[3 * i + 2] = z[i];
+}
+
 extern "C" {
 
 const char *awk_last_error(void) { return g_err.c_str(); }
@@ -367,6 +387,28 @@ awk_status awk_tune_get(const char *name, int *value) {
   else if (n == "xinv_unroll") *value = t.xinv_unroll;
   else if (n == "splat_tile") *value = t.splat_tile;
   else return fail(AWK_ERR_INVALID, "awk_tune_get: unknown knob " + n);
+  return AWK_OK;
+}
+
+awk_status awk_vf_aos_to_soa(const float *aos, float *x, float *y, float *z, long long n,
+                             awk_stream_t stream) {
+  if (n < 0) return fail(AWK_ERR_INVALID, "awk_vf_aos_to_soa: negative size");
+  if (n == 0) return AWK_OK;
+  if (!aos || !x || !y || !z) return fail(AWK_ERR_INVALID, "awk_vf_aos_to_soa: null pointer");
+  AWK_LAUNCH(k_aos_to_soa, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+             as_stream(stream), aos, x, y, z, n);
+  AWK_LAUNCH_CHECK("k_aos_to_soa");
+  return AWK_OK;
+}
+
+awk_status awk_vf_soa_to_aos(const float *x, const float *y, const float *z, float *aos, long long n,
+                             awk_stream_t stream) {
+  if (n < 0) return fail(AWK_ERR_INVALID, "awk_vf_soa_to_aos: negative size");
+  if (n == 0) return AWK_OK;
+  if (!aos || !x || !y || !z) return fail(AWK_ERR_INVALID, "awk_vf_soa_to_aos: null pointer");
+  AWK_LAUNCH(k_soa_to_aos, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+             as_stream(stream), x, y, z, aos, n);
+  AWK_LAUNCH_CHECK("k_soa_to_aos");
   return AWK_OK;
 }
 

@@ -2,30 +2,21 @@
 // KernelInterfaceGPU (Algorithms/Common/KernelInterfaceGPU.h:20-43) and
 // DiffOperGPU (Algorithms/Common/DiffOperGPU.h:27-135), over awk_fluid_kernel_*.
 // Unlike the reference GPU operator, LPow != 1, the eigen table path and the
-// divergence-free projection are supported rather than throwing.
+// divergence-free projection are supported rather than throwing
+// (DiffOperGPU.cxx:127-161).
 #ifndef AWK_DIFFOPERGPU_H
 #define AWK_DIFFOPERGPU_H
 
+#include "awk_drivers.h"
 #include "cplVector3DArray.h"
 
-// DiffOperParam (Algorithms/Common/DiffOperParam.h:24-86): defaults alpha =
-// 0.01, beta = 0.01, gamma = 0.001, LPow = 1.
-struct DiffOperParam {
-  float alpha = 0.01f, beta = 0.01f, gamma = 0.001f, lpow = 1.0f;
-  bool useEigenLUT = false, divergenceFree = false;
-  float &Alpha() { return alpha; }
-  float &Beta() { return beta; }
-  float &Gamma() { return gamma; }
-  float &LPow() { return lpow; }
-  bool &UseEigenLUT() { return useEigenLUT; }
-  bool &DivergenceFree() { return divergenceFree; }
-};
-
+// KernelInterfaceGPU (KernelInterfaceGPU.h:20-43): SetSize with the KernelParam
+// the iterators hold; ApplyOperator / ApplyInverseOperator overwrite their input.
 class KernelInterfaceGPU {
  public:
   virtual ~KernelInterfaceGPU() {}
   virtual void SetSize(const SizeType &logicalSize, const SpacingType &spacing,
-                       const DiffOperParam &params) = 0;
+                       const KernelParam &params) = 0;
   virtual void ApplyOperator(cplVector3DArray &dV) = 0;         // f = L v, in place
   virtual void ApplyInverseOperator(cplVector3DArray &dF) = 0;  // v = L^-1 f, in place
 };
@@ -37,33 +28,39 @@ class DiffOperGPU : public KernelInterfaceGPU {
   DiffOperGPU(const DiffOperGPU &) = delete;
   DiffOperGPU &operator=(const DiffOperGPU &) = delete;
 
+  // DiffOperGPU::SetSize (DiffOperGPU.cxx:32-60): the KernelParam must hold a
+  // DiffOper (KernelFactory pairs a MultiGaussian param with another operator).
   void SetSize(const SizeType &logicalSize, const SpacingType &spacing,
-               const DiffOperParam &params) override {
+               const KernelParam &params) override {
+    if (!params.IsDiffOperParam())
+      throw AtlasWerksException(__FILE__, __LINE__, "DiffOperGPU::SetSize: not a DiffOper parameter");
+    SetSize(logicalSize, spacing, *params.AsDiffOperParam());
+  }
+  void SetSize(const SizeType &logicalSize, const SpacingType &spacing, const DiffOperParam &p) {
     release();
     mSize = logicalSize;
     mSpacing = spacing;
-    mParam = params;
+    copyParams(p);
     AWK_CHECK(awk_fluid_kernel_create(&mH, (int)logicalSize.x, (int)logicalSize.y,
-                                      (int)logicalSize.z, spacing.x, spacing.y, spacing.z,
-                                      mParam.alpha, mParam.beta, mParam.gamma, mParam.lpow, flags()));
+                                      (int)logicalSize.z, spacing.x, spacing.y, spacing.z, mAlpha,
+                                      mBeta, mGamma, mLPow, flags()));
   }
   void SetParams(const DiffOperParam &p) {
-    mParam = p;
+    copyParams(p);
     push();
   }
-  DiffOperParam GetParams() const { return mParam; }
-  void SetAlpha(float a) { mParam.alpha = a; push(); }
-  float GetAlpha() const { return mParam.alpha; }
-  void SetBeta(float b) { mParam.beta = b; push(); }
-  float GetBeta() const { return mParam.beta; }
-  void SetGamma(float g) { mParam.gamma = g; push(); }
-  float GetGamma() const { return mParam.gamma; }
-  void SetLPow(float p) { mParam.lpow = p; push(); }
-  float GetLPow() const { return mParam.lpow; }
-  void SetUseEigenLUT(bool b) { mParam.useEigenLUT = b; push(); }
-  bool GetUseEigenLUT() const { return mParam.useEigenLUT; }
-  void SetDivergenceFree(bool b) { mParam.divergenceFree = b; push(); }
-  bool GetDivergenceFree() const { return mParam.divergenceFree; }
+  void SetAlpha(Real a) { mAlpha = a; push(); }
+  Real GetAlpha() const { return mAlpha; }
+  void SetBeta(Real b) { mBeta = b; push(); }
+  Real GetBeta() const { return mBeta; }
+  void SetGamma(Real g) { mGamma = g; push(); }
+  Real GetGamma() const { return mGamma; }
+  void SetLPow(Real p) { mLPow = p; push(); }
+  Real GetLPow() const { return mLPow; }
+  void SetUseEigenLUT(bool b) { mEigen = b; push(); }
+  bool GetUseEigenLUT() const { return mEigen; }
+  void SetDivergenceFree(bool b) { mDivFree = b; push(); }
+  bool GetDivergenceFree() const { return mDivFree; }
 
   void ApplyOperator(float *dVx, float *dVy, float *dVz, awkStream_t s = nullptr) {
     check();
@@ -77,11 +74,17 @@ class DiffOperGPU : public KernelInterfaceGPU {
   void ApplyInverseOperator(cplVector3DArray &dF) override { ApplyInverseOperator(dF.x, dF.y, dF.z); }
 
  private:
-  int flags() const {
-    return (mParam.useEigenLUT ? AWK_FK_EIGEN : 0) | (mParam.divergenceFree ? AWK_FK_DIVFREE : 0);
+  void copyParams(const DiffOperParam &p) {
+    mAlpha = p.Alpha();
+    mBeta = p.Beta();
+    mGamma = p.Gamma();
+    mLPow = p.LPow();
+    mEigen = p.UseEigenLUT();
+    mDivFree = p.DivergenceFree();
   }
+  int flags() const { return (mEigen ? AWK_FK_EIGEN : 0) | (mDivFree ? AWK_FK_DIVFREE : 0); }
   void push() {
-    if (mH) AWK_CHECK(awk_fluid_kernel_set_params(mH, mParam.alpha, mParam.beta, mParam.gamma, mParam.lpow, flags()));
+    if (mH) AWK_CHECK(awk_fluid_kernel_set_params(mH, mAlpha, mBeta, mGamma, mLPow, flags()));
   }
   void check() const {
     if (!mH) throw AtlasWerksException(__FILE__, __LINE__, "DiffOperGPU: SetSize must be called first");
@@ -93,7 +96,8 @@ class DiffOperGPU : public KernelInterfaceGPU {
   awk_fk_t mH = nullptr;
   SizeType mSize;
   SpacingType mSpacing;
-  DiffOperParam mParam;
+  Real mAlpha = 0.01f, mBeta = 0.01f, mGamma = 0.001f, mLPow = 1.f;
+  bool mEigen = false, mDivFree = false;
 };
 
 // FluidKernelFFT is the name this operator goes by in the registration drivers.

@@ -1,6 +1,7 @@
 // cplVector3DArray.h -- device vector field as three SoA planes, the reference's
-// cplVector3DArray (CUDACommon/inc/cudaVector3DArray.h:32-60).  Planes are
-// separate allocations here (x, y, z may be any three device buffers).
+// cplVector3DArray (CUDACommon/inc/cudaVector3DArray.h:32-70): x, y, z plane
+// pointers, n elements, nAlign the plane stride of one allocation.  Planes may
+// also be any three separate device buffers (nAlign = n then).
 #ifndef AWK_CPLVECTOR3DARRAY_H
 #define AWK_CPLVECTOR3DARRAY_H
 
@@ -10,14 +11,20 @@
 
 struct cplVector3DArray {
   float *x = nullptr, *y = nullptr, *z = nullptr;
-  unsigned int n = 0;
+  unsigned int n = 0;       // number of elements
+  unsigned int nAlign = 0;  // plane stride when the planes share one allocation
   cplVector3DArray() {}
-  cplVector3DArray(float *px, float *py, float *pz, unsigned int count) : x(px), y(py), z(pz), n(count) {}
+  cplVector3DArray(float *px, float *py, float *pz, unsigned int count)
+      : x(px), y(py), z(pz), n(count), nAlign(count) {}
   unsigned int size() const { return n; }
+  unsigned int capacity() const { return nAlign; }
+  bool isContinous() const { return n == nAlign; }
   float *elementArray(int i) { return i == 0 ? x : i == 1 ? y : i == 2 ? z : nullptr; }
+  const float *elementArray(int i) const { return i == 0 ? x : i == 1 ? y : i == 2 ? z : nullptr; }
 };
 
-// allocateDeviceVector3DArray / freeDeviceVector3DArray (cudaVector3DArray.cu)
+// allocateDeviceVector3DArray / freeDeviceVector3DArray (cudaVector3DArray.cu):
+// one allocation, planes at x, x + n, x + 2n.
 inline void allocateDeviceVector3DArray(cplVector3DArray &a, unsigned int n) {
   float *p = nullptr;
   if (hipMalloc(&p, 3 * (size_t)n * sizeof(float)) != hipSuccess)

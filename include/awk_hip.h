@@ -115,6 +115,16 @@ awk_status awk_fluid_kernel_apply(awk_fk_t k, float *x, float *y, float *z, int 
 /* HField3DUtils (cudaHField3DUtils namespace, CUDACommon/inc/cudaHField3DUtils.h) */
 /* ====================================================================== */
 
+/* Vector-field layout change on the device, for host AoS fields copied as one
+ * block (Vector3D<float> per voxel, the reference VectorField) to and from the
+ * device SoA planes: CUDAUtilities::CopyVectorFieldToDevice / FromDevice
+ * (Base/CUDAUtilities.cxx:22-72), which transpose on the host one component at
+ * a time.  aos: 3n floats. */
+awk_status awk_vf_aos_to_soa(const float *aos, float *x, float *y, float *z, long long n,
+                             awk_stream_t stream);
+awk_status awk_vf_soa_to_aos(const float *x, const float *y, const float *z, float *aos, long long n,
+                             awk_stream_t stream);
+
 /* h(x) = x (HField3DUtils::setToIdentity, HField3DUtils.txx:19-35). */
 awk_status awk_hfield_set_to_identity(float *hx, float *hy, float *hz, int nx, int ny, int nz,
                                       awk_stream_t stream);

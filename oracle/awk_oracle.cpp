@@ -558,7 +558,8 @@ void trilerp_vec_T(const T *h, const int *size, double x, double y, double z, do
 template <class T, class TI1 = float>
 double greedy_iterate(GreedyState<T> &st, const orc_greedy_params *p, const float *I0,
                       const TI1 *I1, T *h, T *hinv, bool update_step, double &step,
-                      double *energy_float_sum, const int *isize = nullptr) {
+                      double *energy_float_sum, const int *isize = nullptr,
+                      const double *prev_energy = nullptr) {
   const int *size = st.size;
   const size_t N = st.N;
   const int sx = size[0], sy = size[1];
@@ -614,6 +615,13 @@ double greedy_iterate(GreedyState<T> &st, const orc_greedy_params *p, const floa
   }
   const double spprod = p->sp[0] * p->sp[1] * p->sp[2];
   if (energy_float_sum) *energy_float_sum = spprod * (double)fsum;  // ImageUtils.txx:55-58
+  // GreedyDeformationData::AddEnergy (GreedyDeformationData.cxx:162-184) runs here,
+  // before the update (GreedyIteratorCPU.cxx:120-136): under AutoStepReduce an
+  // energy above the previous iteration's halves the step this iteration uses.
+  if (p->auto_step_reduce && prev_energy) {
+    const double e = (sizeof(T) == sizeof(float)) ? spprod * (double)fsum : spprod * dsum;
+    if (e - *prev_energy > 0) step = step / 2.0;
+  }
   // 5. v = L^-1 f
   diffoper<T>(size, p->sp, p->alpha, p->beta, p->gamma, 1.0, 0, st.vf.data(), true);
   // 6. step size at the first iteration of a scale level (GreedyIteratorCPU.cxx:65-79,129-134)
@@ -668,14 +676,11 @@ int greedy_run(const orc_greedy_params *p, const float *I0, const float *I1, T *
   double prev = 0;
   for (int it = 0; it < p->n_iter; ++it) {
     double ef = 0;
-    double ed = greedy_iterate<T>(st, p, I0, I1, h, hinv, it == 0, step, &ef);
+    double ed = greedy_iterate<T>(st, p, I0, I1, h, hinv, it == 0, step, &ef, nullptr,
+                                  it > 0 ? &prev : nullptr);
     if (energy_f) energy_f[it] = ef;
     if (energy_d) energy_d[it] = ed;
-    // GreedyDeformationData::AddEnergy (GreedyDeformationData.cxx:162-184): the
-    // halved step takes effect from the next iteration.
-    double e = (sizeof(T) == sizeof(float)) ? ef : ed;
-    if (it > 0 && p->auto_step_reduce && e - prev > 0) step = step / 2.0;
-    prev = e;
+    prev = (sizeof(T) == sizeof(float)) ? ef : ed;
   }
   if (step_out) *step_out = step;
   return 0;

@@ -127,6 +127,47 @@ def test_auto_step_reduce_halves_on_increase():
     assert abs(step_auto - step_plain / 2 ** increases) < 1e-9 * step_plain
 
 
+def _greedy_asr_from_primitives(I0, I1, n_iter, max_pert, late=False):
+    """GreedyIteratorCPU::Iterate with AutoStepReduce written out from the pinned
+    apply, gradient, L^-1 and composeHV: the energy is taken before the update and
+    GreedyDeformationData::AddEnergy halves the step on an increase before
+    updateDeformation uses it (GreedyIteratorCPU.cxx:120-136,
+    GreedyDeformationData.cxx:175-183); late=True plants the mistake of halving
+    after the update."""
+    shape = I0.shape
+    h = oracle.identity(shape, np.float64)
+    step, prev, energies = None, None, []
+    for it in range(n_iter):
+        idef = oracle.apply(I0, h)
+        r = I1.astype(np.float64) - idef
+        e = float((r * r).sum())
+        energies.append(e)
+        v = oracle.diffoper(oracle.gradient(idef) * r[..., None], inverse=True)
+        increase = prev is not None and e > prev
+        if increase and not late:
+            step /= 2
+        if it == 0:
+            step = max_pert / np.sqrt((v * v).sum(-1)).max()
+        h = oracle.compose_hv(h, step * v)
+        if increase and late:
+            step /= 2
+        prev = e
+    return np.array(energies), step
+
+
+def test_auto_step_reduce_applies_to_the_rising_iteration():
+    shape = (16, 16, 16)
+    t, h, _ = synth.pair(shape, seed=2, max_disp=2.0)
+    target = oracle.apply(t, np.moveaxis(h, 0, -1), dtype=np.float32)
+    _, _, _, ed, step = oracle.greedy_run(t, target, 12, max_pert=3.0, auto_step_reduce=True,
+                                          dtype=np.float64)
+    e_ref, s_ref = _greedy_asr_from_primitives(t, target, 12, 3.0)
+    assert np.max(np.abs(ed - e_ref) / e_ref) < 1e-4
+    assert abs(step - s_ref) < 1e-6 * s_ref
+    e_late, _ = _greedy_asr_from_primitives(t, target, 12, 3.0, late=True)
+    assert np.max(np.abs(ed - e_late) / e_late) > 0.1  # the planted mistake is caught
+
+
 def test_metaimage_roundtrip(tmp_path, golden_dir):
     im = read_metaimage(os.path.join(golden_dir, "BullseyeTest00.mha"))
     assert im.data.shape == (32, 32, 32) and im.spacing == (1.0, 1.0, 1.0)
