@@ -9,8 +9,16 @@
 // stage touches consecutive addresses: no bank conflicts.
 //
 // Radices 2, 3, 4, 5, 8 and 16 have register butterflies; any other prime
-// factor uses a table-driven direct DFT (one thread per output).  Twiddles come
-// from a per-axis table tw[k] = exp(-2 pi i k / n) computed in double on the host.
+// factor up to 64 uses a table-driven direct DFT (one thread per output).  Twiddles
+// come from a per-axis table tw[k] = exp(-2 pi i k / n) computed in double on the
+// host.  An axis with a larger prime factor is transformed by Bluestein's chirp-z
+// method (L. Bluestein, "A linear filtering approach to the computation of the
+// discrete Fourier transform", 1970): with c_j = exp(-i pi j^2 / n),
+//   X_k = c_k sum_j (x_j c_j) conj(c_{k-j}),
+// a cyclic convolution of length m (a power of two >= 2n - 1) done as two
+// power-of-two FFTs in an LDS work area and a product with the precomputed
+// spectrum of conj(c) (scaled by 1/m); the inverse transform is
+// conj(DFT(conj x)).
 #pragma once
 
 #include "awk_common.h"
@@ -25,6 +33,12 @@ struct Axis {
   int n;
   int nst;
   int rad[kMaxStages];
+  // Bluestein (m > 0): chirp c_j (n), the scaled spectrum of conj(c) (m), and
+  // the power-of-two sub-transform of length m (twiddles, radices)
+  int m;
+  const float2 *chirp, *bhat, *mtw;
+  int mst;
+  int mrad[8];
 };
 
 // multiply by -i (forward) or +i (inverse)
@@ -205,24 +219,60 @@ __device__ __forceinline__ void stage_generic(float2 *buf, int n, int R, int NL,
   __syncthreads();
 }
 
-// Full unnormalised transform of NL interleaved lines held in LDS `buf`.
-// Callers must __syncthreads() after filling buf; the result is in buf and
-// visible to all threads on return.
-__device__ __forceinline__ void lds_fft(float2 *buf, const Axis &ax, int NL, int LS, bool inv) {
+// The Stockham stages of one transform of length n with radices rad[0..nst).
+__device__ __forceinline__ void lds_fft_stages(float2 *buf, const float2 *__restrict__ tw, int n,
+                                               int nst, const int *rad, int NL, int LS, bool inv) {
   int Ns = 1;
-  for (int s = 0; s < ax.nst; ++s) {
-    const int R = ax.rad[s];
+  for (int s = 0; s < nst; ++s) {
+    const int R = rad[s];
     switch (R) {
-      case 2: stage_reg<2>(buf, ax.n, NL, LS, Ns, ax.tw, inv); break;
-      case 3: stage_reg<3>(buf, ax.n, NL, LS, Ns, ax.tw, inv); break;
-      case 4: stage_reg<4>(buf, ax.n, NL, LS, Ns, ax.tw, inv); break;
-      case 5: stage_reg<5>(buf, ax.n, NL, LS, Ns, ax.tw, inv); break;
-      case 8: stage_reg<8>(buf, ax.n, NL, LS, Ns, ax.tw, inv); break;
-      case 16: stage_reg<16>(buf, ax.n, NL, LS, Ns, ax.tw, inv); break;
-      default: stage_generic(buf, ax.n, R, NL, LS, Ns, ax.tw, inv); break;
+      case 2: stage_reg<2>(buf, n, NL, LS, Ns, tw, inv); break;
+      case 3: stage_reg<3>(buf, n, NL, LS, Ns, tw, inv); break;
+      case 4: stage_reg<4>(buf, n, NL, LS, Ns, tw, inv); break;
+      case 5: stage_reg<5>(buf, n, NL, LS, Ns, tw, inv); break;
+      case 8: stage_reg<8>(buf, n, NL, LS, Ns, tw, inv); break;
+      case 16: stage_reg<16>(buf, n, NL, LS, Ns, tw, inv); break;
+      default: stage_generic(buf, n, R, NL, LS, Ns, tw, inv); break;
     }
     Ns *= R;
   }
+}
+
+// Bluestein's transform of the NL lines of buf through the work area work
+// (m * NL complex, interleaved work[i * NL + line]).
+__device__ __forceinline__ void lds_fft_bluestein(float2 *buf, float2 *work, const Axis &ax,
+                                                  int NL, int LS, bool inv) {
+  const int n = ax.n, m = ax.m, bd = (int)blockDim.x;
+  for (int w = (int)threadIdx.x; w < m * NL; w += bd) {
+    const int line = w % NL, j = w / NL;
+    float2 v = make_float2(0.f, 0.f);
+    if (j < n) {
+      const float2 x = buf[j * LS + line];
+      v = cmul(inv ? cconj(x) : x, ax.chirp[j]);
+    }
+    work[w] = v;
+  }
+  __syncthreads();
+  lds_fft_stages(work, ax.mtw, m, ax.mst, ax.mrad, NL, NL, false);
+  for (int w = (int)threadIdx.x; w < m * NL; w += bd) work[w] = cmul(work[w], ax.bhat[w / NL]);
+  __syncthreads();
+  lds_fft_stages(work, ax.mtw, m, ax.mst, ax.mrad, NL, NL, true);
+  for (int w = (int)threadIdx.x; w < n * NL; w += bd) {
+    const int line = w % NL, k = w / NL;
+    const float2 y = cmul(work[w], ax.chirp[k]);
+    buf[k * LS + line] = inv ? cconj(y) : y;
+  }
+  __syncthreads();
+}
+
+// Full unnormalised transform of NL interleaved lines held in LDS `buf`.
+// Callers must __syncthreads() after filling buf; the result is in buf and
+// visible to all threads on return.  work: the Bluestein work area (m * NL
+// complex) for axes that need it.
+__device__ __forceinline__ void lds_fft(float2 *buf, const Axis &ax, int NL, int LS, bool inv,
+                                        float2 *work = nullptr) {
+  if (ax.m) lds_fft_bluestein(buf, work, ax, NL, LS, inv);
+  else lds_fft_stages(buf, ax.tw, ax.n, ax.nst, ax.rad, NL, LS, inv);
 }
 
 }  // namespace awk

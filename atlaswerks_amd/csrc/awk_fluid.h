@@ -14,6 +14,14 @@ struct AxisPlan {
   std::vector<int> rad;
   Axis dev() const;
   float2 *tw = nullptr;  // device
+  // Bluestein's method (axes with a prime factor above 64): m = the power-of-two
+  // convolution length (0: not used), its radices, and device tables inside the
+  // operator's twiddle allocation
+  int m = 0;
+  std::vector<int> mrad;
+  const float2 *chirp = nullptr, *bhat = nullptr, *mtw = nullptr;
+  // the line length the LDS stages hold (the work area of m when used)
+  int stage_len() const { return m > n ? m : n; }
 };
 
 // Epilogue of the inverse x pass.

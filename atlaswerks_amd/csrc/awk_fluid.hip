@@ -17,6 +17,7 @@
 //      or the fused greedy hfield update).
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -27,11 +28,68 @@ namespace awk {
 
 Axis AxisPlan::dev() const {
   Axis a;
+  std::memset(&a, 0, sizeof(a));
   a.tw = tw;
   a.n = n;
   a.nst = (int)rad.size();
   for (int i = 0; i < a.nst; ++i) a.rad[i] = rad[i];
+  a.m = m;
+  a.chirp = chirp;
+  a.bhat = bhat;
+  a.mtw = mtw;
+  a.mst = (int)mrad.size();
+  for (int i = 0; i < a.mst; ++i) a.mrad[i] = mrad[i];
   return a;
+}
+
+// Unnormalised forward DFT of a power-of-two length in double (iterative
+// radix 2), for the Bluestein kernel spectrum.
+static void host_fft_pow2(std::vector<std::complex<double>> &a) {
+  const size_t n = a.size();
+  for (size_t i = 1, j = 0; i < n; ++i) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (size_t len = 2; len <= n; len <<= 1) {
+    const double ang = -2.0 * M_PI / (double)len;
+    for (size_t i = 0; i < n; i += len)
+      for (size_t k = 0; k < len / 2; ++k) {
+        const std::complex<double> w(cos(ang * k), sin(ang * k));
+        const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+      }
+  }
+}
+
+// Bluestein tables of one axis appended to the host table `tw`: the chirp
+// c_j = exp(-i pi j^2 / n) (the angle from j^2 mod 2n, exact in integers), the
+// spectrum of b_l = conj(c_|l|) laid cyclically over m and scaled by 1/m, and
+// the m twiddles.  Returns the offsets of the three tables.
+static void bluestein_tables(int n, int m, std::vector<float2> &tw, size_t off[3]) {
+  off[0] = tw.size();
+  std::vector<std::complex<double>> c(n), b(m, 0.0);
+  for (int j = 0; j < n; ++j) {
+    const long long r = ((long long)j * j) % (2LL * n);
+    const double a = -M_PI * (double)r / (double)n;
+    c[j] = std::complex<double>(cos(a), sin(a));
+    tw.push_back(make_float2((float)c[j].real(), (float)c[j].imag()));
+  }
+  for (int j = 0; j < n; ++j) {
+    b[j] = std::conj(c[j]);
+    if (j > 0) b[m - j] = std::conj(c[j]);
+  }
+  host_fft_pow2(b);
+  off[1] = tw.size();
+  for (int k = 0; k < m; ++k)
+    tw.push_back(make_float2((float)(b[k].real() / m), (float)(b[k].imag() / m)));
+  off[2] = tw.size();
+  for (int k = 0; k < m; ++k) {
+    const double a = -2.0 * M_PI * (double)k / (double)m;
+    tw.push_back(make_float2((float)cos(a), (float)sin(a)));
+  }
 }
 
 static std::vector<int> factor(int n) {
@@ -238,7 +296,7 @@ k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
   }
   __syncthreads();
   if constexpr (NC != 0) ct_fft<NC, 3 * RTC / 2, 3 * RTC / 2 + 1, BDC>(sm, ax.tw, false);
-  else lds_fft(sm, ax, NL, LS, false);
+  else lds_fft(sm, ax, NL, LS, false, sm + (size_t)nx * LS);
 #pragma unroll
   for (int w = threadIdx.x; w < NL * nxh; w += bdim) {
     const int line = w / nxh, k = w - line * nxh;
@@ -302,7 +360,7 @@ k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
   }
   __syncthreads();
   if constexpr (NC != 0) ct_fft<NC, 3 * RTC / 2, 3 * RTC / 2 + 1, BDC>(sm, ax.tw, true);
-  else lds_fft(sm, ax, NL, LS, true);
+  else lds_fft(sm, ax, NL, LS, true, sm + (size_t)nx * LS);
   float mx = 0.f;
   if (a.mode == XSTORE_COMPOSE) {
     // updateDeformation (GreedyIteratorCPU.cxx:81-95): v *= step; h <- composeHV(h, v, sp, PARTIAL_ID).
@@ -431,7 +489,7 @@ k_ypass(Axis ax, int ny, int nz, int nxh, int nxh_pad, int TX, float2 *__restric
     sm[y * TX + t] = (kx0 + t < nxh) ? base[(size_t)y * nxh_pad + t] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  lds_fft(sm, ax, TX, TX, inv != 0);
+  lds_fft(sm, ax, TX, TX, inv != 0, sm + (size_t)ny * TX);
   for (int w = threadIdx.x; w < TX * ny; w += blockDim.x) {
     const int t = w % TX, y = w / TX;
     if (kx0 + t < nxh) base[(size_t)y * nxh_pad + t] = sm[y * TX + t];
@@ -647,7 +705,7 @@ k_zsolve(Axis az, SolveArgs s, int ny, int nz, int nxh, int nxh_pad, int TX,
                                   : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  lds_fft(sm, az, NL, NL, false);
+  lds_fft(sm, az, NL, NL, false, sm + (size_t)nz * NL);
   for (int w = threadIdx.x; w < TX * nz; w += blockDim.x) {
     const int t = w % TX, kz = w / TX;
     const int kx = kx0 + t;
@@ -659,7 +717,7 @@ k_zsolve(Axis az, SolveArgs s, int ny, int nz, int nxh, int nxh_pad, int TX,
     sm[kz * NL + 2 * TX + t] = b2;
   }
   __syncthreads();
-  lds_fft(sm, az, NL, NL, true);
+  lds_fft(sm, az, NL, NL, true, sm + (size_t)nz * NL);
   for (int w = threadIdx.x; w < NL * nz; w += blockDim.x) {
     const int t = w % TX, rest = w / TX;
     const int c = rest % 3, z = rest / 3;
@@ -735,16 +793,25 @@ __global__ void k_eigen_table(float *__restrict__ out, int nx, int ny, int nz, i
 
 // ------------------------------------------------------------------ host
 
-static int pick_rt(int nx) {
-  // largest even RT <= 8 with LDS nx * (3RT/2 + 1) * 8 B <= 64 KiB
+// LDS of the x passes: the row buffer (LS = NL + 1) plus the Bluestein work area.
+static size_t x_lds(const AxisPlan &a, int rt) {
+  return ((size_t)a.n * (3 * rt / 2 + 1) + (size_t)a.m * (3 * rt / 2)) * sizeof(float2);
+}
+// LDS of the y / z passes: n lines-per-column * tx lines, plus the work area.
+static size_t yz_lds(const AxisPlan &a, int lines) {
+  return ((size_t)a.n + (size_t)a.m) * lines * sizeof(float2);
+}
+
+static int pick_rt(const AxisPlan &a) {
+  // largest even RT <= 8 whose LDS fits 64 KiB
   for (int rt = 8; rt >= 2; rt -= 2)
-    if ((size_t)nx * (3 * rt / 2 + 1) * sizeof(float2) <= 65536) return rt;
+    if (x_lds(a, rt) <= 65536) return rt;
   return 0;
 }
 
-static int pick_tx(int n, int lines_per_col, int max_tx) {
+static int pick_tx(const AxisPlan &a, int lines_per_col, int max_tx) {
   for (int tx = max_tx; tx >= 1; tx /= 2)
-    if ((size_t)n * lines_per_col * tx * sizeof(float2) <= 65536) return tx;
+    if (yz_lds(a, lines_per_col * tx) <= 65536) return tx;
   return 0;
 }
 
@@ -767,22 +834,32 @@ awk_status fk_init(FluidKernel &k, int nx, int ny, int nz, const float sp[3]) {
   for (int i = 0; i < 3; ++i) {
     k.ax[i].n = ns[i];
     k.ax[i].rad = factor(ns[i]);
-    if ((int)k.ax[i].rad.size() > kMaxStages) return fail(AWK_ERR_INVALID, "fluid kernel: too many FFT stages");
-    for (int r : k.ax[i].rad)
-      if (r > 64) return fail(AWK_ERR_UNSUPPORTED, "fluid kernel: axis length has a prime factor > 64");
+    k.ax[i].m = 0;
+    k.ax[i].mrad.clear();
+    bool big = false;
+    for (int r : k.ax[i].rad) big = big || r > 64;
+    if (big) {
+      // Bluestein: a power-of-two convolution of length m >= 2n - 1
+      int m = 1;
+      while (m < 2 * ns[i] - 1) m *= 2;
+      k.ax[i].m = m;
+      k.ax[i].mrad = factor(m);
+    }
+    if ((int)k.ax[i].rad.size() > kMaxStages || (int)k.ax[i].mrad.size() > 8)
+      return fail(AWK_ERR_INVALID, "fluid kernel: too many FFT stages");
   }
-  k.rt = pick_rt(nx);
-  k.tx_y = pick_tx(ny, 1, 16);
-  k.tx_z = pick_tx(nz, 3, 8);
+  k.rt = pick_rt(k.ax[0]);
+  k.tx_y = pick_tx(k.ax[1], 1, 16);
+  k.tx_z = pick_tx(k.ax[2], 3, 8);
   if (!k.rt || !k.tx_y || !k.tx_z) return fail(AWK_ERR_UNSUPPORTED, "fluid kernel: axis too long for LDS staging");
-  k.bd_x = pick_bd(nx, 3 * k.rt / 2);
-  k.bd_y = pick_bd(ny, k.tx_y);
-  k.bd_z = pick_bd(nz, 3 * k.tx_z);
+  k.bd_x = pick_bd(k.ax[0].stage_len(), 3 * k.rt / 2);
+  k.bd_y = pick_bd(k.ax[1].stage_len(), k.tx_y);
+  k.bd_z = pick_bd(k.ax[2].stage_len(), 3 * k.tx_z);
   if (k.bd_x > 1024 || k.bd_y > 1024 || k.bd_z > 1024)
     return fail(AWK_ERR_UNSUPPORTED, "fluid kernel: block too large");
-  // twiddles and LUT (double -> float)
-  size_t ntw = (size_t)nx + ny + nz;
-  std::vector<float2> tw(ntw);
+  // twiddles and LUT (double -> float); the Bluestein tables follow the three
+  // axes' twiddles in the same allocation
+  std::vector<float2> tw((size_t)nx + ny + nz);
   size_t off = 0;
   for (int i = 0; i < 3; ++i) {
     for (int j = 0; j < ns[i]; ++j) {
@@ -791,6 +868,10 @@ awk_status fk_init(FluidKernel &k, int nx, int ny, int nz, const float sp[3]) {
     }
     off += ns[i];
   }
+  size_t boff[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int i = 0; i < 3; ++i)
+    if (k.ax[i].m) bluestein_tables(ns[i], k.ax[i].m, tw, boff[i]);
+  const size_t ntw = tw.size();
   const size_t nlut = 2 * (size_t)k.nxh + 2 * (size_t)ny + 2 * (size_t)nz;
   std::vector<float> lut(nlut);
   {
@@ -814,6 +895,12 @@ awk_status fk_init(FluidKernel &k, int nx, int ny, int nz, const float sp[3]) {
   k.ax[0].tw = twd;
   k.ax[1].tw = twd + nx;
   k.ax[2].tw = twd + nx + ny;
+  for (int i = 0; i < 3; ++i)
+    if (k.ax[i].m) {
+      k.ax[i].chirp = twd + boff[i][0];
+      k.ax[i].bhat = twd + boff[i][1];
+      k.ax[i].mtw = twd + boff[i][2];
+    }
   AWK_HIP(hipMalloc(&k.lut, nlut * sizeof(float)));
   AWK_HIP(hipMemcpy(k.lut, lut.data(), nlut * sizeof(float), hipMemcpyHostToDevice));
   k.scratch_bytes = spec_bytes + ntw * sizeof(float2) + nlut * sizeof(float);
@@ -854,7 +941,7 @@ int fk_xblocks(const FluidKernel &k) {
 
 awk_status fk_xfwd(FluidKernel &k, const XFwdArgs &a, hipStream_t s, int *nblocks) {
   const unsigned grid = grid_for((long long)k.ny * k.nz, k.rt);
-  const size_t lds = (size_t)k.nx * (3 * k.rt / 2 + 1) * sizeof(float2);
+  const size_t lds = x_lds(k.ax[0], k.rt);
   const Axis ax = k.ax[0].dev();
 #define AWK_XF(NC, RTC, BDC)                                                                     \
   AWK_LAUNCH((k_xfwd<NC, RTC, BDC>), dim3(grid), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \
@@ -888,7 +975,7 @@ awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s, const ReduceArgs *r
   else {
     const int ntile = (k.nxh + k.tx_y - 1) / k.tx_y;
     const unsigned grid = grid_for((long long)3 * k.nz * ntile, 1);
-    const size_t lds = std::max((size_t)k.ny * k.tx_y * sizeof(float2), (size_t)k.bd_y * sizeof(double));
+    const size_t lds = std::max(yz_lds(k.ax[1], k.tx_y), (size_t)k.bd_y * sizeof(double));
     AWK_LAUNCH(k_ypass, dim3(grid), dim3(k.bd_y), lds, s, k.ax[1].dev(), k.ny, k.nz, k.nxh,
                        k.nxh_pad, k.tx_y, k.spec, k.spec_plane(), inv ? 1 : 0, ra);
   }
@@ -930,7 +1017,7 @@ awk_status fk_zsolve(FluidKernel &k, bool inverse_op, hipStream_t s) {
   else {
     const int ntile = (k.nxh + k.tx_z - 1) / k.tx_z;
     const unsigned grid = grid_for((long long)k.ny * ntile, 1);
-    const size_t lds = (size_t)k.nz * 3 * k.tx_z * sizeof(float2);
+    const size_t lds = yz_lds(k.ax[2], 3 * k.tx_z);
     AWK_LAUNCH(k_zsolve, dim3(grid), dim3(k.bd_z), lds, s, k.ax[2].dev(), sa, k.ny, k.nz,
                        k.nxh, k.nxh_pad, k.tx_z, k.spec, k.spec_plane());
   }
@@ -941,7 +1028,7 @@ awk_status fk_zsolve(FluidKernel &k, bool inverse_op, hipStream_t s) {
 
 awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblocks) {
   const unsigned grid = grid_for((long long)k.ny * k.nz, k.rt);
-  const size_t lds = (size_t)k.nx * (3 * k.rt / 2 + 1) * sizeof(float2);
+  const size_t lds = x_lds(k.ax[0], k.rt);
   const Axis ax = k.ax[0].dev();
 #define AWK_XI(NC, RTC, BDC, XU)                                                                 \
   AWK_LAUNCH((k_xinv<NC, RTC, BDC, XU>), dim3(grid), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \

@@ -60,8 +60,10 @@ struct Fft {
     if (n == 1) return;
     C *a = x, *b = buf.data();
     int Ns = 1;
-    C v[128];
-    C o[128];
+    // butterfly scratch sized to the largest radix (any prime factor, e.g. 181)
+    const int rmax = *std::max_element(rad.begin(), rad.end());
+    std::vector<C> vv(rmax), oo(rmax);
+    C *v = vv.data(), *o = oo.data();
     for (int R : rad) {
       const int m = n / R;
       const int twstep = n / (Ns * R);

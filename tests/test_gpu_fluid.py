@@ -15,13 +15,17 @@ TOL = 1e-5
 SIZES = [(8, 8, 8), (6, 10, 12), (24, 40, 36), (32, 31, 36), (64, 64, 64), (17, 9, 5),
          (96, 48, 40), (1, 16, 16), (128, 128, 128), (32, 64, 256), (256, 256, 16),
          (128, 32, 64)]
+# axes with a prime factor above 64 (Bluestein's transform): 67, 71, 127, 181 on each
+# axis; 217 = 7 * 31 and 61 take direct prime stages
+PRIME_SIZES = [(12, 20, 67), (16, 127, 10), (181, 12, 14), (61, 67, 71), (36, 217, 181),
+               (1, 1, 181)]
 
 
 def _field(shape, seed):
     return synth.smooth_noise_field(shape, sigma=1.5, seed=seed)
 
 
-@pytest.mark.parametrize("shape", SIZES)
+@pytest.mark.parametrize("shape", SIZES + PRIME_SIZES)
 @pytest.mark.parametrize("inverse", [True, False])
 def test_operator_matches_oracle(shape, inverse):
     import torch

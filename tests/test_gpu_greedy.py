@@ -22,7 +22,8 @@ def _config0(shape=(64, 64, 64), seed=0):
 
 
 @pytest.mark.parametrize("shape,n_iter", [((32, 32, 32), 20), ((64, 64, 64), 50), ((24, 40, 36), 30),
-                                          ((16, 32, 256), 10), ((8, 24, 128), 10)])
+                                          ((16, 32, 256), 10), ((8, 24, 128), 10),
+                                          ((12, 20, 67), 8), ((181, 14, 12), 6)])
 def test_greedy_energy_trace(shape, n_iter):
     import torch
     from atlaswerks_amd.greedy import GreedyIteratorParam, GreedyWarp

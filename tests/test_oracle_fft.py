@@ -13,7 +13,8 @@ def dft(x, inverse=False):
     return W @ x
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16, 31, 36, 40, 64, 96])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16, 31, 36, 40, 64, 96, 67, 127, 181,
+                               217, 362])
 def test_brute_force(n):
     rng = np.random.default_rng(n)
     x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
