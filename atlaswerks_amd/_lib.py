@@ -51,6 +51,7 @@ SIGNATURES = {
     "awk_gaussian_downsample": [_P, _I, _I, _I, _P, _I, _I, _I, _P],
     "awk_hfield_resample": [_P, _P, _P, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _I, _P],
     "awk_hfield_add_affine": [_P, _P, _P, _I, _I, _I, _P, _F, _P],
+    "awk_hfield_affine_displace": [_P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P],
     "awk_atlas_create": [ctypes.POINTER(_P), _I, _I, _I, _F, _F, _F, _F, _F, _F, _F, _I],
     "awk_atlas_destroy": [_P],
     "awk_atlas_set_subject": [_P, _I, _P, _I, _I, _I, _F, ctypes.POINTER(_P3), _P],
@@ -64,8 +65,10 @@ SIGNATURES = {
     "awk_atlas_deformed_image": [_P, _I, _P, _P],
     "awk_unet_weight_count": [],
     "awk_unet_create": [ctypes.POINTER(_P), _I, _I, _I, _P, ctypes.c_size_t],
+    "awk_unet_create_batch": [ctypes.POINTER(_P), _I, _I, _I, _I, _P, ctypes.c_size_t],
     "awk_unet_destroy": [_P],
     "awk_unet_forward": [_P, _P, _P, ctypes.POINTER(_P3), _P],
+    "awk_unet_forward_batch": [_P, _I, _P, _P, _P, _P],
     "awk_unet_mfma_probe": [_P, _P],
     "awk_unet_trainer_create": [ctypes.POINTER(_P), _P, _P, ctypes.c_size_t],
     "awk_unet_trainer_destroy": [_P],

@@ -80,7 +80,8 @@ class _AtlasLevel:
 class GreedyAtlasBuilderGPU:
     def __init__(self, subjects, template_size, template_origin=(0.0, 0.0, 0.0),
                  template_spacing=(1.0, 1.0, 1.0), max_pert=0.1, alpha=0.01, beta=0.01,
-                 gamma=0.001, use_graph=True, scale_levels=None, compute_inverse=False):
+                 gamma=0.001, use_graph=True, scale_levels=None, compute_inverse=False,
+                 initial_hfields=None):
         if not subjects:
             raise ValueError("an atlas needs at least one subject")
         self.subjects = list(subjects)
@@ -100,6 +101,11 @@ class GreedyAtlasBuilderGPU:
                 if ((ix, iy, iz) != self.size or s.spacing != self.spacing or
                         s.origin != self.origin or s.affine is not None):
                     raise ValueError("inverse fields need every subject on the template grid")
+        # initial hfields on the full-resolution template grid (default: the grid
+        # maps, i.e. identity starts as AtlasThread::AtlasThreadMain makes them)
+        self.initial_hfields = list(initial_hfields) if initial_hfields is not None else None
+        if self.initial_hfields is not None and len(self.initial_hfields) != len(self.subjects):
+            raise ValueError("one initial hfield per subject")
         self._energy = []
         self._energy_level = []
         self.flags = 0  # numerical flags of the run (FLAG_* of _lib)
@@ -140,7 +146,15 @@ class GreedyAtlasBuilderGPU:
                 hfield.gaussianDownsample(im, s.image)
             images.append(im)
         for i, s in enumerate(self.subjects):
-            if self._level is None:
+            if self._level is None and self.initial_hfields is not None:
+                g = self.initial_hfields[i]
+                if tuple(g.shape) != (3,) + tuple(reversed(self.size)):
+                    raise ValueError("initial hfields live on the template grid")
+                h0 = g.clone() if f == 1 else hfield.upsampleHField(
+                    g, size, rescale=True, base_in=self._base_map(s, 1), base_out=self._base_map(s, f))
+                if self.compute_inverse:
+                    raise ValueError("initial hfields with compute_inverse are not supported")
+            elif self._level is None:
                 h0 = torch.empty((3, nz, ny, nx), dtype=torch.float32, device="cuda")
                 self.initial_hfield(s, h0, f)
                 if self.compute_inverse:

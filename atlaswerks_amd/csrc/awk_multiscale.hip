@@ -131,6 +131,24 @@ k_add_affine(float *__restrict__ hx, float *__restrict__ hy, float *__restrict__
   hz[id] += sign * (m.m[8] * x + m.m[9] * y + m.m[10] * z + m.m[11]);
 }
 
+// h = M [x + u(x), 1]: a displacement on a grid mapped through an affine base map
+// (h may alias u)
+__global__ void __launch_bounds__(256)
+k_affine_displace(float *hx, float *hy, float *hz, const float *ux, const float *uy, const float *uz,
+                  int nx, int ny, int nz, Affine34f m) {
+  const int N = nx * ny * nz;
+  const int id = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (id >= N) return;
+  const float x = (float)(id % nx) + ux[id], y = (float)((id / nx) % ny) + uy[id],
+              z = (float)(id / (nx * ny)) + uz[id];
+  const float a = m.m[0] * x + m.m[1] * y + m.m[2] * z + m.m[3];
+  const float b = m.m[4] * x + m.m[5] * y + m.m[6] * z + m.m[7];
+  const float c = m.m[8] * x + m.m[9] * y + m.m[10] * z + m.m[11];
+  hx[id] = a;
+  hy[id] = b;
+  hz[id] = c;
+}
+
 static awk_status gauss_passes(float *out, const float *in, float *tmp, int nx, int ny, int nz,
                                const float sigma[3], const int half[3], hipStream_t s) {
   const int N = nx * ny * nz;
@@ -237,6 +255,21 @@ awk_status awk_hfield_resample(float *ox, float *oy, float *oz, int onx, int ony
   }
 #undef AWK_RS
   AWK_LAUNCH_CHECK("k_resample_field");
+  return AWK_OK;
+}
+
+awk_status awk_hfield_affine_displace(float *hx, float *hy, float *hz, const float *ux,
+                                      const float *uy, const float *uz, int nx, int ny, int nz,
+                                      const float m[12], awk_stream_t stream) {
+  if (awk_status st = check_grid(nx, ny, nz, "awk_hfield_affine_displace")) return st;
+  if (!hx || !hy || !hz || !ux || !uy || !uz || !m)
+    return fail(AWK_ERR_INVALID, "awk_hfield_affine_displace: null pointer");
+  Affine34f a;
+  for (int i = 0; i < 12; ++i) a.m[i] = m[i];
+  const unsigned grid = (unsigned)(((long long)nx * ny * nz + 255) / 256);
+  AWK_LAUNCH(k_affine_displace, dim3(grid), dim3(256), 0, as_stream(stream), hx, hy, hz, ux, uy, uz,
+             nx, ny, nz, a);
+  AWK_LAUNCH_CHECK("k_affine_displace");
   return AWK_OK;
 }
 

@@ -22,7 +22,8 @@ struct ConvArgs {
   bf16 *out;          // [N][cout]
   int C, cout, kpad, nchunks;
   unsigned c8magic;   // floor(2^32 / (C/8)) + 1 for C > 8: chunk / (C/8) as a multiply-high
-  int nx, ny, nz;
+  int nx, ny, nz;     // one volume's grid
+  int batch;          // volumes stacked along z (positions p = ((b*nz + z)*ny + y)*nx + x)
   float slope;        // LeakyReLU slope (forward: applied; gradient mode: derivative)
 };
 
@@ -56,6 +57,7 @@ constexpr float kUnetSlope = 0.2f;
 
 struct awk_unet {
   int nx = 0, ny = 0, nz = 0;
+  int max_batch = 1;  // activations hold this many stacked volumes
   awk::bf16 *w[14] = {};
   float *bias[14] = {};
   int kpad[14] = {};
@@ -71,8 +73,9 @@ struct awk_unet {
 };
 
 namespace awk {
-awk_status unet_forward(awk_unet *u, const float *templ, const float *subject, float *const disp[3],
-                        hipStream_t s);
+// nb volumes (nb <= u->max_batch): templ[b], subject[b] -> disp[3b..3b+2]
+awk_status unet_forward(awk_unet *u, int nb, const float *const *templ, const float *const *subject,
+                        float *const *disp, hipStream_t s);
 awk_status unet_pool(const bf16 *in, bf16 *out, int nx, int ny, int nz, int C, hipStream_t s);
 awk_status unet_upcat(bf16 *out, const bf16 *a, int ca, const bf16 *b, int cb, int nx, int ny,
                       int nz, hipStream_t s);

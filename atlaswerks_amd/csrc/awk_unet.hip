@@ -37,7 +37,7 @@ k_conv3(ConvArgs g) {
   __shared__ f32x4 red[SK > 1 ? (SK - 1) * MT * PT * 64 : 1];
   const int mbase = (int)blockIdx.y * MT * 16;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int N = g.nx * g.ny * g.nz;
+  const int N = g.nx * g.ny * g.nz * g.batch;
   const int slot = SK > 1 ? (int)blockIdx.x : (int)blockIdx.x * kConvWaves + wave;
   const int p0 = slot * (kPosPerWave * PT);
   if (p0 >= N) return;  // uniform over the block when SK > 1
@@ -50,7 +50,7 @@ k_conv3(ConvArgs g) {
     const int p = p0 + t * 16 + col;
     unsigned m = 0u;
     if (p < N) {
-      const int px = p % g.nx, pr = p / g.nx, py = pr % g.ny, pz = pr / g.ny;
+      const int px = p % g.nx, pr = p / g.nx, py = pr % g.ny, pz = (pr / g.ny) % g.nz;
       const unsigned xm = (px > 0 ? 1u : 0u) | 2u | (px < g.nx - 1 ? 4u : 0u);
       const unsigned ym = (py > 0 ? 1u : 0u) | 2u | (py < g.ny - 1 ? 4u : 0u);
       const unsigned zm = (pz > 0 ? 1u : 0u) | 2u | (pz < g.nz - 1 ? 4u : 0u);
@@ -258,21 +258,30 @@ __global__ void k_mfma_probe(float *__restrict__ out) {
 // output channels per wave (MT up to 4) and split K over the block's 4 waves,
 // splitting the channels over blockIdx.y only if that is still too few.
 awk_status launch_conv3(const ConvArgs &g, hipStream_t s) {
-  const long long N = (long long)g.nx * g.ny * g.nz;
+  const long long N1 = (long long)g.nx * g.ny * g.nz, N = N1 * (g.batch < 1 ? 1 : g.batch);
   const int mt_all = g.cout / 16;
   if (g.cout % 16 || mt_all < 1 || mt_all > 16 || g.C % 8 || g.C < 8)
     return fail(AWK_ERR_INVALID, "unet: channels must be multiples of 16 (outputs, at most 256) and 8 (inputs)");
+  if (g.batch < 1 || N > (1LL << 30)) return fail(AWK_ERR_INVALID, "unet: bad batch");
   // channel tiles per wave: the largest power of two (<= 8) dividing the tile count
   int mt_max = 1;
   while (mt_max < 8 && mt_all % (mt_max * 2) == 0) mt_max *= 2;
   const long long target = 8192;
-  int mt = mt_max, pt = mt_max >= 8 ? 2 : 4, sk = 1;
-  auto tiles = [&](int p) { return (N + kPosPerWave * p - 1) / (kPosPerWave * p); };
-  while (tiles(pt) * (mt_all / mt) < target && pt > 1) pt >>= 1;
-  if (tiles(pt) * (mt_all / mt) < target) {
-    sk = 4;
+  auto tiles = [&](long long n, int p) { return (n + kPosPerWave * p - 1) / (kPosPerWave * p); };
+  // Split K or not is decided from ONE volume's size: it fixes each output's
+  // summation order, so a batch gives bit for bit the outputs of single volumes.
+  // The tile shape (which outputs a wave holds) is then sized for the whole batch.
+  int pt = mt_max >= 8 ? 2 : 4;
+  while (tiles(N1, pt) * (mt_all / mt_max) < target && pt > 1) pt >>= 1;
+  const int sk = tiles(N1, pt) * (mt_all / mt_max) < target ? 4 : 1;
+  int mt = mt_max;
+  if (sk == 1) {
+    pt = mt_max >= 8 ? 2 : 4;
+    while (tiles(N, pt) * (mt_all / mt) < target && pt > 1) pt >>= 1;
+  } else {
+    pt = 1;
     mt = mt_max > 4 ? 4 : mt_max;
-    while (tiles(1) * 4 * (mt_all / mt) < target && mt > 1) mt >>= 1;
+    while (tiles(N, 1) * 4 * (mt_all / mt) < target && mt > 1) mt >>= 1;
   }
   const long long per_block = sk > 1 ? (long long)kPosPerWave * pt : (long long)kConvWaves * kPosPerWave * pt;
   const dim3 grid((unsigned)((N + per_block - 1) / per_block), (unsigned)(mt_all / mt));
@@ -318,7 +327,7 @@ static std::vector<bf16> pack_weights(const float *w, int cout, int cin, int blo
 }
 
 static awk_status conv(const awk_unet *u, int l, const bf16 *x, bf16 *out, int nx, int ny, int nz,
-                       hipStream_t s) {
+                       int nb, hipStream_t s) {
   ConvArgs g;
   g.x = x;
   g.w = u->w[l];
@@ -333,6 +342,7 @@ static awk_status conv(const awk_unet *u, int l, const bf16 *x, bf16 *out, int n
   g.nx = nx;
   g.ny = ny;
   g.nz = nz;
+  g.batch = nb;
   g.slope = kUnetSlope;
   return launch_conv3(g, s);
 }
@@ -353,38 +363,46 @@ awk_status awk::unet_upcat(bf16 *out, const bf16 *a, int ca, const bf16 *b, int 
   return AWK_OK;
 }
 
-awk_status awk::unet_forward(awk_unet *u, const float *templ, const float *subject,
-                             float *const disp[3], hipStream_t s) {
+awk_status awk::unet_forward(awk_unet *u, int nb, const float *const *templ,
+                             const float *const *subject, float *const *disp, hipStream_t s) {
   const int nx = u->nx, ny = u->ny, nz = u->nz;
   const int N0 = nx * ny * nz;
+  if (nb < 1 || nb > u->max_batch) return fail(AWK_ERR_INVALID, "unet: batch exceeds the network's");
   awk_status st;
-  AWK_LAUNCH(k_pack_input, dim3((N0 + 255) / 256), dim3(256), 0, s, u->x0, templ, subject, N0);
-  AWK_LAUNCH_CHECK("k_pack_input");
+  for (int b = 0; b < nb; ++b) {
+    AWK_LAUNCH(k_pack_input, dim3((N0 + 255) / 256), dim3(256), 0, s, u->x0 + (size_t)b * N0 * 8,
+               templ[b], subject[b], N0);
+    AWK_LAUNCH_CHECK("k_pack_input");
+  }
+  // the nb volumes are stacked along z: every pooling and upsampling pairs planes
+  // of one volume, since each level's nz is even
   const int a1 = nx / 2, b1 = ny / 2, c1 = nz / 2, a2 = nx / 4, b2 = ny / 4, c2 = nz / 4,
             a3 = nx / 8, b3 = ny / 8, c3 = nz / 8;
-  if ((st = conv(u, 0, u->x0, u->t0, nx, ny, nz, s))) return st;
-  if ((st = conv(u, 1, u->t0, u->s0, nx, ny, nz, s))) return st;
-  if ((st = unet_pool(u->s0, u->p0, nx, ny, nz, 16, s))) return st;
-  if ((st = conv(u, 2, u->p0, u->t1, a1, b1, c1, s))) return st;
-  if ((st = conv(u, 3, u->t1, u->s1, a1, b1, c1, s))) return st;
-  if ((st = unet_pool(u->s1, u->p1, a1, b1, c1, 32, s))) return st;
-  if ((st = conv(u, 4, u->p1, u->t2, a2, b2, c2, s))) return st;
-  if ((st = conv(u, 5, u->t2, u->s2, a2, b2, c2, s))) return st;
-  if ((st = unet_pool(u->s2, u->p2, a2, b2, c2, 64, s))) return st;
-  if ((st = conv(u, 6, u->p2, u->t3, a3, b3, c3, s))) return st;
-  if ((st = conv(u, 7, u->t3, u->s3, a3, b3, c3, s))) return st;
-  if ((st = unet_upcat(u->c2, u->s3, 128, u->s2, 64, a2, b2, c2, s))) return st;
-  if ((st = conv(u, 8, u->c2, u->d2a, a2, b2, c2, s))) return st;
-  if ((st = conv(u, 9, u->d2a, u->d2, a2, b2, c2, s))) return st;
-  if ((st = unet_upcat(u->c1, u->d2, 64, u->s1, 32, a1, b1, c1, s))) return st;
-  if ((st = conv(u, 10, u->c1, u->d1a, a1, b1, c1, s))) return st;
-  if ((st = conv(u, 11, u->d1a, u->d1, a1, b1, c1, s))) return st;
-  if ((st = unet_upcat(u->c0, u->d1, 32, u->s0, 16, nx, ny, nz, s))) return st;
-  if ((st = conv(u, 12, u->c0, u->d0a, nx, ny, nz, s))) return st;
-  if ((st = conv(u, 13, u->d0a, u->d0, nx, ny, nz, s))) return st;
-  AWK_LAUNCH(k_head, dim3((N0 + 255) / 256), dim3(256), 0, s, disp[0], disp[1], disp[2], u->d0,
-             u->head_w, u->head_b, N0, 16);
-  AWK_LAUNCH_CHECK("k_head");
+  if ((st = conv(u, 0, u->x0, u->t0, nx, ny, nz, nb, s))) return st;
+  if ((st = conv(u, 1, u->t0, u->s0, nx, ny, nz, nb, s))) return st;
+  if ((st = unet_pool(u->s0, u->p0, nx, ny, nz * nb, 16, s))) return st;
+  if ((st = conv(u, 2, u->p0, u->t1, a1, b1, c1, nb, s))) return st;
+  if ((st = conv(u, 3, u->t1, u->s1, a1, b1, c1, nb, s))) return st;
+  if ((st = unet_pool(u->s1, u->p1, a1, b1, c1 * nb, 32, s))) return st;
+  if ((st = conv(u, 4, u->p1, u->t2, a2, b2, c2, nb, s))) return st;
+  if ((st = conv(u, 5, u->t2, u->s2, a2, b2, c2, nb, s))) return st;
+  if ((st = unet_pool(u->s2, u->p2, a2, b2, c2 * nb, 64, s))) return st;
+  if ((st = conv(u, 6, u->p2, u->t3, a3, b3, c3, nb, s))) return st;
+  if ((st = conv(u, 7, u->t3, u->s3, a3, b3, c3, nb, s))) return st;
+  if ((st = unet_upcat(u->c2, u->s3, 128, u->s2, 64, a2, b2, c2 * nb, s))) return st;
+  if ((st = conv(u, 8, u->c2, u->d2a, a2, b2, c2, nb, s))) return st;
+  if ((st = conv(u, 9, u->d2a, u->d2, a2, b2, c2, nb, s))) return st;
+  if ((st = unet_upcat(u->c1, u->d2, 64, u->s1, 32, a1, b1, c1 * nb, s))) return st;
+  if ((st = conv(u, 10, u->c1, u->d1a, a1, b1, c1, nb, s))) return st;
+  if ((st = conv(u, 11, u->d1a, u->d1, a1, b1, c1, nb, s))) return st;
+  if ((st = unet_upcat(u->c0, u->d1, 32, u->s0, 16, nx, ny, nz * nb, s))) return st;
+  if ((st = conv(u, 12, u->c0, u->d0a, nx, ny, nz, nb, s))) return st;
+  if ((st = conv(u, 13, u->d0a, u->d0, nx, ny, nz, nb, s))) return st;
+  for (int b = 0; b < nb; ++b) {
+    AWK_LAUNCH(k_head, dim3((N0 + 255) / 256), dim3(256), 0, s, disp[3 * b], disp[3 * b + 1],
+               disp[3 * b + 2], u->d0 + (size_t)b * N0 * 16, u->head_w, u->head_b, N0, 16);
+    AWK_LAUNCH_CHECK("k_head");
+  }
   return AWK_OK;
 }
 
@@ -399,11 +417,19 @@ size_t awk_unet_weight_count(void) {
 
 awk_status awk_unet_create(awk_unet_t *out, int nx, int ny, int nz, const float *weights,
                            size_t n_weights) {
+  return awk_unet_create_batch(out, nx, ny, nz, 1, weights, n_weights);
+}
+
+awk_status awk_unet_create_batch(awk_unet_t *out, int nx, int ny, int nz, int max_batch,
+                                 const float *weights, size_t n_weights) {
   if (!out || !weights) return fail(AWK_ERR_INVALID, "awk_unet_create: null argument");
   *out = nullptr;
+  if (max_batch < 1 || max_batch > 64) return fail(AWK_ERR_INVALID, "awk_unet_create: batch must be 1..64");
   if (nx < 8 || ny < 8 || nz < 8 || nx % 8 || ny % 8 || nz % 8)
     return fail(AWK_ERR_INVALID, "awk_unet_create: every axis must be a multiple of 8 (three poolings)");
   if ((long long)nx * ny * nz > (1LL << 24)) return fail(AWK_ERR_INVALID, "awk_unet_create: at most 2^24 voxels");
+  if ((long long)nx * ny * nz * max_batch > (1LL << 26))
+    return fail(AWK_ERR_INVALID, "awk_unet_create: at most 2^26 voxels over the batch");
   if (n_weights != awk_unet_weight_count())
     return fail(AWK_ERR_INVALID, "awk_unet_create: weight count does not match the network");
   awk_unet *u = new (std::nothrow) awk_unet();
@@ -411,6 +437,7 @@ awk_status awk_unet_create(awk_unet_t *out, int nx, int ny, int nz, const float 
   u->nx = nx;
   u->ny = ny;
   u->nz = nz;
+  u->max_batch = max_batch;
   hipError_t e = hipSuccess;
   size_t off = 0;
   for (int l = 0; l < 14 && e == hipSuccess; ++l) {
@@ -429,7 +456,7 @@ awk_status awk_unet_create(awk_unet_t *out, int nx, int ny, int nz, const float 
   if (e == hipSuccess) e = hipMemcpy(u->head_w, weights + off, 48 * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(&u->head_b, 3 * sizeof(float));
   if (e == hipSuccess) e = hipMemcpy(u->head_b, weights + off + 48, 3 * sizeof(float), hipMemcpyHostToDevice);
-  const size_t N0 = (size_t)nx * ny * nz, N1 = N0 / 8, N2 = N1 / 8, N3 = N2 / 8;
+  const size_t N0 = (size_t)nx * ny * nz * max_batch, N1 = N0 / 8, N2 = N1 / 8, N3 = N2 / 8;
   struct {
     bf16 **p;
     size_t n;
@@ -462,7 +489,19 @@ awk_status awk_unet_forward(awk_unet_t u, const float *templ, const float *subje
                             float *const disp[3], awk_stream_t stream) {
   if (!u || !templ || !subject || !disp || !disp[0] || !disp[1] || !disp[2])
     return fail(AWK_ERR_INVALID, "awk_unet_forward: null argument");
-  return unet_forward(u, templ, subject, disp, as_stream(stream));
+  return unet_forward(u, 1, &templ, &subject, disp, as_stream(stream));
+}
+
+awk_status awk_unet_forward_batch(awk_unet_t u, int batch, const float *const templ[],
+                                  const float *const subject[], float *const disp[],
+                                  awk_stream_t stream) {
+  if (!u || !templ || !subject || !disp) return fail(AWK_ERR_INVALID, "awk_unet_forward_batch: null argument");
+  if (batch < 1 || batch > u->max_batch)
+    return fail(AWK_ERR_INVALID, "awk_unet_forward_batch: batch must be 1..max_batch of awk_unet_create_batch");
+  for (int b = 0; b < batch; ++b)
+    if (!templ[b] || !subject[b] || !disp[3 * b] || !disp[3 * b + 1] || !disp[3 * b + 2])
+      return fail(AWK_ERR_INVALID, "awk_unet_forward_batch: null volume pointer");
+  return unet_forward(u, batch, templ, subject, disp, as_stream(stream));
 }
 
 awk_status awk_unet_mfma_probe(float *out, awk_stream_t stream) {

@@ -202,6 +202,16 @@ def addAffine(h, m, sign=1.0, stream=None):
 IDENTITY_AFFINE = ((1.0, 0.0, 0.0, 0.0), (0.0, 1.0, 0.0, 0.0), (0.0, 0.0, 1.0, 0.0))
 
 
+def affineDisplace(h, u, m=IDENTITY_AFFINE, stream=None):
+    """h(x) = M [x + u(x), 1] (velocityToH with unit spacing, then the 3x4 voxel
+    affine M); h may be u itself."""
+    import numpy as np
+    nx, ny, nz = _same(h, u)
+    arr = (ctypes.c_float * 12)(*[float(v) for v in np.asarray(m, dtype=np.float64).reshape(12)])
+    _lib.call("awk_hfield_affine_displace", *planes(h), *planes(u), nx, ny, nz, arr,
+              stream_handle(stream))
+
+
 def upsampleHField(h, new_size, rescale=True, base_in=IDENTITY_AFFINE, base_out=IDENTITY_AFFINE,
                    stream=None):
     """Scale-level change of an hfield (GreedyDeformationData::SetScaleLevel,

@@ -50,14 +50,18 @@ def split_weights(blob):
 
 
 class UNetWarmStart:
-    def __init__(self, size, weights=None, seed=0):
+    """The network on an (nx, ny, nz) grid, with activations for up to
+    ``max_batch`` pairs per forward pass."""
+
+    def __init__(self, size, weights=None, seed=0, max_batch=1):
         self.size = tuple(int(v) for v in size)  # (nx, ny, nz)
         w = init_weights(seed) if weights is None else np.ascontiguousarray(weights, np.float32)
         if w.size != weight_count():
             raise ValueError("weight blob has %d values, the network takes %d" % (w.size, weight_count()))
         self.weights = w
+        self.max_batch = int(max_batch)
         h = ctypes.c_void_p()
-        _lib.call("awk_unet_create", ctypes.byref(h), *self.size,
+        _lib.call("awk_unet_create_batch", ctypes.byref(h), *self.size, self.max_batch,
                   w.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(w.size))
         self._h = h
 
@@ -68,6 +72,30 @@ class UNetWarmStart:
             out = torch.empty((3, nz, ny, nx), dtype=torch.float32, device=template.device)
         _lib.call("awk_unet_forward", self._h, ptr(template), ptr(subject), planes_arr(out),
                   stream_handle(stream))
+        return out
+
+    def forward_batch(self, templates, subjects, out=None, stream=None):
+        """Displacements [B, 3, nz, ny, nx] of B pairs in one pass (B <= max_batch).
+        ``templates`` is one image shared by every pair or a list of B images."""
+        subjects = list(subjects)
+        B = len(subjects)
+        if not isinstance(templates, (list, tuple)):
+            templates = [templates] * B
+        if len(templates) != B:
+            raise ValueError("one template per subject (or a single shared one)")
+        nx, ny, nz = self.size
+        if out is None:
+            out = torch.empty((B, 3, nz, ny, nx), dtype=torch.float32, device=subjects[0].device)
+        for t in list(templates) + subjects:
+            ptr(t)
+            if tuple(t.shape) != (nz, ny, nx):
+                raise ValueError("images must be [nz, ny, nx] = %s" % ((nz, ny, nx),))
+        if tuple(out.shape) != (B, 3, nz, ny, nx) or not out.is_contiguous():
+            raise ValueError("out must be a contiguous [B, 3, nz, ny, nx] tensor")
+        tp = (ctypes.c_void_p * B)(*[t.data_ptr() for t in templates])
+        sp = (ctypes.c_void_p * B)(*[t.data_ptr() for t in subjects])
+        dp = (ctypes.c_void_p * (3 * B))(*[out[b, c].data_ptr() for b in range(B) for c in range(3)])
+        _lib.call("awk_unet_forward_batch", self._h, B, tp, sp, dp, stream_handle(stream))
         return out
 
     def warm_start_hfield(self, template, subject, stream=None):

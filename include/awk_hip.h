@@ -313,6 +313,13 @@ awk_status awk_hfield_resample(float *ox, float *oy, float *oz, int nx_out, int 
  * for M = identity, generalised to an affine base map. */
 awk_status awk_hfield_add_affine(float *hx, float *hy, float *hz, int nx, int ny, int nz,
                                  const float m[12], float sign, awk_stream_t stream);
+/* h(x) = M [x + u(x); 1]: velocityToH with unit spacing (HField3DUtils.txx:19-57)
+ * followed by an affine voxel-to-voxel map, e.g. the grid map of a subject whose
+ * displacement u is given in template voxels (the warm start's initial hfield).
+ * h may alias u. */
+awk_status awk_hfield_affine_displace(float *hx, float *hy, float *hz, const float *ux,
+                                      const float *uy, const float *uz, int nx, int ny, int nz,
+                                      const float m[12], awk_stream_t stream);
 
 /* ====================================================================== */
 /* Greedy atlas                                                             */
@@ -492,11 +499,23 @@ size_t awk_unet_weight_count(void);
  * Weights are rounded to bf16 on upload. */
 awk_status awk_unet_create(awk_unet_t *out, int nx, int ny, int nz, const float *weights,
                            size_t n_weights);
+/* As awk_unet_create, with activations for up to max_batch volumes (1..64, at most
+ * 2^26 voxels over the batch) for awk_unet_forward_batch. */
+awk_status awk_unet_create_batch(awk_unet_t *out, int nx, int ny, int nz, int max_batch,
+                                 const float *weights, size_t n_weights);
 awk_status awk_unet_destroy(awk_unet_t u);
 /* disp = U-Net(template, subject): bf16 activations, fp32 accumulation (MFMA),
  * LeakyReLU(0.2), 2x2x2 average pooling, nearest x2 upsampling; fp32 output. */
 awk_status awk_unet_forward(awk_unet_t u, const float *templ, const float *subject,
                             float *const disp[3], awk_stream_t stream);
+/* batch volumes in one pass (batch <= max_batch): disp[3b + c] receives component c
+ * of pair b's displacement.  Each output equals, bit for bit, the single-pair
+ * forward of its pair: the volumes are stacked as one grid whose convolution taps
+ * stay inside their own volume, and the summation order of every output is chosen
+ * from one volume's size. */
+awk_status awk_unet_forward_batch(awk_unet_t u, int batch, const float *const templ[],
+                                  const float *const subject[], float *const disp[],
+                                  awk_stream_t stream);
 /* Training on the device (no reference counterpart): MSE between the predicted
  * and a target displacement, backward on MFMA, Adam (beta 0.9 / 0.999, eps 1e-8)
  * on fp32 master weights initialised from `weights`; every step refreshes the

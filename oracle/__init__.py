@@ -49,6 +49,7 @@ def lib():
         _LIB = ctypes.CDLL(path)
         _LIB.orc_trilerp_scalar_d.restype = ctypes.c_double
         _LIB.orc_greedy_time_f.restype = ctypes.c_double
+        _LIB.orc_greedy_time_threads_f.restype = ctypes.c_double
     return _LIB
 
 
@@ -493,6 +494,15 @@ def greedy_time(shape3, n_iter=1, seed=0):
     nz, ny, nx = shape3
     size = (ctypes.c_int * 3)(nx, ny, nz)
     return lib().orc_greedy_time_f(size, int(n_iter), int(seed))
+
+
+def greedy_time_threads(shape3, n_threads, n_iter=1, seed=0):
+    """Wall seconds of n_threads concurrent FP32 greedy registrations (one host
+    thread each, n_iter iterations each), as the reference's CPU atlas runs one
+    thread per image (GreedyAtlasThreadCPU.cxx:19-37)."""
+    nz, ny, nx = shape3
+    size = (ctypes.c_int * 3)(nx, ny, nz)
+    return lib().orc_greedy_time_threads_f(size, int(n_iter), int(seed), int(n_threads))
 
 
 def fluidwarp_asym(fixed, moving, n_iter=20, alpha=0.01, beta=0.01, gamma=1e-4, max_pert=0.5):

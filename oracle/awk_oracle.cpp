@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <thread>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -1526,7 +1527,30 @@ void orc_ldmm_alphas_d(const orc_ldmm_params *p, const float *I0, const float *I
   ldmm_iterate<double>(p, I0, I1, vc.data(), false, step, nullptr, nullptr, alpha);
 }
 
+static void greedy_time_pair(const int *size, int n_iter, int seed, double *seconds);
+
 double orc_greedy_time_f(const int *size, int n_iter, int seed) {
+  double s = 0.0;
+  greedy_time_pair(size, n_iter, seed, &s);
+  return s / n_iter;
+}
+
+// Subject-parallel rate, as GreedyAtlasThreadCPU runs one thread per image
+// (GreedyAtlasThreadCPU.cxx:19-37): n_threads registrations of their own pairs,
+// n_iter FP32 iterations each, all threads started together.  Returns the wall
+// seconds from the first start to the last finish.
+double orc_greedy_time_threads_f(const int *size, int n_iter, int seed, int n_threads) {
+  std::vector<std::thread> th;
+  std::vector<double> secs(n_threads, 0.0);
+  auto t0 = std::chrono::steady_clock::now();
+  for (int k = 0; k < n_threads; ++k)
+    th.emplace_back(greedy_time_pair, size, n_iter, seed + k, &secs[k]);
+  for (auto &t : th) t.join();
+  auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double>(t1 - t0).count();
+}
+
+static void greedy_time_pair(const int *size, int n_iter, int seed, double *seconds) {
   // Synthetic blob pair, identity start; times n_iter FP32 iterations (1 thread).
   orc_greedy_params p;
   for (int i = 0; i < 3; ++i) { p.size[i] = size[i]; p.sp[i] = 1.0; }
@@ -1551,7 +1575,7 @@ double orc_greedy_time_f(const int *size, int n_iter, int seed) {
   auto t0 = std::chrono::steady_clock::now();
   greedy_run<float>(&p, I0.data(), I1.data(), h.data(), nullptr, nullptr, nullptr, nullptr);
   auto t1 = std::chrono::steady_clock::now();
-  return std::chrono::duration<double>(t1 - t0).count() / n_iter;
+  *seconds = std::chrono::duration<double>(t1 - t0).count();
 }
 
 int orc_fluidwarp_asym_f(const float *fixed, const float *moving, const int *size, double alpha,

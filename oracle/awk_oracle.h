@@ -123,6 +123,8 @@ int orc_greedy_run_d(const orc_greedy_params *p, const float *I0, const float *I
 /* One FP32 iteration timed for the CPU baseline (same as orc_greedy_run_f with
  * n_iter=1 but with preallocated state; returns seconds per iteration). */
 double orc_greedy_time_f(const int *size, int n_iter, int seed);
+/* Wall seconds of n_threads concurrent registrations (one thread each), n_iter each. */
+double orc_greedy_time_threads_f(const int *size, int n_iter, int seed, int n_threads);
 
 /* ---- Multiscale (ImageUtils.txx:255-347, GaussianFilter3D.cxx, HField3DUtils.txx:1814-1885) ---- */
 /* GaussianFilter3D with sigma[3] and half kernel sizes ksize[3] (float). */
