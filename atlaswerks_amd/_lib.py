@@ -44,6 +44,7 @@ SIGNATURES = {
                               _P],
     "awk_gradient": [_P, _P, _P, _P, _I, _I, _I, _F, _F, _F, _I, _P],
     "awk_splat": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _P],
+    "awk_splat_noalloc": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _P],
     "awk_splat_sorted": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _P, _P, _P],
     "awk_greedy_create": [ctypes.POINTER(_P), _I, _I, _I, _F, _F, _F, _F, _F, _F, _F],
     "awk_greedy_destroy": [_P],

@@ -14,6 +14,8 @@
 //   FLOAT    float atomics (fast; summation order, hence rounding, varies).
 // Buffers are cleared by fill kernels rather than memset nodes, so a captured
 // graph replays them correctly.
+#include <mutex>
+
 #include "awk_ops.h"
 #include "awk_splat_tile.h"
 
@@ -150,6 +152,73 @@ awk_status launch_splat(float *dst, const float *src, const float *hx, const flo
 }  // namespace awk
 
 using namespace awk;
+
+// ---- the splat without allocations (awk_splat_noalloc)
+//
+// FIXED64 needs an int64 accumulator beside dst.  Callers that cannot allocate
+// (code under graph capture, serving loops) get it from this file-scope array:
+// one copy per GPU, held until the process ends.  Host code never passes the
+// array's name to a kernel: its device address is fetched once per device with
+// hipGetSymbolAddress and kept in `g_scratch`.  Every call on a device shares that
+// device's array, so calls that overlap on one GPU race.  Each call's conversion
+// pass leaves the part it used cleared; a device's array is cleared once when it
+// is first set up and again after any call that failed.
+constexpr long long kNoAllocMaxVoxels = 1LL << 22;  // 32 MiB of int64 accumulators
+static __device__ __align__(256) unsigned long long scratch[kNoAllocMaxVoxels];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long *g_scratch[kMaxDevices] = {};
+bool g_scratch_dirty[kMaxDevices] = {};
+}  // namespace
+
+static awk_status noalloc_scratch(hipStream_t s, unsigned long long **out, int *dev_out) {
+  int dev = -1;
+  AWK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(AWK_ERR_UNSUPPORTED, "awk_splat_noalloc: device index too large");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_check(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                                                "awk_splat_noalloc: scratch symbol address");
+    g_scratch[dev] = reinterpret_cast<unsigned long long *>(p);
+    g_scratch_dirty[dev] = true;
+  }
+  if (g_scratch_dirty[dev]) {
+    awk_status st = clear_splat_acc(g_scratch[dev], kNoAllocMaxVoxels, s);
+    if (st != AWK_OK) return st;
+    g_scratch_dirty[dev] = false;
+  }
+  *out = g_scratch[dev];
+  *dev_out = dev;
+  return AWK_OK;
+}
+
+extern "C" awk_status awk_splat_noalloc(float *dst, const float *src, const float *hx,
+                                        const float *hy, const float *hz, int nx, int ny, int nz,
+                                        int mode, awk_stream_t stream) {
+  if (nx < 1 || ny < 1 || nz < 1)
+    return fail(AWK_ERR_INVALID, "awk_splat_noalloc: bad sizes");
+  if ((long long)nx * ny * nz > kNoAllocMaxVoxels)
+    return fail(AWK_ERR_INVALID, "awk_splat_noalloc: more than 2^22 voxels (the scratch holds 2^22 accumulators)");
+  if (!dst || !src || !hx || !hy || !hz) return fail(AWK_ERR_INVALID, "awk_splat_noalloc: null pointer");
+  if (mode != AWK_SPLAT_FIXED64 && mode != AWK_SPLAT_FIXED20 && mode != AWK_SPLAT_FLOAT)
+    return fail(AWK_ERR_INVALID, "awk_splat_noalloc: unknown mode");
+  hipStream_t s = as_stream(stream);
+  if (mode != AWK_SPLAT_FIXED64)  // these accumulate in dst itself and allocate nothing
+    return launch_splat(dst, src, hx, hy, hz, nx, ny, nz, mode, s);
+  unsigned long long *acc = nullptr;
+  int dev = -1;
+  awk_status st = noalloc_scratch(s, &acc, &dev);
+  if (st != AWK_OK) return st;
+  st = launch_splat_acc(dst, src, hx, hy, hz, nx, ny, nz, acc, s);
+  if (st != AWK_OK) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    g_scratch_dirty[dev] = true;
+  }
+  return st;
+}
 
 extern "C" awk_status awk_splat(float *dst, const float *src, const float *hx, const float *hy,
                                 const float *hz, int nx, int ny, int nz, int mode,

@@ -210,6 +210,17 @@ enum {
 awk_status awk_splat(float *dst, const float *src, const float *hx, const float *hy,
                      const float *hz, int nx, int ny, int nz, int mode, awk_stream_t stream);
 
+/* awk_splat without allocations: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under hipGraph capture, serving loops).
+ * FIXED64 accumulates in a file-scope device array of 2^22 int64 cells (32 MiB per
+ * GPU, held until the process ends; its address is fetched once per device with
+ * hipGetSymbolAddress); FIXED20 and FLOAT accumulate in dst as awk_splat does.
+ * Grids above 2^22 voxels return AWK_ERR_INVALID.  Results equal awk_splat's bit
+ * for bit (FIXED64, FIXED20).  Calls that overlap on one GPU share its array and
+ * race; each call leaves the cells it used cleared. */
+awk_status awk_splat_noalloc(float *dst, const float *src, const float *hx, const float *hy,
+                             const float *hz, int nx, int ny, int nz, int mode,
+                             awk_stream_t stream);
 /* Sort/scan splatting, cplSplating (CUDACommon/inc/cudaSplat.h:32-34,
  * cudaSplat.cu:175-238): positions are voxel + (vx, vy, vz) (a velocity in
  * voxels).  Every voxel emits 8 (cell, weight * src) pairs in the createIndex
