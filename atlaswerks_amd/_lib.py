@@ -61,6 +61,7 @@ SIGNATURES = {
     "awk_atlas_get_hfield": [_P, _I, ctypes.POINTER(_P3), _P],
     "awk_atlas_update_step_next": [_P],
     "awk_atlas_set_graph": [_P, _I],
+    "awk_atlas_set_batch": [_P, _I],
     "awk_atlas_iterate": [_P, _I, _P, _P],
     "awk_atlas_mean": [_P, _P, _P],
     "awk_atlas_deformed_image": [_P, _I, _P, _P],

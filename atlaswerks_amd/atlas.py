@@ -41,7 +41,7 @@ class _AtlasLevel:
     """One awk_atlas handle: template grid, subjects' (level) images, hfields."""
 
     def __init__(self, images, weights, size, spacing, max_pert, alpha, beta, gamma, use_graph,
-                 h0s, hinv0s=None):
+                 h0s, hinv0s=None, batch=None):
         self.size = tuple(int(v) for v in size)
         self.images = images  # kept alive: the library holds the pointers
         h = ctypes.c_void_p()
@@ -49,6 +49,8 @@ class _AtlasLevel:
                   alpha, beta, gamma, max_pert, len(images))
         self._h = h
         _lib.call("awk_atlas_set_graph", self._h, int(bool(use_graph)))
+        if batch is not None:
+            _lib.call("awk_atlas_set_batch", self._h, int(batch))
         for i, (im, w, h0) in enumerate(zip(images, weights, h0s)):
             iz, iy, ix = im.shape
             _lib.call("awk_atlas_set_subject", self._h, i, ptr(im), ix, iy, iz, float(w),
@@ -81,7 +83,7 @@ class GreedyAtlasBuilderGPU:
     def __init__(self, subjects, template_size, template_origin=(0.0, 0.0, 0.0),
                  template_spacing=(1.0, 1.0, 1.0), max_pert=0.1, alpha=0.01, beta=0.01,
                  gamma=0.001, use_graph=True, scale_levels=None, compute_inverse=False,
-                 initial_hfields=None):
+                 initial_hfields=None, batch=None):
         if not subjects:
             raise ValueError("an atlas needs at least one subject")
         self.subjects = list(subjects)
@@ -89,6 +91,10 @@ class GreedyAtlasBuilderGPU:
         self.origin = tuple(float(v) for v in template_origin)
         self.spacing = tuple(float(v) for v in template_spacing)
         self.prm = (max_pert, alpha, beta, gamma, use_graph)
+        # subjects per batched launch of the steady-state greedy step: None = the
+        # library's default, 0 = one launch per subject on concurrent stream lanes,
+        # an int, or {level factor: int} (autotune.tune_atlas_batch picks these)
+        self.batch = batch
         ns = len(self.subjects)
         self.weights = [s.weight if s.weight is not None else 1.0 / ns for s in self.subjects]
         self.scale_levels = list(scale_levels) if scale_levels else None
@@ -174,7 +180,8 @@ class GreedyAtlasBuilderGPU:
         max_pert, alpha, beta, gamma, use_graph = self.prm
         self._level_index += 1
         self._level = _AtlasLevel(images, self.weights, size, sp, max_pert, alpha, beta, gamma,
-                                  use_graph, h0s, hinv0s if self.compute_inverse else None)
+                                  use_graph, h0s, hinv0s if self.compute_inverse else None,
+                                  self.batch.get(f) if isinstance(self.batch, dict) else self.batch)
         self._factor = f
 
     @property

@@ -3,9 +3,9 @@
 Each knob (``awk_tune_set``) selects between kernel variants that compute the
 same result with a different amount of work per thread.  ``tune()`` times every
 variant on the actual GPU at the workload's grid size, keeps the fastest, and
-returns the choice.  FFT radix schedules are fixed per axis length by the
-library's planner; the knobs here are the gather kernels' memory-level
-parallelism.
+returns the choice.  ``tune_atlas_batch()`` picks, per scale level of an atlas,
+how many subjects each batched launch of the greedy step covers (bit-identical
+results for every choice).
 """
 import ctypes
 
@@ -123,4 +123,85 @@ def tune(size=(256, 256, 256), reps=10, device="cuda", verbose=False):
     del it, data
     if verbose:
         print("autotune", chosen, timings)
+    return chosen, timings
+
+
+def tune_atlas_batch(subjects, template_size, levels=((1, 0),), candidates=(0, 1, 2, 4, 8, 16),
+                     reps=5, **atlas_kw):
+    """Subjects per batched launch for each scale level factor of an atlas.
+
+    For every level a scratch atlas of the subjects (identity start) runs its
+    step-size iteration, then each candidate (0: one launch per subject on the
+    stream lanes) is timed over ``reps`` hipGraph-replayed iterations; the
+    fastest wins.  Returns ({factor: batch}, {factor: {batch: ms}})."""
+    from . import atlas
+    chosen, timings = {}, {}
+    ns = len(subjects)
+    for f, _ in levels:
+        b = atlas.GreedyAtlasBuilderGPU(subjects, template_size, use_graph=True,
+                                        scale_levels=[(f, 1)], **atlas_kw)
+        b.Run()  # enters the level, sets the step sizes
+        t = {}
+        for c in candidates:
+            if c > ns:
+                continue
+            _lib.call("awk_atlas_set_batch", b._h, int(c))
+            b.Iterate(1)  # graph capture for this batch size
+            t[c] = _events(lambda: b.Iterate(1), reps)
+        del b
+        best = min(t, key=t.get)
+        chosen[f], timings[f] = best, t
+    return chosen, timings
+
+
+def _events(fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+PLAN_KNOBS = {
+    "fft_order": (0, 1),
+    "x_rt": (0, 2, 4, 6, 8),
+    "y_tx": (0, 1, 2, 4, 8, 16),
+    "z_tx": (0, 1, 2, 4, 8),
+}
+
+
+def tune_fft(size, reps=10, device="cuda", verbose=False):
+    """FFT plan for one grid (nx, ny, nz): the radix order of the generic LDS
+    stages and the x / y / z pass tiles, chosen one knob after the other by timing
+    the operator L^-1 (DiffOperGPU::ApplyInverseOperator) through graph replays.
+    Plans are read when an operator is created, so the chosen knobs stay set for
+    the operators made afterwards.  They change results only at FP32 rounding
+    level (another butterfly order or kernel).  Returns (chosen, timings)."""
+    from .fluid import DiffOperGPU
+    nx, ny, nz = size
+    f = torch.randn((3, nz, ny, nx), device=device)
+    chosen, timings = {}, {}
+    op = DiffOperGPU((nx, ny, nz))  # warm-up: the first operator's first-use costs
+    _time(lambda: op.ApplyInverseOperator(f), reps)
+    del op
+    for name, values in PLAN_KNOBS.items():
+        timings[name] = {}
+        best = None
+        for v in values:
+            set_knob(name, v)
+            op = DiffOperGPU((nx, ny, nz))
+            ms = _time(lambda: op.ApplyInverseOperator(f), reps)
+            del op
+            timings[name][v] = ms
+            if best is None or ms < best[1] * 0.99:  # keep the earlier (simpler) value on a tie
+                best = (v, ms)
+        set_knob(name, best[0])
+        chosen[name] = best[0]
+    if verbose:
+        print("tune_fft", size, chosen, timings)
     return chosen, timings

@@ -14,7 +14,13 @@
 // (ComputeThreadMean) is the same sum of the Idef_s that iteration k+1 needs, so
 // stage A serves both.  Steady-state iterations (step sizes known) are captured
 // once into a hipGraph per hfield parity and replayed: one graph launch per atlas
-// iteration for all subjects.
+// iteration for all subjects.  In those iterations stage B runs batched: each of its
+// five passes is one launch over all subjects (blockIdx.y = subject), every
+// subject's hfields, deformed image, step and energy state lying at a fixed stride
+// in per-atlas slabs, each with its own spectrum.  The kernels and their
+// block-to-data mapping are those of the per-subject launches, so the results are
+// bit-identical; iterations that set step sizes or track inverse fields take the
+// per-subject path on the stream lanes.
 #include <algorithm>
 #include <cstring>
 #include <new>
@@ -107,6 +113,16 @@ struct awk_atlas {
   bool use_graph = true;
   bool tables_dirty = true;
   hipGraphExec_t exec[2] = {nullptr, nullptr};
+  // slabs holding every subject's buffers at fixed strides (batched stage B)
+  float *hslab[2] = {nullptr, nullptr};  // ns x 3 planes
+  float *idef_slab = nullptr;            // ns x N
+  float *step_slab = nullptr;            // ns
+  double *prev_slab = nullptr;           // ns
+  int *have_prev_slab = nullptr;         // ns
+  int batch = 0;                         // subjects per batched launch (0: per-subject path)
+  int batch_cap = 0;                     // subjects the batch spectra hold
+  float2 *bspec = nullptr;               // batch_cap spectra
+  double *bpart = nullptr;               // batch_cap x x-pass partials
 };
 
 using namespace awk;
@@ -123,12 +139,13 @@ static void atlas_free(awk_atlas *a) {
   atlas_drop_graphs(a);
   fk_release(a->k);
   for (Subject &s : a->subj) {
-    void *b[] = {s.h[0], s.h[1], s.hinv[0], s.hinv[1], s.idef, s.step, s.prev, s.have_prev};
+    void *b[] = {s.hinv[0], s.hinv[1]};
     for (void *p : b)
       if (p) (void)hipFree(p);
   }
   void *b[] = {a->mean, (void *)a->idef_tab, a->w_dev, a->esub, a->partials, a->maxpart,
-               a->hist, a->cursor, a->vbuf, a->flags};
+               a->hist, a->cursor, a->vbuf, a->flags, a->hslab[0], a->hslab[1], a->idef_slab,
+               a->step_slab, a->prev_slab, a->have_prev_slab, a->bspec, a->bpart};
   for (void *p : b)
     if (p) (void)hipFree(p);
   for (Lane &L : a->lanes) {
@@ -160,6 +177,55 @@ static awk_status atlas_mean_stage(awk_atlas *a, hipStream_t s) {
   return AWK_OK;
 }
 
+// Stage B batched: subjects [s0, s0 + n) of the slabs in one launch per pass.
+static awk_status atlas_stage_b_batched(awk_atlas *a, int s0, int n, hipStream_t s) {
+  awk_status st;
+  const long long N = (long long)a->nx * a->ny * a->nz;
+  const int nxt = 1 - a->cur;
+  FluidKernel k = a->k;  // the atlas operator's tables, the batch's spectra
+  k.spec = a->bspec;
+  Batch b;
+  b.n = n;
+  b.spec = 3 * (long long)a->k.spec_plane();
+  b.img = N;
+  b.field = 3 * N;
+  b.part = fk_xblocks(a->k);
+  b.scalar = 1;
+  const Subject &sb = a->subj[s0];
+  XFwdArgs fa;
+  std::memset(&fa, 0, sizeof(fa));
+  fa.mode = XLOAD_FORCE;
+  fa.idef = sb.idef;
+  fa.i1 = a->mean;
+  xfwd_spacing(fa, a->sp);
+  fa.partials = a->bpart;
+  int nb = 0;
+  if ((st = fk_xfwd(k, fa, s, &nb, &b))) return st;
+  ReduceArgs ra;
+  ra.partials = a->bpart;
+  ra.n = (a->ny * a->nz + k.rt - 1) / k.rt;
+  ra.scale = (double)a->sp[0] * a->sp[1] * a->sp[2];
+  ra.energy_out = a->esub + s0;
+  ra.prev = sb.prev;
+  ra.have_prev = sb.have_prev;
+  ra.step = sb.step;
+  ra.auto_reduce = 0;
+  ra.flags = a->flags;
+  if ((st = fk_ypass(k, false, s, &ra, &b))) return st;
+  if ((st = fk_zsolve(k, true, s, &b))) return st;
+  if ((st = fk_ypass(k, true, s, nullptr, &b))) return st;
+  XInvArgs ia;
+  std::memset(&ia, 0, sizeof(ia));
+  for (int c = 0; c < 3; ++c) ia.sp[c] = a->sp[c];
+  ia.mode = XSTORE_COMPOSE;
+  for (int c = 0; c < 3; ++c) {
+    ia.h[c] = sb.h[a->cur] + c * N;
+    ia.hnew[c] = sb.h[nxt] + c * N;
+  }
+  ia.step = sb.step;
+  return fk_xinv(k, ia, s, &nb, &b);
+}
+
 // One atlas iteration on stream s (stages A, M, B, E).  Leaves the new hfields in
 // buffer 1 - cur; the caller flips cur.
 static awk_status atlas_iteration(awk_atlas *a, hipStream_t s) {
@@ -167,6 +233,16 @@ static awk_status atlas_iteration(awk_atlas *a, hipStream_t s) {
   const int N = a->nx * a->ny * a->nz;
   const int nxt = 1 - a->cur;
   if ((st = atlas_mean_stage(a, s))) return st;
+  bool batched = a->batch > 0;
+  for (const Subject &sb : a->subj) batched = batched && !sb.update_step && !sb.hinv[0];
+  if (batched) {
+    for (int s0 = 0; s0 < a->ns; s0 += a->batch)
+      if ((st = atlas_stage_b_batched(a, s0, std::min(a->batch, a->ns - s0), s))) return st;
+    AWK_LAUNCH(k_atlas_energy, dim3(1), dim3(64), 0, s, a->esub, a->w_dev, a->ns, a->hist,
+               a->cursor, kHistCap);
+    AWK_LAUNCH_CHECK("k_atlas_energy");
+    return AWK_OK;
+  }
   const int nl = 1 + (int)a->lanes.size();
   if (nl > 1) {
     AWK_HIP(hipEventRecord(a->ev_fork, s));
@@ -304,15 +380,37 @@ awk_status awk_atlas_create(awk_atlas_t *out, int nx, int ny, int nz, float spx,
   const size_t N = (size_t)nx * ny * nz;
   const int nblk = fk_xblocks(a->k);
   hipError_t e = hipSuccess;
-  for (Subject &s : a->subj) {
-    if (e == hipSuccess) e = hipMalloc(&s.h[0], 3 * N * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&s.h[1], 3 * N * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&s.idef, N * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&s.step, sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&s.prev, sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&s.have_prev, sizeof(int));
-    if (e == hipSuccess) e = hipMemset(s.step, 0, sizeof(float));
-    if (e == hipSuccess) e = hipMemset(s.have_prev, 0, sizeof(int));
+  const size_t ns = (size_t)n_subjects;
+  if (e == hipSuccess) e = hipMalloc(&a->hslab[0], ns * 3 * N * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&a->hslab[1], ns * 3 * N * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&a->idef_slab, ns * N * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&a->step_slab, ns * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&a->prev_slab, ns * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&a->have_prev_slab, ns * sizeof(int));
+  if (e == hipSuccess) e = hipMemset(a->step_slab, 0, ns * sizeof(float));
+  if (e == hipSuccess) e = hipMemset(a->have_prev_slab, 0, ns * sizeof(int));
+  if (e == hipSuccess)
+    for (size_t i = 0; i < ns; ++i) {
+      Subject &s = a->subj[i];
+      s.h[0] = a->hslab[0] + i * 3 * N;
+      s.h[1] = a->hslab[1] + i * 3 * N;
+      s.idef = a->idef_slab + i * N;
+      s.step = a->step_slab + i;
+      s.prev = a->prev_slab + i;
+      s.have_prev = a->have_prev_slab + i;
+    }
+  // batch spectra for up to every subject (at most 8 GiB of them); by default a
+  // launch takes as many subjects as keep their spectra within 128 MiB, half the
+  // memory-side cache, so each pass still reads what the previous one wrote from
+  // that cache (16 subjects at 128^3 in one launch: 1.81 ms per iteration against
+  // 1.70 ms one launch per subject on 3 lanes, measured)
+  {
+    const size_t sb = 3 * a->k.spec_plane() * sizeof(float2);
+    size_t cap = std::min(ns, std::max<size_t>(1, (8ull << 30) / sb));
+    if (e == hipSuccess) e = hipMalloc(&a->bspec, cap * sb);
+    if (e == hipSuccess) e = hipMalloc(&a->bpart, cap * (size_t)nblk * sizeof(double));
+    a->batch_cap = (int)cap;
+    a->batch = (int)std::min(cap, std::max<size_t>(1, (128ull << 20) / sb));
   }
   if (e == hipSuccess) e = hipMalloc(&a->mean, N * sizeof(float));
   if (e == hipSuccess) e = hipMalloc((void **)&a->idef_tab, n_subjects * sizeof(float *));
@@ -454,6 +552,15 @@ awk_status awk_atlas_set_graph(awk_atlas_t a, int on) {
   if (!a) return fail(AWK_ERR_INVALID, "awk_atlas_set_graph: null handle");
   a->use_graph = on != 0;
   if (!a->use_graph) atlas_drop_graphs(a);
+  return AWK_OK;
+}
+
+awk_status awk_atlas_set_batch(awk_atlas_t a, int n) {
+  if (!a) return fail(AWK_ERR_INVALID, "awk_atlas_set_batch: null handle");
+  if (n < 0) return fail(AWK_ERR_INVALID, "awk_atlas_set_batch: negative batch");
+  const int b = std::min(n, a->batch_cap);
+  if (b != a->batch) atlas_drop_graphs(a);
+  a->batch = b;
   return AWK_OK;
 }
 

@@ -159,6 +159,12 @@ struct Tuning {
   int compose_vpt = 1;  // voxels per thread, compose family
   int xinv_unroll = 1;  // voxels per epilogue step of the inverse x pass
   int splat_tile = 1;   // fixed-point splat through per-block LDS windows (0: direct atomics)
+  // FFT plan knobs, read when an operator is created; they change results at
+  // FP32 rounding level (another butterfly order or kernel), never beyond
+  int fft_order = 0;    // radix order of the generic LDS stages: 0 largest first, 1 reversed
+  int x_rt = 0;         // rows per x-pass block (0: the largest even <= 8 that fits LDS)
+  int y_tx = 0;         // kx columns per generic y-pass block (0: the largest <= 16 that fits)
+  int z_tx = 0;         // kx columns per generic z-pass block (0: the largest <= 8 that fits)
 };
 Tuning &tuning();
 

@@ -85,6 +85,19 @@ struct XInvArgs {
   float sigma2;             // AXPBY / MAXAXPY: sigma^2
 };
 
+// Subjects of one batched launch: blockIdx.y selects the subject, and each of its
+// buffers lies at a fixed stride from subject 0's (every stride 0 and n = 1: one
+// subject).  The operator's tables are shared; only the spectrum, the images,
+// fields, partial sums and per-subject scalars move.
+struct Batch {
+  int n = 1;             // subjects in the launch (gridDim.y)
+  long long spec = 0;    // float2 elements between spectra
+  long long img = 0;     // floats between deformed images (forward x pass, FORCE)
+  long long field = 0;   // floats between hfields (inverse x pass, COMPOSE: h and hnew)
+  long long part = 0;    // doubles between the x-pass partial arrays
+  int scalar = 0;        // step, energy, prev, have_prev: one per subject (stride 1)
+};
+
 struct FluidKernel {
   int nx = 0, ny = 0, nz = 0;
   int nxh = 0, nxh_pad = 0;
@@ -109,16 +122,19 @@ struct FluidKernel {
 awk_status fk_init(FluidKernel &k, int nx, int ny, int nz, const float sp[3]);
 void fk_release(FluidKernel &k);
 awk_status fk_prepare_eigen(FluidKernel &k, int dir, hipStream_t s);
-// Pass launchers.
-awk_status fk_xfwd(FluidKernel &k, const XFwdArgs &a, hipStream_t s, int *nblocks);
-awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s, const ReduceArgs *red);
+// Pass launchers; b (may be null: one subject) batches subjects over blockIdx.y.
+awk_status fk_xfwd(FluidKernel &k, const XFwdArgs &a, hipStream_t s, int *nblocks,
+                   const Batch *b = nullptr);
+awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s, const ReduceArgs *red,
+                    const Batch *b = nullptr);
 // step = MaxPert / max over n per-block maxima (GreedyIteratorCPU.cxx:65-79,
 // 129-134; LDMMIteratorGPU.cxx:286-289); a NaN, infinite or zero maximum raises
 // AWK_FLAG_BAD_STEP in flags (may be null).  The step is stored as computed.
 awk_status launch_step_from_max(const float *maxpart, int n, float max_pert, float *step,
                                 unsigned *flags, hipStream_t s);
-awk_status fk_zsolve(FluidKernel &k, bool inverse_op, hipStream_t s);
-awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblocks);
+awk_status fk_zsolve(FluidKernel &k, bool inverse_op, hipStream_t s, const Batch *b = nullptr);
+awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblocks,
+                   const Batch *b = nullptr);
 int fk_xblocks(const FluidKernel &k);
 
 }  // namespace awk

@@ -120,6 +120,18 @@ __device__ __forceinline__ void line_targets(int line, int r0, int &rowA, int &c
   else { rowA = ra + 1; cA = 0; rowB = ra + 1; cB = 1; }
 }
 
+// Batched launches: subject blockIdx.y's reduction arguments.
+__device__ __forceinline__ void batch_reduce(ReduceArgs &ra, const Batch &bt) {
+  const long long sb = blockIdx.y;
+  ra.partials += sb * bt.part;
+  if (bt.scalar) {
+    if (ra.energy_out) ra.energy_out += sb;
+    ra.prev += sb;
+    ra.have_prev += sb;
+    ra.step += sb;
+  }
+}
+
 // Energy bookkeeping of one greedy iteration, run by logical block 0 of the
 // forward y pass (the kernel boundary after the x pass makes every partial
 // visible): fixed-order FP64 sum of the x-pass block partials, then
@@ -167,9 +179,15 @@ __device__ __forceinline__ float load_mul(const XFwdArgs &a, size_t i) {
 template <int NC, int RTC, int BDC>
 __global__ void __launch_bounds__(BDC ? BDC : 1024)
 k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, int RT_rt,
-       float2 *__restrict__ spec, size_t plane) {
+       float2 *__restrict__ spec, size_t plane, Batch bt) {
   extern __shared__ float2 sm[];
   __shared__ double red[16];
+  {  // subject blockIdx.y of a batched launch
+    const long long sb = blockIdx.y;
+    spec += sb * bt.spec;
+    if (a.idef) a.idef += sb * bt.img;
+    if (a.partials) a.partials += sb * bt.part;
+  }
   const int nx = NC ? NC : nx_rt, nxh = NC ? NC / 2 + 1 : nxh_rt, RT = RTC ? RTC : RT_rt;
   const int bdim = BDC ? BDC : (int)blockDim.x;
   const int NL = 3 * RT / 2, LS = NL + 1;
@@ -330,9 +348,20 @@ k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
 template <int NC, int RTC, int BDC, int XU>
 __global__ void __launch_bounds__(BDC ? BDC : 1024)
 k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, int RT_rt,
-       const float2 *__restrict__ spec, size_t plane) {
+       const float2 *__restrict__ spec, size_t plane, Batch bt) {
   extern __shared__ float2 sm[];
   __shared__ float redf[16];
+  {  // subject blockIdx.y of a batched launch
+    const long long sb = blockIdx.y;
+    spec += sb * bt.spec;
+    for (int c = 0; c < 3; ++c) {
+      if (a.h[c]) a.h[c] += sb * bt.field;
+      if (a.hnew[c]) a.hnew[c] += sb * bt.field;
+      if (a.out[c]) a.out[c] += sb * bt.field;
+    }
+    if (bt.scalar) a.step += sb;
+    if (a.maxpart) a.maxpart += sb * (long long)gridDim.x;
+  }
   const int nx = NC ? NC : nx_rt, nxh = NC ? NC / 2 + 1 : nxh_rt, RT = RTC ? RTC : RT_rt;
   const int bdim = BDC ? BDC : (int)blockDim.x;
   const int NL = 3 * RT / 2, LS = NL + 1;
@@ -475,8 +504,9 @@ k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
 // Y pass: complex FFT along y for TX kx columns of one (component, z) slab.
 __global__ void __launch_bounds__(1024)
 k_ypass(Axis ax, int ny, int nz, int nxh, int nxh_pad, int TX, float2 *__restrict__ spec,
-        size_t plane, int inv, ReduceArgs ra) {
+        size_t plane, int inv, ReduceArgs ra, Batch bt) {
   extern __shared__ float2 sm[];
+  spec += (long long)blockIdx.y * bt.spec;
   const int ntile = (nxh + TX - 1) / TX;
   const int lb = (int)xcd_block();
   if (lb >= 3 * nz * ntile) return;
@@ -496,6 +526,7 @@ k_ypass(Axis ax, int ny, int nz, int nxh, int nxh_pad, int TX, float2 *__restric
   }
   if (ra.partials && lb == 0) {
     __syncthreads();
+    batch_reduce(ra, bt);
     energy_block(ra, reinterpret_cast<double *>(sm), blockDim.x);
   }
 }
@@ -512,9 +543,10 @@ k_ypass(Axis ax, int ny, int nz, int nxh, int nxh_pad, int TX, float2 *__restric
 template <int N, int RA, int RB, int TX>
 __global__ void __launch_bounds__(TX *(N / RA > N / RB ? N / RA : N / RB))
 k_ypass_reg(const float2 *__restrict__ tw, int nz, int nxh, int nxh_pad, float2 *__restrict__ spec,
-            size_t plane, int inv, ReduceArgs ra) {
+            size_t plane, int inv, ReduceArgs ra, Batch bt) {
   constexpr int PA = N / RA, PB = N / RB;
   __shared__ float2 sm[N * TX];
+  spec += (long long)blockIdx.y * bt.spec;
   const int t = threadIdx.x % TX, j = threadIdx.x / TX;
   const int ntile = (nxh + TX - 1) / TX;
   const int lb = (int)xcd_block();
@@ -552,6 +584,7 @@ k_ypass_reg(const float2 *__restrict__ tw, int nz, int nxh, int nxh_pad, float2 
   }
   if (ra.partials && lb == 0) {
     __syncthreads();
+    batch_reduce(ra, bt);
     energy_block(ra, reinterpret_cast<double *>(sm), TX * (PA > PB ? PA : PB));
   }
 }
@@ -621,9 +654,10 @@ __device__ __forceinline__ void solve_one(const SolveArgs &s, int kx, int ky, in
 template <int N, int RA, int RB, int TX>
 __global__ void __launch_bounds__(TX *(N / RA > N / RB ? N / RA : N / RB), N >= 256 ? 3 : 4)
 k_zsolve_reg(const float2 *__restrict__ tw, SolveArgs sa, int ny, int nxh, int nxh_pad,
-             float2 *__restrict__ spec, size_t plane) {
+             float2 *__restrict__ spec, size_t plane, Batch bt) {
   constexpr int PA = N / RA, PB = N / RB;  // PB = RA: threads holding the spectrum
   __shared__ float2 sm[N * TX];
+  spec += (long long)blockIdx.y * bt.spec;
   const int t = threadIdx.x % TX, j = threadIdx.x / TX;
   const int ntile = (nxh + TX - 1) / TX;
   const int lb = (int)xcd_block();
@@ -689,8 +723,9 @@ k_zsolve_reg(const float2 *__restrict__ tw, SolveArgs sa, int ny, int nxh, int n
 // per-frequency solve, inverse FFT along z.
 __global__ void __launch_bounds__(1024)
 k_zsolve(Axis az, SolveArgs s, int ny, int nz, int nxh, int nxh_pad, int TX,
-         float2 *__restrict__ spec, size_t plane) {
+         float2 *__restrict__ spec, size_t plane, Batch bt) {
   extern __shared__ float2 sm[];
+  spec += (long long)blockIdx.y * bt.spec;
   const int NL = 3 * TX;
   const int ntile = (nxh + TX - 1) / TX;
   const int lb = (int)xcd_block();
@@ -803,13 +838,16 @@ static size_t yz_lds(const AxisPlan &a, int lines) {
 }
 
 static int pick_rt(const AxisPlan &a) {
-  // largest even RT <= 8 whose LDS fits 64 KiB
+  // the tuned RT if it fits, else the largest even RT <= 8 whose LDS fits 64 KiB
+  const int want = tuning().x_rt;
+  if (want > 0 && x_lds(a, want) <= 65536) return want;
   for (int rt = 8; rt >= 2; rt -= 2)
     if (x_lds(a, rt) <= 65536) return rt;
   return 0;
 }
 
-static int pick_tx(const AxisPlan &a, int lines_per_col, int max_tx) {
+static int pick_tx(const AxisPlan &a, int lines_per_col, int max_tx, int want) {
+  if (want > 0 && want <= max_tx && yz_lds(a, lines_per_col * want) <= 65536) return want;
   for (int tx = max_tx; tx >= 1; tx /= 2)
     if (yz_lds(a, lines_per_col * tx) <= 65536) return tx;
   return 0;
@@ -845,12 +883,16 @@ awk_status fk_init(FluidKernel &k, int nx, int ny, int nz, const float sp[3]) {
       k.ax[i].m = m;
       k.ax[i].mrad = factor(m);
     }
+    if (tuning().fft_order) {  // the stages in the reverse order (a tuned plan)
+      std::reverse(k.ax[i].rad.begin(), k.ax[i].rad.end());
+      std::reverse(k.ax[i].mrad.begin(), k.ax[i].mrad.end());
+    }
     if ((int)k.ax[i].rad.size() > kMaxStages || (int)k.ax[i].mrad.size() > 8)
       return fail(AWK_ERR_INVALID, "fluid kernel: too many FFT stages");
   }
   k.rt = pick_rt(k.ax[0]);
-  k.tx_y = pick_tx(k.ax[1], 1, 16);
-  k.tx_z = pick_tx(k.ax[2], 3, 8);
+  k.tx_y = pick_tx(k.ax[1], 1, 16, tuning().y_tx);
+  k.tx_z = pick_tx(k.ax[2], 3, 8, tuning().z_tx);
   if (!k.rt || !k.tx_y || !k.tx_z) return fail(AWK_ERR_UNSUPPORTED, "fluid kernel: axis too long for LDS staging");
   k.bd_x = pick_bd(k.ax[0].stage_len(), 3 * k.rt / 2);
   k.bd_y = pick_bd(k.ax[1].stage_len(), k.tx_y);
@@ -939,13 +981,14 @@ int fk_xblocks(const FluidKernel &k) {
   return (int)grid_for((long long)k.ny * k.nz, k.rt);
 }
 
-awk_status fk_xfwd(FluidKernel &k, const XFwdArgs &a, hipStream_t s, int *nblocks) {
+awk_status fk_xfwd(FluidKernel &k, const XFwdArgs &a, hipStream_t s, int *nblocks, const Batch *b) {
+  const Batch bt = b ? *b : Batch();
   const unsigned grid = grid_for((long long)k.ny * k.nz, k.rt);
   const size_t lds = x_lds(k.ax[0], k.rt);
   const Axis ax = k.ax[0].dev();
 #define AWK_XF(NC, RTC, BDC)                                                                     \
-  AWK_LAUNCH((k_xfwd<NC, RTC, BDC>), dim3(grid), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \
-                     k.nx, k.ny, k.nz, k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane())
+  AWK_LAUNCH((k_xfwd<NC, RTC, BDC>), dim3(grid, bt.n), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \
+                     k.nx, k.ny, k.nz, k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane(), bt)
   // block size = row length selects column mode (128^3 LDMM: 17.2 -> 16.3 us
   // against the generic 256-thread form; the spectrum is the same bit for bit)
   if (k.nx == 256 && k.rt == 8) AWK_XF(256, 8, 256);
@@ -958,7 +1001,8 @@ awk_status fk_xfwd(FluidKernel &k, const XFwdArgs &a, hipStream_t s, int *nblock
   return AWK_OK;
 }
 
-awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s, const ReduceArgs *red) {
+awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s, const ReduceArgs *red, const Batch *b) {
+  const Batch bt = b ? *b : Batch();
   ReduceArgs ra;
   std::memset(&ra, 0, sizeof(ra));
   if (red) ra = *red;
@@ -966,8 +1010,8 @@ awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s, const ReduceArgs *r
   const int ntile16 = (k.nxh + 15) / 16;
   const unsigned g16 = grid_for((long long)3 * k.nz * ntile16, 1);
 #define AWK_YR(N, RA, RB)                                                                     \
-  AWK_LAUNCH((k_ypass_reg<N, RA, RB, 16>), dim3(g16), dim3(16 * (N / RA > N / RB ? N / RA : N / RB)), \
-                     0, s, tw, k.nz, k.nxh, k.nxh_pad, k.spec, k.spec_plane(), inv ? 1 : 0, ra)
+  AWK_LAUNCH((k_ypass_reg<N, RA, RB, 16>), dim3(g16, bt.n), dim3(16 * (N / RA > N / RB ? N / RA : N / RB)), \
+                     0, s, tw, k.nz, k.nxh, k.nxh_pad, k.spec, k.spec_plane(), inv ? 1 : 0, ra, bt)
   if (k.ny == 256) AWK_YR(256, 16, 16);
   else if (k.ny == 128) AWK_YR(128, 16, 8);
   else if (k.ny == 64) AWK_YR(64, 8, 8);
@@ -976,15 +1020,16 @@ awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s, const ReduceArgs *r
     const int ntile = (k.nxh + k.tx_y - 1) / k.tx_y;
     const unsigned grid = grid_for((long long)3 * k.nz * ntile, 1);
     const size_t lds = std::max(yz_lds(k.ax[1], k.tx_y), (size_t)k.bd_y * sizeof(double));
-    AWK_LAUNCH(k_ypass, dim3(grid), dim3(k.bd_y), lds, s, k.ax[1].dev(), k.ny, k.nz, k.nxh,
-                       k.nxh_pad, k.tx_y, k.spec, k.spec_plane(), inv ? 1 : 0, ra);
+    AWK_LAUNCH(k_ypass, dim3(grid, bt.n), dim3(k.bd_y), lds, s, k.ax[1].dev(), k.ny, k.nz, k.nxh,
+                       k.nxh_pad, k.tx_y, k.spec, k.spec_plane(), inv ? 1 : 0, ra, bt);
   }
 #undef AWK_YR
   AWK_LAUNCH_CHECK("k_ypass");
   return AWK_OK;
 }
 
-awk_status fk_zsolve(FluidKernel &k, bool inverse_op, hipStream_t s) {
+awk_status fk_zsolve(FluidKernel &k, bool inverse_op, hipStream_t s, const Batch *b) {
+  const Batch bt = b ? *b : Batch();
   const bool eigen = (k.flags & AWK_FK_EIGEN) || k.lpow != 1.0f;
   if (eigen) {
     awk_status st = fk_prepare_eigen(k, inverse_op ? -1 : 1, s);
@@ -1008,8 +1053,8 @@ awk_status fk_zsolve(FluidKernel &k, bool inverse_op, hipStream_t s) {
   const int ntile16 = (k.nxh + 15) / 16;
   const unsigned g16 = grid_for((long long)k.ny * ntile16, 1);
 #define AWK_ZR(N, RA, RB)                                                                     \
-  AWK_LAUNCH((k_zsolve_reg<N, RA, RB, 16>), dim3(g16), dim3(16 * (N / RA > N / RB ? N / RA : N / RB)), \
-                     0, s, tw, sa, k.ny, k.nxh, k.nxh_pad, k.spec, k.spec_plane())
+  AWK_LAUNCH((k_zsolve_reg<N, RA, RB, 16>), dim3(g16, bt.n), dim3(16 * (N / RA > N / RB ? N / RA : N / RB)), \
+                     0, s, tw, sa, k.ny, k.nxh, k.nxh_pad, k.spec, k.spec_plane(), bt)
   if (k.nz == 256) AWK_ZR(256, 16, 16);
   else if (k.nz == 128) AWK_ZR(128, 16, 8);
   else if (k.nz == 64) AWK_ZR(64, 8, 8);
@@ -1018,21 +1063,22 @@ awk_status fk_zsolve(FluidKernel &k, bool inverse_op, hipStream_t s) {
     const int ntile = (k.nxh + k.tx_z - 1) / k.tx_z;
     const unsigned grid = grid_for((long long)k.ny * ntile, 1);
     const size_t lds = yz_lds(k.ax[2], 3 * k.tx_z);
-    AWK_LAUNCH(k_zsolve, dim3(grid), dim3(k.bd_z), lds, s, k.ax[2].dev(), sa, k.ny, k.nz,
-                       k.nxh, k.nxh_pad, k.tx_z, k.spec, k.spec_plane());
+    AWK_LAUNCH(k_zsolve, dim3(grid, bt.n), dim3(k.bd_z), lds, s, k.ax[2].dev(), sa, k.ny, k.nz,
+                       k.nxh, k.nxh_pad, k.tx_z, k.spec, k.spec_plane(), bt);
   }
 #undef AWK_ZR
   AWK_LAUNCH_CHECK("k_zsolve");
   return AWK_OK;
 }
 
-awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblocks) {
+awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblocks, const Batch *b) {
+  const Batch bt = b ? *b : Batch();
   const unsigned grid = grid_for((long long)k.ny * k.nz, k.rt);
   const size_t lds = x_lds(k.ax[0], k.rt);
   const Axis ax = k.ax[0].dev();
 #define AWK_XI(NC, RTC, BDC, XU)                                                                 \
-  AWK_LAUNCH((k_xinv<NC, RTC, BDC, XU>), dim3(grid), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \
-                     k.nx, k.ny, k.nz, k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane())
+  AWK_LAUNCH((k_xinv<NC, RTC, BDC, XU>), dim3(grid, bt.n), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \
+                     k.nx, k.ny, k.nz, k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane(), bt)
   const int xu = tuning().xinv_unroll;
   if (k.nx == 256 && k.rt == 8) {
     if (xu == 4) AWK_XI(256, 8, 256, 4);

@@ -370,9 +370,21 @@ awk_status awk_tune_set(const char *name, int value) {
     else t.xinv_unroll = value;
     return AWK_OK;
   }
-  if (n == "splat_tile") {
-    if (value != 0 && value != 1) return fail(AWK_ERR_INVALID, "awk_tune_set: splat_tile must be 0 or 1");
-    t.splat_tile = value;
+  if (n == "splat_tile" || n == "fft_order") {
+    if (value != 0 && value != 1) return fail(AWK_ERR_INVALID, "awk_tune_set: " + n + " must be 0 or 1");
+    (n == "splat_tile" ? t.splat_tile : t.fft_order) = value;
+    return AWK_OK;
+  }
+  if (n == "x_rt") {
+    if (value < 0 || value > 8 || value % 2) return fail(AWK_ERR_INVALID, "awk_tune_set: x_rt must be 0, 2, 4, 6 or 8");
+    t.x_rt = value;
+    return AWK_OK;
+  }
+  if (n == "y_tx" || n == "z_tx") {
+    const int mx = n == "y_tx" ? 16 : 8;
+    if (value < 0 || value > mx || (value & (value - 1)))
+      return fail(AWK_ERR_INVALID, "awk_tune_set: " + n + " must be 0 or a power of two up to " + std::to_string(mx));
+    (n == "y_tx" ? t.y_tx : t.z_tx) = value;
     return AWK_OK;
   }
   return fail(AWK_ERR_INVALID, "awk_tune_set: unknown knob " + n);
@@ -386,6 +398,10 @@ awk_status awk_tune_get(const char *name, int *value) {
   else if (n == "compose_vpt") *value = t.compose_vpt;
   else if (n == "xinv_unroll") *value = t.xinv_unroll;
   else if (n == "splat_tile") *value = t.splat_tile;
+  else if (n == "fft_order") *value = t.fft_order;
+  else if (n == "x_rt") *value = t.x_rt;
+  else if (n == "y_tx") *value = t.y_tx;
+  else if (n == "z_tx") *value = t.z_tx;
   else return fail(AWK_ERR_INVALID, "awk_tune_get: unknown knob " + n);
   return AWK_OK;
 }

@@ -197,14 +197,16 @@ class WarmStartGreedyAtlas:
 
     def __init__(self, net, subjects, template_size, template_origin=(0.0, 0.0, 0.0),
                  template_spacing=(1.0, 1.0, 1.0), max_pert=0.1, alpha=0.01, beta=0.01,
-                 gamma=0.001, use_graph=True, plain_levels=PLAIN_LEVELS, max_refine=50):
+                 gamma=0.001, use_graph=True, plain_levels=PLAIN_LEVELS, max_refine=50,
+                 batch=None):
         self.net = net
         self.subjects = list(subjects)
         self.size = tuple(int(v) for v in template_size)
         if tuple(net.size) != self.size:
             raise ValueError("the network's grid %s is not the template grid %s" % (net.size, self.size))
         self.kw = dict(template_origin=template_origin, template_spacing=template_spacing,
-                       max_pert=max_pert, alpha=alpha, beta=beta, gamma=gamma, use_graph=use_graph)
+                       max_pert=max_pert, alpha=alpha, beta=beta, gamma=gamma, use_graph=use_graph,
+                       batch=batch)
         self.plain_levels = tuple(plain_levels)
         self.max_refine = int(max_refine)
 
@@ -220,6 +222,7 @@ class WarmStartGreedyAtlas:
 
     def _base_map(self, s):
         o, sp = self.kw["template_origin"], self.kw["template_spacing"]
+        # (subjects' grid maps at full resolution, as the builder forms them)
         return hfield.grid_affine(o, sp, s.origin, s.spacing, s.affine)
 
     def predict(self):

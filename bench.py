@@ -165,31 +165,42 @@ def atlas_subjects(dev):
 
 def bench_atlas(dev, subs, reps=10):
     """The 16-subject atlas on a 128^3 template grid: per-iteration time at full
-    resolution (eager and hipGraph replay), then one whole 3-level build (1/4,
+    resolution (batched launches eager and through hipGraph replays, and one launch
+    per subject and pass through graph replays), then one whole 3-level build (1/4,
     1/2, 1 with 100 / 50 / 25 iterations)."""
     import torch
     from atlaswerks_amd import atlas
+    from atlaswerks_amd import autotune
     out = {}
-    for use_graph in (False, True):
-        b = atlas.GreedyAtlasBuilderGPU(subs, (128, 128, 128), max_pert=0.1, use_graph=use_graph)
+    levels = [(4, 100), (2, 50), (1, 25)]
+    # subjects per batched launch of the greedy step, per level (autotuned on a
+    # scratch atlas; 0 = one launch per subject and pass on 3 stream lanes)
+    batch, bt_ms = autotune.tune_atlas_batch(subs, (128, 128, 128), levels, max_pert=0.1)
+    out["batch_tuning"] = {"chosen": {str(f): v for f, v in batch.items()},
+                           "ms": {str(f): {str(c): round(m, 4) for c, m in t.items()}
+                                  for f, t in bt_ms.items()}}
+    for name, use_graph, bsz in (("eager", False, batch[1]), ("graph", True, batch[1]),
+                                 ("graph_per_subject", True, 0)):
+        b = atlas.GreedyAtlasBuilderGPU(subs, (128, 128, 128), max_pert=0.1, use_graph=use_graph,
+                                        batch=bsz)
         b.RunScaleLevel(2)  # step sizes + graph capture
         ms = _events_ms(lambda: b.Iterate(1), reps)
-        out["graph" if use_graph else "eager"] = {"ms_per_atlas_iteration": round(ms, 3)}
+        out[name] = {"ms_per_atlas_iteration": round(ms, 3)}
         del b
     out["subjects"] = len(subs)
     out["template"] = [128, 128, 128]
     out["subject_iter_per_s_graph"] = round(1000.0 * len(subs) / out["graph"]["ms_per_atlas_iteration"], 1)
-    levels = [(4, 100), (2, 50), (1, 25)]
-    b = atlas.GreedyAtlasBuilderGPU(subs, (128, 128, 128), max_pert=0.1, use_graph=True,
-                                    scale_levels=levels)
-    torch.cuda.synchronize()
-    t0 = time.time()
-    b.Run()
-    torch.cuda.synchronize()
-    e = b.energies()
-    out["multiscale_build"] = {"levels": [list(x) for x in levels], "build_s": round(time.time() - t0, 3),
-                               "energy_first_last": [round(float(e[0]), 4), round(float(e[-1]), 4)]}
-    del b
+    for name, bsz in (("multiscale_build", batch), ("multiscale_build_per_subject", 0)):
+        b = atlas.GreedyAtlasBuilderGPU(subs, (128, 128, 128), max_pert=0.1, use_graph=True,
+                                        scale_levels=levels, batch=bsz)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        b.Run()
+        torch.cuda.synchronize()
+        e = b.energies()
+        out[name] = {"levels": [list(x) for x in levels], "build_s": round(time.time() - t0, 4),
+                     "energy_first_last": [round(float(e[0]), 4), round(float(e[-1]), 4)]}
+        del b
     return out
 
 
@@ -236,7 +247,7 @@ def bench_unet(dev, n=128, reps=10, train_reps=5):
     return r
 
 
-def bench_warm_start(dev, subs, max_refine=50):
+def bench_warm_start(dev, subs, max_refine=50, batch=None):
     """Time-to-SSD of the 16-subject atlas, plain vs U-Net warm start.
 
     Plain: the greedy atlas from the identity over scale levels 1/4, 1/2, 1 with
@@ -259,7 +270,8 @@ def bench_warm_start(dev, subs, max_refine=50):
     n = 128
     net = UNetWarmStart((n, n, n), weights=np.load(path), max_batch=16)
     ws = warmstart.WarmStartGreedyAtlas(net, subs, (n, n, n), max_pert=0.1,
-                                        plain_levels=warmstart.PLAIN_LEVELS, max_refine=max_refine)
+                                        plain_levels=warmstart.PLAIN_LEVELS, max_refine=max_refine,
+                                        batch=batch)
 
     def wall(fn):
         torch.cuda.synchronize()
@@ -363,9 +375,12 @@ def main():
     tuning = None
     if not args.no_autotune:
         from atlaswerks_amd import autotune
+        # FFT plan of the grid (radix order, pass tiles), then the gather knobs
+        plan, plan_ms = autotune.tune_fft((nx, ny, nz), reps=5)
         tuning, tune_ms = autotune.tune((nx, ny, nz), reps=5)
-        tuning = {"chosen": tuning, "ms": {k: {str(a): round(b, 4) for a, b in v.items()}
-                                          for k, v in tune_ms.items()}}
+        tuning = {"chosen": dict(tuning, **plan),
+                  "ms": {k: {str(a): round(b, 4) for a, b in v.items()}
+                         for k, v in list(tune_ms.items()) + list(plan_ms.items())}}
     param = GreedyIteratorParam(max_pert=0.1, alpha=0.01, beta=0.01, gamma=0.001)
     it = GreedyIteratorGPU((nx, ny, nz), (1.0, 1.0, 1.0), param)
     data = GreedyDeformationDataGPU(I0, I1, max_iters=args.warmup + args.steps + 1)
@@ -480,6 +495,28 @@ def main():
         vel = torch.from_numpy(synth.sinusoidal_displacement(shape, 8.0, seed=3)).to(dev)
         ms = timeit(lambda: hfield.splat_sorted(outi, I0, vel), reps=5)
         mb["splat_sorted"] = {"ms": round(ms, 4), "gbs_20N": round(20 * N / (ms * 1e-3) / 1e9, 1)}
+        # FFT plan autotuning on a grid of mixed radices (the largest subject grid
+        # of the atlas workload), then back to the default plan
+        if not args.no_autotune:
+            from atlaswerks_amd import autotune
+            saved = {k: autotune.get_knob(k) for k in autotune.PLAN_KNOBS}
+            g = (160, 192, 144)
+            fm = torch.randn((3, g[2], g[1], g[0]), device=dev)
+            for k in autotune.PLAN_KNOBS:
+                autotune.set_knob(k, 0)
+            op = DiffOperGPU(g)
+            d_ms = timeit(lambda: op.ApplyInverseOperator(fm))
+            del op
+            plan, _ = autotune.tune_fft(g, reps=5)
+            op = DiffOperGPU(g)
+            t_ms = timeit(lambda: op.ApplyInverseOperator(fm))
+            del op, fm
+            Ng = g[0] * g[1] * g[2]
+            mb["L_inverse_160x192x144"] = {"default_plan_ms": round(d_ms, 4), "tuned_plan_ms": round(t_ms, 4),
+                                           "tuned_plan": plan,
+                                           "gbs_120N_tuned": round(120 * Ng / (t_ms * 1e-3) / 1e9, 1)}
+            for k, v in saved.items():
+                autotune.set_knob(k, v)
         out["microbench"] = mb
         del f, g1, g2, outi, outv, vel
 
@@ -492,7 +529,8 @@ def main():
         subs = atlas_subjects(dev)
         out["atlas_16_uneven"] = bench_atlas(dev, subs)
         out["unet_128"] = bench_unet(dev)
-        out["warm_start_atlas_16"] = bench_warm_start(dev, subs)
+        tuned = out["atlas_16_uneven"]["batch_tuning"]["chosen"]
+        out["warm_start_atlas_16"] = bench_warm_start(dev, subs, batch={int(f): v for f, v in tuned.items()})
         del subs
 
     if args.copy_ref:

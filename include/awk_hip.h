@@ -373,6 +373,12 @@ awk_status awk_atlas_update_step_next(awk_atlas_t a);
 awk_status awk_atlas_get_flags(awk_atlas_t a, unsigned *flags, int clear, awk_stream_t stream);
 /* Steady-state iterations are captured into hipGraphs and replayed (default on). */
 awk_status awk_atlas_set_graph(awk_atlas_t a, int on);
+/* Subjects per batched launch in steady-state iterations (those that set no step
+ * size and track no inverse): each pass of the greedy step is one launch over up
+ * to n subjects (at most as many as 8 GiB of spectra hold; 0: one launch per
+ * subject and pass, on 3 concurrent stream lanes).  Default: as many as keep the
+ * batch's spectra within 128 MiB.  Results are bit-identical for every n. */
+awk_status awk_atlas_set_batch(awk_atlas_t a, int n);
 /* n_iter atlas iterations: each subject's greedy step against the current
  * weighted mean, then the new mean (AtlasThreadMain's loop).  energy_dev (n_iter
  * device doubles, may be NULL): sum_s w_s E_s per iteration (FP64, subject

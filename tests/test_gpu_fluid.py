@@ -80,3 +80,29 @@ def test_round_trip_identity():
     op.ApplyInverseOperator(d)
     op.ApplyOperator(d)
     assert rel_l2(host(d), f) <= 1e-5
+
+
+@pytest.mark.parametrize("knobs", [{"fft_order": 1}, {"x_rt": 2}, {"x_rt": 4, "y_tx": 2, "z_tx": 1},
+                                   {"fft_order": 1, "x_rt": 6, "y_tx": 4, "z_tx": 2}])
+@pytest.mark.parametrize("shape", [(24, 40, 36), (36, 48, 60), (12, 20, 67), (128, 128, 128)])
+def test_tuned_fft_plans_match_oracle(shape, knobs):
+    """The plans the autotuner chooses between (radix order of the generic stages,
+    x / y / z pass tiles): L^-1 within the same 1e-5 of the FP64 oracle."""
+    from atlaswerks_amd import autotune
+    from atlaswerks_amd.fluid import DiffOperGPU, DiffOperParam
+    from gpu_util import aos, host, rel_l2, soa, to_dev
+    nz, ny, nx = shape
+    print("fluid plan", shape, knobs, flush=True)
+    f = _field(shape, seed=11)
+    sp = (1.0, 1.25, 0.8)
+    try:
+        for k, v in knobs.items():
+            autotune.set_knob(k, v)
+        op = DiffOperGPU((nx, ny, nz), sp, DiffOperParam(alpha=0.01, beta=0.01, gamma=0.001))
+    finally:
+        for k in autotune.PLAN_KNOBS:
+            autotune.set_knob(k, 0)
+    d = to_dev(f)
+    op.ApplyInverseOperator(d)
+    ref = oracle.diffoper(aos(f), sp=sp, alpha=0.01, beta=0.01, gamma=0.001, inverse=True)
+    assert rel_l2(host(d), soa(ref)) <= TOL
