@@ -159,6 +159,9 @@ struct Tuning {
   int compose_vpt = 1;  // voxels per thread, compose family
   int xinv_unroll = 1;  // voxels per epilogue step of the inverse x pass
   int splat_tile = 1;   // fixed-point splat through per-block LDS windows (0: direct atomics)
+  int unet_lds = 1;     // U-Net convolutions on grids >= 32^3 through LDS-staged input bricks
+  int unet_pipe = 2;    // ... K steps' fragment loads software-pipelined: 0 never, 1 always, 2 CC >= 32
+  int unet_cc = 0;      // ... cap on the channels staged per chunk (0: all up to 48, else 32)
   // FFT plan knobs, read when an operator is created; they change results at
   // FP32 rounding level (another butterfly order or kernel), never beyond
   int fft_order = 0;    // radix order of the generic LDS stages: 0 largest first, 1 reversed

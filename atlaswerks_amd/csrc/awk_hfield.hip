@@ -370,9 +370,20 @@ awk_status awk_tune_set(const char *name, int value) {
     else t.xinv_unroll = value;
     return AWK_OK;
   }
-  if (n == "splat_tile" || n == "fft_order") {
+  if (n == "unet_cc") {
+    if (value != 0 && value != 8 && value != 16 && value != 32)
+      return fail(AWK_ERR_INVALID, "awk_tune_set: unet_cc must be 0, 8, 16 or 32");
+    t.unet_cc = value;
+    return AWK_OK;
+  }
+  if (n == "unet_pipe") {
+    if (value < 0 || value > 2) return fail(AWK_ERR_INVALID, "awk_tune_set: unet_pipe must be 0, 1 or 2");
+    t.unet_pipe = value;
+    return AWK_OK;
+  }
+  if (n == "splat_tile" || n == "fft_order" || n == "unet_lds") {
     if (value != 0 && value != 1) return fail(AWK_ERR_INVALID, "awk_tune_set: " + n + " must be 0 or 1");
-    (n == "splat_tile" ? t.splat_tile : t.fft_order) = value;
+    (n == "splat_tile" ? t.splat_tile : n == "fft_order" ? t.fft_order : t.unet_lds) = value;
     return AWK_OK;
   }
   if (n == "x_rt") {
@@ -399,6 +410,9 @@ awk_status awk_tune_get(const char *name, int *value) {
   else if (n == "xinv_unroll") *value = t.xinv_unroll;
   else if (n == "splat_tile") *value = t.splat_tile;
   else if (n == "fft_order") *value = t.fft_order;
+  else if (n == "unet_lds") *value = t.unet_lds;
+  else if (n == "unet_pipe") *value = t.unet_pipe;
+  else if (n == "unet_cc") *value = t.unet_cc;
   else if (n == "x_rt") *value = t.x_rt;
   else if (n == "y_tx") *value = t.y_tx;
   else if (n == "z_tx") *value = t.z_tx;

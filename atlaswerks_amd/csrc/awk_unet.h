@@ -25,9 +25,17 @@ struct ConvArgs {
   int nx, ny, nz;     // one volume's grid
   int batch;          // volumes stacked along z (positions p = ((b*nz + z)*ny + y)*nx + x)
   float slope;        // LeakyReLU slope (forward: applied; gradient mode: derivative)
+  // Concatenated input read in place (LDS form only): channels [0, ca) are the
+  // half-resolution tensor xa at (x/2, y/2, z/2) (nearest x2 upsampling), channels
+  // [ca, C) are x with C - ca channels.  xa = null: x holds all C channels.
+  const bf16 *xa;
+  int ca;
 };
 
 awk_status launch_conv3(const ConvArgs &g, hipStream_t s);
+// whether launch_conv3 takes the LDS-staged form for g (the only form that reads a
+// concatenated input in place)
+bool conv3_lds_ok(const ConvArgs &g);
 
 __device__ __forceinline__ bf16x8 zero8() {
   bf16x8 z;
@@ -73,9 +81,11 @@ struct awk_unet {
 };
 
 namespace awk {
-// nb volumes (nb <= u->max_batch): templ[b], subject[b] -> disp[3b..3b+2]
+// nb volumes (nb <= u->max_batch): templ[b], subject[b] -> disp[3b..3b+2].
+// keep_concat: write the decoder's concatenated inputs (c2, c1, c0), which the
+// weight gradients read; else the convolutions that can read them in place do.
 awk_status unet_forward(awk_unet *u, int nb, const float *const *templ, const float *const *subject,
-                        float *const *disp, hipStream_t s);
+                        float *const *disp, hipStream_t s, bool keep_concat = false);
 awk_status unet_pool(const bf16 *in, bf16 *out, int nx, int ny, int nz, int C, hipStream_t s);
 awk_status unet_upcat(bf16 *out, const bf16 *a, int ca, const bf16 *b, int cb, int nx, int ny,
                       int nz, hipStream_t s);

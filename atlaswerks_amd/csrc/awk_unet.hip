@@ -160,6 +160,162 @@ k_conv3(ConvArgs g) {
   }
 }
 
+// LDS-staged form for grids of at least 32^3 per volume (nx % 16 == 0, ny and
+// nz % 4 == 0).  A block owns a 16 x 4 x 4 brick of output positions (wave w: the
+// brick's plane z0 + w, its 4 rows as the 4 position tiles) and MT 16-channel
+// output tiles.  The brick's 18 x 6 x 6 input halo is staged in LDS one chunk of
+// CC channels at a time (positions outside the volume as zeros: the convolution's
+// zero padding), so every tap's fragment is an LDS read and each input value is
+// fetched from memory once per brick instead of once per tap.  K runs chunk by
+// chunk, and within a chunk over (tap, 8-channel group) in the weights' tap-major
+// order.  Positions p = ((b*nz + z)*ny + y)*nx + x of the stacked volumes; a brick
+// never straddles two volumes.
+constexpr int kLdsBX = 16, kLdsBY = 4, kLdsBZ = 4;
+constexpr int kLdsHX = kLdsBX + 2, kLdsHY = kLdsBY + 2, kLdsHZ = kLdsBZ + 2;
+constexpr int kLdsHalo = kLdsHX * kLdsHY * kLdsHZ;  // 648 positions
+
+template <int MT, bool PIPE>
+__global__ void __launch_bounds__(kConvBlock)
+k_conv3_lds(ConvArgs g, int CC) {
+  extern __shared__ bf16 tile[];  // kLdsHalo x PS, PS = CC + 8, then the K-group table
+  const int PS = CC + 8;          // the 8 padding channels are zeros (dead K groups read them)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, grp = lane >> 4;
+  const int mbase = (int)blockIdx.y * MT * 16;
+  const int nbx = g.nx / kLdsBX, nby = g.ny / kLdsBY;
+  const int b = (int)blockIdx.x;
+  const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+  const int x0 = bx * kLdsBX, y0 = by * kLdsBY, zs = bz * kLdsBZ;  // zs: stacked z
+  const int vol = zs / g.nz, z0 = zs - vol * g.nz;
+  const size_t vbase = (size_t)vol * g.nz * g.ny * g.nx;
+  f32x4 acc[MT][kLdsBY];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int t = 0; t < kLdsBY; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int groups = CC >> 3;         // 8-channel groups per position in a chunk
+  const int cgroups = 27 * groups;    // (tap, group) pairs per chunk
+  const int ksteps = (cgroups + 3) >> 2;
+  const int nchunk = g.C / CC;
+  // c / groups as a multiply-high (groups > 1): exact for c < 2^32 / groups^2
+  const unsigned gmagic = groups > 1 ? (unsigned)(4294967296ull / (unsigned)groups) + 1u : 0u;
+  auto divg = [&](int c) { return groups == 1 ? c : (int)__umulhi((unsigned)c, gmagic); };
+  // K-group table: for c = 4 ks + grp, the LDS offset of its fragment relative to
+  // the lane's own position and the weight offset within a chunk; past the last
+  // group, the lane's own zero padding (and weight offset 0: finite weights times
+  // zeros add nothing)
+  int2 *koff = reinterpret_cast<int2 *>(tile + kLdsHalo * PS);
+  for (int c = threadIdx.x; c < ksteps * 4; c += kConvBlock) {
+    int2 o = make_int2(CC, 0);
+    if (c < cgroups) {
+      const int tap = divg(c), gi = c - tap * groups;
+      const int tz = tap / 9, tr = tap - tz * 9, ty = tr / 3, tx = tr - ty * 3;
+      o = make_int2(((tz * kLdsHY + ty) * kLdsHX + tx) * PS + gi * 8, tap * g.C + gi * 8);
+    }
+    koff[c] = o;
+  }
+  for (int hp = threadIdx.x; hp < kLdsHalo; hp += kConvBlock)
+    *reinterpret_cast<bf16x8 *>(tile + hp * PS + CC) = zero8();
+  // the lane's positions (4 rows of its plane) and weight rows
+  int bpos[kLdsBY];
+#pragma unroll
+  for (int t = 0; t < kLdsBY; ++t) bpos[t] = ((wave * kLdsHY + t) * kLdsHX + col) * PS;
+  const bf16 *wrow[MT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m) wrow[m] = g.w + (size_t)(mbase + m * 16 + col) * g.kpad;
+  for (int ch = 0; ch < nchunk; ++ch) {
+    if (ch) __syncthreads();  // the previous chunk's reads are done
+    for (int w = threadIdx.x; w < kLdsHalo * groups; w += kConvBlock) {
+      const int hp = divg(w), gi = w - hp * groups;
+      const int hx = hp % kLdsHX, hr = hp / kLdsHX, hy = hr % kLdsHY, hz = hr / kLdsHY;
+      const int x = x0 - 1 + hx, y = y0 - 1 + hy, z = z0 - 1 + hz;
+      bf16x8 v = zero8();
+      if (x >= 0 && x < g.nx && y >= 0 && y < g.ny && z >= 0 && z < g.nz) {
+        const int cidx = ch * CC + gi * 8;
+        if (!g.xa) {
+          v = *reinterpret_cast<const bf16x8 *>(g.x + (vbase + ((size_t)z * g.ny + y) * g.nx + x) * g.C + cidx);
+        } else if (cidx < g.ca) {  // the upsampled coarse tensor
+          const int hx2 = g.nx >> 1, hy2 = g.ny >> 1, hz2 = g.nz >> 1;
+          const size_t q = (size_t)vol * hz2 * hy2 * hx2 + ((size_t)(z >> 1) * hy2 + (y >> 1)) * hx2 + (x >> 1);
+          v = *reinterpret_cast<const bf16x8 *>(g.xa + q * g.ca + cidx);
+        } else {                   // the skip tensor
+          const int cb = g.C - g.ca;
+          v = *reinterpret_cast<const bf16x8 *>(g.x + (vbase + ((size_t)z * g.ny + y) * g.nx + x) * cb +
+                                                (cidx - g.ca));
+        }
+      }
+      *reinterpret_cast<bf16x8 *>(tile + hp * PS + gi * 8) = v;
+    }
+    __syncthreads();
+    const int cw = ch * CC;
+    auto load = [&](int ks, bf16x8 *av, bf16x8 *bv) {
+      const int2 o = koff[ks * 4 + grp];
+#pragma unroll
+      for (int t = 0; t < kLdsBY; ++t) bv[t] = *reinterpret_cast<const bf16x8 *>(tile + bpos[t] + o.x);
+#pragma unroll
+      for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const bf16x8 *>(wrow[m] + cw + o.y);
+    };
+    if constexpr (!PIPE) {
+      for (int ks = 0; ks < ksteps; ++ks) {
+        bf16x8 av[MT], bv[kLdsBY];
+        load(ks, av, bv);
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+          for (int t = 0; t < kLdsBY; ++t)
+            acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[t], acc[m][t], 0, 0, 0);
+      }
+    } else {
+      // the next step's fragments are loaded before this step's MFMAs, into the
+      // other of two register sets (two steps per trip: no copies)
+      bf16x8 a0[MT], b0[kLdsBY], a1[MT], b1[kLdsBY];
+      auto mma = [&](const bf16x8 *av, const bf16x8 *bv) {
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+          for (int t = 0; t < kLdsBY; ++t)
+            acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[t], acc[m][t], 0, 0, 0);
+      };
+      load(0, a0, b0);
+      for (int ks = 0; ks < ksteps; ks += 2) {  // ks + 1, ks + 2 < ksteps: uniform
+        if (ks + 1 < ksteps) load(ks + 1, a1, b1);
+        mma(a0, b0);
+        if (ks + 1 < ksteps) {
+          if (ks + 2 < ksteps) load(ks + 2, a0, b0);
+          mma(a1, b1);
+        }
+      }
+    }
+  }
+  // epilogue: lane holds channels m*16 + 4*grp + 0..3 of position (x0 + col, y0 + t, zs + wave)
+#pragma unroll
+  for (int t = 0; t < kLdsBY; ++t) {
+    const size_t p = ((size_t)(zs + wave) * g.ny + (y0 + t)) * g.nx + x0 + col;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const int c0 = mbase + m * 16 + grp * 4;
+      const size_t o0 = p * g.cout + c0;
+      bf16x4 o;
+      if (g.dact) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4 *>(g.dact + o0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float v = acc[m][t][i];
+          o[i] = (bf16)((float)a[i] > 0.f ? v : v * g.slope);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          float v = acc[m][t][i] + (g.bias ? g.bias[c0 + i] : 0.f);
+          v = v >= 0.f ? v : v * g.slope;
+          o[i] = (bf16)v;
+        }
+      }
+      *reinterpret_cast<bf16x4 *>(g.out + o0) = o;
+    }
+  }
+}
+
 // two fp32 images -> bf16 NDHWC with 8 channels (template, subject, 6 zeros)
 __global__ void __launch_bounds__(256)
 k_pack_input(bf16 *__restrict__ out, const float *__restrict__ t, const float *__restrict__ s, int N) {
@@ -257,12 +413,77 @@ __global__ void k_mfma_probe(float *__restrict__ out) {
 // 16) that still gives at least 8192 waves.  Grids too small for that keep all
 // output channels per wave (MT up to 4) and split K over the block's 4 waves,
 // splitting the channels over blockIdx.y only if that is still too few.
+// Channels staged per chunk.  Chunks of 16 keep 5 blocks per CU resident where the
+// output tile is narrow; 32-channel chunks (software-pipelined) pay off from 64
+// outputs or more than 48 inputs on (measured per layer at 128^3: 48 -> 16
+// 302 -> 225 us, 32 -> 32 37 -> 32 us with 16-channel chunks; 192 -> 64 70 -> 85 us,
+// 64 -> 64 27 -> 32 us the other way).  unet_cc, when set, caps the chunk.
+static int lds_chunk(const ConvArgs &g) {
+  int CC = g.C <= 16 ? g.C : (g.C > 48 || (g.cout >= 64 && g.C % 32 == 0)) ? 32 : 16;
+  const int ccap = tuning().unet_cc;
+  if (ccap && CC > ccap && g.C % ccap == 0) CC = ccap;
+  return CC;
+}
+
+bool conv3_lds_ok(const ConvArgs &g) {
+  const long long N1 = (long long)g.nx * g.ny * g.nz;
+  const int CC = lds_chunk(g);
+  const int mt_all = g.cout / 16;
+  const int mt = mt_all % 4 == 0 ? 4 : mt_all % 2 == 0 ? 2 : 1;
+  return tuning().unet_lds && N1 >= 32768 && g.nx % kLdsBX == 0 && g.ny % kLdsBY == 0 &&
+         g.nz % kLdsBZ == 0 && g.C % CC == 0 && g.cout % 16 == 0 && (mt_all / mt) * mt * 16 == g.cout &&
+         (!g.xa || (g.ca % 8 == 0 && g.ca < g.C && g.nx % 2 == 0 && g.ny % 2 == 0 && g.nz % 2 == 0));
+}
+
 awk_status launch_conv3(const ConvArgs &g, hipStream_t s) {
   const long long N1 = (long long)g.nx * g.ny * g.nz, N = N1 * (g.batch < 1 ? 1 : g.batch);
   const int mt_all = g.cout / 16;
   if (g.cout % 16 || mt_all < 1 || mt_all > 16 || g.C % 8 || g.C < 8)
     return fail(AWK_ERR_INVALID, "unet: channels must be multiples of 16 (outputs, at most 256) and 8 (inputs)");
   if (g.batch < 1 || N > (1LL << 30)) return fail(AWK_ERR_INVALID, "unet: bad batch");
+  // the LDS-staged form where one volume is at least 32^3 (chosen per volume, so a
+  // batch rounds as its single volumes do); CC channels staged at a time
+  if (g.xa && !conv3_lds_ok(g)) return fail(AWK_ERR_INVALID, "unet: in-place concatenation needs the LDS form");
+  const int CC = lds_chunk(g);
+  if (conv3_lds_ok(g)) {
+    // output-channel tiles per wave: the largest of 4, 2, 1 dividing the tile
+    // count (so that blockIdx.y * MT * 16 covers exactly cout) that still gives
+    // 1024 blocks, 4 per CU, where the grid allows (MT changes no result)
+    const long long bricks = N / (kLdsBX * kLdsBY * kLdsBZ);
+    int mt = mt_all % 4 == 0 ? 4 : mt_all % 2 == 0 ? 2 : 1;
+    while (mt > 1 && bricks * (mt_all / mt) < 1024) mt >>= 1;
+    if ((mt_all / mt) * mt * 16 == g.cout) {
+      const int ksteps = (27 * (CC / 8) + 3) / 4;
+      const size_t lds = (size_t)kLdsHalo * (CC + 8) * sizeof(bf16) + (size_t)ksteps * 4 * sizeof(int2);
+      static bool attr_set = false;  // dynamic LDS above 64 KiB (CC = 48: 71 KiB)
+      if (!attr_set) {
+        const int mx = kLdsHalo * (48 + 8) * (int)sizeof(bf16) + 164 * (int)sizeof(int2);
+        const void *fns[] = {reinterpret_cast<const void *>(&k_conv3_lds<1, false>),
+                             reinterpret_cast<const void *>(&k_conv3_lds<2, false>),
+                             reinterpret_cast<const void *>(&k_conv3_lds<4, false>),
+                             reinterpret_cast<const void *>(&k_conv3_lds<1, true>),
+                             reinterpret_cast<const void *>(&k_conv3_lds<2, true>),
+                             reinterpret_cast<const void *>(&k_conv3_lds<4, true>)};
+        for (const void *f : fns)
+          AWK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+        attr_set = true;
+      }
+      const dim3 grid((unsigned)(N / (kLdsBX * kLdsBY * kLdsBZ)), (unsigned)(mt_all / mt));
+#define AWK_CL(M, P) AWK_LAUNCH((k_conv3_lds<M, P>), grid, dim3(kConvBlock), lds, s, g, CC)
+      // software pipelining pays where a K step is long enough to hide the
+      // prefetch registers' occupancy cost: chunks of 32 or more channels
+      // (measured per layer at 128^3: 96 -> 32 150 -> 118 us, 16 -> 16 109 -> 123 us)
+      const bool pipe = tuning().unet_pipe == 2 ? CC >= 32 : tuning().unet_pipe == 1;
+      if (pipe) {
+        if (mt == 1) AWK_CL(1, true); else if (mt == 2) AWK_CL(2, true); else AWK_CL(4, true);
+      } else {
+        if (mt == 1) AWK_CL(1, false); else if (mt == 2) AWK_CL(2, false); else AWK_CL(4, false);
+      }
+#undef AWK_CL
+      AWK_LAUNCH_CHECK("k_conv3_lds");
+      return AWK_OK;
+    }
+  }
   // channel tiles per wave: the largest power of two (<= 8) dividing the tile count
   int mt_max = 1;
   while (mt_max < 8 && mt_all % (mt_max * 2) == 0) mt_max *= 2;
@@ -326,8 +547,8 @@ static std::vector<bf16> pack_weights(const float *w, int cout, int cin, int blo
   return wt;
 }
 
-static awk_status conv(const awk_unet *u, int l, const bf16 *x, bf16 *out, int nx, int ny, int nz,
-                       int nb, hipStream_t s) {
+static ConvArgs conv_args(const awk_unet *u, int l, const bf16 *x, bf16 *out, int nx, int ny,
+                          int nz, int nb) {
   ConvArgs g;
   g.x = x;
   g.w = u->w[l];
@@ -344,7 +565,29 @@ static awk_status conv(const awk_unet *u, int l, const bf16 *x, bf16 *out, int n
   g.nz = nz;
   g.batch = nb;
   g.slope = kUnetSlope;
-  return launch_conv3(g, s);
+  g.xa = nullptr;
+  g.ca = 0;
+  return g;
+}
+
+static awk_status conv(const awk_unet *u, int l, const bf16 *x, bf16 *out, int nx, int ny, int nz,
+                       int nb, hipStream_t s) {
+  return launch_conv3(conv_args(u, l, x, out, nx, ny, nz, nb), s);
+}
+
+// Decoder layer l on [nearest x2 upsampling of a (ca channels), b (cb channels)]:
+// read in place by the LDS form unless the concatenation must be kept (training)
+// or the grid takes the direct form; else written to cat first.
+static awk_status conv_cat(const awk_unet *u, int l, const bf16 *a, int ca, const bf16 *b, int cb,
+                           bf16 *cat, bf16 *out, int nx, int ny, int nz, int nb, bool keep,
+                           hipStream_t s) {
+  ConvArgs g = conv_args(u, l, b, out, nx, ny, nz, nb);
+  g.xa = a;
+  g.ca = ca;
+  if (!keep && conv3_lds_ok(g)) return launch_conv3(g, s);
+  awk_status st = unet_upcat(cat, a, ca, b, cb, nx, ny, nz * nb, s);
+  if (st) return st;
+  return conv(u, l, cat, out, nx, ny, nz, nb, s);
 }
 
 awk_status awk::unet_pool(const bf16 *in, bf16 *out, int nx, int ny, int nz, int C, hipStream_t s) {
@@ -364,7 +607,8 @@ awk_status awk::unet_upcat(bf16 *out, const bf16 *a, int ca, const bf16 *b, int 
 }
 
 awk_status awk::unet_forward(awk_unet *u, int nb, const float *const *templ,
-                             const float *const *subject, float *const *disp, hipStream_t s) {
+                             const float *const *subject, float *const *disp, hipStream_t s,
+                             bool keep_concat) {
   const int nx = u->nx, ny = u->ny, nz = u->nz;
   const int N0 = nx * ny * nz;
   if (nb < 1 || nb > u->max_batch) return fail(AWK_ERR_INVALID, "unet: batch exceeds the network's");
@@ -389,14 +633,12 @@ awk_status awk::unet_forward(awk_unet *u, int nb, const float *const *templ,
   if ((st = unet_pool(u->s2, u->p2, a2, b2, c2 * nb, 64, s))) return st;
   if ((st = conv(u, 6, u->p2, u->t3, a3, b3, c3, nb, s))) return st;
   if ((st = conv(u, 7, u->t3, u->s3, a3, b3, c3, nb, s))) return st;
-  if ((st = unet_upcat(u->c2, u->s3, 128, u->s2, 64, a2, b2, c2 * nb, s))) return st;
-  if ((st = conv(u, 8, u->c2, u->d2a, a2, b2, c2, nb, s))) return st;
+  const bool kc = keep_concat;
+  if ((st = conv_cat(u, 8, u->s3, 128, u->s2, 64, u->c2, u->d2a, a2, b2, c2, nb, kc, s))) return st;
   if ((st = conv(u, 9, u->d2a, u->d2, a2, b2, c2, nb, s))) return st;
-  if ((st = unet_upcat(u->c1, u->d2, 64, u->s1, 32, a1, b1, c1 * nb, s))) return st;
-  if ((st = conv(u, 10, u->c1, u->d1a, a1, b1, c1, nb, s))) return st;
+  if ((st = conv_cat(u, 10, u->d2, 64, u->s1, 32, u->c1, u->d1a, a1, b1, c1, nb, kc, s))) return st;
   if ((st = conv(u, 11, u->d1a, u->d1, a1, b1, c1, nb, s))) return st;
-  if ((st = unet_upcat(u->c0, u->d1, 32, u->s0, 16, nx, ny, nz * nb, s))) return st;
-  if ((st = conv(u, 12, u->c0, u->d0a, nx, ny, nz, nb, s))) return st;
+  if ((st = conv_cat(u, 12, u->d1, 32, u->s0, 16, u->c0, u->d0a, nx, ny, nz, nb, kc, s))) return st;
   if ((st = conv(u, 13, u->d0a, u->d0, nx, ny, nz, nb, s))) return st;
   for (int b = 0; b < nb; ++b) {
     AWK_LAUNCH(k_head, dim3((N0 + 255) / 256), dim3(256), 0, s, disp[3 * b], disp[3 * b + 1],

@@ -387,6 +387,8 @@ static awk_status dgrad(awk_unet_trainer *t, int l, const bf16 *dz, const bf16 *
   g.nz = nz;
   g.batch = 1;
   g.slope = act ? kUnetSlope : 1.f;
+  g.xa = nullptr;
+  g.ca = 0;
   return launch_conv3(g, s);
 }
 
@@ -434,7 +436,7 @@ static awk_status train_backward(awk_unet_trainer *t, const float *templ, const 
   const size_t n0 = (size_t)N0;
   float *up[3] = {t->u, t->u + n0, t->u + 2 * n0};
   awk_status st;
-  if ((st = unet_forward(u, 1, &templ, &subject, up, s))) return st;
+  if ((st = unet_forward(u, 1, &templ, &subject, up, s, true))) return st;
   const int hb = (N0 + 255) / 256;
   AWK_LAUNCH(k_head_bwd, dim3(hb), dim3(256), 0, s, t->dz13, t->hpart, up[0], up[1], up[2],
              target[0], target[1], target[2], u->d0, u->head_w, N0, kUnetSlope);

@@ -327,7 +327,14 @@ def main():
                     help="run one secondary workload alone (for profiling) and print its JSON")
     ap.add_argument("--copy-ref", action="store_true",
                     help="also run a plain 1 GiB device copy (counter scaling reference)")
+    ap.add_argument("--tune", action="append", default=[], metavar="KNOB=VALUE",
+                    help="set a library knob (awk_tune_set) before running")
     args = ap.parse_args()
+    if args.tune:
+        from atlaswerks_amd import autotune
+        for kv in args.tune:
+            k, v = kv.split("=")
+            autotune.set_knob(k, int(v))
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU only (--gpus %d requested); across GPUs it "
               "would shard an atlas by subject, one process per GPU, with one all-reduce of the "
