@@ -17,6 +17,7 @@ KNOBS = {
     "apply_vpt": (1, 2, 4),
     "compose_vpt": (1, 2, 4),
     "xinv_unroll": (1, 2, 4),
+    "xinv_pairs": (0, 1),
 }
 
 
@@ -120,6 +121,7 @@ def tune(size=(256, 256, 256), reps=10, device="cuda", verbose=False):
     it.UpdateStepSizeNextIteration()
     it.Iterate(data)
     sweep("xinv_unroll", lambda: it.Iterate(data))
+    sweep("xinv_pairs", lambda: it.Iterate(data))
     del it, data
     if verbose:
         print("autotune", chosen, timings)

@@ -158,6 +158,7 @@ struct Tuning {
   int apply_vpt = 1;    // voxels per thread, apply
   int compose_vpt = 1;  // voxels per thread, compose family
   int xinv_unroll = 1;  // voxels per epilogue step of the inverse x pass
+  int xinv_pairs = 0;   // inverse x pass: x-neighbour corners as 8-byte pairs inside the volume
   int splat_tile = 1;   // fixed-point splat through per-block LDS windows (0: direct atomics)
   int unet_lds = 1;     // U-Net convolutions on grids >= 32^3 through LDS-staged input bricks
   int unet_pipe = 2;    // ... K steps' fragment loads software-pipelined: 0 never, 1 always, 2 CC >= 32

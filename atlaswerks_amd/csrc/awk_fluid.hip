@@ -345,7 +345,7 @@ k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
 }
 
 // X inverse: c2r of RT rows (3 components) with an epilogue.
-template <int NC, int RTC, int BDC, int XU>
+template <int NC, int RTC, int BDC, int XU, bool PAIRS = false>
 __global__ void __launch_bounds__(BDC ? BDC : 1024)
 k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, int RT_rt,
        const float2 *__restrict__ spec, size_t plane, Batch bt) {
@@ -431,7 +431,7 @@ k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
       float3 hn[XU];
 #pragma unroll
       for (int u = 0; u < XU; ++u)
-        if (row_[u] < NR) hn[u] = trilerp3g<AWK_BG_PARTIAL_ID, decltype(hx), false>(hx, hy, hz, d, px[u], py[u], pz[u]);
+        if (row_[u] < NR) hn[u] = trilerp3g<AWK_BG_PARTIAL_ID, decltype(hx), PAIRS>(hx, hy, hz, d, px[u], py[u], pz[u]);
 #pragma unroll
       for (int u = 0; u < XU; ++u)
         if (row_[u] < NR) {
@@ -837,12 +837,19 @@ static size_t yz_lds(const AxisPlan &a, int lines) {
   return ((size_t)a.n + (size_t)a.m) * lines * sizeof(float2);
 }
 
+// Dynamic LDS a pass block may take: gfx950's 160 KiB per workgroup, less 1 KiB
+// for the kernels' static reduction arrays (the generic pass kernels are set up
+// for it once; above 64 KiB a block has the CU's LDS to itself)
+constexpr size_t kPassLdsMax = 160 * 1024 - 1024;
+
 static int pick_rt(const AxisPlan &a) {
   // the tuned RT if it fits, else the largest even RT <= 8 whose LDS fits 64 KiB
   const int want = tuning().x_rt;
   if (want > 0 && x_lds(a, want) <= 65536) return want;
-  for (int rt = 8; rt >= 2; rt -= 2)
+  for (int rt = 8; rt >= 2; rt -= 2)  // within 64 KiB where possible (several blocks per CU)
     if (x_lds(a, rt) <= 65536) return rt;
+  for (int rt = 8; rt >= 2; rt -= 2)
+    if (x_lds(a, rt) <= kPassLdsMax) return rt;
   return 0;
 }
 
@@ -850,6 +857,8 @@ static int pick_tx(const AxisPlan &a, int lines_per_col, int max_tx, int want) {
   if (want > 0 && want <= max_tx && yz_lds(a, lines_per_col * want) <= 65536) return want;
   for (int tx = max_tx; tx >= 1; tx /= 2)
     if (yz_lds(a, lines_per_col * tx) <= 65536) return tx;
+  for (int tx = max_tx; tx >= 1; tx /= 2)
+    if (yz_lds(a, lines_per_col * tx) <= kPassLdsMax) return tx;
   return 0;
 }
 
@@ -899,6 +908,19 @@ awk_status fk_init(FluidKernel &k, int nx, int ny, int nz, const float sp[3]) {
   k.bd_z = pick_bd(k.ax[2].stage_len(), 3 * k.tx_z);
   if (k.bd_x > 1024 || k.bd_y > 1024 || k.bd_z > 1024)
     return fail(AWK_ERR_UNSUPPORTED, "fluid kernel: block too large");
+  {
+    // the generic pass kernels may take more than the default 64 KiB of dynamic LDS
+    static bool lds_attr = false;
+    if (!lds_attr) {
+      const void *fns[] = {reinterpret_cast<const void *>(&k_xfwd<0, 0, 0>),
+                           reinterpret_cast<const void *>(&k_xinv<0, 0, 0, 1>),
+                           reinterpret_cast<const void *>(&k_ypass),
+                           reinterpret_cast<const void *>(&k_zsolve)};
+      for (const void *f : fns)
+        AWK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPassLdsMax));
+      lds_attr = true;
+    }
+  }
   // twiddles and LUT (double -> float); the Bluestein tables follow the three
   // axes' twiddles in the same allocation
   std::vector<float2> tw((size_t)nx + ny + nz);
@@ -1076,18 +1098,30 @@ awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblock
   const unsigned grid = grid_for((long long)k.ny * k.nz, k.rt);
   const size_t lds = x_lds(k.ax[0], k.rt);
   const Axis ax = k.ax[0].dev();
+#define AWK_XI2(NC, RTC, BDC, XU, PR)                                                            \
+  AWK_LAUNCH((k_xinv<NC, RTC, BDC, XU, PR>), dim3(grid, bt.n), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \
+                     k.nx, k.ny, k.nz, k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane(), bt)
 #define AWK_XI(NC, RTC, BDC, XU)                                                                 \
   AWK_LAUNCH((k_xinv<NC, RTC, BDC, XU>), dim3(grid, bt.n), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \
                      k.nx, k.ny, k.nz, k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane(), bt)
   const int xu = tuning().xinv_unroll;
+  // xinv_pairs: waves whose points all lie inside the volume gather the
+  // x-neighbour corners as 8-byte pairs (12 loads per point instead of 24; the
+  // same values, so the same result)
+  const bool pr = a.mode == XSTORE_COMPOSE && tuning().xinv_pairs;
   if (k.nx == 256 && k.rt == 8) {
-    if (xu == 4) AWK_XI(256, 8, 256, 4);
+    if (pr) {
+      if (xu == 4) AWK_XI2(256, 8, 256, 4, true);
+      else if (xu == 2) AWK_XI2(256, 8, 256, 2, true);
+      else AWK_XI2(256, 8, 256, 1, true);
+    } else if (xu == 4) AWK_XI(256, 8, 256, 4);
     else if (xu == 2) AWK_XI(256, 8, 256, 2);
     else AWK_XI(256, 8, 256, 1);
   } else if (k.nx == 128 && k.rt == 8) AWK_XI(128, 8, 256, 2);
   else if (k.nx == 64 && k.rt == 8) AWK_XI(64, 8, 256, 2);
   else AWK_XI(0, 0, 0, 1);
 #undef AWK_XI
+#undef AWK_XI2
   AWK_LAUNCH_CHECK("k_xinv");
   if (nblocks) *nblocks = (int)grid;
   return AWK_OK;

@@ -381,9 +381,10 @@ awk_status awk_tune_set(const char *name, int value) {
     t.unet_pipe = value;
     return AWK_OK;
   }
-  if (n == "splat_tile" || n == "fft_order" || n == "unet_lds") {
+  if (n == "splat_tile" || n == "fft_order" || n == "unet_lds" || n == "xinv_pairs") {
     if (value != 0 && value != 1) return fail(AWK_ERR_INVALID, "awk_tune_set: " + n + " must be 0 or 1");
-    (n == "splat_tile" ? t.splat_tile : n == "fft_order" ? t.fft_order : t.unet_lds) = value;
+    (n == "splat_tile" ? t.splat_tile : n == "fft_order" ? t.fft_order : n == "unet_lds" ? t.unet_lds
+                                                                                          : t.xinv_pairs) = value;
     return AWK_OK;
   }
   if (n == "x_rt") {
@@ -411,6 +412,7 @@ awk_status awk_tune_get(const char *name, int *value) {
   else if (n == "splat_tile") *value = t.splat_tile;
   else if (n == "fft_order") *value = t.fft_order;
   else if (n == "unet_lds") *value = t.unet_lds;
+  else if (n == "xinv_pairs") *value = t.xinv_pairs;
   else if (n == "unet_pipe") *value = t.unet_pipe;
   else if (n == "unet_cc") *value = t.unet_cc;
   else if (n == "x_rt") *value = t.x_rt;

@@ -19,13 +19,16 @@ SIZES = [(8, 8, 8), (6, 10, 12), (24, 40, 36), (32, 31, 36), (64, 64, 64), (17, 
 # axis; 217 = 7 * 31 and 61 take direct prime stages
 PRIME_SIZES = [(12, 20, 67), (16, 127, 10), (181, 12, 14), (61, 67, 71), (36, 217, 181),
                (1, 1, 181)]
+# axes whose pass blocks stage more than 64 KiB of LDS (up to 159 KiB): 2500 along x,
+# Bluestein primes 1021 along x and 3001 along y, 1499 along z
+LONG_SIZES = [(3, 4, 2500), (4, 5, 1021), (2, 3001, 3), (1499, 3, 2)]
 
 
 def _field(shape, seed):
     return synth.smooth_noise_field(shape, sigma=1.5, seed=seed)
 
 
-@pytest.mark.parametrize("shape", SIZES + PRIME_SIZES)
+@pytest.mark.parametrize("shape", SIZES + PRIME_SIZES + LONG_SIZES)
 @pytest.mark.parametrize("inverse", [True, False])
 def test_operator_matches_oracle(shape, inverse):
     import torch

@@ -101,3 +101,33 @@ def test_fluidwarp_fixture(golden_dir):
     print("fixture sq-diff", d_h, d_hi, d_img)
     # reference tolerance is 10 (FluidWarpRoutineTests.cxx:28); hold the GPU to 1e-3
     assert d_h < 1e-3 and d_hi < 1e-3 and d_img < 1e-3
+
+
+def test_inverse_x_pass_launch_variants_bit_identical():
+    """The inverse x pass's launch variants (1, 2 or 4 voxels per epilogue step;
+    gathers one corner at a time or in x-neighbour pairs) give the same hfields
+    and energies bit for bit over 6 iterations (256-long rows, the specialised
+    pass)."""
+    import torch
+    from atlaswerks_amd import autotune
+    from atlaswerks_amd.greedy import GreedyIteratorParam, GreedyWarp
+    from gpu_util import to_dev
+    t, target = _config0((8, 16, 256), seed=3)
+    print("xinv variants", flush=True)
+    saved = {k: autotune.get_knob(k) for k in ("xinv_unroll", "xinv_pairs")}
+    runs = []
+    try:
+        for xu in (1, 2, 4):
+            for pr in (0, 1):
+                autotune.set_knob("xinv_unroll", xu)
+                autotune.set_knob("xinv_pairs", pr)
+                w = GreedyWarp(to_dev(t), to_dev(target), 6, param=GreedyIteratorParam(max_pert=0.1))
+                d = w.RunWarp()
+                torch.cuda.synchronize()
+                runs.append((d.Def1To0.cpu().numpy(), d.energies()))
+    finally:
+        for k, v in saved.items():
+            autotune.set_knob(k, v)
+    for h, e in runs[1:]:
+        np.testing.assert_array_equal(h, runs[0][0])
+        np.testing.assert_array_equal(e, runs[0][1])
