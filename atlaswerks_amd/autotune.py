@@ -17,8 +17,11 @@ KNOBS = {
     "apply_vpt": (1, 2, 4),
     "compose_vpt": (1, 2, 4),
     "xinv_unroll": (1, 2, 4),
-    "xinv_pairs": (0, 1),
 }
+# Knobs left out of the sweep: "xinv_pairs" (the inverse x pass's corner pairs) ties
+# on any proxy field here but measures 8 % slower in that pass on the 256^3
+# registration (0.210-0.214 against 0.195-0.199 ms at every unroll), where more
+# waves reach past the volume's edge; it stays settable through set_knob.
 
 
 def set_knob(name, value):
@@ -109,19 +112,27 @@ def tune(size=(256, 256, 256), reps=10, device="cuda", verbose=False):
             set_knob(name, v)
             ms = _time(fn, reps)
             timings[name][v] = ms
-            if best is None or ms < best[1]:
+            if best is None or ms < best[1] * 0.99:  # a tie keeps the earlier (simpler) value
                 best = (v, ms)
         set_knob(name, best[0])
         chosen[name] = best[0]
 
     sweep("apply_vpt", lambda: hfield.apply(out, img, h))
     sweep("compose_vpt", lambda: hfield.compose(outv, h, h2, hfield.BACKGROUND_STRATEGY_PARTIAL_ID))
+    # the greedy iteration on a registration's state: smooth images, an hfield
+    # displaced by up to n/32 voxels (8 at 256^3) as a registration's is, so that
+    # as many gathers reach past the volume's edge as in the workloads (random
+    # images from the identity keep every point inside and favour other variants)
+    amp = max(2.0, max(size) / 32.0)
+    sm = _smooth_field(shape, 1.0, 3, device)
+    I0 = sm[0].contiguous()
+    I1 = torch.roll(I0, 2, dims=2).contiguous()
+    h0 = ident + _smooth_field(shape, amp, 4, device)
     it = GreedyIteratorGPU(size)
-    data = GreedyDeformationDataGPU(img, torch.roll(img, 2, dims=2), max_iters=4 * reps + 8)
+    data = GreedyDeformationDataGPU(I0, I1, max_iters=4 * reps + 8, h0=h0)
     it.UpdateStepSizeNextIteration()
     it.Iterate(data)
     sweep("xinv_unroll", lambda: it.Iterate(data))
-    sweep("xinv_pairs", lambda: it.Iterate(data))
     del it, data
     if verbose:
         print("autotune", chosen, timings)
