@@ -7,15 +7,13 @@
 //          the output gradient with the weights flipped in space and transposed
 //          in channels, with the previous activation's LeakyReLU derivative
 //          folded into its epilogue;
-//   wgrad  dW[co][ci][tap] = sum_p dZ[p][co] X[p + tap][ci]: blocks take one tap
-//          and a range of positions, stage 64 positions of dZ and of the shifted
-//          X in LDS transposed (channel-major), and run MFMAs with K = positions;
-//          partial sums are added to the fp32 gradient with atomics;
+//   wgrad  dW[co][ci][tap] = sum_p dZ[p][co] X[p + tap][ci]: MFMAs with K =
+//          positions over 3-D bricks whose dZ and X halo are staged once for all
+//          27 taps (k_wgrad below); block partial sums are added in a fixed order;
 //   pooling, upsampling + concatenation, the head and the loss have their own
 //   small kernels.
-// Gradients are stored in bf16 between layers and accumulated in fp32.  The
-// gradient sums use float atomics, so training is not bit-reproducible run to
-// run (inference is).
+// Gradients are stored in bf16 between layers and accumulated in fp32.  Every
+// reduction runs in a fixed order, so a training step is reproducible bit for bit.
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -25,127 +23,192 @@
 
 namespace awk {
 
-constexpr int kWgStage = 128;     // positions staged per LDS round (4 K steps of 32)
-constexpr int kWgPitch = 136;     // LDS row pitch (bf16): 272 B keeps 16-byte reads aligned
-constexpr int kWgMaxCi = 192;
-constexpr int kWgMaxCo = 128;
-constexpr int kWgMaxTiles = 16;   // accumulator tiles per wave
+// Weight gradient.  A block takes a brick of 16 x 4 x 4 output positions and
+// stages, once per brick, dZ for its output-channel tiles ([tile][256 positions][16])
+// and the 18 x 6 x 6 halo of one 16-channel tile of X ([648 positions][16]), both
+// channels-last as in memory.  The MFMA operands need 8 consecutive positions of one
+// channel per lane; ds_read_b64_tr_b16 delivers exactly that from the channels-last
+// rows (within each group of 16 lanes, lane l receives element l % 4 of the 8 bytes
+// addressed by lanes 4j + l / 4, j = 0..3), and because every lane supplies its own
+// row address, a tap's shift is only a different row: no staging per tap.
+// Wave w accumulates taps w, w + 4, ... for all positions of the brick; each block
+// walks a strided list of bricks and stores its partial sums once, and
+// k_wgrad_reduce adds the blocks' partials in a fixed order.
+constexpr int kWbX = 16, kWbY = 4, kWbZ = 4;
+constexpr int kWbP = kWbX * kWbY * kWbZ;  // 256 positions = 8 K steps of 32
+constexpr int kWhX = kWbX + 2, kWhY = kWbY + 2, kWhZ = kWbZ + 2;
+constexpr int kWhP = kWhX * kWhY * kWhZ;  // 648 halo positions
+constexpr long long kWgWsFloats = 8LL << 20;  // partial sums per layer (32 MiB)
+constexpr int kWgMaxBlocks = 2048;        // position blocks per group
+constexpr int kWgSumRows = 32;           // partial rows per thread in the first reduction stage
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) s16x4 lds_s16x4;
+
+// 8 positions (rows r and r + 4 of the lane's K group) of one channel, transposed
+// out of channels-last LDS rows of 16 channels (32 bytes).
+__device__ __forceinline__ bf16x8 tr_rows(const bf16 *p) {
+  const s16x4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4 *)(p));
+  const s16x4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4 *)(p + 4 * 16));
+  const short __attribute__((ext_vector_type(8))) v =
+      __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_bit_cast(bf16x8, v);
+}
 
 struct WgradArgs {
   const bf16 *dz;  // [N][CO]
   const bf16 *x;   // [N][CI]
-  float *dw;       // [CO][CIb][27] (blob OIDHW), accumulated
-  float *db;       // [CO], accumulated by the centre-tap blocks; may be null
-  int CO, CI, CIb;
+  float *ws;       // partials [nb][G][E], then bias partials [nb][CO]
+  int CO, CI;
   int nx, ny, nz;
-  int chunk;       // positions per block (multiple of kWgStage)
+  int bxn, byn;    // bricks along x and y
+  int nbricks;
+  int citiles;     // G = (CO / (16 COT)) * citiles; blockIdx.y = cog * citiles + cit
 };
 
-// Work split: with at least 4 output tiles, wave w owns tiles w, w+4, ... over all
-// K steps; with fewer, every wave owns all tiles and the K step ks = w of each
-// staged round, and the waves' partial sums are added through LDS at the end.
+template <int COT>
 __global__ void __launch_bounds__(256)
 k_wgrad(WgradArgs g) {
   extern __shared__ bf16 lds[];
-  bf16 *zt = lds;
-  bf16 *xt = lds + g.CO * kWgPitch;
-  const int tap = (int)blockIdx.y;
-  const int N = g.nx * g.ny * g.nz;
-  const int pbeg = (int)blockIdx.x * g.chunk;
-  const int pend = min(pbeg + g.chunk, N);
-  const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+  bf16 *xl = lds;               // [kWhP][16]
+  bf16 *zl = lds + kWhP * 16;   // [COT][kWbP][16]
+  constexpr int E = 27 * COT * 16 * 16;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int col = lane & 15, grp = lane >> 4;
-  const int CIt = (g.CI + 15) / 16, COt = g.CO / 16;
-  const int ntiles = COt * CIt;
-  const bool split_k = ntiles < 4;
-  const bool do_bias = g.db && tap == 13;
-  float bsum = 0.f;
-  f32x4 acc[kWgMaxTiles];
+  const int t = lane & 15, grp = lane >> 4;
+  const int cit = (int)blockIdx.y % g.citiles, cog = (int)blockIdx.y / g.citiles;
+  const int ci0 = cit * 16, co0 = cog * COT * 16;
+  const int ntap = wave < 3 ? 7 : 6;
+  // per-lane row addresses (elements): K index k = grp*8 + h*4 + (t>>2), channel quad t&3
+  const int a_lane = (grp * 8 + (t >> 2)) * 16 + 4 * (t & 3);
+  const int b_lane = ((1 * kWhY + (grp >> 1) + 1) * kWhX + (grp & 1) * 8 + (t >> 2) + 1) * 16 + 4 * (t & 3);
+  int boff[7];
 #pragma unroll
-  for (int j = 0; j < kWgMaxTiles; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int zchunks = g.CO / 8, xchunks = CIt * 2;  // 8-channel chunks per position
-  for (int s0 = pbeg; s0 < pend; s0 += kWgStage) {
-    // stage dZ and the tap-shifted X, transposed to [channel][position]
-    for (int w = threadIdx.x; w < kWgStage * zchunks; w += 256) {
-      const int q = w / zchunks, c8 = w - q * zchunks;
-      const int p = s0 + q;
-      bf16x8 v = zero8();
-      if (p < pend) v = *reinterpret_cast<const bf16x8 *>(g.dz + (size_t)p * g.CO + c8 * 8);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) zt[(c8 * 8 + j) * kWgPitch + q] = v[j];
-    }
-    for (int w = threadIdx.x; w < kWgStage * xchunks; w += 256) {
-      const int q = w / xchunks, c8 = w - q * xchunks;
-      const int p = s0 + q;
-      bf16x8 v = zero8();
-      if (p < pend && c8 * 8 < g.CI) {
-        const int x = p % g.nx, r = p / g.nx, y = r % g.ny, z = r / g.ny;
-        const int X = x + dx, Y = y + dy, Z = z + dz;
-        if (X >= 0 && X < g.nx && Y >= 0 && Y < g.ny && Z >= 0 && Z < g.nz)
-          v = *reinterpret_cast<const bf16x8 *>(g.x + ((size_t)(Z * g.ny + Y) * g.nx + X) * g.CI + c8 * 8);
-      }
-#pragma unroll
-      for (int j = 0; j < 8; ++j) xt[(c8 * 8 + j) * kWgPitch + q] = v[j];
-    }
-    __syncthreads();
-    if (do_bias && (int)threadIdx.x < g.CO)
-      for (int q = 0; q < kWgStage; ++q) bsum += (float)zt[threadIdx.x * kWgPitch + q];
-    if (split_k) {
-      const int ks = wave;
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        if (j >= ntiles) break;
-        const int m = j / CIt, n = j - m * CIt;
-        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&zt[(m * 16 + col) * kWgPitch + ks * 32 + grp * 8]);
-        const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&xt[(n * 16 + col) * kWgPitch + ks * 32 + grp * 8]);
-        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
-      }
-    } else {
-#pragma unroll
-      for (int ks = 0; ks < kWgStage / 32; ++ks) {
-#pragma unroll
-        for (int j = 0; j < kWgMaxTiles; ++j) {
-          const int tile = wave + 4 * j;
-          if (tile >= ntiles) break;
-          const int m = tile / CIt, n = tile - m * CIt;
-          const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&zt[(m * 16 + col) * kWgPitch + ks * 32 + grp * 8]);
-          const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&xt[(n * 16 + col) * kWgPitch + ks * 32 + grp * 8]);
-          acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
-        }
-      }
-    }
-    __syncthreads();
+  for (int i = 0; i < 7; ++i) {
+    const int tap = wave + 4 * i < 27 ? wave + 4 * i : 13;
+    const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+    boff[i] = b_lane + ((dz * kWhY + dy) * kWhX + dx) * 16;
   }
-  if (do_bias && (int)threadIdx.x < g.CO) atomicAdd(&g.db[threadIdx.x], bsum);
-  if (split_k) {
-    // add the 4 waves' partial tiles (the staging space is free after the last sync)
-    f32x4 *red = reinterpret_cast<f32x4 *>(lds);
-    if (wave > 0)
+  f32x4 acc[7][COT];
 #pragma unroll
-      for (int j = 0; j < 3; ++j)
-        if (j < ntiles) red[((wave - 1) * 3 + j) * 64 + lane] = acc[j];
-    __syncthreads();
-    if (wave > 0) return;
+  for (int i = 0; i < 7; ++i)
 #pragma unroll
-    for (int w = 0; w < 3; ++w)
+    for (int c = 0; c < COT; ++c) acc[i][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float bsum[COT];
 #pragma unroll
-      for (int j = 0; j < 3; ++j)
-        if (j < ntiles) acc[j] += red[(w * 3 + j) * 64 + lane];
-  }
-  // D[co][ci]: lane holds co = m*16 + 4*grp + i, ci = n*16 + col
-#pragma unroll
-  for (int j = 0; j < kWgMaxTiles; ++j) {
-    const int tile = split_k ? j : wave + 4 * j;
-    if (tile >= ntiles) break;
-    const int m = tile / CIt, n = tile - m * CIt;
-    const int ci = n * 16 + col;
-    if (ci >= g.CIb) continue;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int co = m * 16 + grp * 4 + i;
-      atomicAdd(&g.dw[((size_t)co * g.CIb + ci) * 27 + tap], acc[j][i]);
+  for (int c = 0; c < COT; ++c) bsum[c] = 0.f;
+  const bool do_bias = cit == 0 && wave == 0;
+  for (int br = (int)blockIdx.x; br < g.nbricks; br += (int)gridDim.x) {
+    const int bxi = br % g.bxn, rr = br / g.bxn, byi = rr % g.byn, bzi = rr / g.byn;
+    const int x0 = bxi * kWbX, y0 = byi * kWbY, z0 = bzi * kWbZ;
+    __syncthreads();  // the previous brick's reads are done
+    for (int w = threadIdx.x; w < kWhP * 2; w += 256) {
+      const int hp = w >> 1, h8 = w & 1;
+      const int hx = hp % kWhX, r = hp / kWhX, hy = r % kWhY, hz = r / kWhY;
+      const int X = x0 + hx - 1, Y = y0 + hy - 1, Z = z0 + hz - 1, c = ci0 + h8 * 8;
+      bf16x8 v = zero8();
+      if (c < g.CI && X >= 0 && X < g.nx && Y >= 0 && Y < g.ny && Z >= 0 && Z < g.nz)
+        v = *reinterpret_cast<const bf16x8 *>(g.x + ((size_t)(Z * g.ny + Y) * g.nx + X) * g.CI + c);
+      *reinterpret_cast<bf16x8 *>(xl + hp * 16 + h8 * 8) = v;
     }
+    for (int w = threadIdx.x; w < kWbP * COT * 2; w += 256) {
+      const int p = w / (COT * 2), rc = w - p * (COT * 2), c = rc >> 1, h8 = rc & 1;
+      const int X = x0 + (p & 15), Y = y0 + ((p >> 4) & 3), Z = z0 + (p >> 6);
+      bf16x8 v = zero8();
+      if (X < g.nx && Y < g.ny && Z < g.nz)
+        v = *reinterpret_cast<const bf16x8 *>(g.dz + ((size_t)(Z * g.ny + Y) * g.nx + X) * g.CO + co0 +
+                                               c * 16 + h8 * 8);
+      *reinterpret_cast<bf16x8 *>(zl + (c * kWbP + p) * 16 + h8 * 8) = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      // K step s: positions 32 s + k, i.e. z = s/2, y = 2 (s%2) + k/16, x = k%16
+      bf16x8 a[COT];
+#pragma unroll
+      for (int c = 0; c < COT; ++c) a[c] = tr_rows(zl + c * kWbP * 16 + s * 32 * 16 + a_lane);
+      if (do_bias) {
+#pragma unroll
+        for (int c = 0; c < COT; ++c)
+#pragma unroll
+          for (int j = 0; j < 8; ++j) bsum[c] += (float)a[c][j];
+      }
+      const int soff = ((s >> 1) * kWhY + 2 * (s & 1)) * kWhX * 16;
+#pragma unroll
+      for (int i = 0; i < 7; ++i) {
+        if (i >= ntap) break;
+        const bf16x8 b = tr_rows(xl + boff[i] + soff);
+#pragma unroll
+        for (int c = 0; c < COT; ++c)
+          acc[i][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[c], b, acc[i][c], 0, 0, 0);
+      }
+    }
+  }
+  // partials: element (tap, co_l, ci_l) at (tap * 16 COT + co_l) * 16 + ci_l; lane (t, grp)
+  // holds co_l = c*16 + grp*4 + r, ci_l = t
+  const int G = (int)gridDim.y;
+  float *ws = g.ws + ((size_t)blockIdx.x * G + blockIdx.y) * E;
+#pragma unroll
+  for (int i = 0; i < 7; ++i) {
+    if (i >= ntap) break;
+    const int tap = wave + 4 * i;
+#pragma unroll
+    for (int c = 0; c < COT; ++c)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ws[(tap * COT * 16 + c * 16 + grp * 4 + r) * 16 + t] = acc[i][c][r];
+  }
+  if (do_bias) {
+    float *wb = g.ws + (size_t)gridDim.x * G * E + (size_t)blockIdx.x * g.CO + co0;
+#pragma unroll
+    for (int c = 0; c < COT; ++c) {
+      float v = bsum[c];
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      if (grp == 0) wb[c * 16 + t] = v;
+    }
+  }
+}
+
+// Partial sums, nb rows of ne weight elements then nb rows of CO bias elements,
+// reduced in fixed order.  k_wgrad_sum adds row ranges [y*per, (y+1)*per) into
+// row y of dst (same layout, gridDim.y rows); k_wgrad_reduce adds all rows into
+// dW[co][ci][tap] and db.
+__global__ void __launch_bounds__(256)
+k_wgrad_sum(const float *__restrict__ src, float *__restrict__ dst, int nb, int per, int ne, int CO) {
+  const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (e >= ne + CO) return;
+  const int y = (int)blockIdx.y, nr = (int)gridDim.y;
+  const int b0 = y * per, b1 = min(nb, b0 + per);
+  float s = 0.f;
+  if (e < ne) {
+    for (int b = b0; b < b1; ++b) s += src[(size_t)b * ne + e];
+    dst[(size_t)y * ne + e] = s;
+  } else {
+    const int co = e - ne;
+    for (int b = b0; b < b1; ++b) s += src[(size_t)nb * ne + (size_t)b * CO + co];
+    dst[(size_t)nr * ne + (size_t)y * CO + co] = s;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_wgrad_reduce(const float *__restrict__ ws, float *__restrict__ dw, float *__restrict__ db, int nb,
+               int G, int cot, int citiles, int CO, int CIb) {
+  const int E = 27 * cot * 256;
+  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long ne = (long long)G * E;
+  if (id < ne) {
+    const int gi = (int)(id / E), e = (int)(id - (long long)gi * E);
+    const int tap = e / (cot * 256), r = e - tap * cot * 256, co_l = r >> 4, ci_l = r & 15;
+    const int ci = (gi % citiles) * 16 + ci_l, co = (gi / citiles) * cot * 16 + co_l;
+    if (ci >= CIb) return;
+    float s = 0.f;
+    for (int b = 0; b < nb; ++b) s += ws[(size_t)b * ne + id];
+    dw[((size_t)co * CIb + ci) * 27 + tap] += s;
+  } else if (id < ne + CO) {
+    const int co = (int)(id - ne);
+    const float *wb = ws + (size_t)nb * ne;
+    float s = 0.f;
+    for (int b = 0; b < nb; ++b) s += wb[(size_t)b * CO + co];
+    db[co] += s;
   }
 }
 
@@ -339,12 +402,14 @@ struct awk_unet_trainer {
        *dz3 = nullptr, *dz2 = nullptr, *dp0 = nullptr, *dz1 = nullptr, *dz0 = nullptr;
   float *u = nullptr;     // 3 planes: the forward output
   float *hpart = nullptr; // head-gradient block partials [blocks][52]
+  float *ws = nullptr;    // weight-gradient block partials (kWgWsFloats + bias partials)
+  float *ws2 = nullptr;   // their first reduction stage
 };
 
 static void trainer_free(awk_unet_trainer *t) {
   void *b[] = {t->w, t->g, t->m, t->v, t->dz13, t->dz12, t->dc0, t->dz11, t->dz10, t->dc1, t->dz9,
                t->dz8, t->dc2, t->dz7, t->dz6, t->dp2, t->dz5, t->dz4, t->dp1, t->dz3, t->dz2,
-               t->dp0, t->dz1, t->dz0, t->u, t->hpart};
+               t->dp0, t->dz1, t->dz0, t->u, t->hpart, t->ws, t->ws2};
   for (void *p : b)
     if (p) (void)hipFree(p);
   for (int l = 0; l < 14; ++l)
@@ -395,32 +460,48 @@ static awk_status dgrad(awk_unet_trainer *t, int l, const bf16 *dz, const bf16 *
 static awk_status wgrad(awk_unet_trainer *t, int l, const bf16 *dz, const bf16 *x, int nx, int ny,
                         int nz, hipStream_t s) {
   const LayerSpec &L = kUnetLayers[l];
+  const int cot = L.cout >= 64 ? 4 : L.cout >= 32 ? 2 : 1;
   WgradArgs g;
   g.dz = dz;
   g.x = x;
-  g.dw = t->g + t->woff[l];
+  g.ws = t->ws;
   g.CO = L.cout;
   g.CI = L.cin;
-  g.CIb = kUnetBlobCin[l];
   g.nx = nx;
   g.ny = ny;
   g.nz = nz;
-  g.db = t->g + t->boff[l];
-  const long long N = (long long)nx * ny * nz;
-  // blocks per tap: up to 256, but few enough that the layer's partial-sum
-  // atomics (blocks x cout x cin) stay around 4M
-  long long per_tap = (4LL << 20) / (27LL * L.cout * kUnetBlobCin[l]);
-  per_tap = per_tap < 1 ? 1 : (per_tap > 256 ? 256 : per_tap);
-  long long chunk = (N + per_tap - 1) / per_tap;
-  chunk = (chunk + kWgStage - 1) / kWgStage * kWgStage;
-  g.chunk = (int)chunk;
-  const dim3 grid((unsigned)((N + chunk - 1) / chunk), 27);
-  const int CIt = (L.cin + 15) / 16;
-  size_t lds = (size_t)(L.cout + CIt * 16) * kWgPitch * sizeof(bf16);
-  const size_t red = (size_t)3 * 3 * 64 * sizeof(f32x4);  // split-K partials reuse the staging space
-  if (lds < red) lds = red;
-  AWK_LAUNCH(k_wgrad, grid, dim3(256), lds, s, g);
+  g.bxn = (nx + kWbX - 1) / kWbX;
+  g.byn = (ny + kWbY - 1) / kWbY;
+  const long long nbr = (long long)g.bxn * g.byn * ((nz + kWbZ - 1) / kWbZ);
+  g.nbricks = (int)nbr;
+  g.citiles = (L.cin + 15) / 16;
+  const int G = L.cout / (16 * cot) * g.citiles;
+  const long long E = 27LL * cot * 256;
+  // position blocks: up to kWgMaxBlocks in all, one brick at least each, and the
+  // partial sums within the workspace
+  long long nb = kWgMaxBlocks / G;
+  if (nb > nbr) nb = nbr;
+  if (nb > kWgWsFloats / (G * E)) nb = kWgWsFloats / (G * E);
+  if (nb < 1) nb = 1;
+  const dim3 grid((unsigned)nb, (unsigned)G);
+  const size_t lds = (size_t)(kWhP + cot * kWbP) * 16 * sizeof(bf16);
+  if (cot == 4) AWK_LAUNCH(k_wgrad<4>, grid, dim3(256), lds, s, g);
+  else if (cot == 2) AWK_LAUNCH(k_wgrad<2>, grid, dim3(256), lds, s, g);
+  else AWK_LAUNCH(k_wgrad<1>, grid, dim3(256), lds, s, g);
   AWK_LAUNCH_CHECK("k_wgrad");
+  // fixed-order reduction: rows of at most kWgSumRows partials, then the rows
+  const long long ne = G * E, nr = ne + L.cout;
+  const float *src = t->ws;
+  int rows = (int)nb;
+  if (nb > kWgSumRows) {
+    rows = (int)((nb + kWgSumRows - 1) / kWgSumRows);
+    AWK_LAUNCH(k_wgrad_sum, dim3((unsigned)((nr + 255) / 256), (unsigned)rows), dim3(256), 0, s, t->ws,
+               t->ws2, (int)nb, kWgSumRows, (int)ne, L.cout);
+    src = t->ws2;
+  }
+  AWK_LAUNCH(k_wgrad_reduce, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, src,
+             t->g + t->woff[l], t->g + t->boff[l], rows, G, cot, g.citiles, L.cout, kUnetBlobCin[l]);
+  AWK_LAUNCH_CHECK("k_wgrad_reduce");
   return AWK_OK;
 }
 
@@ -503,14 +584,12 @@ awk_status awk_unet_trainer_create(awk_unet_trainer_t *out, awk_unet_t net, cons
   *out = nullptr;
   if (n_weights != awk_unet_weight_count())
     return fail(AWK_ERR_INVALID, "awk_unet_trainer_create: weight count does not match the network");
+  // the gradient kernel's shapes: 16-channel output tiles, 8-channel input groups,
+  // at most 128 outputs (bias partials), one block's partial sums within the workspace
   for (int l = 0; l < 14; ++l)
-    if (kUnetLayers[l].cout > kWgMaxCo || kUnetLayers[l].cin > kWgMaxCi ||
-        (kUnetLayers[l].cout / 16) * ((kUnetLayers[l].cin + 15) / 16) > 4 * kWgMaxTiles)
-      return fail(AWK_ERR_UNSUPPORTED, "awk_unet_trainer_create: layer too wide for the gradient kernel");
-  // the widest layers stage (128 + 128) x 136 bf16 = 68 KiB of LDS per wgrad block
-  AWK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_wgrad),
-                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (kWgMaxCo + kWgMaxCi) * kWgPitch * (int)sizeof(bf16)));
+    if (kUnetLayers[l].cout % 16 || kUnetLayers[l].cout > 128 || kUnetLayers[l].cin % 8 ||
+        27LL * 256 * (kUnetLayers[l].cout / 16) * ((kUnetLayers[l].cin + 15) / 16) > kWgWsFloats)
+      return fail(AWK_ERR_UNSUPPORTED, "awk_unet_trainer_create: layer shape not supported by the gradient kernel");
   awk_unet_trainer *t = new (std::nothrow) awk_unet_trainer();
   if (!t) return fail(AWK_ERR_ALLOC, "awk_unet_trainer_create: host allocation failed");
   t->net = net;
@@ -551,6 +630,15 @@ awk_status awk_unet_trainer_create(awk_unet_trainer_t *out, awk_unet_t net, cons
     if (e == hipSuccess) e = hipMalloc((void **)b.p, b.n * sizeof(bf16));
   if (e == hipSuccess) e = hipMalloc(&t->u, 3 * N0 * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&t->hpart, (N0 + 255) / 256 * 52 * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&t->ws, (kWgWsFloats + (size_t)kWgMaxBlocks * 128) * sizeof(float));
+  // rows = ceil(nb / kWgSumRows) <= nb / kWgSumRows + 1, so rows * ne <= ws / kWgSumRows + ne
+  size_t max_ne = 0;
+  for (int l = 0; l < 14; ++l) {
+    const size_t ne = (size_t)27 * 256 * (kUnetLayers[l].cout / 16) * ((kUnetLayers[l].cin + 15) / 16);
+    if (ne > max_ne) max_ne = ne;
+  }
+  if (e == hipSuccess)
+    e = hipMalloc(&t->ws2, (kWgWsFloats / kWgSumRows + max_ne + (size_t)kWgMaxBlocks * 128) * sizeof(float));
   awk_status st = AWK_OK;
   if (e != hipSuccess) st = hip_check(e, "awk_unet_trainer_create: allocation");
   if (!st) st = repack_all(t, nullptr);
