@@ -103,7 +103,7 @@ SIGNATURES = {
     "awk_greedy_get_step": [_P, ctypes.POINTER(_F), _P],
     "awk_greedy_set_auto_step_reduce": [_P, _I],
     "awk_greedy_set_profiling": [_P, _I],
-    "awk_greedy_get_profile": [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)],
+    "awk_greedy_get_profile": [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I), ctypes.POINTER(_I)],
     "awk_greedy_get_flags": [_P, ctypes.POINTER(ctypes.c_uint), _I, _P],
     "awk_vf_aos_to_soa": [_P, _P, _P, _P, ctypes.c_longlong, _P],
     "awk_vf_soa_to_aos": [_P, _P, _P, _P, ctypes.c_longlong, _P],
@@ -112,6 +112,7 @@ SIGNATURES = {
     "awk_ldmm_atlas_get_flags": [_P, ctypes.POINTER(ctypes.c_uint), _I, _P],
     "awk_greedy_iterate": [_P, _P, _P, ctypes.POINTER(_P3), ctypes.POINTER(_P3),
                            ctypes.POINTER(_P3), ctypes.POINTER(_P3), _P, _P],
+    "awk_greedy_iterate_n": [_P, _P, _P, ctypes.POINTER(_P3), ctypes.POINTER(_P3), _I, _P, _P],
 }
 _RESTYPE = {"awk_last_error": ctypes.c_char_p, "awk_version": ctypes.c_int,
             "awk_unet_weight_count": ctypes.c_size_t}

@@ -132,7 +132,9 @@ def tune(size=(256, 256, 256), reps=10, device="cuda", verbose=False):
     data = GreedyDeformationDataGPU(I0, I1, max_iters=4 * reps + 8, h0=h0)
     it.UpdateStepSizeNextIteration()
     it.Iterate(data)
-    sweep("xinv_unroll", lambda: it.Iterate(data))
+    # back-to-back iterations, as GreedyWarp runs them (the composition also forms
+    # the next deformed image)
+    sweep("xinv_unroll", lambda: it.IterateN(data, 4))
     del it, data
     if verbose:
         print("autotune", chosen, timings)

@@ -83,6 +83,9 @@ struct XInvArgs {
   const float *step;        // COMPOSE / AXPBY: step size (device scalar)
   float sp[3];              // COMPOSE: spacing
   float sigma2;             // AXPBY / MAXAXPY: sigma^2
+  const float *i0;          // COMPOSE: when set, also idef = I0(hnew) (the next
+  float *idef;              //   iteration's deformed image; VAL background i0_bg,
+  float i0_bg;              //   I0 on the field's grid; one subject per launch)
 };
 
 // Subjects of one batched launch: blockIdx.y selects the subject, and each of its

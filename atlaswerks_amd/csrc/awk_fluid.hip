@@ -344,8 +344,10 @@ k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
   }
 }
 
-// X inverse: c2r of RT rows (3 components) with an epilogue.
-template <int NC, int RTC, int BDC, int XU, bool PAIRS = false>
+// X inverse: c2r of RT rows (3 components) with an epilogue.  APPLY (COMPOSE
+// only): the epilogue also samples I0 at the new field (a.i0 -> a.idef), the next
+// iteration's deformed image.
+template <int NC, int RTC, int BDC, int XU, bool PAIRS = false, bool APPLY = false>
 __global__ void __launch_bounds__(BDC ? BDC : 1024)
 k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, int RT_rt,
        const float2 *__restrict__ spec, size_t plane, Batch bt) {
@@ -436,11 +438,24 @@ k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
       for (int u = 0; u < XU; ++u)
         if (row_[u] < NR) {
           const size_t base = (size_t)row_[u] * nx + x_[u];
-          // streamed: the next reader is the next iteration's apply
+          // streamed: the next reader is the next iteration's apply (or, with
+          // APPLY, the next iteration's composition gathers)
           __builtin_nontemporal_store(hn[u].x, a.hnew[0] + base);
           __builtin_nontemporal_store(hn[u].y, a.hnew[1] + base);
           __builtin_nontemporal_store(hn[u].z, a.hnew[2] + base);
         }
+      if constexpr (APPLY) {
+        // the next iteration's deformed image I0(hnew), from the new field still in
+        // registers: the same sampling as the apply kernel on the stored field
+        const BufLoad ib = make_buf(a.i0, ntot);
+        float r[XU];
+#pragma unroll
+        for (int u = 0; u < XU; ++u)
+          if (row_[u] < NR) r[u] = trilerp1g<AWK_BG_VAL>(ib, d, hn[u].x, hn[u].y, hn[u].z, a.i0_bg);
+#pragma unroll
+        for (int u = 0; u < XU; ++u)
+          if (row_[u] < NR) a.idef[(size_t)row_[u] * nx + x_[u]] = r[u];
+      }
     }
   } else {
     // LDMM coefficients, in float as the reference forms them
@@ -914,6 +929,7 @@ awk_status fk_init(FluidKernel &k, int nx, int ny, int nz, const float sp[3]) {
     if (!lds_attr) {
       const void *fns[] = {reinterpret_cast<const void *>(&k_xfwd<0, 0, 0>),
                            reinterpret_cast<const void *>(&k_xinv<0, 0, 0, 1>),
+                           reinterpret_cast<const void *>(&k_xinv<0, 0, 0, 1, false, true>),
                            reinterpret_cast<const void *>(&k_ypass),
                            reinterpret_cast<const void *>(&k_zsolve)};
       for (const void *f : fns)
@@ -1098,9 +1114,9 @@ awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblock
   const unsigned grid = grid_for((long long)k.ny * k.nz, k.rt);
   const size_t lds = x_lds(k.ax[0], k.rt);
   const Axis ax = k.ax[0].dev();
-#define AWK_XI2(NC, RTC, BDC, XU, PR)                                                            \
-  AWK_LAUNCH((k_xinv<NC, RTC, BDC, XU, PR>), dim3(grid, bt.n), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \
-                     k.nx, k.ny, k.nz, k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane(), bt)
+#define AWK_XI3(NC, RTC, BDC, XU, PR, AP)                                                        \
+  AWK_LAUNCH((k_xinv<NC, RTC, BDC, XU, PR, AP>), dim3(grid, bt.n), dim3(BDC ? BDC : k.bd_x), lds, s, a, \
+                     ax, k.nx, k.ny, k.nz, k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane(), bt)
 #define AWK_XI(NC, RTC, BDC, XU)                                                                 \
   AWK_LAUNCH((k_xinv<NC, RTC, BDC, XU>), dim3(grid, bt.n), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \
                      k.nx, k.ny, k.nz, k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane(), bt)
@@ -1109,11 +1125,27 @@ awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblock
   // x-neighbour corners as 8-byte pairs (12 loads per point instead of 24; the
   // same values, so the same result)
   const bool pr = a.mode == XSTORE_COMPOSE && tuning().xinv_pairs;
-  if (k.nx == 256 && k.rt == 8) {
+  // the composition with the next iteration's deformed image (one subject per launch)
+  const bool ap = a.mode == XSTORE_COMPOSE && a.i0;
+  if (ap && (bt.n != 1 || !a.idef))
+    return fail(AWK_ERR_INVALID, "inverse x pass: the fused apply takes one subject and an output image");
+  if (ap) {
+    if (k.nx == 256 && k.rt == 8) {
+      if (pr) {
+        if (xu == 4) AWK_XI3(256, 8, 256, 4, true, true);
+        else if (xu == 2) AWK_XI3(256, 8, 256, 2, true, true);
+        else AWK_XI3(256, 8, 256, 1, true, true);
+      } else if (xu == 4) AWK_XI3(256, 8, 256, 4, false, true);
+      else if (xu == 2) AWK_XI3(256, 8, 256, 2, false, true);
+      else AWK_XI3(256, 8, 256, 1, false, true);
+    } else if (k.nx == 128 && k.rt == 8) AWK_XI3(128, 8, 256, 2, false, true);
+    else if (k.nx == 64 && k.rt == 8) AWK_XI3(64, 8, 256, 2, false, true);
+    else AWK_XI3(0, 0, 0, 1, false, true);
+  } else if (k.nx == 256 && k.rt == 8) {
     if (pr) {
-      if (xu == 4) AWK_XI2(256, 8, 256, 4, true);
-      else if (xu == 2) AWK_XI2(256, 8, 256, 2, true);
-      else AWK_XI2(256, 8, 256, 1, true);
+      if (xu == 4) AWK_XI3(256, 8, 256, 4, true, false);
+      else if (xu == 2) AWK_XI3(256, 8, 256, 2, true, false);
+      else AWK_XI3(256, 8, 256, 1, true, false);
     } else if (xu == 4) AWK_XI(256, 8, 256, 4);
     else if (xu == 2) AWK_XI(256, 8, 256, 2);
     else AWK_XI(256, 8, 256, 1);
@@ -1121,7 +1153,7 @@ awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblock
   else if (k.nx == 64 && k.rt == 8) AWK_XI(64, 8, 256, 2);
   else AWK_XI(0, 0, 0, 1);
 #undef AWK_XI
-#undef AWK_XI2
+#undef AWK_XI3
   AWK_LAUNCH_CHECK("k_xinv");
   if (nblocks) *nblocks = (int)grid;
   return AWK_OK;

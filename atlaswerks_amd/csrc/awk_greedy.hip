@@ -12,7 +12,9 @@
 // On the first iteration of a scale level (step not yet known) step 5 writes v
 // and a per-block max |v|; the step MaxPert / max|v| is then formed on the device
 // and a separate composeHV applies it.  The same unfused tail serves the inverse
-// hfield update.
+// hfield update.  Iterations run back to back (awk_greedy_iterate_n) fold step 1
+// of the next iteration into step 4: the composition samples I0 at the new field
+// it holds in registers.
 #include <array>
 #include <utility>
 #include <cstring>
@@ -74,9 +76,13 @@ struct awk_greedy {
   // attached to each stage's kernel dispatch
   bool prof = false;
   std::vector<std::array<awk::LaunchEvents, AWK_GREEDY_NSTAGES>> ev;
+  std::vector<unsigned> ev_mask;  // stages launched in each recorded iteration
   int ev_used = 0;
   void stage(int i) {
-    if (prof) awk::launch_events() = &ev[ev_used][i];
+    if (prof) {
+      awk::launch_events() = &ev[ev_used][i];
+      ev_mask[ev_used] |= 1u << i;
+    }
   }
 };
 
@@ -89,6 +95,7 @@ static void greedy_free(awk_greedy *g) {
       (void)hipEventDestroy(e.stop);
     }
   g->ev.clear();
+  g->ev_mask.clear();
   fk_release(g->k);
   if (g->idef) (void)hipFree(g->idef);
   if (g->v) (void)hipFree(g->v);
@@ -191,18 +198,19 @@ awk_status awk_greedy_set_auto_step_reduce(awk_greedy_t g, int on) {
   return AWK_OK;
 }
 
-awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
+}  // extern "C"
+
+// One iteration.  have_idef: g->idef already holds I0(h_in), written by the
+// previous iteration's composition.  fuse_apply: a steady-state composition (step
+// known, no inverse field) also writes I0(h_out) into g->idef, and *fused says
+// that it did; h_out must not overlap h_in then.
+static awk_status iterate_one(awk_greedy *g, const float *I0, const float *I1,
                               const float *const h_in[3], float *const h_out[3],
                               const float *const hinv_in[3], float *const hinv_out[3],
-                              double *energy_dev, awk_stream_t stream) {
-  if (!g || !I0 || !I1 || !h_in || !h_out) return fail(AWK_ERR_INVALID, "awk_greedy_iterate: null argument");
-  for (int c = 0; c < 3; ++c)
-    if (!h_in[c] || !h_out[c]) return fail(AWK_ERR_INVALID, "awk_greedy_iterate: null hfield plane");
+                              double *energy_dev, hipStream_t s, bool have_idef, bool fuse_apply,
+                              bool *fused) {
+  *fused = false;
   const bool inv = hinv_in && hinv_out;
-  if (inv)
-    for (int c = 0; c < 3; ++c)
-      if (!hinv_in[c] || !hinv_out[c]) return fail(AWK_ERR_INVALID, "awk_greedy_iterate: null hinv plane");
-  hipStream_t s = as_stream(stream);
   const int nx = g->nx, ny = g->ny, nz = g->nz;
   const size_t N = (size_t)nx * ny * nz;
   awk_status st;
@@ -214,15 +222,19 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
         AWK_HIP(hipEventCreate(&e.stop));
       }
       g->ev.push_back(set);
+      g->ev_mask.push_back(0u);
     }
+    g->ev_mask[g->ev_used] = 0u;
   }
   // the launches of this call are timed only while launch_events() is set
   struct Clear {
     ~Clear() { launch_events() = nullptr; }
   } clear_events;
-  g->stage(0);
-  // 1. deformed image (VAL background 0, HField3DUtils::apply defaults)
-  if ((st = launch_apply(g->idef, I0, h_in[0], h_in[1], h_in[2], nx, ny, nz, AWK_BG_VAL, 0.f, s))) return st;
+  if (!have_idef) {
+    g->stage(0);
+    // 1. deformed image (VAL background 0, HField3DUtils::apply defaults)
+    if ((st = launch_apply(g->idef, I0, h_in[0], h_in[1], h_in[2], nx, ny, nz, AWK_BG_VAL, 0.f, s))) return st;
+  }
   g->stage(1);
   XFwdArgs fa;
   std::memset(&fa, 0, sizeof(fa));
@@ -250,7 +262,6 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
   if ((st = fk_zsolve(g->k, true, s))) return st;
   g->stage(4);
   if ((st = fk_ypass(g->k, true, s, nullptr))) return st;
-  g->stage(5);
   // 5. velocity -> hfield update
   const bool in_place = planes_alias(h_out, h_in, N);
   float *dst[3];
@@ -259,6 +270,7 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
   std::memset(&ia, 0, sizeof(ia));
   for (int c = 0; c < 3; ++c) ia.sp[c] = g->sp[c];
   if (g->update_step || inv) {
+    g->stage(5);
     ia.mode = g->update_step ? XSTORE_MAXNORM : XSTORE_PLAIN;
     for (int c = 0; c < 3; ++c) ia.out[c] = g->v + c * N;
     ia.maxpart = g->maxpart;
@@ -295,12 +307,61 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
       ia.hnew[c] = dst[c];
     }
     ia.step = g->step;
+    if (fuse_apply && !in_place) {
+      ia.i0 = I0;
+      ia.idef = g->idef;
+      ia.i0_bg = 0.f;
+      *fused = true;
+    }
+    g->stage(*fused ? 6 : 5);
     if ((st = fk_xinv(g->k, ia, s, &nb))) return st;
   }
   if (in_place)
     for (int c = 0; c < 3; ++c)
       AWK_HIP(hipMemcpyAsync(h_out[c], g->hbuf + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, s));
   if (g->prof) ++g->ev_used;
+  return AWK_OK;
+}
+
+extern "C" {
+
+awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
+                              const float *const h_in[3], float *const h_out[3],
+                              const float *const hinv_in[3], float *const hinv_out[3],
+                              double *energy_dev, awk_stream_t stream) {
+  if (!g || !I0 || !I1 || !h_in || !h_out) return fail(AWK_ERR_INVALID, "awk_greedy_iterate: null argument");
+  for (int c = 0; c < 3; ++c)
+    if (!h_in[c] || !h_out[c]) return fail(AWK_ERR_INVALID, "awk_greedy_iterate: null hfield plane");
+  if (hinv_in && hinv_out)
+    for (int c = 0; c < 3; ++c)
+      if (!hinv_in[c] || !hinv_out[c]) return fail(AWK_ERR_INVALID, "awk_greedy_iterate: null hinv plane");
+  bool fused = false;
+  return iterate_one(g, I0, I1, h_in, h_out, hinv_in, hinv_out, energy_dev, as_stream(stream), false,
+                     false, &fused);
+}
+
+awk_status awk_greedy_iterate_n(awk_greedy_t g, const float *I0, const float *I1, float *const h_a[3],
+                                float *const h_b[3], int n, double *energies_dev,
+                                awk_stream_t stream) {
+  if (!g || !I0 || !I1 || !h_a || !h_b) return fail(AWK_ERR_INVALID, "awk_greedy_iterate_n: null argument");
+  if (n < 0) return fail(AWK_ERR_INVALID, "awk_greedy_iterate_n: negative iteration count");
+  for (int c = 0; c < 3; ++c)
+    if (!h_a[c] || !h_b[c]) return fail(AWK_ERR_INVALID, "awk_greedy_iterate_n: null hfield plane");
+  const size_t N = (size_t)g->nx * g->ny * g->nz;
+  if (planes_alias(h_b, h_a, N) || planes_alias(h_a, h_b, N))
+    return fail(AWK_ERR_INVALID, "awk_greedy_iterate_n: the two hfields must not overlap");
+  hipStream_t s = as_stream(stream);
+  bool have_idef = false;
+  for (int i = 0; i < n; ++i) {
+    float *const *src = (i & 1) ? h_b : h_a;
+    float *const *dst = (i & 1) ? h_a : h_b;
+    bool fused = false;
+    awk_status st = iterate_one(g, I0, I1, src, dst, nullptr, nullptr,
+                                energies_dev ? energies_dev + i : nullptr, s, have_idef, i + 1 < n,
+                                &fused);
+    if (st) return st;
+    have_idef = fused;
+  }
   return AWK_OK;
 }
 
@@ -311,15 +372,21 @@ awk_status awk_greedy_set_profiling(awk_greedy_t g, int on) {
   return AWK_OK;
 }
 
-awk_status awk_greedy_get_profile(awk_greedy_t g, double *stage_ms, int *iterations) {
+awk_status awk_greedy_get_profile(awk_greedy_t g, double *stage_ms, int *stage_launches,
+                                  int *iterations) {
   if (!g || !stage_ms || !iterations) return fail(AWK_ERR_INVALID, "awk_greedy_get_profile: null argument");
-  for (int i = 0; i < AWK_GREEDY_NSTAGES; ++i) stage_ms[i] = 0.0;
+  for (int i = 0; i < AWK_GREEDY_NSTAGES; ++i) {
+    stage_ms[i] = 0.0;
+    if (stage_launches) stage_launches[i] = 0;
+  }
   for (int it = 0; it < g->ev_used; ++it) {
     for (int i = 0; i < AWK_GREEDY_NSTAGES; ++i) {
+      if (!(g->ev_mask[it] >> i & 1u)) continue;  // not launched in this iteration
       AWK_HIP(hipEventSynchronize(g->ev[it][i].stop));
       float ms = 0.f;
       AWK_HIP(hipEventElapsedTime(&ms, g->ev[it][i].start, g->ev[it][i].stop));
       stage_ms[i] += ms;
+      if (stage_launches) ++stage_launches[i];
     }
   }
   *iterations = g->ev_used;

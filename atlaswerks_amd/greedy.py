@@ -5,7 +5,10 @@ Mirrors the reference driver layer for the greedy path:
 ``GreedyDeformationDataGPU`` (per-registration device state) and
 ``GreedyWarp<IteratorT>::RunWarp`` (Algorithms/Greedy/GreedyWarp.txx:69-125).
 Each ``Iterate`` is one call of ``awk_greedy_iterate`` (six HIP kernels, no host
-sync); energies stay on the device until ``energies()`` is read.
+sync); ``IterateN`` runs several iterations in one call of ``awk_greedy_iterate_n``,
+where each composition also forms the next iteration's deformed image (five
+kernels per iteration after the first, the same results bit for bit).  Energies
+stay on the device until ``energies()`` is read.
 """
 import ctypes
 import warnings
@@ -52,12 +55,14 @@ class GreedyDeformationDataGPU:
         self._energy = torch.zeros(max_iters, dtype=torch.float64, device=dev)
         self.cur_iter = 0
 
-    def energy_slot(self):
-        if self.cur_iter >= self._energy.numel():
-            grown = torch.zeros(2 * self._energy.numel(), dtype=torch.float64, device=self._energy.device)
+    def energy_slot(self, n=1):
+        """Device slots for the energies of the next n iterations."""
+        if self.cur_iter + n > self._energy.numel():
+            size = max(2 * self._energy.numel(), self.cur_iter + n)
+            grown = torch.zeros(size, dtype=torch.float64, device=self._energy.device)
             grown[: self._energy.numel()] = self._energy
             self._energy = grown
-        return self._energy[self.cur_iter:self.cur_iter + 1]
+        return self._energy[self.cur_iter:self.cur_iter + n]
 
     def energies(self):
         """Per-iteration energies (host numpy, synchronises)."""
@@ -100,16 +105,21 @@ class GreedyIteratorGPU:
     def SetStepSize(self, step, stream=None):
         _lib.call("awk_greedy_set_step", self._h, float(step), stream_handle(stream))
 
-    STAGES = ("apply", "xfwd_force", "y_fwd", "z_solve", "y_inv", "xinv_compose")
+    STAGES = ("apply", "xfwd_force", "y_fwd", "z_solve", "y_inv", "xinv_compose",
+              "xinv_compose_apply")
 
     def SetProfiling(self, on):
         _lib.call("awk_greedy_set_profiling", self._h, int(bool(on)))
 
-    def GetProfile(self):
-        """{stage: total ms} over the iterations recorded since the last read, and the count."""
+    def GetProfile(self, launches=False):
+        """{stage: total ms} over the iterations recorded since the last read, and the
+        iteration count; with launches=True also {stage: timed launches}."""
         ms = (ctypes.c_double * len(self.STAGES))()
+        cnt = (ctypes.c_int * len(self.STAGES))()
         n = ctypes.c_int()
-        _lib.call("awk_greedy_get_profile", self._h, ms, ctypes.byref(n))
+        _lib.call("awk_greedy_get_profile", self._h, ms, cnt, ctypes.byref(n))
+        if launches:
+            return dict(zip(self.STAGES, list(ms))), n.value, dict(zip(self.STAGES, list(cnt)))
         return dict(zip(self.STAGES, list(ms))), n.value
 
     def Iterate(self, data, stream=None):
@@ -126,6 +136,23 @@ class GreedyIteratorGPU:
         if data.compute_inverse:
             data.Def0To1, data._hinvbuf = data._hinvbuf, data.Def0To1
         data.cur_iter += 1
+
+    def IterateN(self, data, n, stream=None):
+        """n iterations back to back (awk_greedy_iterate_n): the same fields and
+        energies as n calls of Iterate.  Not with an inverse field."""
+        if dims(data.I0) != self.size:
+            raise ValueError("deformation data size does not match the iterator")
+        if data.compute_inverse:
+            raise ValueError("IterateN does not update Def0To1; use Iterate")
+        if n <= 0:
+            return
+        e = data.energy_slot(n)
+        _lib.call("awk_greedy_iterate_n", self._h, ptr(data.I0), ptr(data.I1),
+                  planes_arr(data.Def1To0), planes_arr(data._hbuf), int(n),
+                  ctypes.c_void_p(e.data_ptr()), stream_handle(stream))
+        if n % 2:
+            data.Def1To0, data._hbuf = data._hbuf, data.Def1To0
+        data.cur_iter += n
 
     def _destroy(self):
         if self._h is not None:
@@ -188,12 +215,15 @@ class GreedyWarp:
                 data.Def0To1.copy_(hfield.upsampleHField(prev.Def0To1, (nx, ny, nz)))
             it = GreedyIteratorGPU((nx, ny, nz), sp, self.param)
             it.SetAutoStepReduce(self.auto_step_reduce)
-            for k in range(n_iter):
-                if k == 0:
-                    it.UpdateStepSizeNextIteration()
+            if n_iter > 0:
+                it.UpdateStepSizeNextIteration()
                 it.Iterate(data, stream)
-                if k == 0:
-                    self.step_history.append(it.StepSize(stream))
+                self.step_history.append(it.StepSize(stream))
+            if self.compute_inverse:
+                for k in range(1, n_iter):
+                    it.Iterate(data, stream)
+            else:
+                it.IterateN(data, n_iter - 1, stream)
             flags = it.GetFlags(stream=stream)
             self.flags |= flags
             if flags & _lib.FLAG_NAN_ENERGY:

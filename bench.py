@@ -2,7 +2,10 @@
 
 One step = one greedy-fluid iteration (GreedyIteratorGPU::Iterate semantics:
 apply, gradient/residual/force/SSD, FluidKernelFFT L^-1, step-scaled composeHV)
-of a 256^3 float32 pairwise registration, inputs resident in HBM.  Prints one
+of a 256^3 float32 pairwise registration, inputs resident in HBM.  The K timed
+iterations run back to back as GreedyWarp's inner loop does (one call of
+awk_greedy_iterate_n: from the second iteration on, each composition also forms
+the next iteration's deformed image, bit-identical to separate iterations).  Prints one
 JSON line with the iteration rate, the roofline of the dominant kernel (measured
 live with HIP events on the iteration's stream), per-stage times, the kernel
 microbenchmarks of the 256^3 workload (L^-1, apply, compose, splat) and the CPU
@@ -50,6 +53,8 @@ def stage_bytes(nx, ny, nz):
         "z_solve": 2 * spec,
         "y_inv": 2 * spec,
         "xinv_compose": spec + 24 * N,           # read spectrum + h, write h
+        # read spectrum + h + I0, write h + the next deformed image
+        "xinv_compose_apply": spec + 32 * N,
     }
 
 
@@ -392,13 +397,13 @@ def main():
     it = GreedyIteratorGPU((nx, ny, nz), (1.0, 1.0, 1.0), param)
     data = GreedyDeformationDataGPU(I0, I1, max_iters=args.warmup + args.steps + 1)
     it.UpdateStepSizeNextIteration()
-    for _ in range(args.warmup):
-        it.Iterate(data)
+    it.Iterate(data)  # sets the step (MaxPert / max |v|)
+    it.IterateN(data, args.warmup)
     torch.cuda.synchronize()
-    # timed region: K iterations, nothing else on the stream
+    # timed region: K iterations back to back (GreedyWarp's inner loop), nothing
+    # else on the stream
     ts = time.perf_counter()
-    for _ in range(args.steps):
-        it.Iterate(data)
+    it.IterateN(data, args.steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - ts
     energies = data.energies()
@@ -406,17 +411,18 @@ def main():
     # each kernel dispatch (kernel durations, as rocprof reports them)
     n_att = min(args.steps, 50)
     it.SetProfiling(True)
-    for _ in range(n_att):
-        it.Iterate(data)
-    prof, n_prof = it.GetProfile()
+    it.IterateN(data, n_att)
+    prof, n_prof, launches = it.GetProfile(launches=True)
     it.SetProfiling(False)
     ms_per_step = 1e3 * elapsed / args.steps
     iters_per_s = args.steps / elapsed
 
+    # per iteration: each stage's share; per launch: the kernel's mean duration
     stage_avg = {k: v / max(n_prof, 1) for k, v in prof.items()}
+    launch_avg = {k: prof[k] / launches[k] for k in prof if launches[k]}
     sb = stage_bytes(nx, ny, nz)
     dom = max(stage_avg, key=lambda k: stage_avg[k])
-    achieved = sb[dom] / (stage_avg[dom] * 1e-3) / 1e9
+    achieved = sb[dom] / (launch_avg[dom] * 1e-3) / 1e9
     # HBM bytes per launch of the dominant kernel, measured by rocprofv3 --pmc
     # FETCH_SIZE / WRITE_SIZE (scaled by the copy calibration) and recorded in
     # profiles/pmc_traffic_<n>.txt; null when no measurement exists for this size.
@@ -459,11 +465,12 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "alg_bytes_per_launch": sb[dom],
-            "avg_launch_ms": round(stage_avg[dom], 5),
+            "avg_launch_ms": round(launch_avg[dom], 5),
         },
         "stages_ms": {k: round(v, 5) for k, v in stage_avg.items()},
+        "stages_launches": {k: launches[k] for k in launches},
         "stages_sum_ms": round(sum(stage_avg.values()), 5),
-        "stages_gbs": {k: round(sb[k] / (v * 1e-3) / 1e9, 1) for k, v in stage_avg.items() if sb[k]},
+        "stages_gbs": {k: round(sb[k] / (v * 1e-3) / 1e9, 1) for k, v in launch_avg.items() if sb[k]},
         "energy_first_last": [float(energies[0]), float(energies[-1])],
         "input_gen_s": round(gen_s, 2),
         "tuning": tuning,

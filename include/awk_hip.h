@@ -275,6 +275,20 @@ awk_status awk_greedy_iterate(awk_greedy_t g, const float *I0, const float *I1,
                               const float *const hinv_in[3], float *const hinv_out[3],
                               double *energy_dev, awk_stream_t stream);
 
+/* n iterations back to back (GreedyWarp::RunWarp's inner loop,
+ * GreedyWarp.txx:94-111, without the host between iterations): equal bit for
+ * bit to n calls of awk_greedy_iterate that alternate the two fields.
+ * Iteration i reads h_a and writes h_b when i is even, the other way round when
+ * it is odd, so the result is in h_b for odd n and in h_a for even n; the fields
+ * must not overlap.  energies_dev (n device doubles, may be NULL) receives each
+ * iteration's energy.  Once the step is known, each iteration's composition
+ * kernel also samples I0 at the new field, which is the next iteration's
+ * deformed image, so the apply kernel (and its read of the field) runs only in
+ * the first iteration.  No inverse field (awk_greedy_iterate computes Def0To1). */
+awk_status awk_greedy_iterate_n(awk_greedy_t g, const float *I0, const float *I1, float *const h_a[3],
+                                float *const h_b[3], int n, double *energies_dev,
+                                awk_stream_t stream);
+
 /* Sticky numerical flags since the last clear (AWK_FLAG_*); waits for stream;
  * clear != 0 resets them in stream order.  Returns AWK_ERR_NUMERIC on a NaN
  * energy or an invalid step.  Replaces the NaN and energy-increase checks of
@@ -286,17 +300,23 @@ awk_status awk_greedy_get_flags(awk_greedy_t g, unsigned *flags, int clear, awk_
  * this iteration's update (done on the device). */
 awk_status awk_greedy_set_auto_step_reduce(awk_greedy_t g, int on);
 
-/* Per-kernel timing of awk_greedy_iterate, for benchmarks: while on, each
- * stage's kernel is launched with start/stop events attached to its dispatch
- * (kernel duration only).  Stages of the fused steady-state iteration:
+/* Per-kernel timing of awk_greedy_iterate / awk_greedy_iterate_n, for
+ * benchmarks: while on, each stage's kernel is launched with start/stop events
+ * attached to its dispatch (kernel duration only).  Stages of the steady-state
+ * iteration:
  *   0 apply, 1 x pass + force + SSD partials, 2 y pass (+ energy reduction),
- *   3 z pass + solve, 4 inverse y pass, 5 inverse x pass + hfield update.
+ *   3 z pass + solve, 4 inverse y pass, 5 inverse x pass + hfield update,
+ *   6 inverse x pass + hfield update + the next iteration's deformed image
+ *     (awk_greedy_iterate_n, in place of 5 and of the next iteration's 0).
  * No counterpart in the reference (its GreedyIteratorGPU has no timers). */
-#define AWK_GREEDY_NSTAGES 6
+#define AWK_GREEDY_NSTAGES 7
 awk_status awk_greedy_set_profiling(awk_greedy_t g, int on);
 /* Sums the recorded stage times (ms) over all iterations since profiling was
- * switched on or last read (synchronises), and resets the record. */
-awk_status awk_greedy_get_profile(awk_greedy_t g, double *stage_ms, int *iterations);
+ * switched on or last read (synchronises), and resets the record.
+ * stage_launches (may be NULL) receives how many launches of each stage were
+ * timed; iterations the number of iterations recorded. */
+awk_status awk_greedy_get_profile(awk_greedy_t g, double *stage_ms, int *stage_launches,
+                                  int *iterations);
 
 /* ====================================================================== */
 /* Scale levels                                                             */

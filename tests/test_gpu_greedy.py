@@ -131,3 +131,64 @@ def test_inverse_x_pass_launch_variants_bit_identical():
     for h, e in runs[1:]:
         np.testing.assert_array_equal(h, runs[0][0])
         np.testing.assert_array_equal(e, runs[0][1])
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (8, 16, 256), (8, 24, 128), (12, 20, 67), (17, 9, 5)])
+def test_iterate_n_equals_single_iterations(shape):
+    """awk_greedy_iterate_n (each composition also forms the next deformed image)
+    gives the fields and energies of the same number of awk_greedy_iterate calls
+    bit for bit: an odd and an even count, called after the iteration that sets
+    the step or with that iteration inside the call, and calls of one iteration; at 256-long rows for every launch
+    variant of the inverse x pass (unroll, corner pairs)."""
+    import torch
+    from atlaswerks_amd import autotune
+    from atlaswerks_amd.greedy import GreedyDeformationDataGPU, GreedyIteratorGPU, GreedyIteratorParam
+    from gpu_util import to_dev
+    print("iterate_n", shape, flush=True)
+    t, target = _config0(shape, seed=5)
+    nz, ny, nx = shape
+    variants = [(1, 0), (2, 0), (4, 0), (1, 1), (2, 1), (4, 1)] if nx == 256 else [None]
+    saved = {k: autotune.get_knob(k) for k in ("xinv_unroll", "xinv_pairs")}
+
+    def run(schedule):
+        it = GreedyIteratorGPU((nx, ny, nz), (1.0, 1.0, 1.0), GreedyIteratorParam(max_pert=0.1))
+        d = GreedyDeformationDataGPU(to_dev(t), to_dev(target), max_iters=16)
+        it.UpdateStepSizeNextIteration()
+        for kind, k in schedule:
+            if kind == "one":
+                for _ in range(k):
+                    it.Iterate(d)
+            else:
+                it.IterateN(d, k)
+        torch.cuda.synchronize()
+        return d.Def1To0.cpu().numpy(), d.energies(), d.cur_iter
+
+    try:
+        for var in variants:
+            if var:
+                autotune.set_knob("xinv_unroll", var[0])
+                autotune.set_knob("xinv_pairs", var[1])
+            for n in (7, 4):
+                h0, e0, c0 = run([("one", n + 1)])
+                for sched in ([("one", 1), ("n", n)], [("n", n + 1)], [("n", 1), ("n", 1), ("n", n - 1)]):
+                    h, e, c = run(sched)
+                    assert c == c0 == n + 1
+                    np.testing.assert_array_equal(h, h0)
+                    np.testing.assert_array_equal(e, e0)
+    finally:
+        for k, v in saved.items():
+            autotune.set_knob(k, v)
+
+
+def test_iterate_n_rejects_overlapping_fields():
+    import ctypes
+    import torch
+    from atlaswerks_amd import _lib
+    from atlaswerks_amd.device import planes_arr, ptr
+    from atlaswerks_amd.greedy import GreedyIteratorGPU
+    it = GreedyIteratorGPU((8, 8, 8))
+    I = torch.zeros((8, 8, 8), dtype=torch.float32, device="cuda")
+    h = torch.zeros((3, 8, 8, 8), dtype=torch.float32, device="cuda")
+    rc = _lib.load().awk_greedy_iterate_n(it._h, ptr(I), ptr(I), planes_arr(h), planes_arr(h), 2,
+                                          ctypes.c_void_p(0), None)
+    assert rc != 0
