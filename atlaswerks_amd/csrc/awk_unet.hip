@@ -225,26 +225,33 @@ k_conv3_lds(ConvArgs g, int CC) {
   for (int m = 0; m < MT; ++m) wrow[m] = g.w + (size_t)(mbase + m * 16 + col) * g.kpad;
   for (int ch = 0; ch < nchunk; ++ch) {
     if (ch) __syncthreads();  // the previous chunk's reads are done
-    for (int w = threadIdx.x; w < kLdsHalo * groups; w += kConvBlock) {
-      const int hp = divg(w), gi = w - hp * groups;
+    // one halo position per thread and step: its coordinates, bounds test and
+    // addresses once, then its CC / 8 channel groups
+    for (int hp = threadIdx.x; hp < kLdsHalo; hp += kConvBlock) {
       const int hx = hp % kLdsHX, hr = hp / kLdsHX, hy = hr % kLdsHY, hz = hr / kLdsHY;
       const int x = x0 - 1 + hx, y = y0 - 1 + hy, z = z0 - 1 + hz;
-      bf16x8 v = zero8();
+      bf16 *dst = tile + hp * PS;
       if (x >= 0 && x < g.nx && y >= 0 && y < g.ny && z >= 0 && z < g.nz) {
-        const int cidx = ch * CC + gi * 8;
+        const size_t pos = vbase + ((size_t)z * g.ny + y) * g.nx + x;
         if (!g.xa) {
-          v = *reinterpret_cast<const bf16x8 *>(g.x + (vbase + ((size_t)z * g.ny + y) * g.nx + x) * g.C + cidx);
-        } else if (cidx < g.ca) {  // the upsampled coarse tensor
+          const bf16 *src = g.x + pos * g.C + ch * CC;
+          for (int gi = 0; gi < groups; ++gi)
+            *reinterpret_cast<bf16x8 *>(dst + gi * 8) = *reinterpret_cast<const bf16x8 *>(src + gi * 8);
+        } else {
+          // [the upsampled coarse tensor (ca channels), the skip tensor]
           const int hx2 = g.nx >> 1, hy2 = g.ny >> 1, hz2 = g.nz >> 1;
           const size_t q = (size_t)vol * hz2 * hy2 * hx2 + ((size_t)(z >> 1) * hy2 + (y >> 1)) * hx2 + (x >> 1);
-          v = *reinterpret_cast<const bf16x8 *>(g.xa + q * g.ca + cidx);
-        } else {                   // the skip tensor
-          const int cb = g.C - g.ca;
-          v = *reinterpret_cast<const bf16x8 *>(g.x + (vbase + ((size_t)z * g.ny + y) * g.nx + x) * cb +
-                                                (cidx - g.ca));
+          const bf16 *sa = g.xa + q * g.ca;
+          const bf16 *sb = g.x + pos * (g.C - g.ca);
+          for (int gi = 0; gi < groups; ++gi) {
+            const int cidx = ch * CC + gi * 8;
+            const bf16 *src = cidx < g.ca ? sa + cidx : sb + (cidx - g.ca);
+            *reinterpret_cast<bf16x8 *>(dst + gi * 8) = *reinterpret_cast<const bf16x8 *>(src);
+          }
         }
+      } else {
+        for (int gi = 0; gi < groups; ++gi) *reinterpret_cast<bf16x8 *>(dst + gi * 8) = zero8();
       }
-      *reinterpret_cast<bf16x8 *>(tile + hp * PS + gi * 8) = v;
     }
     __syncthreads();
     const int cw = ch * CC;
@@ -256,9 +263,26 @@ k_conv3_lds(ConvArgs g, int CC) {
       for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const bf16x8 *>(wrow[m] + cw + o.y);
     };
     if constexpr (!PIPE) {
+      // the weight fragments (global, L1/L2 hits) of the next step are requested
+      // before this step's MFMAs; the input fragments come from LDS within the step
+      bf16x8 an[MT];
+      {
+        const int oy = koff[grp].y;
+#pragma unroll
+        for (int m = 0; m < MT; ++m) an[m] = *reinterpret_cast<const bf16x8 *>(wrow[m] + cw + oy);
+      }
       for (int ks = 0; ks < ksteps; ++ks) {
         bf16x8 av[MT], bv[kLdsBY];
-        load(ks, av, bv);
+#pragma unroll
+        for (int m = 0; m < MT; ++m) av[m] = an[m];
+        if (ks + 1 < ksteps) {
+          const int oy = koff[(ks + 1) * 4 + grp].y;
+#pragma unroll
+          for (int m = 0; m < MT; ++m) an[m] = *reinterpret_cast<const bf16x8 *>(wrow[m] + cw + oy);
+        }
+        const int ox = koff[ks * 4 + grp].x;
+#pragma unroll
+        for (int t = 0; t < kLdsBY; ++t) bv[t] = *reinterpret_cast<const bf16x8 *>(tile + bpos[t] + ox);
 #pragma unroll
         for (int m = 0; m < MT; ++m)
 #pragma unroll
@@ -382,11 +406,18 @@ k_head(float *__restrict__ ux, float *__restrict__ uy, float *__restrict__ uz,
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= N) return;
   float a0 = b[0], a1 = b[1], a2 = b[2];
-  for (int c = 0; c < C; ++c) {
-    const float v = (float)d[(size_t)i * C + c];
-    a0 += w[c] * v;
-    a1 += w[C + c] * v;
-    a2 += w[2 * C + c] * v;
+  // the position's C channels (a multiple of 8) as 16-byte loads, summed in order
+  const bf16 *dp = d + (size_t)i * C;
+  for (int c0 = 0; c0 < C; c0 += 8) {
+    const bf16x8 v8 = *reinterpret_cast<const bf16x8 *>(dp + c0);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int c = c0 + j;
+      const float v = (float)v8[j];
+      a0 += w[c] * v;
+      a1 += w[C + c] * v;
+      a2 += w[2 * C + c] * v;
+    }
   }
   ux[i] = a0;
   uy[i] = a1;

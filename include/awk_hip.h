@@ -66,9 +66,12 @@ enum {
 
 /* Launch-shape knobs chosen by the autotuner (atlaswerks_amd/autotune.py):
  * "apply_vpt", "compose_vpt" (voxels per thread: 1, 2, 4), "xinv_unroll"
- * (voxels per step of the fused inverse-x epilogue: 1, 2, 4), "splat_tile"
- * (0 or 1: the fixed-point splat through per-block LDS windows).  Process-wide;
- * results do not depend on them.  No counterpart in the reference. */
+ * (voxels per step of the fused inverse-x epilogue: 1, 2, 4), "xinv_pairs" (0/1),
+ * "splat_tile" (0 or 1: the fixed-point splat through per-block LDS windows),
+ * U-Net convolutions "unet_lds", "unet_pipe", "unet_cc"; results do
+ * not depend on them.  FFT plan knobs read when an operator is created:
+ * "fft_order", "x_rt", "y_tx", "z_tx" (another butterfly order or tile, results
+ * within FP32 rounding).  Process-wide.  No counterpart in the reference. */
 awk_status awk_tune_set(const char *name, int value);
 awk_status awk_tune_get(const char *name, int *value);
 
