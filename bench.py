@@ -5,7 +5,9 @@ apply, gradient/residual/force/SSD, FluidKernelFFT L^-1, step-scaled composeHV)
 of a 256^3 float32 pairwise registration, inputs resident in HBM.  The K timed
 iterations run back to back as GreedyWarp's inner loop does (one call of
 awk_greedy_iterate_n: from the second iteration on, each composition also forms
-the next iteration's deformed image, bit-identical to separate iterations).  Prints one
+the next iteration's deformed image, bit-identical to separate iterations); the
+same K iterations as K calls of the per-iteration entry point (awk_greedy_iterate)
+are reported beside them ("per_iteration_calls").  Prints one
 JSON line with the iteration rate, the roofline of the dominant kernel (measured
 live with HIP events on the iteration's stream), per-stage times, the kernel
 microbenchmarks of the 256^3 workload (L^-1, apply, compose, splat) and the CPU
@@ -400,8 +402,9 @@ def main():
     it.Iterate(data)  # sets the step (MaxPert / max |v|)
     it.IterateN(data, args.warmup)
     torch.cuda.synchronize()
-    # timed region: K iterations back to back (GreedyWarp's inner loop), nothing
-    # else on the stream
+    # timed region: K iterations back to back in one awk_greedy_iterate_n call
+    # (GreedyWarp's inner loop: from the second iteration on each composition also
+    # forms the next deformed image), nothing else on the stream
     ts = time.perf_counter()
     it.IterateN(data, args.steps)
     torch.cuda.synchronize()
@@ -416,6 +419,20 @@ def main():
     it.SetProfiling(False)
     ms_per_step = 1e3 * elapsed / args.steps
     iters_per_s = args.steps / elapsed
+    # the same K iterations as K calls of the per-iteration entry point
+    # (awk_greedy_iterate, the reference's GreedyIteratorGPU::Iterate), and their
+    # kernel attribution
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        it.Iterate(data)
+    torch.cuda.synchronize()
+    elapsed_1 = time.perf_counter() - ts
+    it.SetProfiling(True)
+    for _ in range(n_att):
+        it.Iterate(data)
+    prof_1, n_prof_1, launches_1 = it.GetProfile(launches=True)
+    it.SetProfiling(False)
 
     # per iteration: each stage's share; per launch: the kernel's mean duration
     stage_avg = {k: v / max(n_prof, 1) for k, v in prof.items()}
@@ -423,17 +440,23 @@ def main():
     sb = stage_bytes(nx, ny, nz)
     dom = max(stage_avg, key=lambda k: stage_avg[k])
     achieved = sb[dom] / (launch_avg[dom] * 1e-3) / 1e9
+    stage_avg_1 = {k: v / max(n_prof_1, 1) for k, v in prof_1.items()}
+    launch_avg_1 = {k: prof_1[k] / launches_1[k] for k in prof_1 if launches_1[k]}
+    dom_1 = max(stage_avg_1, key=lambda k: stage_avg_1[k])
+    achieved_1 = sb[dom_1] / (launch_avg_1[dom_1] * 1e-3) / 1e9
     # HBM bytes per launch of the dominant kernel, measured by rocprofv3 --pmc
     # FETCH_SIZE / WRITE_SIZE (scaled by the copy calibration) and recorded in
     # profiles/pmc_traffic_<n>.txt; null when no measurement exists for this size.
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "pmc_traffic_%d.txt" % n)
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            for line in f:
-                parts = line.split()
-                if len(parts) == 4 and parts[0] == dom:
-                    traffic = int(parts[2]) + int(parts[3])
+    def traffic_of(stage):
+        tpath = os.path.join(ROOT, "profiles", "pmc_traffic_%d.txt" % n)
+        if os.path.exists(tpath):
+            with open(tpath) as f:
+                for line in f:
+                    parts = line.split()
+                    if len(parts) == 4 and parts[0] == stage:
+                        return int(parts[2]) + int(parts[3])
+        return None
+    traffic = traffic_of(dom)
     yard = 196 * N  # fixed yardstick bytes of one greedy iteration
     out = {
         "metric": "greedy-fluid iter/s @256^3 (% HBM peak); time-to-SSD plain vs warm-start",
@@ -471,6 +494,16 @@ def main():
         "stages_launches": {k: launches[k] for k in launches},
         "stages_sum_ms": round(sum(stage_avg.values()), 5),
         "stages_gbs": {k: round(sb[k] / (v * 1e-3) / 1e9, 1) for k, v in launch_avg.items() if sb[k]},
+        "per_iteration_calls": {
+            "entry_point": "awk_greedy_iterate",
+            "iter_per_s": round(args.steps / elapsed_1, 3),
+            "ms_per_step": round(1e3 * elapsed_1 / args.steps, 4),
+            "roofline": {"bound": "hbm", "kernel": dom_1, "achieved": round(achieved_1, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved_1 / HBM_PEAK_GBS, 4),
+                         "traffic": traffic_of(dom_1), "alg_bytes_per_launch": sb[dom_1],
+                         "avg_launch_ms": round(launch_avg_1[dom_1], 5)},
+            "stages_ms": {k: round(v, 5) for k, v in stage_avg_1.items() if launches_1[k]},
+        },
         "energy_first_last": [float(energies[0]), float(energies[-1])],
         "input_gen_s": round(gen_s, 2),
         "tuning": tuning,
