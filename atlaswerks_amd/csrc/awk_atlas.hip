@@ -5,8 +5,8 @@
 // GreedyAtlasThreadGPU.cxx:161-206).  Every subject's hfield lives on the template
 // (mean) grid and holds coordinates in that subject's own image grid, so subject
 // images may have different sizes.  One atlas iteration:
-//   A  Idef_s = I0_s(h_s) for every subject                    (gather, per subject)
-//   M  mean = sum_s w_s Idef_s in subject order                 (one kernel)
+//   A  Idef_s = I0_s(h_s) for every subject                    (gather on s's grid)
+//   M  mean = sum_s w_s Idef_s in subject order                 (A and M: one kernel)
 //   B  per subject the greedy step against I1 = mean: force + FFT passes + Navier
 //      solve + composeHV in the inverse x pass, energy E_s on the device
 //   E  energy = sum_s w_s E_s (FP64, subject order) into a device history
@@ -39,8 +39,69 @@ k_atlas_mean(float *__restrict__ mean, const float *const *__restrict__ idef,
   const int i = (int)(xcd_block() * 256 + threadIdx.x);
   if (i >= N) return;
   // GreedyAtlasThreadGPU::ComputeThreadMean: mean = 0; mean += Idef_s * w_s, s in order
+  // (one explicit fma per subject, as in k_atlas_apply_mean)
   float m = 0.f;
-  for (int s = 0; s < ns; ++s) m = m + idef[s][i] * w[s];
+  for (int s = 0; s < ns; ++s) m = __builtin_fmaf(idef[s][i], w[s], m);
+  mean[i] = m;
+}
+
+// One subject's image as the fused mean stage samples it.
+struct AtlasSrc {
+  const float *i0;  // image on the subject's own grid
+  int nx, ny, nz;
+  float w;          // the subject's weight in the mean
+};
+
+// Stages A and M in one pass over the template grid: for every subject s in
+// order, Idef_s = I0_s(h_s) (trilinear, background 0; the gather of k_apply) and
+// m = fma(Idef_s, w_s, m), then mean = m.  Same samples and the same summation order
+// as one apply launch per subject followed by k_atlas_mean, so the results are bit
+// for bit those; the mean pass's re-read of every Idef_s and ns - 1 launches go.
+// h_s and Idef_s lie at fixed strides (3N and N floats) in the atlas slabs.  SU
+// subjects per group: their field loads are all issued before the gathers.
+template <int SU>
+__global__ void __launch_bounds__(256)
+k_atlas_apply_mean(float *__restrict__ mean, float *__restrict__ idef, const float *__restrict__ h,
+                   const AtlasSrc *__restrict__ tab, int ns, int N) {
+  const int i = (int)(xcd_block() * 256 + threadIdx.x);
+  if (i >= N) return;
+  const size_t n = (size_t)N;
+  float m = 0.f;
+  int s = 0;
+  for (; s + SU <= ns; s += SU) {
+    float px[SU], py[SU], pz[SU];
+#pragma unroll
+    for (int u = 0; u < SU; ++u) {
+      // each field is read once: streamed past the memory-side cache
+      const float *hs = h + (size_t)(s + u) * 3 * n + i;
+      px[u] = __builtin_nontemporal_load(hs);
+      py[u] = __builtin_nontemporal_load(hs + n);
+      pz[u] = __builtin_nontemporal_load(hs + 2 * n);
+    }
+    float r[SU];
+#pragma unroll
+    for (int u = 0; u < SU; ++u) {
+      const AtlasSrc t = tab[s + u];
+      const BufLoad ib = make_buf(t.i0, t.nx * t.ny * t.nz);
+      r[u] = trilerp1g<AWK_BG_VAL>(ib, Dims{t.nx, t.ny, t.nz}, px[u], py[u], pz[u], 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < SU; ++u) {
+      idef[(size_t)(s + u) * n + i] = r[u];
+      m = __builtin_fmaf(r[u], tab[s + u].w, m);
+    }
+  }
+  for (; s < ns; ++s) {
+    const float *hs = h + (size_t)s * 3 * n + i;
+    const float px = __builtin_nontemporal_load(hs);
+    const float py = __builtin_nontemporal_load(hs + n);
+    const float pz = __builtin_nontemporal_load(hs + 2 * n);
+    const AtlasSrc t = tab[s];
+    const BufLoad ib = make_buf(t.i0, t.nx * t.ny * t.nz);
+    const float r = trilerp1g<AWK_BG_VAL>(ib, Dims{t.nx, t.ny, t.nz}, px, py, pz, 0.f);
+    idef[(size_t)s * n + i] = r;
+    m = __builtin_fmaf(r, t.w, m);
+  }
   mean[i] = m;
 }
 
@@ -99,6 +160,7 @@ struct awk_atlas {
   float *mean = nullptr;
   const float **idef_tab = nullptr;  // device table of Idef pointers
   float *w_dev = nullptr;
+  awk::AtlasSrc *src_tab = nullptr;  // device table of the subjects' images and weights
   double *esub = nullptr;
   double *partials = nullptr;
   float *maxpart = nullptr;
@@ -143,7 +205,7 @@ static void atlas_free(awk_atlas *a) {
     for (void *p : b)
       if (p) (void)hipFree(p);
   }
-  void *b[] = {a->mean, (void *)a->idef_tab, a->w_dev, a->esub, a->partials, a->maxpart,
+  void *b[] = {a->mean, (void *)a->idef_tab, a->w_dev, a->src_tab, a->esub, a->partials, a->maxpart,
                a->hist, a->cursor, a->vbuf, a->flags, a->hslab[0], a->hslab[1], a->idef_slab,
                a->step_slab, a->prev_slab, a->have_prev_slab, a->bspec, a->bpart};
   for (void *p : b)
@@ -161,10 +223,25 @@ static void atlas_free(awk_atlas *a) {
   if (a->as) (void)hipStreamDestroy(a->as);
 }
 
-// Stage A + M: Idef_s = I0_s(h_s), mean = sum_s w_s Idef_s.
+// Stage A + M: Idef_s = I0_s(h_s), mean = sum_s w_s Idef_s.  One fused launch
+// (knob atlas_mean = subjects per group, 1, 2 or 4), or with atlas_mean = 0 one
+// apply launch per subject and then the mean; the results are the same bits.
 static awk_status atlas_mean_stage(awk_atlas *a, hipStream_t s) {
   awk_status st;
   const int N = a->nx * a->ny * a->nz;
+  const int su = tuning().atlas_mean;
+  if (su) {
+    const dim3 grid(grid_for(N, 256)), block(256);
+    const float *h = a->hslab[a->cur];
+    if (su == 4)
+      AWK_LAUNCH(k_atlas_apply_mean<4>, grid, block, 0, s, a->mean, a->idef_slab, h, a->src_tab, a->ns, N);
+    else if (su == 2)
+      AWK_LAUNCH(k_atlas_apply_mean<2>, grid, block, 0, s, a->mean, a->idef_slab, h, a->src_tab, a->ns, N);
+    else
+      AWK_LAUNCH(k_atlas_apply_mean<1>, grid, block, 0, s, a->mean, a->idef_slab, h, a->src_tab, a->ns, N);
+    AWK_LAUNCH_CHECK("k_atlas_apply_mean");
+    return AWK_OK;
+  }
   for (Subject &sb : a->subj) {
     const float *h = sb.h[a->cur];
     if ((st = launch_apply_grid(sb.idef, sb.I0, sb.dims[0], sb.dims[1], sb.dims[2], h, h + N,
@@ -336,12 +413,16 @@ static awk_status atlas_sync_tables(awk_atlas *a) {
     if (!a->subj[i].set) return fail(AWK_ERR_INVALID, "awk_atlas: subject " + std::to_string(i) + " not set");
   std::vector<const float *> tab(a->ns);
   std::vector<float> w(a->ns);
+  std::vector<AtlasSrc> src(a->ns);
   for (int i = 0; i < a->ns; ++i) {
-    tab[i] = a->subj[i].idef;
-    w[i] = a->subj[i].w;
+    const Subject &sb = a->subj[i];
+    tab[i] = sb.idef;
+    w[i] = sb.w;
+    src[i] = AtlasSrc{sb.I0, sb.dims[0], sb.dims[1], sb.dims[2], sb.w};
   }
   AWK_HIP(hipMemcpyAsync((void *)a->idef_tab, tab.data(), a->ns * sizeof(float *), hipMemcpyHostToDevice, a->as));
   AWK_HIP(hipMemcpyAsync(a->w_dev, w.data(), a->ns * sizeof(float), hipMemcpyHostToDevice, a->as));
+  AWK_HIP(hipMemcpyAsync(a->src_tab, src.data(), a->ns * sizeof(AtlasSrc), hipMemcpyHostToDevice, a->as));
   AWK_HIP(hipStreamSynchronize(a->as));
   a->tables_dirty = false;
   return AWK_OK;
@@ -415,6 +496,7 @@ awk_status awk_atlas_create(awk_atlas_t *out, int nx, int ny, int nz, float spx,
   if (e == hipSuccess) e = hipMalloc(&a->mean, N * sizeof(float));
   if (e == hipSuccess) e = hipMalloc((void **)&a->idef_tab, n_subjects * sizeof(float *));
   if (e == hipSuccess) e = hipMalloc(&a->w_dev, n_subjects * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&a->src_tab, n_subjects * sizeof(AtlasSrc));
   if (e == hipSuccess) e = hipMalloc(&a->esub, n_subjects * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&a->partials, (size_t)nblk * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&a->maxpart, (size_t)nblk * sizeof(float));

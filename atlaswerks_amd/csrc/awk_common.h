@@ -159,6 +159,8 @@ struct Tuning {
   int compose_vpt = 1;  // voxels per thread, compose family
   int xinv_unroll = 1;  // voxels per epilogue step of the inverse x pass
   int xinv_pairs = 0;   // inverse x pass: x-neighbour corners as 8-byte pairs inside the volume
+  int atlas_mean = 4;   // greedy atlas deformed images + mean: subjects per group of one fused
+                        // launch (1, 2, 4), or 0: one apply launch per subject, then the mean
   int splat_tile = 1;   // fixed-point splat through per-block LDS windows (0: direct atomics)
   int unet_lds = 1;     // U-Net convolutions on grids >= 32^3 through LDS-staged input bricks
   int unet_pipe = 2;    // ... K steps' fragment loads software-pipelined: 0 never, 1 always, 2 CC >= 32

@@ -376,6 +376,12 @@ awk_status awk_tune_set(const char *name, int value) {
     t.unet_cc = value;
     return AWK_OK;
   }
+  if (n == "atlas_mean") {
+    if (value != 0 && value != 1 && value != 2 && value != 4)
+      return fail(AWK_ERR_INVALID, "awk_tune_set: atlas_mean must be 0, 1, 2 or 4");
+    t.atlas_mean = value;
+    return AWK_OK;
+  }
   if (n == "unet_pipe") {
     if (value < 0 || value > 2) return fail(AWK_ERR_INVALID, "awk_tune_set: unet_pipe must be 0, 1 or 2");
     t.unet_pipe = value;
@@ -415,6 +421,7 @@ awk_status awk_tune_get(const char *name, int *value) {
   else if (n == "xinv_pairs") *value = t.xinv_pairs;
   else if (n == "unet_pipe") *value = t.unet_pipe;
   else if (n == "unet_cc") *value = t.unet_cc;
+  else if (n == "atlas_mean") *value = t.atlas_mean;
   else if (n == "x_rt") *value = t.x_rt;
   else if (n == "y_tx") *value = t.y_tx;
   else if (n == "z_tx") *value = t.z_tx;

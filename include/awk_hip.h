@@ -68,8 +68,10 @@ enum {
  * "apply_vpt", "compose_vpt" (voxels per thread: 1, 2, 4), "xinv_unroll"
  * (voxels per step of the fused inverse-x epilogue: 1, 2, 4), "xinv_pairs" (0/1),
  * "splat_tile" (0 or 1: the fixed-point splat through per-block LDS windows),
- * U-Net convolutions "unet_lds", "unet_pipe", "unet_cc"; results do
- * not depend on them.  FFT plan knobs read when an operator is created:
+ * U-Net convolutions "unet_lds", "unet_pipe", "unet_cc", the greedy atlas's
+ * deformed images and mean "atlas_mean" (0: one launch per subject; 1, 2, 4:
+ * subjects per group of one fused launch; read when an atlas iteration is
+ * launched or captured); results do not depend on them.  FFT plan knobs read when an operator is created:
  * "fft_order", "x_rt", "y_tx", "z_tx" (another butterfly order or tile, results
  * within FP32 rounding).  Process-wide.  No counterpart in the reference. */
 awk_status awk_tune_set(const char *name, int value);

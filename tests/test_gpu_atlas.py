@@ -264,3 +264,59 @@ def test_atlas_batched_uneven_grids_bit_identical():
     for a, c in zip(out[0][0], out[1][0]):
         np.testing.assert_array_equal(a, c)
     np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("grids", ["same", "uneven"])
+def test_atlas_fused_mean_stage_bit_identical(grids):
+    """The deformed images and the mean in one launch (knob atlas_mean = 1, 2 or 4
+    subjects per group; 5 subjects, so groups of 2 and 4 leave a remainder) against
+    one apply launch per subject followed by the mean (atlas_mean = 0): hfields,
+    mean, every deformed image and the energies bit for bit over 5 iterations,
+    eager and through hipGraph replays."""
+    import torch
+    from atlaswerks_amd import atlas, autotune
+    from gpu_util import to_dev
+    print("fused atlas mean", grids, flush=True)
+    if grids == "same":
+        shape = (20, 24, 28)
+        ims = _subjects(shape, 5, seed=140)
+        subs = [atlas.AtlasSubject(to_dev(im), weight=w) for im, w in zip(ims, [0.3, 0.1, 0.2, 0.25, 0.15])]
+        kw = dict(max_pert=0.2)
+        nz, ny, nx = shape
+        tgrid = (nx, ny, nz)
+    else:
+        shps = [((32, 32, 32), (1.0, 1.0, 1.0)), ((24, 24, 24), (4 / 3, 4 / 3, 4 / 3)),
+                ((20, 28, 16), (1.6, 8 / 7, 2.0)), ((16, 16, 16), (2.0, 2.0, 2.0)),
+                ((30, 18, 22), (16 / 15, 16 / 9, 16 / 11))]
+        base = synth.blobs((32, 32, 32), seed=151)
+        subs = []
+        for k, (shp, sp) in enumerate(shps):
+            nz, ny, nx = shp
+            z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+            pts = np.stack([x * sp[0], y * sp[1], z * sp[2]], -1)
+            u = np.moveaxis(synth.sinusoidal_displacement(shp, max_disp=1.0, seed=161 + k), 0, -1)
+            subs.append(atlas.AtlasSubject(to_dev(oracle.apply(base, pts + u, dtype=np.float32)), spacing=sp))
+        kw = dict(template_spacing=(2.0, 2.0, 2.0), max_pert=0.2)
+        tgrid = (16, 16, 16)
+    saved = autotune.get_knob("atlas_mean")
+
+    def run(su, use_graph):
+        autotune.set_knob("atlas_mean", su)
+        b = atlas.GreedyAtlasBuilderGPU(subs, tgrid, use_graph=use_graph, **kw)
+        b.UpdateStepSizeNextIteration()
+        b.Iterate(5)
+        torch.cuda.synchronize()
+        f = [b.HField(i).cpu().numpy() for i in range(5)] + [b.Mean().cpu().numpy()]
+        f += [b.DeformedImage(i).cpu().numpy() for i in range(5)]
+        return f, b.energies()
+
+    try:
+        ref_f, ref_e = run(0, False)
+        for su in (1, 2, 4):
+            for use_graph in (False, True):
+                f, e = run(su, use_graph)
+                for a, c in zip(f, ref_f):
+                    np.testing.assert_array_equal(a, c)
+                np.testing.assert_array_equal(e, ref_e)
+    finally:
+        autotune.set_knob("atlas_mean", saved)
