@@ -86,6 +86,8 @@ struct XInvArgs {
   const float *i0;          // COMPOSE: when set, also idef = I0(hnew) (the next
   float *idef;              //   iteration's deformed image; VAL background i0_bg,
   float i0_bg;              //   I0 on the field's grid; one subject per launch)
+  int band;                 // COMPOSE: row groups per band of the block order (0: plane
+                            //   order); set by fk_xinv from the xinv_band knob
 };
 
 // Subjects of one batched launch: blockIdx.y selects the subject, and each of its

@@ -346,8 +346,10 @@ k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
 
 // X inverse: c2r of RT rows (3 components) with an epilogue.  APPLY (COMPOSE
 // only): the epilogue also samples I0 at the new field (a.i0 -> a.idef), the next
-// iteration's deformed image.
-template <int NC, int RTC, int BDC, int XU, bool PAIRS = false, bool APPLY = false>
+// iteration's deformed image.  STG (COMPOSE only; NC, RTC, BDC set, ny % RTC == 0):
+// the composition reads the current field from the block's rows and their
+// neighbours staged in LDS.
+template <int NC, int RTC, int BDC, int XU, bool PAIRS = false, bool APPLY = false, bool STG = false>
 __global__ void __launch_bounds__(BDC ? BDC : 1024)
 k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, int RT_rt,
        const float2 *__restrict__ spec, size_t plane, Batch bt) {
@@ -369,7 +371,18 @@ k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
   const int NL = 3 * RT / 2, LS = NL + 1;
   const int NR = ny * nz;
   const int lb = (int)xcd_block();
-  const int r0 = lb * RT;
+  int g = lb;
+  if (a.band) {
+    // band order (the host checked that whole planes of G row groups fill each
+    // XCD's share): an XCD takes its z-planes band by band, a band being
+    // a.band adjacent row groups, so its resident blocks cover a narrow y range
+    // over many z-planes and their gathers' footprint shrinks in its L2
+    const int G = ny / RT, per = (int)gridDim.x / kNumXcd, Zc = per / G, nbb = Zc * a.band;
+    const int c = lb / per, j = lb - c * per;
+    const int bnd = j / nbb, rem = j - bnd * nbb;
+    g = (c * Zc + rem / a.band) * G + bnd * a.band + rem % a.band;
+  }
+  const int r0 = g * RT;
   if (r0 >= NR) {
     // a padding block of the grid: its max partial is still read
     if ((a.mode == XSTORE_MAXNORM || a.mode == XSTORE_MAXAXPY) && threadIdx.x == 0) a.maxpart[lb] = 0.f;
@@ -403,6 +416,87 @@ k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
     const float isx = 1.f / a.sp[0], isy = 1.f / a.sp[1], isz = 1.f / a.sp[2];
     const int ntot = nx * ny * nz;
     const BufLoad hx = make_buf(a.h[0], ntot), hy = make_buf(a.h[1], ntot), hz = make_buf(a.h[2], ntot);
+    if constexpr (STG) {
+      // The block's RT rows are y = yb .. yb + RT - 1 of plane zr.  A point whose 8
+      // corners lie inside the volume, in rows yb - 1 .. yb + RT of planes zr - 1 ..
+      // zr + 1 (the common case: the step moves points by less than a voxel), reads
+      // them from those rows staged in LDS, one component at a time; any other
+      // point gathers from memory.  The same corners and the same lerp8 as the
+      // gathers, so the same results bit for bit.
+      constexpr int VP = RTC * NC / BDC;  // voxels per thread
+      constexpr int WR = RTC + 2;         // staged rows per plane
+      const int zr = r0 / ny, yb = r0 - zr * ny;
+      float px[VP], py[VP], pz[VP];
+      int win[VP];  // LDS offset of corner c0 in the staged rows (-1: gather from memory)
+#pragma unroll
+      for (int u = 0; u < VP; ++u) {
+        const int w = threadIdx.x + u * BDC;
+        const int q = w / NC, x = w - q * NC;
+        const int p = q >> 1;
+        float v0, v1, v2;
+        if ((q & 1) == 0) {
+          const float2 t = sm[x * LS + 3 * p];
+          v0 = t.x; v1 = t.y; v2 = sm[x * LS + 3 * p + 1].x;
+        } else {
+          const float2 t = sm[x * LS + 3 * p + 2];
+          v0 = t.x; v1 = t.y; v2 = sm[x * LS + 3 * p + 1].y;
+        }
+        px[u] = fmaf(v0 * st, isx, (float)x);
+        py[u] = fmaf(v1 * st, isy, (float)(yb + q));
+        pz[u] = fmaf(v2 * st, isz, (float)zr);
+        const int x0 = (int)floorf(px[u]), y0 = (int)floorf(py[u]), z0 = (int)floorf(pz[u]);
+        const bool ok = x0 >= 0 && x0 < nx - 1 && y0 >= 0 && y0 < ny - 1 && z0 >= 0 && z0 < nz - 1 &&
+                        y0 >= yb - 1 && y0 <= yb + RTC - 1 && z0 >= zr - 1 && z0 <= zr;
+        win[u] = ok ? ((z0 - zr + 1) * WR + (y0 - yb + 1)) * NC + x0 : -1;
+      }
+      __syncthreads();  // every read of the transformed rows is done: sm is reused
+      float hc[VP][3];
+#pragma unroll
+      for (int u = 0; u < VP; ++u)
+        if (win[u] < 0) {
+          const float3 g = trilerp3g<AWK_BG_PARTIAL_ID, BufLoad, false>(hx, hy, hz, d, px[u], py[u], pz[u]);
+          hc[u][0] = g.x; hc[u][1] = g.y; hc[u][2] = g.z;
+        }
+      float *stg = reinterpret_cast<float *>(sm);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (c) __syncthreads();  // the previous component's reads are done
+        // rows outside the volume are not staged: no point in the window reads them
+        const float *hs = a.h[c];
+        for (int e = threadIdx.x; e < 3 * WR * (NC / 4); e += BDC) {
+          const int row = e / (NC / 4), x4 = e - row * (NC / 4);
+          const int zz = zr - 1 + row / WR, yy = yb - 1 + row % WR;
+          if (zz >= 0 && zz < nz && yy >= 0 && yy < ny)
+            *reinterpret_cast<float4 *>(stg + row * NC + 4 * x4) =
+                *reinterpret_cast<const float4 *>(hs + ((size_t)zz * ny + yy) * NC + 4 * x4);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < VP; ++u)
+          if (win[u] >= 0) {
+            const float *b = stg + win[u];
+            const float t = px[u] - floorf(px[u]), uu = py[u] - floorf(py[u]), vv = pz[u] - floorf(pz[u]);
+            hc[u][c] = lerp8(b[0], b[1], b[NC + 1], b[NC], b[WR * NC + NC], b[WR * NC + NC + 1],
+                             b[WR * NC + 1], b[WR * NC], t, uu, vv);
+          }
+      }
+#pragma unroll
+      for (int u = 0; u < VP; ++u) {
+        // rows r0 .. r0 + RT - 1 are contiguous in memory
+        const size_t base = (size_t)r0 * NC + threadIdx.x + u * BDC;
+        __builtin_nontemporal_store(hc[u][0], a.hnew[0] + base);
+        __builtin_nontemporal_store(hc[u][1], a.hnew[1] + base);
+        __builtin_nontemporal_store(hc[u][2], a.hnew[2] + base);
+      }
+      if constexpr (APPLY) {
+        const BufLoad ib = make_buf(a.i0, ntot);
+#pragma unroll
+        for (int u = 0; u < VP; ++u) {
+          const size_t base = (size_t)r0 * NC + threadIdx.x + u * BDC;
+          a.idef[base] = trilerp1g<AWK_BG_VAL>(ib, d, hc[u][0], hc[u][1], hc[u][2], a.i0_bg);
+        }
+      }
+    } else {
 #pragma unroll
     for (int w0 = threadIdx.x; w0 < RT * nx; w0 += bdim * XU) {
       float px[XU], py[XU], pz[XU];
@@ -456,6 +550,7 @@ k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
         for (int u = 0; u < XU; ++u)
           if (row_[u] < NR) a.idef[(size_t)row_[u] * nx + x_[u]] = r[u];
       }
+    }
     }
   } else {
     // LDMM coefficients, in float as the reference forms them
@@ -1109,14 +1204,28 @@ awk_status fk_zsolve(FluidKernel &k, bool inverse_op, hipStream_t s, const Batch
   return AWK_OK;
 }
 
-awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblocks, const Batch *b) {
+awk_status fk_xinv(FluidKernel &k, const XInvArgs &a_in, hipStream_t s, int *nblocks, const Batch *b) {
   const Batch bt = b ? *b : Batch();
   const unsigned grid = grid_for((long long)k.ny * k.nz, k.rt);
-  const size_t lds = x_lds(k.ax[0], k.rt);
+  XInvArgs a = a_in;
+  a.band = 0;
+  {
+    // the band block order of the composition, where whole planes of row groups
+    // fill each XCD's share of the grid and the band divides a plane
+    const int bd = tuning().xinv_band;
+    const long long G = k.ny / k.rt, per = grid / kNumXcd;
+    if (a.mode == XSTORE_COMPOSE && bd > 0 && k.ny % k.rt == 0 && G * k.nz == (long long)grid &&
+        per % G == 0 && G % bd == 0)
+      a.band = bd;
+  }
+  size_t lds = x_lds(k.ax[0], k.rt);
   const Axis ax = k.ax[0].dev();
 #define AWK_XI3(NC, RTC, BDC, XU, PR, AP)                                                        \
   AWK_LAUNCH((k_xinv<NC, RTC, BDC, XU, PR, AP>), dim3(grid, bt.n), dim3(BDC ? BDC : k.bd_x), lds, s, a, \
                      ax, k.nx, k.ny, k.nz, k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane(), bt)
+#define AWK_XIS(NC, BD, AP)                                                                      \
+  AWK_LAUNCH((k_xinv<NC, 8, BD, 1, false, AP, true>), dim3(grid, bt.n), dim3(BD), lds, s, a, ax, \
+                     k.nx, k.ny, k.nz, k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane(), bt)
 #define AWK_XI(NC, RTC, BDC, XU)                                                                 \
   AWK_LAUNCH((k_xinv<NC, RTC, BDC, XU>), dim3(grid, bt.n), dim3(BDC ? BDC : k.bd_x), lds, s, a, ax, \
                      k.nx, k.ny, k.nz, k.nxh, k.nxh_pad, k.rt, k.spec, k.spec_plane(), bt)
@@ -1129,7 +1238,26 @@ awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblock
   const bool ap = a.mode == XSTORE_COMPOSE && a.i0;
   if (ap && (bt.n != 1 || !a.idef))
     return fail(AWK_ERR_INVALID, "inverse x pass: the fused apply takes one subject and an output image");
-  if (ap) {
+  // xinv_stage: the composition reads the field from the block's rows and their
+  // neighbours staged in LDS (specialised row lengths, whole blocks in one plane).
+  // Measured on the greedy step: 128^3 atlas iteration 1.52 -> 1.35 ms, but at
+  // 256-long rows 0.26 -> 0.36 ms per composition (three staged rows per row read
+  // through L2, with 4 blocks per CU; 0.30 ms with 512 threads of 4 points each,
+  // whose FFT rounds differently), so by default rows up to 128 only.
+  const int sk = tuning().xinv_stage;
+  const bool stg = a.mode == XSTORE_COMPOSE && (sk == 1 || (sk == 2 && k.nx <= 128)) && k.rt == 8 &&
+                   k.ny % 8 == 0 && (k.nx == 256 || k.nx == 128 || k.nx == 64);
+  if (stg) {
+    lds = std::max(lds, (size_t)3 * (k.rt + 2) * k.nx * sizeof(float));
+    if (k.nx == 256) {
+      if (ap) AWK_XIS(256, 256, true);
+      else AWK_XIS(256, 256, false);
+    } else if (k.nx == 128) {
+      if (ap) AWK_XIS(128, 256, true);
+      else AWK_XIS(128, 256, false);
+    } else if (ap) AWK_XIS(64, 256, true);
+    else AWK_XIS(64, 256, false);
+  } else if (ap) {
     if (k.nx == 256 && k.rt == 8) {
       if (pr) {
         if (xu == 4) AWK_XI3(256, 8, 256, 4, true, true);
@@ -1154,6 +1282,7 @@ awk_status fk_xinv(FluidKernel &k, const XInvArgs &a, hipStream_t s, int *nblock
   else AWK_XI(0, 0, 0, 1);
 #undef AWK_XI
 #undef AWK_XI3
+#undef AWK_XIS
   AWK_LAUNCH_CHECK("k_xinv");
   if (nblocks) *nblocks = (int)grid;
   return AWK_OK;

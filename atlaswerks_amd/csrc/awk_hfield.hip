@@ -376,6 +376,12 @@ awk_status awk_tune_set(const char *name, int value) {
     t.unet_cc = value;
     return AWK_OK;
   }
+  if (n == "xinv_band") {
+    if (value != 0 && value != 2 && value != 4 && value != 8 && value != 16 && value != 32)
+      return fail(AWK_ERR_INVALID, "awk_tune_set: xinv_band must be 0, 2, 4, 8, 16 or 32");
+    t.xinv_band = value;
+    return AWK_OK;
+  }
   if (n == "atlas_mean") {
     if (value != 0 && value != 1 && value != 2 && value != 4)
       return fail(AWK_ERR_INVALID, "awk_tune_set: atlas_mean must be 0, 1, 2 or 4");
@@ -391,6 +397,11 @@ awk_status awk_tune_set(const char *name, int value) {
     if (value != 0 && value != 1) return fail(AWK_ERR_INVALID, "awk_tune_set: " + n + " must be 0 or 1");
     (n == "splat_tile" ? t.splat_tile : n == "fft_order" ? t.fft_order : n == "unet_lds" ? t.unet_lds
                                                                                           : t.xinv_pairs) = value;
+    return AWK_OK;
+  }
+  if (n == "xinv_stage") {
+    if (value < 0 || value > 2) return fail(AWK_ERR_INVALID, "awk_tune_set: xinv_stage must be 0, 1 or 2");
+    t.xinv_stage = value;
     return AWK_OK;
   }
   if (n == "x_rt") {
@@ -422,6 +433,8 @@ awk_status awk_tune_get(const char *name, int *value) {
   else if (n == "unet_pipe") *value = t.unet_pipe;
   else if (n == "unet_cc") *value = t.unet_cc;
   else if (n == "atlas_mean") *value = t.atlas_mean;
+  else if (n == "xinv_band") *value = t.xinv_band;
+  else if (n == "xinv_stage") *value = t.xinv_stage;
   else if (n == "x_rt") *value = t.x_rt;
   else if (n == "y_tx") *value = t.y_tx;
   else if (n == "z_tx") *value = t.z_tx;

@@ -320,3 +320,35 @@ def test_atlas_fused_mean_stage_bit_identical(grids):
                 np.testing.assert_array_equal(e, ref_e)
     finally:
         autotune.set_knob("atlas_mean", saved)
+
+
+def test_atlas_staged_composition_bit_identical():
+    """The batched composing pass (one launch over a batch of subjects) reading the
+    fields from rows staged in LDS (knob xinv_stage) against its gathers from
+    memory: hfields, mean and energies bit for bit over 6 iterations."""
+    import torch
+    from atlaswerks_amd import atlas, autotune
+    from gpu_util import to_dev
+    print("staged atlas", flush=True)
+    shape = (16, 32, 64)
+    ims = _subjects(shape, 4, seed=170)
+    nz, ny, nx = shape
+    subs = [atlas.AtlasSubject(to_dev(im)) for im in ims]
+    saved = autotune.get_knob("xinv_stage")
+
+    def run(stage):
+        autotune.set_knob("xinv_stage", stage)
+        b = atlas.GreedyAtlasBuilderGPU(subs, (nx, ny, nz), max_pert=0.2, batch=None)
+        b.UpdateStepSizeNextIteration()
+        b.Iterate(6)
+        torch.cuda.synchronize()
+        return [b.HField(i).cpu().numpy() for i in range(4)] + [b.Mean().cpu().numpy()], b.energies()
+
+    try:
+        ref_f, ref_e = run(0)
+        f, e = run(1)
+        for a, c in zip(f, ref_f):
+            np.testing.assert_array_equal(a, c)
+        np.testing.assert_array_equal(e, ref_e)
+    finally:
+        autotune.set_knob("xinv_stage", saved)

@@ -180,6 +180,85 @@ def test_iterate_n_equals_single_iterations(shape):
             autotune.set_knob(k, v)
 
 
+@pytest.mark.parametrize("shape", [(32, 32, 32), (16, 32, 256), (32, 64, 128), (64, 64, 64)])
+def test_composition_band_order_bit_identical(shape):
+    """The composing inverse x pass taking its row groups band by band (knob
+    xinv_band: 2, 4, 8 adjacent row groups per band) gives the plane-order
+    results bit for bit: GreedyWarp (one iteration, then the rest back to back,
+    each composition also forming the next deformed image) over 8 iterations,
+    and the per-iteration entry point over 3."""
+    import torch
+    from atlaswerks_amd import autotune
+    from atlaswerks_amd.greedy import (GreedyDeformationDataGPU, GreedyIteratorGPU, GreedyIteratorParam,
+                                       GreedyWarp)
+    from gpu_util import to_dev
+    print("xinv band", shape, flush=True)
+    t, target = _config0(shape, seed=7)
+    nz, ny, nx = shape
+    saved = autotune.get_knob("xinv_band")
+
+    def run(band):
+        autotune.set_knob("xinv_band", band)
+        w = GreedyWarp(to_dev(t), to_dev(target), 8, param=GreedyIteratorParam(max_pert=0.1))
+        d = w.RunWarp()
+        it = GreedyIteratorGPU((nx, ny, nz), (1.0, 1.0, 1.0), GreedyIteratorParam(max_pert=0.1))
+        d2 = GreedyDeformationDataGPU(to_dev(t), to_dev(target), max_iters=3)
+        it.UpdateStepSizeNextIteration()
+        for _ in range(3):
+            it.Iterate(d2)
+        torch.cuda.synchronize()
+        return d.Def1To0.cpu().numpy(), d.energies(), d2.Def1To0.cpu().numpy(), d2.energies()
+
+    try:
+        ref = run(0)
+        for band in (2, 4, 8):
+            got = run(band)
+            for a, c in zip(got, ref):
+                np.testing.assert_array_equal(a, c)
+    finally:
+        autotune.set_knob("xinv_band", saved)
+
+
+@pytest.mark.parametrize("shape", [(16, 32, 256), (8, 24, 128), (32, 64, 128), (64, 64, 64)])
+@pytest.mark.parametrize("max_pert", [0.1, 1.5])
+def test_composition_staged_rows_bit_identical(shape, max_pert):
+    """The composing inverse x pass reading the field from rows staged in LDS
+    (knob xinv_stage) against its gathers from memory: GreedyWarp over 8
+    iterations (back to back, each composition also forming the next deformed
+    image) and 3 per-iteration calls, bit for bit.  max_pert 1.5 moves many points
+    more than a voxel, outside the staged rows, where the staged pass gathers from
+    memory as well."""
+    import torch
+    from atlaswerks_amd import autotune
+    from atlaswerks_amd.greedy import (GreedyDeformationDataGPU, GreedyIteratorGPU, GreedyIteratorParam,
+                                       GreedyWarp)
+    from gpu_util import to_dev
+    print("xinv stage", shape, max_pert, flush=True)
+    t, target = _config0(shape, seed=9)
+    nz, ny, nx = shape
+    saved = autotune.get_knob("xinv_stage")
+
+    def run(stage):
+        autotune.set_knob("xinv_stage", stage)
+        w = GreedyWarp(to_dev(t), to_dev(target), 8, param=GreedyIteratorParam(max_pert=max_pert))
+        d = w.RunWarp()
+        it = GreedyIteratorGPU((nx, ny, nz), (1.0, 1.0, 1.0), GreedyIteratorParam(max_pert=max_pert))
+        d2 = GreedyDeformationDataGPU(to_dev(t), to_dev(target), max_iters=3)
+        it.UpdateStepSizeNextIteration()
+        for _ in range(3):
+            it.Iterate(d2)
+        torch.cuda.synchronize()
+        return d.Def1To0.cpu().numpy(), d.energies(), d2.Def1To0.cpu().numpy(), d2.energies()
+
+    try:
+        ref = run(0)
+        got = run(1)
+        for a, c in zip(got, ref):
+            np.testing.assert_array_equal(a, c)
+    finally:
+        autotune.set_knob("xinv_stage", saved)
+
+
 def test_iterate_n_rejects_overlapping_fields():
     import ctypes
     import torch
