@@ -50,6 +50,8 @@ struct XFwdArgs {
   float rsp[3], r2sp[3];  // 1 / sp and 1 / (2 sp), rounded (set by xfwd_spacing)
   double *partials;     // FORCE: one double per block (may be null)
   long long *acc;       // GRADMUL_FX: fixed-point multiplier, cleared after reading
+  int band;             // row groups per band of the block order (0: plane order); set by
+                        //   fk_xfwd from the xfwd_band knob
 };
 
 // The gradient's spacing and the reciprocals of its two divisors.

@@ -193,7 +193,16 @@ k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
   const int NL = 3 * RT / 2, LS = NL + 1;
   const int NR = ny * nz;
   const int lb = (int)xcd_block();
-  const int r0 = lb * RT;
+  int g = lb;  // the block's row group (its SSD partial's index)
+  if (a.band) {
+    // band order, as in the inverse x pass: each XCD takes its planes one band of
+    // a.band row groups at a time (the host checked the grid's shape)
+    const int G = ny / RT, per = (int)gridDim.x / kNumXcd, Zc = per / G, nbb = Zc * a.band;
+    const int c = lb / per, j = lb - c * per;
+    const int bnd = j / nbb, rem = j - bnd * nbb;
+    g = (c * Zc + rem / a.band) * G + bnd * a.band + rem % a.band;
+  }
+  const int r0 = g * RT;
   if (r0 >= NR) return;
   double ssd = 0.0;
   float *smf = reinterpret_cast<float *>(sm);
@@ -339,7 +348,7 @@ k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
       double s = 0.0;
       const int nw = (bdim + 63) >> 6;
       for (int i = 0; i < nw; ++i) s += red[i];
-      a.partials[lb] = s;
+      a.partials[g] = s;
     }
   }
 }
@@ -1114,9 +1123,17 @@ int fk_xblocks(const FluidKernel &k) {
   return (int)grid_for((long long)k.ny * k.nz, k.rt);
 }
 
-awk_status fk_xfwd(FluidKernel &k, const XFwdArgs &a, hipStream_t s, int *nblocks, const Batch *b) {
+awk_status fk_xfwd(FluidKernel &k, const XFwdArgs &a_in, hipStream_t s, int *nblocks, const Batch *b) {
   const Batch bt = b ? *b : Batch();
   const unsigned grid = grid_for((long long)k.ny * k.nz, k.rt);
+  XFwdArgs a = a_in;
+  a.band = 0;
+  {
+    const int bd = tuning().xfwd_band;
+    const long long G = k.ny / k.rt, per = grid / kNumXcd;
+    if (bd > 0 && k.ny % k.rt == 0 && G * k.nz == (long long)grid && per % G == 0 && G % bd == 0)
+      a.band = bd;
+  }
   const size_t lds = x_lds(k.ax[0], k.rt);
   const Axis ax = k.ax[0].dev();
 #define AWK_XF(NC, RTC, BDC)                                                                     \

@@ -376,10 +376,10 @@ awk_status awk_tune_set(const char *name, int value) {
     t.unet_cc = value;
     return AWK_OK;
   }
-  if (n == "xinv_band") {
+  if (n == "xinv_band" || n == "xfwd_band") {
     if (value != 0 && value != 2 && value != 4 && value != 8 && value != 16 && value != 32)
-      return fail(AWK_ERR_INVALID, "awk_tune_set: xinv_band must be 0, 2, 4, 8, 16 or 32");
-    t.xinv_band = value;
+      return fail(AWK_ERR_INVALID, "awk_tune_set: " + n + " must be 0, 2, 4, 8, 16 or 32");
+    (n == "xinv_band" ? t.xinv_band : t.xfwd_band) = value;
     return AWK_OK;
   }
   if (n == "atlas_mean") {
@@ -434,6 +434,7 @@ awk_status awk_tune_get(const char *name, int *value) {
   else if (n == "unet_cc") *value = t.unet_cc;
   else if (n == "atlas_mean") *value = t.atlas_mean;
   else if (n == "xinv_band") *value = t.xinv_band;
+  else if (n == "xfwd_band") *value = t.xfwd_band;
   else if (n == "xinv_stage") *value = t.xinv_stage;
   else if (n == "x_rt") *value = t.x_rt;
   else if (n == "y_tx") *value = t.y_tx;

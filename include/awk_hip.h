@@ -68,7 +68,8 @@ enum {
  * "apply_vpt", "compose_vpt" (voxels per thread: 1, 2, 4), "xinv_unroll"
  * (voxels per step of the fused inverse-x epilogue: 1, 2, 4), "xinv_pairs" (0/1),
  * "xinv_band" (row groups per band of the composing inverse x pass's block order:
- * 0 plane by plane, 2, 4, 8, 16, 32), "xinv_stage" (that pass reads the field
+ * 0 plane by plane, 2, 4, 8, 16, 32), "xfwd_band" (the same for the forward x
+ * pass), "xinv_stage" (that pass reads the field
  * from the block's rows and their neighbours staged in LDS: 0 never, 1 always,
  * 2 rows of at most 128),
  * "splat_tile" (0 or 1: the fixed-point splat through per-block LDS windows),

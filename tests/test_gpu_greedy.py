@@ -182,8 +182,8 @@ def test_iterate_n_equals_single_iterations(shape):
 
 @pytest.mark.parametrize("shape", [(32, 32, 32), (16, 32, 256), (32, 64, 128), (64, 64, 64)])
 def test_composition_band_order_bit_identical(shape):
-    """The composing inverse x pass taking its row groups band by band (knob
-    xinv_band: 2, 4, 8 adjacent row groups per band) gives the plane-order
+    """The inverse and forward x passes taking their row groups band by band
+    (knobs xinv_band, xfwd_band: 2, 4, 8 adjacent row groups per band) give the plane-order
     results bit for bit: GreedyWarp (one iteration, then the rest back to back,
     each composition also forming the next deformed image) over 8 iterations,
     and the per-iteration entry point over 3."""
@@ -195,10 +195,11 @@ def test_composition_band_order_bit_identical(shape):
     print("xinv band", shape, flush=True)
     t, target = _config0(shape, seed=7)
     nz, ny, nx = shape
-    saved = autotune.get_knob("xinv_band")
+    saved = {k: autotune.get_knob(k) for k in ("xinv_band", "xfwd_band")}
 
     def run(band):
         autotune.set_knob("xinv_band", band)
+        autotune.set_knob("xfwd_band", band)
         w = GreedyWarp(to_dev(t), to_dev(target), 8, param=GreedyIteratorParam(max_pert=0.1))
         d = w.RunWarp()
         it = GreedyIteratorGPU((nx, ny, nz), (1.0, 1.0, 1.0), GreedyIteratorParam(max_pert=0.1))
@@ -216,7 +217,8 @@ def test_composition_band_order_bit_identical(shape):
             for a, c in zip(got, ref):
                 np.testing.assert_array_equal(a, c)
     finally:
-        autotune.set_knob("xinv_band", saved)
+        for k, v in saved.items():
+            autotune.set_knob(k, v)
 
 
 @pytest.mark.parametrize("shape", [(16, 32, 256), (8, 24, 128), (32, 64, 128), (64, 64, 64)])
