@@ -162,7 +162,8 @@ struct Tuning {
   int xinv_stage = 2;   // composing inverse x pass reads the field's rows staged in LDS:
                         // 0 never, 1 always, 2 rows of at most 128 (where it measured faster)
   int xinv_band = 8;
-  int xfwd_band = 4;    // forward x pass: row groups per band of its block order (0: planes)    // composing inverse x pass: row groups per band of its block order (0: planes)
+  int xfwd_band = 4;
+  int yfwd_rev = 0;     // forward y pass visits its planes in reverse (0: in order)    // forward x pass: row groups per band of its block order (0: planes)    // composing inverse x pass: row groups per band of its block order (0: planes)
   int atlas_mean = 4;   // greedy atlas deformed images + mean: subjects per group of one fused
                         // launch (1, 2, 4), or 0: one apply launch per subject, then the mean
   int splat_tile = 1;   // fixed-point splat through per-block LDS windows (0: direct atomics)

@@ -671,15 +671,18 @@ k_ypass_reg(const float2 *__restrict__ tw, int nz, int nxh, int nxh_pad, float2 
   const int lb = (int)xcd_block();
   if (lb >= 3 * nz * ntile) return;
   const int tile = lb % ntile, rest = lb / ntile;
-  // the forward pass visits (c, z) in reverse: the x pass wrote the spectrum in
-  // increasing row order, so its last-written planes, the likeliest still held by
-  // the memory-side cache, are read first (256^3: 78.5 -> 75.0 us, measured);
-  // every block transforms its own columns, so the order changes no result
-  const int z = inv ? rest % nz : nz - 1 - rest % nz, c = inv ? rest / nz : 2 - rest / nz;
+  // inv = 0: the forward pass visits (c, z) in reverse, which pays when the x pass
+  // wrote the spectrum plane by plane (its last-written planes, the likeliest still
+  // held by the memory-side cache, are read first; 256^3: 78.5 -> 75.0 us,
+  // measured); inv = 2: in plane order, faster after the x pass's band order
+  // (79.9 -> 78.0 us).  Every block transforms its own columns, so the order
+  // changes no result.
+  const bool rev = inv == 0;
+  const int z = rev ? nz - 1 - rest % nz : rest % nz, c = rev ? 2 - rest / nz : rest / nz;
   const int kx = tile * TX + t;
   const bool valid = kx < nxh;
   float2 *base = spec + c * plane + (size_t)z * N * nxh_pad + kx;
-  const bool iv = inv != 0;
+  const bool iv = inv == 1;
   if (j < PA) {  // stage 1: radix RA, Ns = 1, inputs y = j + r * PA
     float2 v[RA];
 #pragma unroll
@@ -1161,7 +1164,8 @@ awk_status fk_ypass(FluidKernel &k, bool inv, hipStream_t s, const ReduceArgs *r
   const unsigned g16 = grid_for((long long)3 * k.nz * ntile16, 1);
 #define AWK_YR(N, RA, RB)                                                                     \
   AWK_LAUNCH((k_ypass_reg<N, RA, RB, 16>), dim3(g16, bt.n), dim3(16 * (N / RA > N / RB ? N / RA : N / RB)), \
-                     0, s, tw, k.nz, k.nxh, k.nxh_pad, k.spec, k.spec_plane(), inv ? 1 : 0, ra, bt)
+                     0, s, tw, k.nz, k.nxh, k.nxh_pad, k.spec, k.spec_plane(),                    \
+                     inv ? 1 : tuning().yfwd_rev ? 0 : 2, ra, bt)
   if (k.ny == 256) AWK_YR(256, 16, 16);
   else if (k.ny == 128) AWK_YR(128, 16, 8);
   else if (k.ny == 64) AWK_YR(64, 8, 8);

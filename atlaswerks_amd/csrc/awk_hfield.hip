@@ -393,10 +393,13 @@ awk_status awk_tune_set(const char *name, int value) {
     t.unet_pipe = value;
     return AWK_OK;
   }
-  if (n == "splat_tile" || n == "fft_order" || n == "unet_lds" || n == "xinv_pairs") {
+  if (n == "splat_tile" || n == "fft_order" || n == "unet_lds" || n == "xinv_pairs" || n == "yfwd_rev") {
     if (value != 0 && value != 1) return fail(AWK_ERR_INVALID, "awk_tune_set: " + n + " must be 0 or 1");
-    (n == "splat_tile" ? t.splat_tile : n == "fft_order" ? t.fft_order : n == "unet_lds" ? t.unet_lds
-                                                                                          : t.xinv_pairs) = value;
+    (n == "splat_tile"   ? t.splat_tile
+     : n == "fft_order"  ? t.fft_order
+     : n == "unet_lds"   ? t.unet_lds
+     : n == "xinv_pairs" ? t.xinv_pairs
+                         : t.yfwd_rev) = value;
     return AWK_OK;
   }
   if (n == "xinv_stage") {
@@ -435,6 +438,7 @@ awk_status awk_tune_get(const char *name, int *value) {
   else if (n == "atlas_mean") *value = t.atlas_mean;
   else if (n == "xinv_band") *value = t.xinv_band;
   else if (n == "xfwd_band") *value = t.xfwd_band;
+  else if (n == "yfwd_rev") *value = t.yfwd_rev;
   else if (n == "xinv_stage") *value = t.xinv_stage;
   else if (n == "x_rt") *value = t.x_rt;
   else if (n == "y_tx") *value = t.y_tx;

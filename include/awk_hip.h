@@ -69,7 +69,7 @@ enum {
  * (voxels per step of the fused inverse-x epilogue: 1, 2, 4), "xinv_pairs" (0/1),
  * "xinv_band" (row groups per band of the composing inverse x pass's block order:
  * 0 plane by plane, 2, 4, 8, 16, 32), "xfwd_band" (the same for the forward x
- * pass), "xinv_stage" (that pass reads the field
+ * pass), "yfwd_rev" (0/1: the forward y pass visits its planes in reverse), "xinv_stage" (that pass reads the field
  * from the block's rows and their neighbours staged in LDS: 0 never, 1 always,
  * 2 rows of at most 128),
  * "splat_tile" (0 or 1: the fixed-point splat through per-block LDS windows),

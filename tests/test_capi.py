@@ -42,7 +42,7 @@ def test_knob_ranges_without_device():
     lib = _lib.load()
     table = {"apply_vpt": (1, 2, 4), "compose_vpt": (1, 2, 4), "xinv_unroll": (1, 2, 4),
              "xinv_pairs": (0, 1), "splat_tile": (0, 1), "unet_lds": (0, 1), "unet_pipe": (0, 1, 2),
-             "unet_cc": (0, 8, 16, 32), "atlas_mean": (0, 1, 2, 4), "xinv_band": (0, 2, 4, 8, 16, 32), "xfwd_band": (0, 2, 4, 8, 16, 32), "xinv_stage": (0, 1, 2), "fft_order": (0, 1),
+             "unet_cc": (0, 8, 16, 32), "atlas_mean": (0, 1, 2, 4), "xinv_band": (0, 2, 4, 8, 16, 32), "xfwd_band": (0, 2, 4, 8, 16, 32), "yfwd_rev": (0, 1), "xinv_stage": (0, 1, 2), "fft_order": (0, 1),
              "x_rt": (0, 2, 4, 6, 8), "y_tx": (0, 1, 2, 4, 8, 16), "z_tx": (0, 1, 2, 4, 8)}
     for name, ok in table.items():
         v = ctypes.c_int()

@@ -195,11 +195,12 @@ def test_composition_band_order_bit_identical(shape):
     print("xinv band", shape, flush=True)
     t, target = _config0(shape, seed=7)
     nz, ny, nx = shape
-    saved = {k: autotune.get_knob(k) for k in ("xinv_band", "xfwd_band")}
+    saved = {k: autotune.get_knob(k) for k in ("xinv_band", "xfwd_band", "yfwd_rev")}
 
     def run(band):
         autotune.set_knob("xinv_band", band)
         autotune.set_knob("xfwd_band", band)
+        autotune.set_knob("yfwd_rev", 1 if band else 0)
         w = GreedyWarp(to_dev(t), to_dev(target), 8, param=GreedyIteratorParam(max_pert=0.1))
         d = w.RunWarp()
         it = GreedyIteratorGPU((nx, ny, nz), (1.0, 1.0, 1.0), GreedyIteratorParam(max_pert=0.1))
