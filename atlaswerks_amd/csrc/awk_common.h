@@ -196,6 +196,20 @@ __device__ __forceinline__ unsigned xcd_block() {
   return (b % kNumXcd) * (nb / kNumXcd) + b / kNumXcd;
 }
 
+// The band block order of a grid of G blocks per plane (the x passes: one block
+// per RT rows): where whole planes fill each XCD's share of the grid (the host
+// checks it), XCD k takes its planes one band of `band` adjacent blocks at a time,
+// so its resident blocks span a narrow y range over many planes and their
+// gathers' footprint in its L2 shrinks; band 0 keeps the plane order.  A
+// bijection of the logical block index.
+__device__ __forceinline__ int band_block(int lb, int band, int G) {
+  if (!band) return lb;
+  const int per = (int)gridDim.x / kNumXcd, Zc = per / G, nbb = Zc * band;
+  const int c = lb / per, j = lb - c * per;
+  const int bnd = j / nbb, rem = j - bnd * nbb;
+  return (c * Zc + rem / band) * G + bnd * band + rem % band;
+}
+
 // ---------------------------------------------------------------- complex
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);

@@ -193,15 +193,7 @@ k_xfwd(XFwdArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
   const int NL = 3 * RT / 2, LS = NL + 1;
   const int NR = ny * nz;
   const int lb = (int)xcd_block();
-  int g = lb;  // the block's row group (its SSD partial's index)
-  if (a.band) {
-    // band order, as in the inverse x pass: each XCD takes its planes one band of
-    // a.band row groups at a time (the host checked the grid's shape)
-    const int G = ny / RT, per = (int)gridDim.x / kNumXcd, Zc = per / G, nbb = Zc * a.band;
-    const int c = lb / per, j = lb - c * per;
-    const int bnd = j / nbb, rem = j - bnd * nbb;
-    g = (c * Zc + rem / a.band) * G + bnd * a.band + rem % a.band;
-  }
+  const int g = band_block(lb, a.band, ny / RT);  // the row group (its SSD partial's index)
   const int r0 = g * RT;
   if (r0 >= NR) return;
   double ssd = 0.0;
@@ -380,17 +372,7 @@ k_xinv(XInvArgs a, Axis ax, int nx_rt, int ny, int nz, int nxh_rt, int nxh_pad, 
   const int NL = 3 * RT / 2, LS = NL + 1;
   const int NR = ny * nz;
   const int lb = (int)xcd_block();
-  int g = lb;
-  if (a.band) {
-    // band order (the host checked that whole planes of G row groups fill each
-    // XCD's share): an XCD takes its z-planes band by band, a band being
-    // a.band adjacent row groups, so its resident blocks cover a narrow y range
-    // over many z-planes and their gathers' footprint shrinks in its L2
-    const int G = ny / RT, per = (int)gridDim.x / kNumXcd, Zc = per / G, nbb = Zc * a.band;
-    const int c = lb / per, j = lb - c * per;
-    const int bnd = j / nbb, rem = j - bnd * nbb;
-    g = (c * Zc + rem / a.band) * G + bnd * a.band + rem % a.band;
-  }
+  const int g = band_block(lb, a.band, ny / RT);  // the row group
   const int r0 = g * RT;
   if (r0 >= NR) {
     // a padding block of the grid: its max partial is still read
